@@ -1,0 +1,198 @@
+/*
+ * mm_hotpath.h -- C ABI of libmmhip.so, the MI355X (gfx950) implementation of
+ * MicroMagnum's effective-field and Runge-Kutta hot path.
+ *
+ * Conventions
+ *   - Every function returns MM_OK (0) or an error code; mm_last_error()
+ *     returns the message of the last failure on the calling thread.  The
+ *     reference raises RuntimeError from C++ exceptions
+ *     (src/magneto/bindings/magneto.i:31-42); a binding maps a non-zero code
+ *     to the same exception.
+ *   - Array arguments are DEVICE pointers unless the name ends in _host.
+ *     Layout is the reference's (src/magneto/matrix/Shape.cpp:64-75):
+ *     x fastest, index x + nx*(y + ny*z); vector fields are SoA, three
+ *     consecutive planes of n = nx*ny*nz values (x, y, z components).
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream).
+ *     Functions that return a scalar (energies, norms) synchronise the
+ *     stream; the others only enqueue work.
+ *   - FP64 entry points compute in double.  *_f32 entry points take float
+ *     arrays and compute in float (the mixed-precision bulk path).
+ */
+#ifndef MM_HOTPATH_H
+#define MM_HOTPATH_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    MM_OK = 0,
+    MM_ERR_INVALID = 1,     /* bad argument (shape, pointer, option)       */
+    MM_ERR_RUNTIME = 2,     /* internal failure                            */
+    MM_ERR_HIP = 3,         /* HIP runtime error                           */
+    MM_ERR_UNSUPPORTED = 4  /* option the library does not implement        */
+};
+
+/* padding policies: src/magneto/bindings/mmm/demag.i:27-34 */
+enum {
+    MM_PADDING_DISABLE = 0,
+    MM_PADDING_ROUND_2 = 1,
+    MM_PADDING_ROUND_4 = 2,
+    MM_PADDING_ROUND_8 = 3,
+    MM_PADDING_ROUND_POT = 4,
+    MM_PADDING_SMALL_PRIME_FACTORS = 5
+};
+
+/* precision of a convolution object / stepper */
+enum { MM_FP64 = 0, MM_FP32 = 1 };
+
+/* Runge-Kutta methods: src/magnum/evolver/tableaus.py:54-139 (+ classical RK4,
+ * src/magnum/evolver/runge_kutta_4.py:28-59) */
+enum { MM_RKF45 = 0, MM_CC45 = 1, MM_DP54 = 2, MM_RK23 = 3, MM_RK4 = 4 };
+
+/* ---------------------------------------------------------------- library */
+/* Last error message of this thread (never NULL). */
+const char *mm_last_error(void);
+/* Select the HIP device and initialise the library on it.  Replaces
+ * magneto.initialize + enableCuda (src/magneto/Magneto.cpp:56-73,153-176). */
+int mm_init(int device);
+int mm_device_alloc(size_t bytes, void **out);
+int mm_device_free(void *ptr);
+int mm_memcpy_htod(void *dst, const void *src_host, size_t bytes, void *stream);
+int mm_memcpy_dtoh(void *dst_host, const void *src, size_t bytes, void *stream);
+int mm_synchronize(void *stream);
+
+/* ------------------------------------------------------- demag tensor setup */
+/* round_tensor_dimension (src/magneto/mmm/demag/tensor_round.cpp:90-106) */
+int mm_round_tensor_dimension(int dim, int periodic, int padding, int *out);
+/* GenerateDemagTensor (src/magneto/mmm/demag/demag_tensor.cpp:116-186), open
+ * boundaries.  exp_out receives (ex, ey, ez).  N_host (may be NULL to query the
+ * size) receives 6*ex*ey*ez doubles, component fastest (xx,xy,xz,yy,yz,zz),
+ * already sign-flipped so that H = N (*) M.  Periodic axes return
+ * MM_ERR_UNSUPPORTED.  nthreads <= 0: one per host core. */
+int mm_generate_demag_tensor(int nx, int ny, int nz, double dx, double dy, double dz,
+                             int px, int py, int pz, int repeat, int padding,
+                             int nthreads, double *N_host, int *exp_out);
+
+/* ------------------------------------------------------ FFT convolution */
+/* SymmetricMatrixVectorConvolution_FFT(N, nx, ny, nz)
+ * (src/magneto/math/conv/SymmetricMatrixVectorConvolution_FFT.cpp:31-65).
+ * N_host: tensor as returned by mm_generate_demag_tensor for the padded size
+ * (ex, ey, ez).  precision MM_FP64 or MM_FP32 selects the arithmetic of the
+ * transforms; the Fourier kernel is computed in FP64 either way. */
+typedef struct mm_conv mm_conv_t;
+int mm_conv_create(const double *N_host, int nx, int ny, int nz, int ex, int ey, int ez,
+                   int precision, mm_conv_t **out);
+/* execute(M, H) (MatrixVectorConvolution_FFT.cpp:110-375): H = N (*) M. */
+int mm_conv_execute(mm_conv_t *conv, const double *M, double *H, void *stream);
+int mm_conv_execute_f32(mm_conv_t *conv, const float *M, float *H, void *stream);
+/* padded FFT sizes actually used (radix-friendly; result-neutral for open
+ * boundaries) */
+int mm_conv_fft_size(const mm_conv_t *conv, int *fft_xyz);
+int mm_conv_destroy(mm_conv_t *conv);
+
+/* ------------------------------------------------------ local field terms */
+/* exchange (src/magneto/mmm/exchange/exchange.cpp:31-77; bindings/mmm/exchange.i:24-39).
+ * energy (may be NULL) receives -mu0/2 * dV * sum_i m_i.H_i. */
+int mm_exchange(int nx, int ny, int nz, double dx, double dy, double dz,
+                int px, int py, int pz, const double *Ms, const double *A,
+                const double *M, double *H, double *energy, void *stream);
+int mm_exchange_f32(int nx, int ny, int nz, double dx, double dy, double dz,
+                    int px, int py, int pz, const float *Ms, const float *A,
+                    const float *M, float *H, void *stream);
+/* uniaxial_anisotropy(axis, k, Ms, M, H) (anisotropy.cpp:33-58;
+ * bindings/mmm/anisotropy.i:24-39); energy = sum_i k |a x m|^2 */
+int mm_uniaxial_anisotropy(int n, const double *axis, const double *k, const double *Ms,
+                           const double *M, double *H, double *energy, void *stream);
+/* cubic_anisotropy(axis1, axis2, k, Ms, M, H) (anisotropy.cpp:60-88) */
+int mm_cubic_anisotropy(int n, const double *axis1, const double *axis2, const double *k,
+                        const double *Ms, const double *M, double *H, double *energy,
+                        void *stream);
+/* llge(f1, f2, M, H, dM) (src/magneto/mmm/llge/llge.cpp:33-59) */
+int mm_llge(int n, const double *f1, const double *f2, const double *M, const double *H,
+            double *dM, void *stream);
+/* fdm_zhangli (src/magneto/mmm/spintorque/fdm_zhangli.cpp:33-53) */
+int mm_fdm_zhangli(int nx, int ny, int nz, double dx, double dy, double dz, int do_precess,
+                   const double *P, const double *xi, const double *Ms, const double *alpha,
+                   const double *j, const double *M, double *dM, void *stream);
+
+/* ------------------------------------------------------ reductions */
+/* scaled_abs_max(M, scale) (src/magneto/math/ScaledAbsMax.cpp:52-68):
+ * uniform scale -> absMax(M)/scale[0]; else max over scale!=0 of |M|/scale. */
+int mm_scaled_abs_max(int n, const double *M, const double *scale, int scale_uniform,
+                      double *out, void *stream);
+/* VectorMatrix::absMax / dotSum / average (VectorMatrix.cpp:187-274) */
+int mm_abs_max(int n, const double *M, double *out, void *stream);
+int mm_dot_sum(int n, const double *A, const double *B, double *out, void *stream);
+int mm_average(int n, const double *M, double *out3, void *stream);
+/* VectorMatrix::normalize(Matrix len) (CPUDevice.cpp:244-271).
+ * mode 0: reference CPU semantics (|M| from float-cast components, zero
+ * vectors kept); mode 1: FP64 norm. */
+int mm_normalize(int n, double *M, const double *len, int mode, void *stream);
+
+/* ------------------------------------------------------ Runge-Kutta algebra */
+typedef struct {
+    int num_steps, order, fsal;
+    double a[8], b[8][8], c[8], ec[8];
+} mm_tableau_t;
+/* ButcherTableau for a method (src/magneto/evolver/runge_kutta.h:31-72). */
+int mm_tableau(int method, mm_tableau_t *out);
+/* rk_prepare_step (runge_kutta.cpp:32-48): ytmp = y + h*sum_{j<step} b[step][j] k_j.
+ * k: host array of num_steps device pointers (entries >= step may be NULL). */
+int mm_rk_prepare_step(int step, double h, const mm_tableau_t *tab, int n,
+                       const double *const *k, const double *y, double *ytmp, void *stream);
+/* rk_combine_result (runge_kutta.cpp:50-96): y += h*sum c_j k_j; y_err = h*sum ec_j k_j */
+int mm_rk_combine_result(double h, const mm_tableau_t *tab, int n, const double *const *k,
+                         double *y, double *y_err, void *stream);
+
+/* ------------------------------------------------------ fused stepper */
+/* Whole LLG system on the device: H_tot = H_stray + H_exch + H_aniso + H_ext
+ * (module order of the reference's LandauLifshitzGilbert.calculate_H_tot,
+ * landau_lifshitz_gilbert.py:92-101), dM/dt = LLG + Zhang-Li, stepped by an
+ * RK method with MicroMagneticsStepSizeController
+ * (micro_magnetics_stepsize_controller.py:35-73) and M renormalised to Ms
+ * after every accepted step (micro_magnetics.py:49-50).  Accept/reject and
+ * the step size stay on the device; one call runs many steps. */
+typedef struct {
+    int nx, ny, nz;
+    double dx, dy, dz;
+    int precision;          /* MM_FP64 or MM_FP32 (FP32 bulk, FP64 step control) */
+    int method;             /* MM_RKF45 ... MM_RK4                                */
+    double eps_abs, eps_rel;
+    double h0;              /* initial / fixed step size [s]                      */
+    int normalize_mode;     /* 0 reference float-norm, 1 FP64                     */
+    int do_precess;
+    /* host arrays of n (scalars) or 3n (vectors, SoA); NULL = term absent
+     * (Ms and alpha are required; A NULL = no exchange). */
+    const double *Ms_host, *alpha_host, *A_host;
+    const double *k_uni_host, *k_cub_host, *axis1_host, *axis2_host;
+    const double *P_host, *xi_host, *j_host;
+    int use_zeeman;
+    double h_ext[3];        /* uniform external field [A/m]                       */
+    const double *N_host;   /* demag tensor (6*ex*ey*ez) or NULL = no stray field */
+    int ex, ey, ez;
+} mm_system_desc_t;
+
+typedef struct mm_stepper mm_stepper_t;
+int mm_stepper_create(const mm_system_desc_t *desc, mm_stepper_t **out);
+int mm_stepper_set_m(mm_stepper_t *st, const double *M_host);
+int mm_stepper_get_m(mm_stepper_t *st, double *M_host);
+/* One evaluation of dM/dt and H_tot at the current M (host outputs, may be NULL). */
+int mm_stepper_dmdt(mm_stepper_t *st, double *dMdt_host, double *Htot_host);
+/* Run `steps` accepted steps (adaptive methods retry rejected attempts on
+ * the device).  Returns when the work is enqueued; mm_stepper_sync waits. */
+int mm_stepper_run(mm_stepper_t *st, int steps);
+int mm_stepper_sync(mm_stepper_t *st);
+/* stats: [0]=t [1]=h of last accepted step [2]=next h [3]=attempts [4]=rejections */
+int mm_stepper_stats(mm_stepper_t *st, double *stats5);
+/* <M> per component (FP64 sums). */
+int mm_stepper_average_m(mm_stepper_t *st, double *avg3);
+void *mm_stepper_stream(mm_stepper_t *st);
+int mm_stepper_destroy(mm_stepper_t *st);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

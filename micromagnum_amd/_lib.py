@@ -1,0 +1,129 @@
+"""ctypes binding of libmmhip.so (the C ABI declared in include/mm_hotpath.h).
+
+The shared library is built in-tree by ``micromagnum_amd.build.build()``.
+Nothing here falls back to another implementation: a missing library or a
+missing GPU raises.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmmhip.so")
+
+MM_OK = 0
+MM_FP64, MM_FP32 = 0, 1
+MM_RKF45, MM_CC45, MM_DP54, MM_RK23, MM_RK4 = 0, 1, 2, 3, 4
+METHODS = {"rkf45": MM_RKF45, "cc45": MM_CC45, "dp54": MM_DP54, "rk23": MM_RK23, "rk4": MM_RK4}
+
+PADDING_DISABLE = 0
+PADDING_ROUND_2 = 1
+PADDING_ROUND_4 = 2
+PADDING_ROUND_8 = 3
+PADDING_ROUND_POT = 4
+PADDING_SMALL_PRIME_FACTORS = 5
+
+_vp = ctypes.c_void_p
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_i = ctypes.c_int
+_d = ctypes.c_double
+
+
+class Tableau(ctypes.Structure):
+    _fields_ = [("num_steps", _i), ("order", _i), ("fsal", _i),
+                ("a", _d * 8), ("b", (_d * 8) * 8), ("c", _d * 8), ("ec", _d * 8)]
+
+
+class SystemDesc(ctypes.Structure):
+    _fields_ = [("nx", _i), ("ny", _i), ("nz", _i),
+                ("dx", _d), ("dy", _d), ("dz", _d),
+                ("precision", _i), ("method", _i),
+                ("eps_abs", _d), ("eps_rel", _d), ("h0", _d),
+                ("normalize_mode", _i), ("do_precess", _i),
+                ("Ms_host", _dp), ("alpha_host", _dp), ("A_host", _dp),
+                ("k_uni_host", _dp), ("k_cub_host", _dp), ("axis1_host", _dp), ("axis2_host", _dp),
+                ("P_host", _dp), ("xi_host", _dp), ("j_host", _dp),
+                ("use_zeeman", _i), ("h_ext", _d * 3),
+                ("N_host", _dp), ("ex", _i), ("ey", _i), ("ez", _i)]
+
+
+# name -> (restype, argtypes); every symbol of include/mm_hotpath.h
+SIGNATURES = {
+    "mm_last_error": (ctypes.c_char_p, []),
+    "mm_init": (_i, [_i]),
+    "mm_device_alloc": (_i, [ctypes.c_size_t, ctypes.POINTER(_vp)]),
+    "mm_device_free": (_i, [_vp]),
+    "mm_memcpy_htod": (_i, [_vp, _vp, ctypes.c_size_t, _vp]),
+    "mm_memcpy_dtoh": (_i, [_vp, _vp, ctypes.c_size_t, _vp]),
+    "mm_synchronize": (_i, [_vp]),
+    "mm_round_tensor_dimension": (_i, [_i, _i, _i, _ip]),
+    "mm_generate_demag_tensor": (_i, [_i, _i, _i, _d, _d, _d, _i, _i, _i, _i, _i, _i, _dp, _ip]),
+    "mm_conv_create": (_i, [_dp, _i, _i, _i, _i, _i, _i, _i, ctypes.POINTER(_vp)]),
+    "mm_conv_execute": (_i, [_vp, _vp, _vp, _vp]),
+    "mm_conv_execute_f32": (_i, [_vp, _vp, _vp, _vp]),
+    "mm_conv_fft_size": (_i, [_vp, _ip]),
+    "mm_conv_destroy": (_i, [_vp]),
+    "mm_exchange": (_i, [_i, _i, _i, _d, _d, _d, _i, _i, _i, _vp, _vp, _vp, _vp, _dp, _vp]),
+    "mm_exchange_f32": (_i, [_i, _i, _i, _d, _d, _d, _i, _i, _i, _vp, _vp, _vp, _vp, _vp]),
+    "mm_uniaxial_anisotropy": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _dp, _vp]),
+    "mm_cubic_anisotropy": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _vp, _dp, _vp]),
+    "mm_llge": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "mm_fdm_zhangli": (_i, [_i, _i, _i, _d, _d, _d, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "mm_scaled_abs_max": (_i, [_i, _vp, _vp, _i, _dp, _vp]),
+    "mm_abs_max": (_i, [_i, _vp, _dp, _vp]),
+    "mm_dot_sum": (_i, [_i, _vp, _vp, _dp, _vp]),
+    "mm_average": (_i, [_i, _vp, _dp, _vp]),
+    "mm_normalize": (_i, [_i, _vp, _vp, _i, _vp]),
+    "mm_tableau": (_i, [_i, ctypes.POINTER(Tableau)]),
+    "mm_rk_prepare_step": (_i, [_i, _d, ctypes.POINTER(Tableau), _i, ctypes.POINTER(_vp), _vp, _vp, _vp]),
+    "mm_rk_combine_result": (_i, [_d, ctypes.POINTER(Tableau), _i, ctypes.POINTER(_vp), _vp, _vp, _vp]),
+    "mm_stepper_create": (_i, [ctypes.POINTER(SystemDesc), ctypes.POINTER(_vp)]),
+    "mm_stepper_set_m": (_i, [_vp, _dp]),
+    "mm_stepper_get_m": (_i, [_vp, _dp]),
+    "mm_stepper_dmdt": (_i, [_vp, _dp, _dp]),
+    "mm_stepper_run": (_i, [_vp, _i]),
+    "mm_stepper_sync": (_i, [_vp]),
+    "mm_stepper_stats": (_i, [_vp, _dp]),
+    "mm_stepper_average_m": (_i, [_vp, _dp]),
+    "mm_stepper_stream": (_vp, [_vp]),
+    "mm_stepper_destroy": (_i, [_vp]),
+}
+
+_LIB = None
+_INIT = False
+
+
+def load():
+    """Load libmmhip.so and declare every entry point (no GPU needed)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libmmhip.so is not built (%s); run micromagnum_amd.build.build()" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def check(rc):
+    if rc != MM_OK:
+        msg = load().mm_last_error().decode("utf-8", "replace")
+        raise RuntimeError("libmmhip error %d: %s" % (rc, msg))
+
+
+def device_lib():
+    """The library with a HIP device initialised; raises without a GPU."""
+    global _INIT
+    L = load()
+    if not _INIT:
+        import torch
+        if not torch.cuda.is_available():
+            raise RuntimeError("micromagnum_amd needs a HIP GPU (torch.cuda.is_available() is False)")
+        dev = torch.cuda.current_device()
+        torch.cuda.init()
+        check(L.mm_init(dev))
+        _INIT = True
+    return L

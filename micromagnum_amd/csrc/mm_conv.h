@@ -1,0 +1,93 @@
+// mm_conv.h -- FFT demag convolution objects (host side of mm_conv.hip).
+#ifndef MM_CONV_H
+#define MM_CONV_H
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "mm_fft_device.h"
+#include "mm_internal.h"
+
+#define MM_HIP(call)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            throw mm_error(MM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// host plan of one FFT length: radix list, threads per line, twiddle tables
+struct FftPlanHost {
+    int L = 1;
+    int tline = 1;
+    std::vector<int> rad;
+    void *tw32 = nullptr;  // float2[L]
+    void *tw64 = nullptr;  // double2[L]
+    void build(int len);
+    void release();
+    template <typename T> FftPlanDev dev() const
+    {
+        FftPlanDev d;
+        d.L = L;
+        d.tline = tline;
+        d.npass = (int)rad.size();
+        for (int i = 0; i < d.npass; ++i) d.rad[i] = rad[i];
+        d.tw = sizeof(T) == 4 ? tw32 : tw64;
+        return d;
+    }
+};
+
+bool mm_is_smooth(int n);                  // n has no prime factor > 5
+int mm_fft_size_for(int n, int given);     // radix-friendly padded size for n cells
+
+// Geometry of a convolution: physical (nx,ny,nz), padded FFT sizes (Px,Py,Pz).
+struct ConvGeom {
+    int nx, ny, nz, Px, Py, Pz, Hx, Hy2, Hz2;
+    bool is2d;
+};
+
+// Tile widths (lines per workgroup) for the strided passes.
+struct ConvLaunch {
+    int lpw_x;  // x lines per WG (plain x passes)
+    int tk_y;   // kx columns per WG, y passes
+    int tk_z;   // kx columns per WG, z pass (3 components each)
+};
+
+template <typename T>
+struct ConvImpl {
+    ConvGeom g;
+    ConvLaunch ln;
+    FftPlanHost px, py, pz;
+    T *kern = nullptr;      // 6 real planes [e][kz<Hz2][ky<Hy2][kx<Hx], scaled 1/(PxPyPz)
+    C2<T> *S1 = nullptr;    // [c][z<nz][y<ny][kx<Hx]
+    C2<T> *S2 = nullptr;    // [c][z<nz][ky<Py][kx<Hx]   (3-D only)
+    size_t kern_elems = 0;
+
+    void init(const double *N_host, int nx, int ny, int nz, int ex, int ey, int ez);
+    void release();
+    // spectral part: pad+x forward .. x inverse up to S1 holding the result
+    // spectrum rows; then x inverse into H.
+    void forward_x(const T *M, hipStream_t s, const T *M_alt = nullptr, const int *sel = nullptr);
+    void spectral(hipStream_t s);  // y fwd, z fwd*mult*inv / 2-D fused, y inv
+    void inverse_x(T *H, hipStream_t s);
+    void execute(const T *M, T *H, hipStream_t s)
+    {
+        forward_x(M, s);
+        spectral(s);
+        inverse_x(H, s);
+    }
+};
+
+// device-side FFT kernels reused by the fused stepper
+template <typename T> size_t conv_lds_xinv(const ConvImpl<T> &c, int lines) { return (size_t)lines * (c.px.L + 1) * sizeof(C2<T>); }
+
+struct mm_conv {
+    int precision;
+    ConvImpl<float> *c32 = nullptr;
+    ConvImpl<double> *c64 = nullptr;
+};
+
+void mm_lds_attr_once();
+int mm_max_lds_bytes();
+
+#endif

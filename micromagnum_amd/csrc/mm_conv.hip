@@ -1,0 +1,630 @@
+// mm_conv.hip -- FFT demag convolution on gfx950.
+//
+// Replaces SymmetricMatrixVectorConvolution_FFT / MatrixVectorConvolution_FFT
+// (src/magneto/math/conv/MatrixVectorConvolution_FFT.cpp:110-375) with five
+// LDS-staged passes per evaluation instead of thirteen:
+//
+//   x fwd  : real rows of M (x < nx) zero-extended to Px, complex FFT,
+//            keep kx < Hx = Px/2+1                    M  -> S1[c][z][y][kx]
+//   y fwd  : columns of S1 (y < ny) zero-extended to Py  S1 -> S2[c][z][ky][kx]
+//   z      : lines of S2 (z < nz) zero-extended to Pz, FFT, 3x3 multiply with
+//            the real octant-stored Fourier kernel, inverse FFT, keep z < nz
+//   y inv  : inverse FFT of S2 columns, keep y < ny     S2 -> S1
+//   x inv  : Hermitian extension, inverse FFT, keep x < nx  S1 -> H
+//
+// 2-D meshes (nz == 1) fuse y fwd + multiply + y inv into one pass on S1.
+// No transpose passes: strided passes load tiles of adjacent kx columns so
+// every global access is a contiguous row segment, and the zero padding is
+// never stored (only nonzero input rows are read, only kept outputs written).
+//
+// Fourier kernel: the padded tensor is even/odd in every axis, so its
+// transform is real; only kx < Hx, ky <= Py/2, kz <= Pz/2 are stored (6 real
+// planes, sign flips for the odd components on the other octants).  It is
+// computed once on the device in FP64 with the same passes and cast to the
+// working precision.  The reference keeps the full complex kernel; the
+// dropped imaginary parts are rounding noise (~1e-17 relative).
+#include "mm_conv.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+// ---------------------------------------------------------------------------
+// FFT plan
+// ---------------------------------------------------------------------------
+void FftPlanHost::build(int len)
+{
+    MM_REQUIRE(len >= 1, "FFT length must be positive");
+    L = len;
+    rad.clear();
+    int n = len;
+    int twos = 0;
+    while (n % 2 == 0) { n /= 2; ++twos; }
+    while (twos >= 3) { rad.push_back(8); twos -= 3; }
+    if (twos == 2) rad.push_back(4);
+    if (twos == 1) rad.push_back(2);
+    while (n % 5 == 0) { rad.push_back(5); n /= 5; }
+    while (n % 3 == 0) { rad.push_back(3); n /= 3; }
+    if (n != 1) throw mm_error(MM_ERR_UNSUPPORTED, "FFT length " + std::to_string(len) + " has a prime factor > 5");
+    MM_REQUIRE(rad.size() <= MM_FFT_MAX_PASSES, "FFT length needs too many passes");
+    const int ept = L > 2048 ? 16 : 8;
+    tline = std::max(1, (L + ept - 1) / ept);
+    std::vector<float2> t32(L);
+    std::vector<double2> t64(L);
+    for (int m = 0; m < L; ++m) {
+        const long double a = -2.0L * MM_PI_L * (long double)m / (long double)L;
+        const long double c = cosl(a), s = sinl(a);
+        t64[m].x = (double)c;
+        t64[m].y = (double)s;
+        t32[m].x = (float)c;
+        t32[m].y = (float)s;
+    }
+    MM_HIP(hipMalloc(&tw32, sizeof(float2) * L));
+    MM_HIP(hipMalloc(&tw64, sizeof(double2) * L));
+    MM_HIP(hipMemcpy(tw32, t32.data(), sizeof(float2) * L, hipMemcpyHostToDevice));
+    MM_HIP(hipMemcpy(tw64, t64.data(), sizeof(double2) * L, hipMemcpyHostToDevice));
+}
+
+void FftPlanHost::release()
+{
+    if (tw32) (void)hipFree(tw32);
+    if (tw64) (void)hipFree(tw64);
+    tw32 = tw64 = nullptr;
+}
+
+bool mm_is_smooth(int n)
+{
+    if (n < 1) return false;
+    for (int f : {2, 3, 5})
+        while (n % f == 0) n /= f;
+    return n == 1;
+}
+
+// Any padded size P >= 2n-1 gives the same linear convolution for open
+// boundaries (the tensor is zero between the +/- offsets), so the given size
+// is kept if it is radix-friendly and even, else the next such size is used.
+int mm_fft_size_for(int n, int given)
+{
+    if (n == 1) return 1;
+    if (given >= 2 * n - 1 && given % 2 == 0 && mm_is_smooth(given)) return given;
+    int p = 2 * n - 1;
+    while (!(p % 2 == 0 && mm_is_smooth(p))) ++p;
+    return p;
+}
+
+static int g_max_lds = 0;
+static std::once_flag g_lds_once;
+
+int mm_max_lds_bytes()
+{
+    std::call_once(g_lds_once, []() {
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+            v = 65536;
+        g_max_lds = v;
+    });
+    return g_max_lds;
+}
+
+template <typename K> static void set_lds(K kernel, size_t bytes)
+{
+    if (bytes > 65536) MM_HIP(hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+
+// x forward: rows (c, z, y); input rows of nx_in reals with row stride nx_in
+// sel (may be null): device index choosing in (0) or in_alt (1)
+template <typename T, typename TIN>
+__global__ void k_xfwd(const TIN *__restrict__ in, size_t in_cstride, C2<T> *__restrict__ out,
+                       size_t out_cstride, int Hx, int ny, int nz, int nrows, int nx_in,
+                       FftPlanDev p, int lpw, const TIN *__restrict__ in_alt = nullptr,
+                       const int *__restrict__ sel = nullptr)
+{
+    if (sel && *sel) in = in_alt;
+    extern __shared__ char smem_raw[];
+    C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
+    const int L = p.L;
+    const int l = threadIdx.x / p.tline, t = threadIdx.x % p.tline;
+    const int g = blockIdx.x * lpw + l;
+    const bool active = g < nrows;
+    C2<T> *line = sm + (size_t)l * (L + 1);
+    int c = 0, r = 0;
+    if (active) {
+        c = g / (ny * nz);
+        r = g % (ny * nz);
+        const TIN *src = in + c * in_cstride + (size_t)r * nx_in;
+        for (int x = t; x < L; x += p.tline) line[x] = cmk<T>(x < nx_in ? (T)src[x] : (T)0, (T)0);
+    }
+    __syncthreads();
+    lds_fft<T, false>(line, p, t, active);
+    if (active) {
+        C2<T> *dst = out + c * out_cstride + (size_t)r * Hx;
+        for (int kx = t; kx < Hx; kx += p.tline) dst[kx] = line[kx];
+    }
+}
+
+// y forward: columns kx of planes (c, z); rows y < ny_in read, Py written
+template <typename T>
+__global__ void k_yfwd(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, int Hx, int ny_in,
+                       int nzl, FftPlanDev p, int tk)
+{
+    extern __shared__ char smem_raw[];
+    C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
+    const int L = p.L, LP = L + 1;
+    const int kx0 = blockIdx.x * tk;
+    const int z = blockIdx.y, c = blockIdx.z;
+    const C2<T> *src = in + ((size_t)c * nzl + z) * (size_t)ny_in * Hx;
+    C2<T> *dst = out + ((size_t)c * nzl + z) * (size_t)L * Hx;
+    for (int e = threadIdx.x; e < tk * L; e += blockDim.x) {
+        const int col = e % tk, y = e / tk, kx = kx0 + col;
+        sm[col * LP + y] = (y < ny_in && kx < Hx) ? src[(size_t)y * Hx + kx] : cmk<T>((T)0, (T)0);
+    }
+    __syncthreads();
+    const int l = threadIdx.x / p.tline, t = threadIdx.x % p.tline;
+    lds_fft<T, false>(sm + (size_t)l * LP, p, t, l < tk);
+    for (int e = threadIdx.x; e < tk * L; e += blockDim.x) {
+        const int col = e % tk, y = e / tk, kx = kx0 + col;
+        if (kx < Hx) dst[(size_t)y * Hx + kx] = sm[col * LP + y];
+    }
+}
+
+// y inverse: columns of S2 planes (c, z), all Py rows read, y < ny_out written
+template <typename T>
+__global__ void k_yinv(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, int Hx, int ny_out,
+                       int nzl, FftPlanDev p, int tk)
+{
+    extern __shared__ char smem_raw[];
+    C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
+    const int L = p.L, LP = L + 1;
+    const int kx0 = blockIdx.x * tk;
+    const int z = blockIdx.y, c = blockIdx.z;
+    const C2<T> *src = in + ((size_t)c * nzl + z) * (size_t)L * Hx;
+    C2<T> *dst = out + ((size_t)c * nzl + z) * (size_t)ny_out * Hx;
+    for (int e = threadIdx.x; e < tk * L; e += blockDim.x) {
+        const int col = e % tk, y = e / tk, kx = kx0 + col;
+        sm[col * LP + y] = (kx < Hx) ? src[(size_t)y * Hx + kx] : cmk<T>((T)0, (T)0);
+    }
+    __syncthreads();
+    const int l = threadIdx.x / p.tline, t = threadIdx.x % p.tline;
+    lds_fft<T, true>(sm + (size_t)l * LP, p, t, l < tk);
+    for (int e = threadIdx.x; e < tk * ny_out; e += blockDim.x) {
+        const int col = e % tk, y = e / tk, kx = kx0 + col;
+        if (kx < Hx) dst[(size_t)y * Hx + kx] = sm[col * LP + y];
+    }
+}
+
+// real octant-stored kernel entry e at (kx, ky, kz) of the full spectrum
+template <typename T>
+__device__ __forceinline__ void kernel_at(const T *__restrict__ K, size_t plane, int Hx, int Hy2,
+                                          int Py, int Pz, int kx, int ky, int kz, T k[6])
+{
+    const bool fy = ky > Py / 2, fz = kz > Pz / 2;
+    const int kyf = fy ? Py - ky : ky;
+    const int kzf = fz ? Pz - kz : kz;
+    const size_t o = ((size_t)kzf * Hy2 + kyf) * Hx + kx;
+    const T sy = fy ? (T)-1 : (T)1, sz = fz ? (T)-1 : (T)1;
+    k[0] = K[o];
+    k[1] = sy * K[plane + o];
+    k[2] = sz * K[2 * plane + o];
+    k[3] = K[3 * plane + o];
+    k[4] = sy * sz * K[4 * plane + o];
+    k[5] = K[5 * plane + o];
+}
+
+template <typename T>
+__device__ __forceinline__ void mult3(C2<T> &mx, C2<T> &my, C2<T> &mz, const T k[6])
+{
+    const C2<T> x = mx, y = my, z = mz;
+    mx = cmk<T>(k[0] * x.x + k[1] * y.x + k[2] * z.x, k[0] * x.y + k[1] * y.y + k[2] * z.y);
+    my = cmk<T>(k[1] * x.x + k[3] * y.x + k[4] * z.x, k[1] * x.y + k[3] * y.y + k[4] * z.y);
+    mz = cmk<T>(k[2] * x.x + k[4] * y.x + k[5] * z.x, k[2] * x.y + k[4] * y.y + k[5] * z.y);
+}
+
+// z pass (3-D): forward over z (nz rows, zero-extended), multiply, inverse,
+// keep z < nz.  Lines: 3 components x tk columns for one ky.
+template <typename T>
+__global__ void k_zmi(C2<T> *__restrict__ S2, const T *__restrict__ K, size_t plane, int Hx, int Hy2,
+                      int Py, int nz, FftPlanDev p, int tk)
+{
+    extern __shared__ char smem_raw[];
+    C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
+    const int L = p.L, LP = L + 1;
+    const int kx0 = blockIdx.x * tk;
+    const int ky = blockIdx.y;
+    const size_t cstride = (size_t)nz * Py * Hx;
+    const size_t zstride = (size_t)Py * Hx;
+    C2<T> *base = S2 + (size_t)ky * Hx;
+    for (int e = threadIdx.x; e < 3 * tk * L; e += blockDim.x) {
+        const int col = e % tk, rest = e / tk, z = rest % L, c = rest / L, kx = kx0 + col;
+        sm[(c * tk + col) * LP + z] = (z < nz && kx < Hx) ? base[c * cstride + z * zstride + kx] : cmk<T>((T)0, (T)0);
+    }
+    __syncthreads();
+    const int l = threadIdx.x / p.tline, t = threadIdx.x % p.tline;
+    const bool active = l < 3 * tk;
+    C2<T> *line = sm + (size_t)l * LP;
+    lds_fft<T, false>(line, p, t, active);
+    for (int e = threadIdx.x; e < tk * L; e += blockDim.x) {
+        const int col = e % tk, kz = e / tk, kx = kx0 + col;
+        if (kx < Hx) {
+            T k[6];
+            kernel_at<T>(K, plane, Hx, Hy2, Py, L, kx, ky, kz, k);
+            mult3<T>(sm[(0 * tk + col) * LP + kz], sm[(1 * tk + col) * LP + kz], sm[(2 * tk + col) * LP + kz], k);
+        }
+    }
+    __syncthreads();
+    lds_fft<T, true>(line, p, t, active);
+    for (int e = threadIdx.x; e < 3 * tk * nz; e += blockDim.x) {
+        const int col = e % tk, rest = e / tk, z = rest % nz, c = rest / nz, kx = kx0 + col;
+        if (kx < Hx) base[c * cstride + z * zstride + kx] = sm[(c * tk + col) * LP + z];
+    }
+}
+
+// 2-D: y forward (ny rows, zero-extended) + multiply (kz = 0) + y inverse,
+// keep y < ny, in place on S1 planes [c][y][kx].
+template <typename T>
+__global__ void k_ymi2d(C2<T> *__restrict__ S1, const T *__restrict__ K, size_t plane, int Hx, int Hy2,
+                        int ny, FftPlanDev p, int tk)
+{
+    extern __shared__ char smem_raw[];
+    C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
+    const int L = p.L, LP = L + 1;
+    const int kx0 = blockIdx.x * tk;
+    const size_t cstride = (size_t)ny * Hx;
+    for (int e = threadIdx.x; e < 3 * tk * L; e += blockDim.x) {
+        const int col = e % tk, rest = e / tk, y = rest % L, c = rest / L, kx = kx0 + col;
+        sm[(c * tk + col) * LP + y] = (y < ny && kx < Hx) ? S1[c * cstride + (size_t)y * Hx + kx] : cmk<T>((T)0, (T)0);
+    }
+    __syncthreads();
+    const int l = threadIdx.x / p.tline, t = threadIdx.x % p.tline;
+    const bool active = l < 3 * tk;
+    C2<T> *line = sm + (size_t)l * LP;
+    lds_fft<T, false>(line, p, t, active);
+    for (int e = threadIdx.x; e < tk * L; e += blockDim.x) {
+        const int col = e % tk, ky = e / tk, kx = kx0 + col;
+        if (kx < Hx) {
+            T k[6];
+            kernel_at<T>(K, plane, Hx, Hy2, L, 1, kx, ky, 0, k);
+            mult3<T>(sm[(0 * tk + col) * LP + ky], sm[(1 * tk + col) * LP + ky], sm[(2 * tk + col) * LP + ky], k);
+        }
+    }
+    __syncthreads();
+    lds_fft<T, true>(line, p, t, active);
+    for (int e = threadIdx.x; e < 3 * tk * ny; e += blockDim.x) {
+        const int col = e % tk, rest = e / tk, y = rest % ny, c = rest / ny, kx = kx0 + col;
+        if (kx < Hx) S1[c * cstride + (size_t)y * Hx + kx] = sm[(c * tk + col) * LP + y];
+    }
+}
+
+// x inverse (C2R): rows (c, z, y) of S1, Hermitian extension, keep x < nx
+template <typename T, typename TOUT>
+__global__ void k_xinv(const C2<T> *__restrict__ in, TOUT *__restrict__ out, size_t out_cstride, int Hx,
+                       int ny, int nz, int nrows, int nx, FftPlanDev p, int lpw)
+{
+    extern __shared__ char smem_raw[];
+    C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
+    const int L = p.L;
+    const int l = threadIdx.x / p.tline, t = threadIdx.x % p.tline;
+    const int g = blockIdx.x * lpw + l;
+    const bool active = g < nrows;
+    C2<T> *line = sm + (size_t)l * (L + 1);
+    int c = 0, r = 0;
+    if (active) {
+        c = g / (ny * nz);
+        r = g % (ny * nz);
+        const C2<T> *src = in + (size_t)g * Hx;
+        for (int k = t; k < L; k += p.tline) line[k] = k < Hx ? src[k] : cconj<T>(src[L - k]);
+    }
+    __syncthreads();
+    lds_fft<T, true>(line, p, t, active);
+    if (active) {
+        TOUT *dst = out + c * out_cstride + (size_t)r * nx;
+        for (int x = t; x < nx; x += p.tline) dst[x] = (TOUT)line[x].x;
+    }
+}
+
+// setup: forward z over full lines (no multiply), all Pz outputs stored
+__global__ void k_zfwd_setup(double2 *__restrict__ S, int Hx, int Py, FftPlanDev p, int tk)
+{
+    extern __shared__ char smem_raw[];
+    double2 *sm = reinterpret_cast<double2 *>(smem_raw);
+    const int L = p.L, LP = L + 1;
+    const int kx0 = blockIdx.x * tk, ky = blockIdx.y;
+    const size_t zstride = (size_t)Py * Hx;
+    double2 *base = S + (size_t)ky * Hx;
+    for (int e = threadIdx.x; e < tk * L; e += blockDim.x) {
+        const int col = e % tk, z = e / tk, kx = kx0 + col;
+        sm[col * LP + z] = kx < Hx ? base[z * zstride + kx] : cmk<double>(0.0, 0.0);
+    }
+    __syncthreads();
+    const int l = threadIdx.x / p.tline, t = threadIdx.x % p.tline;
+    lds_fft<double, false>(sm + (size_t)l * LP, p, t, l < tk);
+    for (int e = threadIdx.x; e < tk * L; e += blockDim.x) {
+        const int col = e % tk, z = e / tk, kx = kx0 + col;
+        if (kx < Hx) base[z * zstride + kx] = sm[col * LP + z];
+    }
+}
+
+// setup: real part of the octant ky <= Py/2, kz <= Pz/2, scaled
+template <typename T>
+__global__ void k_extract_kernel(const double2 *__restrict__ S, T *__restrict__ K, int Hx, int Py,
+                                 int Hy2, int Hz2, double scale)
+{
+    const size_t n = (size_t)Hx * Hy2 * Hz2;
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+        const int kx = (int)(q % Hx);
+        const size_t r = q / Hx;
+        const int ky = (int)(r % Hy2), kz = (int)(r / Hy2);
+        K[q] = (T)(scale * S[((size_t)kz * Py + ky) * Hx + kx].x);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static int grid_1d(size_t n, int bs)
+{
+    size_t b = (n + bs - 1) / bs;
+    return (int)std::min<size_t>(b, 65535 * 64);
+}
+
+static ConvLaunch choose_launch(const ConvGeom &g, const FftPlanHost &px, const FftPlanHost &py,
+                                const FftPlanHost &pz, size_t csize)
+{
+    const int lds = mm_max_lds_bytes();
+    ConvLaunch ln;
+    ln.lpw_x = std::max(1, 256 / px.tline);
+    while (ln.lpw_x > 1 && (size_t)ln.lpw_x * (px.L + 1) * csize > (size_t)lds) --ln.lpw_x;
+    // y: 128-byte row segments, <= 1024 threads, fits LDS; 2-D needs 3 lines per column
+    const int nl_y = g.is2d ? 3 : 1;
+    int tk = std::max(1, (int)(128 / csize));
+    while (tk > 1 && ((size_t)nl_y * tk * py.tline > 1024 || (size_t)nl_y * tk * (py.L + 1) * csize > (size_t)lds)) tk /= 2;
+    ln.tk_y = tk;
+    MM_REQUIRE((size_t)nl_y * tk * py.tline <= 1024 && (size_t)nl_y * tk * (py.L + 1) * csize <= (size_t)lds,
+               "y transform length too large for one workgroup");
+    int tz = std::max(1, (int)(512 / csize));
+    while (tz > 1 && ((size_t)3 * tz * pz.tline > 1024 || (size_t)3 * tz * (pz.L + 1) * csize > (size_t)lds)) tz /= 2;
+    ln.tk_z = tz;
+    return ln;
+}
+
+template <typename T>
+void ConvImpl<T>::init(const double *N_host, int nx, int ny, int nz, int ex, int ey, int ez)
+{
+    MM_REQUIRE(nx >= 1 && ny >= 1 && nz >= 1, "bad mesh size");
+    MM_REQUIRE(ex >= 2 * nx - 1 && ey >= 2 * ny - 1 && ez >= 2 * nz - 1,
+               "tensor padding too small for open boundaries (periodic axes are not supported)");
+    g.nx = nx; g.ny = ny; g.nz = nz;
+    g.Px = mm_fft_size_for(nx, ex);
+    g.Py = mm_fft_size_for(ny, ey);
+    g.Pz = mm_fft_size_for(nz, ez);
+    g.Hx = g.Px / 2 + 1;
+    g.Hy2 = g.Py / 2 + 1;
+    g.Hz2 = g.Pz / 2 + 1;
+    g.is2d = (nz == 1 && g.Pz == 1);
+    px.build(g.Px);
+    py.build(g.Py);
+    pz.build(g.Pz);
+    ln = choose_launch(g, px, py, pz, sizeof(C2<T>));
+    MM_REQUIRE((size_t)px.tline <= 1024, "x transform length too large");
+
+    // re-embed the tensor into the radix-friendly padded grid
+    const int Px = g.Px, Py = g.Py, Pz = g.Pz;
+    auto map_axis = [](int X, int e, int n, int P) -> int {
+        if (X < n) return X;
+        if (X > e - n) return P - (e - X);
+        return -1;
+    };
+    const size_t plane_full = (size_t)Px * Py * Pz;
+    std::vector<double> Nr((size_t)6 * plane_full, 0.0);
+    for (int Z = 0; Z < ez; ++Z) {
+        const int z2 = map_axis(Z, ez, nz, Pz);
+        if (z2 < 0) continue;
+        for (int Y = 0; Y < ey; ++Y) {
+            const int y2 = map_axis(Y, ey, ny, Py);
+            if (y2 < 0) continue;
+            for (int X = 0; X < ex; ++X) {
+                const int x2 = map_axis(X, ex, nx, Px);
+                if (x2 < 0) continue;
+                const double *src = N_host + 6 * ((size_t)X + (size_t)ex * ((size_t)Y + (size_t)ey * Z));
+                const size_t dst = (size_t)x2 + (size_t)Px * ((size_t)y2 + (size_t)Py * z2);
+                for (int e = 0; e < 6; ++e) Nr[e * plane_full + dst] = src[e];
+            }
+        }
+    }
+
+    // Fourier transform of each component in FP64 on the device
+    const size_t plane = (size_t)g.Hx * g.Hy2 * g.Hz2;
+    kern_elems = 6 * plane;
+    MM_HIP(hipMalloc(&kern, sizeof(T) * kern_elems));
+    double *dN = nullptr;
+    double2 *T1 = nullptr, *T2 = nullptr;
+    const size_t nspec = (size_t)g.Hx * Py * Pz;
+    MM_HIP(hipMalloc(&dN, sizeof(double) * plane_full));
+    MM_HIP(hipMalloc(&T1, sizeof(double2) * nspec));
+    MM_HIP(hipMalloc(&T2, sizeof(double2) * nspec));
+    const int lds = mm_max_lds_bytes();
+    const FftPlanDev dx = px.dev<double>(), dy = py.dev<double>(), dz = pz.dev<double>();
+    int lpw = std::max(1, 256 / px.tline);
+    while (lpw > 1 && (size_t)lpw * (px.L + 1) * 16 > (size_t)lds) --lpw;
+    int tky = 8;
+    while (tky > 1 && ((size_t)tky * py.tline > 1024 || (size_t)tky * (py.L + 1) * 16 > (size_t)lds)) tky /= 2;
+    int tkz = 8;
+    while (tkz > 1 && ((size_t)tkz * pz.tline > 1024 || (size_t)tkz * (pz.L + 1) * 16 > (size_t)lds)) tkz /= 2;
+    const size_t shx = (size_t)lpw * (px.L + 1) * 16, shy = (size_t)tky * (py.L + 1) * 16,
+                 shz = (size_t)tkz * (pz.L + 1) * 16;
+    set_lds(k_xfwd<double, double>, shx);
+    set_lds(k_yfwd<double>, shy);
+    set_lds(k_zfwd_setup, shz);
+    const double scale = 1.0 / ((double)Px * (double)Py * (double)Pz);
+    for (int e = 0; e < 6; ++e) {
+        MM_HIP(hipMemcpy(dN, Nr.data() + e * plane_full, sizeof(double) * plane_full, hipMemcpyHostToDevice));
+        const int nrows = Py * Pz;
+        hipLaunchKernelGGL((k_xfwd<double, double>), dim3((nrows + lpw - 1) / lpw), dim3(lpw * px.tline), shx, 0,
+                           dN, 0, T1, 0, g.Hx, Py, Pz, nrows, Px, dx, lpw, (const double *)nullptr,
+                           (const int *)nullptr);
+        MM_HIP(hipGetLastError());
+        hipLaunchKernelGGL((k_yfwd<double>), dim3((g.Hx + tky - 1) / tky, Pz, 1), dim3(tky * py.tline), shy, 0,
+                           T1, T2, g.Hx, Py, Pz, dy, tky);
+        MM_HIP(hipGetLastError());
+        if (Pz > 1) {
+            hipLaunchKernelGGL(k_zfwd_setup, dim3((g.Hx + tkz - 1) / tkz, Py, 1), dim3(tkz * pz.tline), shz, 0,
+                               T2, g.Hx, Py, dz, tkz);
+            MM_HIP(hipGetLastError());
+        }
+        hipLaunchKernelGGL((k_extract_kernel<T>), dim3(grid_1d(plane, 256)), dim3(256), 0, 0, T2, kern + e * plane,
+                           g.Hx, Py, g.Hy2, g.Hz2, scale);
+        MM_HIP(hipGetLastError());
+    }
+    MM_HIP(hipDeviceSynchronize());
+    (void)hipFree(dN);
+    (void)hipFree(T1);
+    (void)hipFree(T2);
+
+    MM_HIP(hipMalloc(&S1, sizeof(C2<T>) * 3 * (size_t)nz * ny * g.Hx));
+    if (!g.is2d) MM_HIP(hipMalloc(&S2, sizeof(C2<T>) * 3 * (size_t)nz * Py * g.Hx));
+
+    const size_t cs = sizeof(C2<T>);
+    set_lds(k_xfwd<T, T>, (size_t)ln.lpw_x * (px.L + 1) * cs);
+    set_lds(k_xinv<T, T>, (size_t)ln.lpw_x * (px.L + 1) * cs);
+    set_lds(k_yfwd<T>, (size_t)ln.tk_y * (py.L + 1) * cs);
+    set_lds(k_yinv<T>, (size_t)ln.tk_y * (py.L + 1) * cs);
+    set_lds(k_ymi2d<T>, (size_t)3 * ln.tk_y * (py.L + 1) * cs);
+    set_lds(k_zmi<T>, (size_t)3 * ln.tk_z * (pz.L + 1) * cs);
+}
+
+template <typename T> void ConvImpl<T>::release()
+{
+    if (kern) (void)hipFree(kern);
+    if (S1) (void)hipFree(S1);
+    if (S2) (void)hipFree(S2);
+    kern = nullptr;
+    S1 = S2 = nullptr;
+    px.release();
+    py.release();
+    pz.release();
+}
+
+template <typename T> void ConvImpl<T>::forward_x(const T *M, hipStream_t s, const T *M_alt, const int *sel)
+{
+    const size_t n = (size_t)g.nx * g.ny * g.nz;
+    const int nrows = 3 * g.ny * g.nz;
+    const size_t sh = (size_t)ln.lpw_x * (px.L + 1) * sizeof(C2<T>);
+    hipLaunchKernelGGL((k_xfwd<T, T>), dim3((nrows + ln.lpw_x - 1) / ln.lpw_x), dim3(ln.lpw_x * px.tline), sh, s,
+                       M, n, S1, (size_t)g.ny * g.nz * g.Hx, g.Hx, g.ny, g.nz, nrows, g.nx, px.dev<T>(), ln.lpw_x,
+                       M_alt, sel);
+    MM_HIP(hipGetLastError());
+}
+
+template <typename T> void ConvImpl<T>::spectral(hipStream_t s)
+{
+    const size_t cs = sizeof(C2<T>);
+    const size_t plane = (size_t)g.Hx * g.Hy2 * g.Hz2;
+    if (g.is2d) {
+        const int tk = ln.tk_y;
+        const size_t sh = (size_t)3 * tk * (py.L + 1) * cs;
+        hipLaunchKernelGGL((k_ymi2d<T>), dim3((g.Hx + tk - 1) / tk), dim3(3 * tk * py.tline), sh, s, S1, kern, plane,
+                           g.Hx, g.Hy2, g.ny, py.dev<T>(), tk);
+        MM_HIP(hipGetLastError());
+        return;
+    }
+    const int tk = ln.tk_y;
+    const size_t shy = (size_t)tk * (py.L + 1) * cs;
+    hipLaunchKernelGGL((k_yfwd<T>), dim3((g.Hx + tk - 1) / tk, g.nz, 3), dim3(tk * py.tline), shy, s, S1, S2, g.Hx,
+                       g.ny, g.nz, py.dev<T>(), tk);
+    MM_HIP(hipGetLastError());
+    const int tz = ln.tk_z;
+    const size_t shz = (size_t)3 * tz * (pz.L + 1) * cs;
+    hipLaunchKernelGGL((k_zmi<T>), dim3((g.Hx + tz - 1) / tz, g.Py), dim3(3 * tz * pz.tline), shz, s, S2, kern, plane,
+                       g.Hx, g.Hy2, g.Py, g.nz, pz.dev<T>(), tz);
+    MM_HIP(hipGetLastError());
+    hipLaunchKernelGGL((k_yinv<T>), dim3((g.Hx + tk - 1) / tk, g.nz, 3), dim3(tk * py.tline), shy, s, S2, S1, g.Hx,
+                       g.ny, g.nz, py.dev<T>(), tk);
+    MM_HIP(hipGetLastError());
+}
+
+template <typename T> void ConvImpl<T>::inverse_x(T *H, hipStream_t s)
+{
+    const size_t n = (size_t)g.nx * g.ny * g.nz;
+    const int nrows = 3 * g.ny * g.nz;
+    const size_t sh = (size_t)ln.lpw_x * (px.L + 1) * sizeof(C2<T>);
+    hipLaunchKernelGGL((k_xinv<T, T>), dim3((nrows + ln.lpw_x - 1) / ln.lpw_x), dim3(ln.lpw_x * px.tline), sh, s, S1,
+                       H, n, g.Hx, g.ny, g.nz, nrows, g.nx, px.dev<T>(), ln.lpw_x);
+    MM_HIP(hipGetLastError());
+}
+
+template struct ConvImpl<float>;
+template struct ConvImpl<double>;
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" int mm_conv_create(const double *N_host, int nx, int ny, int nz, int ex, int ey, int ez,
+                              int precision, mm_conv_t **out)
+{
+    MM_CAPI_BEGIN
+    MM_REQUIRE(N_host && out, "null argument");
+    MM_REQUIRE(precision == MM_FP64 || precision == MM_FP32, "precision must be MM_FP64 or MM_FP32");
+    mm_conv *c = new mm_conv();
+    c->precision = precision;
+    try {
+        if (precision == MM_FP64) {
+            c->c64 = new ConvImpl<double>();
+            c->c64->init(N_host, nx, ny, nz, ex, ey, ez);
+        } else {
+            c->c32 = new ConvImpl<float>();
+            c->c32->init(N_host, nx, ny, nz, ex, ey, ez);
+        }
+    } catch (...) {
+        if (c->c64) { c->c64->release(); delete c->c64; }
+        if (c->c32) { c->c32->release(); delete c->c32; }
+        delete c;
+        throw;
+    }
+    *out = c;
+    MM_CAPI_END
+}
+
+extern "C" int mm_conv_execute(mm_conv_t *conv, const double *M, double *H, void *stream)
+{
+    MM_CAPI_BEGIN
+    MM_REQUIRE(conv && M && H, "null argument");
+    MM_REQUIRE(conv->precision == MM_FP64, "convolution was created for FP32; use mm_conv_execute_f32");
+    conv->c64->execute(M, H, (hipStream_t)stream);
+    MM_CAPI_END
+}
+
+extern "C" int mm_conv_execute_f32(mm_conv_t *conv, const float *M, float *H, void *stream)
+{
+    MM_CAPI_BEGIN
+    MM_REQUIRE(conv && M && H, "null argument");
+    MM_REQUIRE(conv->precision == MM_FP32, "convolution was created for FP64; use mm_conv_execute");
+    conv->c32->execute(M, H, (hipStream_t)stream);
+    MM_CAPI_END
+}
+
+extern "C" int mm_conv_fft_size(const mm_conv_t *conv, int *fft_xyz)
+{
+    MM_CAPI_BEGIN
+    MM_REQUIRE(conv && fft_xyz, "null argument");
+    const ConvGeom &g = conv->c64 ? conv->c64->g : conv->c32->g;
+    fft_xyz[0] = g.Px;
+    fft_xyz[1] = g.Py;
+    fft_xyz[2] = g.Pz;
+    MM_CAPI_END
+}
+
+extern "C" int mm_conv_destroy(mm_conv_t *conv)
+{
+    MM_CAPI_BEGIN
+    if (!conv) return MM_OK;
+    if (conv->c64) { conv->c64->release(); delete conv->c64; }
+    if (conv->c32) { conv->c32->release(); delete conv->c32; }
+    delete conv;
+    MM_CAPI_END
+}

@@ -1,0 +1,202 @@
+// mm_fft_device.h -- LDS-resident Stockham FFT building blocks for gfx950.
+//
+// One line of length L lives in LDS; `tline` threads of the workgroup work on
+// it.  Each pass of radix R (8, 5, 4, 3, 2) reads the R inputs of every
+// butterfly into registers, synchronises the workgroup, applies the twiddles
+// W_{Ns R}^{r k} from a per-length table and a hand-written size-R DFT, and
+// writes the results to their autosorted positions (Stockham), so no
+// bit-reversal pass is needed and the line stays in place.  Forward uses
+// e^{-i}, inverse e^{+i}; neither is normalised (the 1/(Px Py Pz) factor lives
+// in the Fourier kernel, as in the reference's TensorFieldSetup.cpp:58).
+#ifndef MM_FFT_DEVICE_H
+#define MM_FFT_DEVICE_H
+
+#include <hip/hip_runtime.h>
+
+#define MM_FFT_MAX_PASSES 16
+
+template <typename T> struct cvec;
+template <> struct cvec<float> { typedef float2 type; };
+template <> struct cvec<double> { typedef double2 type; };
+
+template <typename T> using C2 = typename cvec<T>::type;
+
+template <typename T> __device__ __forceinline__ C2<T> cmk(T r, T i)
+{
+    C2<T> v;
+    v.x = r;
+    v.y = i;
+    return v;
+}
+template <typename T> __device__ __forceinline__ C2<T> cadd(C2<T> a, C2<T> b) { return cmk<T>(a.x + b.x, a.y + b.y); }
+template <typename T> __device__ __forceinline__ C2<T> csub(C2<T> a, C2<T> b) { return cmk<T>(a.x - b.x, a.y - b.y); }
+template <typename T> __device__ __forceinline__ C2<T> cmul(C2<T> a, C2<T> b)
+{
+    return cmk<T>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename T> __device__ __forceinline__ C2<T> cconj(C2<T> a) { return cmk<T>(a.x, -a.y); }
+// multiply by -i (forward) or +i (inverse)
+template <typename T, bool INV> __device__ __forceinline__ C2<T> cmul_mi(C2<T> a)
+{
+    return INV ? cmk<T>(-a.y, a.x) : cmk<T>(a.y, -a.x);
+}
+
+// device-side plan of one transform length (passed by value)
+struct FftPlanDev {
+    int L;
+    int tline;             // threads per line
+    int npass;
+    int rad[MM_FFT_MAX_PASSES];
+    const void *tw;        // L entries, tw[m] = exp(-2 pi i m / L), precision T
+};
+
+// ---- size-R DFTs on registers --------------------------------------------
+template <typename T, bool INV> __device__ __forceinline__ void dft2(C2<T> *v)
+{
+    const C2<T> a = v[0], b = v[1];
+    v[0] = cadd<T>(a, b);
+    v[1] = csub<T>(a, b);
+}
+
+template <typename T, bool INV> __device__ __forceinline__ void dft4(C2<T> *v)
+{
+    const C2<T> t0 = cadd<T>(v[0], v[2]), t1 = csub<T>(v[0], v[2]);
+    const C2<T> t2 = cadd<T>(v[1], v[3]), t3 = cmul_mi<T, INV>(csub<T>(v[1], v[3]));
+    v[0] = cadd<T>(t0, t2);
+    v[2] = csub<T>(t0, t2);
+    v[1] = cadd<T>(t1, t3);
+    v[3] = csub<T>(t1, t3);
+}
+
+template <typename T, bool INV> __device__ __forceinline__ void dft8(C2<T> *v)
+{
+    C2<T> e[4] = {v[0], v[2], v[4], v[6]};
+    C2<T> o[4] = {v[1], v[3], v[5], v[7]};
+    dft4<T, INV>(e);
+    dft4<T, INV>(o);
+    const T h = (T)0.70710678118654752440084436210484903928483593768847;
+    // o[k] *= W8^k
+    const C2<T> o1 = INV ? cmk<T>(h * (o[1].x - o[1].y), h * (o[1].x + o[1].y))
+                         : cmk<T>(h * (o[1].x + o[1].y), h * (o[1].y - o[1].x));
+    const C2<T> o2 = cmul_mi<T, INV>(o[2]);
+    const C2<T> o3 = INV ? cmk<T>(-h * (o[3].x + o[3].y), h * (o[3].x - o[3].y))
+                         : cmk<T>(h * (o[3].y - o[3].x), -h * (o[3].x + o[3].y));
+    v[0] = cadd<T>(e[0], o[0]);
+    v[4] = csub<T>(e[0], o[0]);
+    v[1] = cadd<T>(e[1], o1);
+    v[5] = csub<T>(e[1], o1);
+    v[2] = cadd<T>(e[2], o2);
+    v[6] = csub<T>(e[2], o2);
+    v[3] = cadd<T>(e[3], o3);
+    v[7] = csub<T>(e[3], o3);
+}
+
+template <typename T, bool INV> __device__ __forceinline__ void dft3(C2<T> *v)
+{
+    const T c = (T)-0.5;
+    const T s = INV ? (T)0.86602540378443864676372317075293618347140262690519
+                    : (T)-0.86602540378443864676372317075293618347140262690519;
+    const C2<T> t = cadd<T>(v[1], v[2]), d = csub<T>(v[1], v[2]);
+    const C2<T> m = cmk<T>(v[0].x + c * t.x, v[0].y + c * t.y);
+    // i*s*d
+    const C2<T> isd = cmk<T>(-s * d.y, s * d.x);
+    v[0] = cadd<T>(v[0], t);
+    v[1] = cadd<T>(m, isd);
+    v[2] = csub<T>(m, isd);
+}
+
+template <typename T, bool INV> __device__ __forceinline__ void dft5(C2<T> *v)
+{
+    const T c1 = (T)0.30901699437494742410229341718281905886436044959787;   // cos(2pi/5)
+    const T c2 = (T)-0.80901699437494742410229341718281905886436044959787;  // cos(4pi/5)
+    const T s1 = (T)0.95105651629515357211643933337938214340569863412575;   // sin(2pi/5)
+    const T s2 = (T)0.58778525229247312916870595463907276859765243764314;   // sin(4pi/5)
+    const T sg = INV ? (T)1 : (T)-1;
+    const C2<T> b1 = cadd<T>(v[1], v[4]), d1 = csub<T>(v[1], v[4]);
+    const C2<T> b2 = cadd<T>(v[2], v[3]), d2 = csub<T>(v[2], v[3]);
+    const C2<T> a0 = v[0];
+    const C2<T> m1 = cmk<T>(a0.x + c1 * b1.x + c2 * b2.x, a0.y + c1 * b1.y + c2 * b2.y);
+    const C2<T> m2 = cmk<T>(a0.x + c2 * b1.x + c1 * b2.x, a0.y + c2 * b1.y + c1 * b2.y);
+    const C2<T> n1 = cmk<T>(sg * (s1 * d1.x + s2 * d2.x), sg * (s1 * d1.y + s2 * d2.y)); // times i below
+    const C2<T> n2 = cmk<T>(sg * (s2 * d1.x - s1 * d2.x), sg * (s2 * d1.y - s1 * d2.y));
+    v[0] = cmk<T>(a0.x + b1.x + b2.x, a0.y + b1.y + b2.y);
+    v[1] = cmk<T>(m1.x - n1.y, m1.y + n1.x);
+    v[4] = cmk<T>(m1.x + n1.y, m1.y - n1.x);
+    v[2] = cmk<T>(m2.x - n2.y, m2.y + n2.x);
+    v[3] = cmk<T>(m2.x + n2.y, m2.y - n2.x);
+}
+
+template <int R, typename T, bool INV> __device__ __forceinline__ void dftR(C2<T> *v)
+{
+    if constexpr (R == 2) dft2<T, INV>(v);
+    else if constexpr (R == 3) dft3<T, INV>(v);
+    else if constexpr (R == 4) dft4<T, INV>(v);
+    else if constexpr (R == 5) dft5<T, INV>(v);
+    else dft8<T, INV>(v);
+}
+
+// maximum butterflies per thread and pass for radix R (16 elements / R)
+template <int R> struct MaxB { static const int value = (16 + R - 1) / R; };
+
+// One Stockham pass over every line of the workgroup.  `active` is false for
+// threads with no line; they still take part in both barriers.
+template <int R, typename T, bool INV>
+__device__ __forceinline__ void fft_pass(C2<T> *line, const FftPlanDev &p, int t, bool active, int Ns)
+{
+    constexpr int MB = MaxB<R>::value;
+    const int L = p.L;
+    const int nb = L / R;
+    C2<T> v[MB][R];
+#pragma unroll
+    for (int q = 0; q < MB; ++q) {
+        const int b = t + q * p.tline;
+        if (active && b < nb) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[q][r] = line[b + r * nb];
+        }
+    }
+    __syncthreads();
+    const C2<T> *tw = (const C2<T> *)p.tw;
+    const int tstep = L / (Ns * R);
+#pragma unroll
+    for (int q = 0; q < MB; ++q) {
+        const int b = t + q * p.tline;
+        if (active && b < nb) {
+            const int k = b % Ns;
+            if (Ns > 1) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    C2<T> w = tw[r * k * tstep];
+                    if (INV) w.y = -w.y;
+                    v[q][r] = cmul<T>(v[q][r], w);
+                }
+            }
+            dftR<R, T, INV>(v[q]);
+            const int base = (b / Ns) * Ns * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) line[base + r * Ns] = v[q][r];
+        }
+    }
+    __syncthreads();
+}
+
+// Full transform of one line per thread group.  All threads of the workgroup
+// must call this with the same plan (barriers inside).
+template <typename T, bool INV>
+__device__ __forceinline__ void lds_fft(C2<T> *line, const FftPlanDev &p, int t, bool active)
+{
+    int Ns = 1;
+    for (int ps = 0; ps < p.npass; ++ps) {
+        const int R = p.rad[ps];
+        switch (R) {
+        case 8: fft_pass<8, T, INV>(line, p, t, active, Ns); break;
+        case 5: fft_pass<5, T, INV>(line, p, t, active, Ns); break;
+        case 4: fft_pass<4, T, INV>(line, p, t, active, Ns); break;
+        case 3: fft_pass<3, T, INV>(line, p, t, active, Ns); break;
+        default: fft_pass<2, T, INV>(line, p, t, active, Ns); break;
+        }
+        Ns *= R;
+    }
+}
+
+#endif

@@ -1,0 +1,863 @@
+// mm_stepper.hip -- fused Runge-Kutta stepping of the micromagnetic system.
+//
+// Replaces the Python-driven loop of the reference (RungeKutta.evolve/apply,
+// src/magnum/evolver/runge_kutta.py:47-137; LandauLifshitzGilbert
+// .calculate_dMdt, landau_lifshitz_gilbert.py:92-125; MicroMagnetics
+// StepSizeController, micro_magnetics_stepsize_controller.py:35-73;
+// finish_step normalisation, micro_magnetics.py:49-50) by a fixed sequence of
+// kernels per attempt:
+//
+//   for every stage s:   x fwd (stage input) -> y fwd -> z*N*z^-1 -> y inv
+//                        -> x inv + EPILOGUE
+//   EPILOGUE per cell:   H = H_demag + H_exch + H_aniso + H_ext,
+//                        k_s = LLG(M, H) + Zhang-Li(M),
+//                        next stage input  ytmp = y + h sum_j b_{s+1,j} k_j,
+//                        or on the last stage y_new = normalise(y + h sum c_j k_j),
+//                        err = h sum ec_j k_j reduced to max |err|/Ms (FP64)
+//   controller (1 thread): accept/reject, next h, swap of the y buffers
+//
+// y is double-buffered and the buffer index, t and h live in device memory,
+// so attempts are enqueued without host round trips and rejected attempts
+// simply leave the index unchanged.  Stage inputs alternate between two
+// buffers because the epilogue reads neighbours of the current input while
+// it writes the next one.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "mm_cell.h"
+#include "mm_conv.h"
+#include "mm_reduce.h"
+
+// device control block
+struct StepCtl {
+    double t, h, h_last;
+    unsigned long long err_bits;   // max of |err|^2 (or |err|^2/Ms^2) as bits
+    unsigned long long dydt_bits;  // max of |k0|^2 (or scaled)
+    long long attempts, rejections, accepted, target;
+    int cur;                       // index of the current y buffer
+    int stop;                      // accepted == target: remaining launches exit
+};
+
+// a per-cell parameter that is either uniform (p == null) or an array
+template <typename T> struct Param {
+    const T *p;
+    T u;
+    __device__ __forceinline__ T at(size_t i) const { return p ? p[i] : u; }
+};
+template <typename T> struct VParam {
+    const T *p; // 3n SoA or null
+    T u[3];
+    __device__ __forceinline__ V3<T> at(size_t n, size_t i) const
+    {
+        return p ? v3<T>(p[i], p[n + i], p[2 * n + i]) : v3<T>(u[0], u[1], u[2]);
+    }
+};
+
+enum { FORM_SUM = 0, FORM_SEQ = 1 };
+
+template <typename T> struct SysDev {
+    MeshDev m;
+    size_t n;
+    T dx, dy, dz;
+    T wx, wy, wz, coef_ex, mu0, cm2mu0, mu_b, e_charge;
+    Param<T> Ms, A, f1, f2, alpha, kuni, kcub, P, xi;
+    VParam<T> axis1, axis2, j;
+    int has_exch, has_uni, has_cub, has_zee, has_stt, do_precess;
+    T hext[3];
+    int Ms_uniform;
+    double Ms_u;
+};
+
+template <typename T> struct StageDev {
+    const T *Ys;          // stage input (null: y[cur])
+    T *yb0, *yb1;         // the two y buffers
+    T *kout;              // k_s (null: not stored)
+    T *Hout;              // H_tot (null: not stored)
+    // next stage input: out = base + (h*) sum coef_j k_j  (k_s last, from registers)
+    int prep;             // 0 none, 1 prepare next stage, 2 final combine
+    int form;             // FORM_SUM / FORM_SEQ
+    T *prep_out;
+    int nk;               // number of stored k_j used (j < s)
+    const T *kin[8];
+    double b[8];          // coefficients for kin
+    double bself;         // coefficient for k_s
+    double ec[8], ecself; // error coefficients (final stage)
+    int want_dydt;        // reduce max|k_s| into ctl->dydt_bits (stage 0)
+    int norm_mode;
+};
+
+template <typename T> __device__ __forceinline__ void atomic_max_bits(unsigned long long *a, double v)
+{
+    atomicMax(a, (unsigned long long)__double_as_longlong(v));
+}
+
+// Per-cell epilogue: everything after the demag field of one cell.
+template <typename T>
+__device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T> &G, const StepCtl *ctl,
+                                           const T *__restrict__ Y, int x, int y, int z, const V3<T> &Hd,
+                                           double &err_max, double &dydt_max)
+{
+    const size_t n = S.n;
+    const size_t i = (size_t)x + (size_t)S.m.nx * ((size_t)y + (size_t)S.m.ny * z);
+    const T ms = S.Ms.at(i);
+    const V3<T> Mv = v3<T>(Y[i], Y[n + i], Y[2 * n + i]);
+    V3<T> H = Hd;
+    if (S.has_exch && ms != (T)0) {
+        const V3<T> he = exchange_cell<T>(S.m, x, y, z, i, n, S.wx, S.wy, S.wz, S.coef_ex, S.Ms.p ? S.Ms.p : nullptr,
+                                          S.A.p ? S.A.p : nullptr, Y, ms);
+        H.x += he.x;
+        H.y += he.y;
+        H.z += he.z;
+    }
+    if (S.has_uni || S.has_cub) {
+        T e0;
+        V3<T> ha = v3<T>(0, 0, 0);
+        if (S.has_cub) ha = cubic_cell<T>(Mv, S.axis1.at(n, i), S.axis2.at(n, i), S.kcub.at(i), ms, S.cm2mu0, &e0);
+        if (S.has_uni) {
+            const V3<T> hu = uniaxial_cell<T>(Mv, S.axis1.at(n, i), S.kuni.at(i), ms, S.mu0, &e0);
+            if (S.has_cub) {
+                ha.x += hu.x;
+                ha.y += hu.y;
+                ha.z += hu.z;
+            } else {
+                ha = hu;
+            }
+        }
+        H.x += ha.x;
+        H.y += ha.y;
+        H.z += ha.z;
+    }
+    if (S.has_zee) {
+        H.x += S.hext[0];
+        H.y += S.hext[1];
+        H.z += S.hext[2];
+    }
+    V3<T> k = llge_cell<T>(Mv, H, S.f1.at(i), S.f2.at(i));
+    if (S.has_stt) {
+        const V3<T> st = zhangli_cell<T>(S.m, x, y, z, i, n, S.dx, S.dy, S.dz, S.do_precess != 0, S.P.at(i), S.xi.at(i),
+                                         ms, S.alpha.at(i), S.j.at(n, i), Y, S.mu_b, S.e_charge);
+        k.x += st.x;
+        k.y += st.y;
+        k.z += st.z;
+    }
+    if (G.kout) store3<T>(G.kout, n, i, k);
+    if (G.Hout) store3<T>(G.Hout, n, i, H);
+    if (G.want_dydt) {
+        const double a = k.x, b = k.y, c = k.z;
+        double q = a * a + b * b + c * c;
+        if (!S.Ms_uniform) {
+            const double s = ms;
+            q = s == 0.0 ? 0.0 : q / (s * s);
+        }
+        dydt_max = fmax(dydt_max, q);
+    }
+    if (!G.prep) return;
+    const double h = ctl->h;
+    const T *yc = ctl->cur ? G.yb1 : G.yb0;
+    T kc[3] = {k.x, k.y, k.z};
+    T out[3];
+    double er[3] = {0.0, 0.0, 0.0};
+    for (int c = 0; c < 3; ++c) {
+        const size_t o = (size_t)c * n + i;
+        if (G.form == FORM_SUM) {
+            T s = (T)0;
+            bool first = true;
+            for (int q = 0; q < G.nk; ++q) {
+                const T term = (T)G.b[q] * G.kin[q][o];
+                s = first ? term : s + term;
+                first = false;
+            }
+            const T ts = (T)G.bself * kc[c];
+            s = first ? ts : s + ts;
+            out[c] = yc[o] + (T)h * s;
+            if (G.prep == 2) {
+                double e = 0.0;
+                bool f2 = true;
+                for (int q = 0; q < G.nk; ++q) {
+                    const double term = G.ec[q] * (double)G.kin[q][o];
+                    e = f2 ? term : e + term;
+                    f2 = false;
+                }
+                const double tse = G.ecself * (double)kc[c];
+                e = f2 ? tse : e + tse;
+                er[c] = h * e;
+            }
+        } else {
+            T v = yc[o];
+            double e = 0.0;
+            for (int q = 0; q < G.nk; ++q) {
+                const T kq = G.kin[q][o];
+                v += kq * (T)(h * G.b[q]);
+                if (G.prep == 2) e += (double)kq * (h * G.ec[q]);
+            }
+            v += kc[c] * (T)(h * G.bself);
+            if (G.prep == 2) e += (double)kc[c] * (h * G.ecself);
+            out[c] = v;
+            er[c] = e;
+        }
+    }
+    if (G.prep == 1) {
+        G.prep_out[i] = out[0];
+        G.prep_out[n + i] = out[1];
+        G.prep_out[2 * n + i] = out[2];
+        return;
+    }
+    // final stage: error norm and normalised new state into the other buffer
+    {
+        double q = er[0] * er[0] + er[1] * er[1] + er[2] * er[2];
+        if (!S.Ms_uniform) {
+            const double s = ms;
+            q = s == 0.0 ? 0.0 : q / (s * s);
+        }
+        err_max = fmax(err_max, q);
+    }
+    T *yn = ctl->cur ? G.yb0 : G.yb1;
+    double norm;
+    if (G.norm_mode == 0) {
+        const float e0 = (float)out[0], e1 = (float)out[1], e2 = (float)out[2];
+        const float s = __fadd_rn(__fadd_rn(__fmul_rn(e0, e0), __fmul_rn(e1, e1)), __fmul_rn(e2, e2));
+        norm = sqrt((double)s);
+    } else {
+        norm = sqrt((double)out[0] * out[0] + (double)out[1] * out[1] + (double)out[2] * out[2]);
+    }
+    if (norm != 0.0) {
+        const double sc = (double)ms / norm;
+        out[0] = (T)(out[0] * sc);
+        out[1] = (T)(out[1] * sc);
+        out[2] = (T)(out[2] * sc);
+    }
+    yn[i] = out[0];
+    yn[n + i] = out[1];
+    yn[2 * n + i] = out[2];
+}
+
+// x inverse of the demag spectrum rows + epilogue.  rpw rows per block, all
+// three components of a row in the same block.  has_demag == 0: rows are
+// processed without the FFT part (H_demag = 0).
+template <typename T>
+__global__ void k_stage(SysDev<T> S, StageDev<T> G, StepCtl *ctl, const C2<T> *__restrict__ S1, FftPlanDev p,
+                        int Hx, int rpw, int has_demag)
+{
+    if (ctl->stop) return;
+    extern __shared__ char smem_raw[];
+    C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
+    const int nx = S.m.nx, ny = S.m.ny, nz = S.m.nz;
+    const int nrows = ny * nz;
+    const int row0 = blockIdx.x * rpw;
+    const T *Y = G.Ys ? G.Ys : (ctl->cur ? G.yb1 : G.yb0);
+    const int L = p.L, LP = L + 1;
+    if (has_demag) {
+        const int l = threadIdx.x / p.tline, t = threadIdx.x % p.tline;
+        const int c = l / rpw, r = l % rpw;
+        const int gr = row0 + r;
+        const bool active = (l < 3 * rpw) && gr < nrows;
+        C2<T> *line = sm + (size_t)l * LP;
+        if (active) {
+            const C2<T> *src = S1 + ((size_t)c * nrows + gr) * Hx;
+            for (int k = t; k < L; k += p.tline) line[k] = k < Hx ? src[k] : cconj<T>(src[L - k]);
+        }
+        __syncthreads();
+        lds_fft<T, true>(line, p, t, active);
+    }
+    double err_max = 0.0, dydt_max = 0.0;
+    for (int e = threadIdx.x; e < rpw * nx; e += blockDim.x) {
+        const int r = e / nx, x = e % nx;
+        const int gr = row0 + r;
+        if (gr >= nrows) continue;
+        V3<T> Hd = v3<T>(0, 0, 0);
+        if (has_demag) Hd = v3<T>(sm[(0 * rpw + r) * LP + x].x, sm[(1 * rpw + r) * LP + x].x, sm[(2 * rpw + r) * LP + x].x);
+        stage_cell<T>(S, G, ctl, Y, x, gr % ny, gr / ny, Hd, err_max, dydt_max);
+    }
+    if (G.prep == 2 || G.want_dydt) {
+        __shared__ double red[2][16];
+        double a = wave_max(err_max), b = wave_max(dydt_max);
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        if (lane == 0) {
+            red[0][w] = a;
+            red[1][w] = b;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double ea = red[0][0], eb = red[1][0];
+            for (int q = 1; q < (int)((blockDim.x + 63) >> 6); ++q) {
+                ea = fmax(ea, red[0][q]);
+                eb = fmax(eb, red[1][q]);
+            }
+            if (G.prep == 2) atomic_max_bits<T>(&ctl->err_bits, ea);
+            if (G.want_dydt) atomic_max_bits<T>(&ctl->dydt_bits, eb);
+        }
+    }
+}
+
+// MicroMagneticsStepSizeController.adjust_stepsize
+// (micro_magnetics_stepsize_controller.py:35-73) + RungeKutta.evolve retry
+// (runge_kutta.py:56-72) + state update (:80-86).
+__global__ void k_controller(StepCtl *ctl, int order, double eps_abs, double eps_rel, int Ms_uniform, double Ms_u,
+                             int adaptive)
+{
+    if (ctl->stop) return;
+    ctl->attempts += 1;
+    const double h = ctl->h;
+    const unsigned long long ebits = ctl->err_bits, dbits = ctl->dydt_bits;
+    ctl->err_bits = 0ull;
+    ctl->dydt_bits = 0ull;
+    if (!adaptive) {
+        ctl->t += h;
+        ctl->h_last = h;
+        ctl->cur = 1 - ctl->cur;
+        ctl->accepted += 1;
+        if (ctl->accepted >= ctl->target) ctl->stop = 1;
+        return;
+    }
+    double e_abs = sqrt(__longlong_as_double((long long)ebits));
+    double dmax = sqrt(__longlong_as_double((long long)dbits));
+    if (Ms_uniform) {
+        e_abs /= Ms_u;
+        dmax /= Ms_u;
+    }
+    const double HEADROOM = 0.85, MINS = 0.2, MAXS = 5.0, MINH = 1e-15, MAXH = 1e-10;
+    const double den = h * dmax;
+    int accept;
+    double hn;
+    if (den == 0.0) {
+        accept = 1;
+        hn = 5e-14;
+    } else {
+        const double e_rel = e_abs / den;
+        accept = (e_abs <= eps_abs) && (e_rel <= eps_rel);
+        const double sa = HEADROOM * pow(eps_abs / e_abs, 1.0 / (order + 1.0));
+        const double sr = HEADROOM * pow(eps_rel / e_rel, 1.0 / order);
+        double sc = sa < sr ? sa : sr;
+        if (sc < MINS) sc = MINS;
+        if (sc > MAXS) sc = MAXS;
+        hn = h * sc;
+        if (hn < MINH) hn = MINH;
+        if (hn > MAXH) hn = MAXH;
+        if (hn == MINH) accept = 1;
+    }
+    if (accept) {
+        ctl->t += h;
+        ctl->h_last = h;
+        ctl->cur = 1 - ctl->cur;
+        ctl->accepted += 1;
+        if (ctl->accepted >= ctl->target) ctl->stop = 1;
+    } else {
+        ctl->rejections += 1;
+    }
+    ctl->h = hn;
+}
+
+template <typename T> __global__ void k_cast_in(size_t n, const double *__restrict__ a, T *__restrict__ b)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        b[i] = (T)a[i];
+}
+template <typename T> __global__ void k_cast_out(size_t n, const T *__restrict__ a, double *__restrict__ b)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        b[i] = (double)a[i];
+}
+template <typename T> __global__ void k_sum3_sel(size_t n, const T *y0, const T *y1, const StepCtl *ctl, double *part)
+{
+    const T *A = ctl->cur ? y1 : y0;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        s0 += (double)A[i];
+        s1 += (double)A[n + i];
+        s2 += (double)A[2 * n + i];
+    }
+    block_sum_store(s0, part);
+    block_sum_store(s1, part + MM_RED_BLOCKS);
+    block_sum_store(s2, part + 2 * MM_RED_BLOCKS);
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct StepperBase {
+    virtual ~StepperBase() {}
+    virtual void set_m(const double *M_host) = 0;
+    virtual void get_m(double *M_host) = 0;
+    virtual void dmdt(double *dM_host, double *H_host) = 0;
+    virtual void run(long long steps) = 0;
+    virtual void sync() = 0;
+    virtual void stats(double *s5) = 0;
+    virtual void average(double *a3) = 0;
+    virtual hipStream_t stream() = 0;
+};
+
+static bool all_equal(const double *a, size_t n)
+{
+    for (size_t i = 1; i < n; ++i)
+        if (a[i] != a[0]) return false;
+    return true;
+}
+
+template <typename T> struct Stepper : public StepperBase {
+    mm_system_desc_t d;
+    size_t n = 0;
+    hipStream_t s = nullptr;
+    ConvImpl<T> *conv = nullptr;
+    SysDev<T> sys;
+    mm_tableau_t tab;
+    std::vector<void *> allocs;
+    T *yb[2] = {nullptr, nullptr};
+    T *k[8] = {nullptr};
+    T *tmp[2] = {nullptr, nullptr};
+    T *Hbuf = nullptr;
+    double *dbuf = nullptr;
+    double *part = nullptr;
+    StepCtl *ctl = nullptr;
+    int rpw = 1;
+    size_t stage_lds = 0;
+    int grid_rows = 1;
+    int threads = 256;
+
+    T *dalloc(size_t count)
+    {
+        void *p = nullptr;
+        MM_HIP(hipMalloc(&p, sizeof(T) * (count ? count : 1)));
+        allocs.push_back(p);
+        return (T *)p;
+    }
+
+    Param<T> param(const double *h, double fallback)
+    {
+        Param<T> p;
+        p.p = nullptr;
+        p.u = (T)fallback;
+        if (!h) return p;
+        if (all_equal(h, n)) {
+            p.u = (T)h[0];
+            return p;
+        }
+        T *dp = dalloc(n);
+        upload(dp, h, n);
+        p.p = dp;
+        return p;
+    }
+
+    VParam<T> vparam(const double *h)
+    {
+        VParam<T> p;
+        p.p = nullptr;
+        p.u[0] = p.u[1] = p.u[2] = (T)0;
+        if (!h) return p;
+        if (all_equal(h, n) && all_equal(h + n, n) && all_equal(h + 2 * n, n)) {
+            for (int c = 0; c < 3; ++c) p.u[c] = (T)h[c * n];
+            return p;
+        }
+        T *dp = dalloc(3 * n);
+        upload(dp, h, 3 * n);
+        p.p = dp;
+        return p;
+    }
+
+    void upload(T *dst, const double *src, size_t cnt)
+    {
+        MM_HIP(hipMemcpyAsync(dbuf, src, sizeof(double) * cnt, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_cast_in<T>, dim3(1024), dim3(256), 0, s, cnt, dbuf, dst);
+        MM_HIP(hipGetLastError());
+        MM_HIP(hipStreamSynchronize(s));
+    }
+
+    void download(double *dst, const T *src, size_t cnt)
+    {
+        hipLaunchKernelGGL(k_cast_out<T>, dim3(1024), dim3(256), 0, s, cnt, src, dbuf);
+        MM_HIP(hipGetLastError());
+        MM_HIP(hipMemcpyAsync(dst, dbuf, sizeof(double) * cnt, hipMemcpyDeviceToHost, s));
+        MM_HIP(hipStreamSynchronize(s));
+    }
+
+    void init(const mm_system_desc_t &desc)
+    {
+        d = desc;
+        MM_REQUIRE(d.nx > 0 && d.ny > 0 && d.nz > 0, "bad mesh size");
+        MM_REQUIRE(d.Ms_host && d.alpha_host, "Ms and alpha are required");
+        n = (size_t)d.nx * d.ny * d.nz;
+        MM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        MM_HIP(hipMalloc(&dbuf, sizeof(double) * 3 * n));
+        MM_HIP(hipMalloc(&part, sizeof(double) * 3 * MM_RED_BLOCKS));
+        MM_HIP(hipMalloc(&ctl, sizeof(StepCtl)));
+        int rc = mm_tableau(d.method, &tab);
+        if (rc != MM_OK) throw mm_error(rc, mm_last_error());
+        if (d.method == MM_DP54 || d.method == MM_RK23)
+            throw mm_error(MM_ERR_UNSUPPORTED, "fused stepper: FSAL methods (dp54, rk23) are not implemented yet");
+
+        // physical parameters; LLG factors as landau_lifshitz_gilbert.py:133-158
+        const double mu0 = MM_MU0_VALUE;
+        std::vector<double> f1(n), f2(n);
+        for (size_t i = 0; i < n; ++i) {
+            const double a = d.alpha_host[i], ms = d.Ms_host[i];
+            if (ms != 0.0) {
+                const double gp = MM_GAMMA / (1.0 + a * a);
+                f1[i] = d.do_precess ? -gp : 0.0;
+                f2[i] = -a * gp / ms;
+            } else {
+                f1[i] = 0.0;
+                f2[i] = 0.0;
+            }
+        }
+        std::memset(&sys, 0, sizeof(sys));
+        sys.m.nx = d.nx;
+        sys.m.ny = d.ny;
+        sys.m.nz = d.nz;
+        sys.n = n;
+        sys.dx = (T)d.dx;
+        sys.dy = (T)d.dy;
+        sys.dz = (T)d.dz;
+        sys.wx = (T)(1.0 / (d.dx * d.dx));
+        sys.wy = (T)(1.0 / (d.dy * d.dy));
+        sys.wz = (T)(1.0 / (d.dz * d.dz));
+        sys.coef_ex = (T)(2.0 / mu0);
+        sys.mu0 = (T)mu0;
+        sys.cm2mu0 = (T)(-2.0 / mu0);
+        sys.mu_b = (T)MM_MU_BOHR;
+        sys.e_charge = (T)MM_E_CHARGE;
+        sys.Ms_uniform = all_equal(d.Ms_host, n);
+        sys.Ms_u = d.Ms_host[0];
+        // exchange and Zhang-Li index Ms/A through arrays in exchange_cell:
+        // keep real arrays for them (uniform values still use scalars elsewhere)
+        {
+            T *dp = dalloc(n);
+            upload(dp, d.Ms_host, n);
+            sys.Ms.p = dp;
+            sys.Ms.u = (T)d.Ms_host[0];
+        }
+        sys.alpha = param(d.alpha_host, 0.0);
+        sys.f1 = param(f1.data(), 0.0);
+        sys.f2 = param(f2.data(), 0.0);
+        sys.has_exch = d.A_host != nullptr;
+        if (sys.has_exch) {
+            T *dp = dalloc(n);
+            upload(dp, d.A_host, n);
+            sys.A.p = dp;
+        }
+        sys.has_uni = d.k_uni_host != nullptr;
+        sys.has_cub = d.k_cub_host != nullptr;
+        if (sys.has_uni) sys.kuni = param(d.k_uni_host, 0.0);
+        if (sys.has_cub) sys.kcub = param(d.k_cub_host, 0.0);
+        if (sys.has_uni || sys.has_cub) {
+            MM_REQUIRE(d.axis1_host, "anisotropy needs axis1");
+            sys.axis1 = vparam(d.axis1_host);
+        }
+        if (sys.has_cub) {
+            MM_REQUIRE(d.axis2_host, "cubic anisotropy needs axis2");
+            sys.axis2 = vparam(d.axis2_host);
+        }
+        sys.has_zee = d.use_zeeman;
+        for (int c = 0; c < 3; ++c) sys.hext[c] = (T)d.h_ext[c];
+        sys.has_stt = d.P_host != nullptr;
+        sys.do_precess = d.do_precess;
+        if (sys.has_stt) {
+            MM_REQUIRE(d.xi_host && d.j_host, "spin torque needs xi and j");
+            sys.P = param(d.P_host, 0.0);
+            sys.xi = param(d.xi_host, 0.0);
+            sys.j = vparam(d.j_host);
+        }
+
+        if (d.N_host) {
+            conv = new ConvImpl<T>();
+            conv->init(d.N_host, d.nx, d.ny, d.nz, d.ex, d.ey, d.ez);
+            const int tl = conv->px.tline;
+            rpw = std::max(1, 512 / (3 * tl));
+            while (rpw > 1 && (size_t)3 * rpw * (conv->px.L + 1) * sizeof(C2<T>) > (size_t)mm_max_lds_bytes()) --rpw;
+            MM_REQUIRE(3 * tl <= 1024, "x transform too long for the fused stage kernel");
+            threads = 3 * rpw * tl;
+            stage_lds = (size_t)3 * rpw * (conv->px.L + 1) * sizeof(C2<T>);
+            if (stage_lds > 65536)
+                MM_HIP(hipFuncSetAttribute((const void *)k_stage<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)stage_lds));
+        } else {
+            rpw = std::max(1, 256 / d.nx);
+            threads = 256;
+            stage_lds = 0;
+        }
+        threads = std::max(64, ((threads + 63) / 64) * 64);
+        grid_rows = (d.ny * d.nz + rpw - 1) / rpw;
+
+        for (int b = 0; b < 2; ++b) yb[b] = dalloc(3 * n);
+        for (int q = 0; q < tab.num_steps; ++q) k[q] = dalloc(3 * n);
+        tmp[0] = dalloc(3 * n);
+        tmp[1] = dalloc(3 * n);
+        Hbuf = dalloc(3 * n);
+        StepCtl c0;
+        std::memset(&c0, 0, sizeof(c0));
+        c0.h = d.h0;
+        c0.h_last = d.h0;
+        MM_HIP(hipMemcpy(ctl, &c0, sizeof(c0), hipMemcpyHostToDevice));
+        MM_HIP(hipMemset(yb[0], 0, sizeof(T) * 3 * n));
+    }
+
+    void release()
+    {
+        if (conv) {
+            conv->release();
+            delete conv;
+            conv = nullptr;
+        }
+        for (void *p : allocs) (void)hipFree(p);
+        allocs.clear();
+        if (dbuf) (void)hipFree(dbuf);
+        if (part) (void)hipFree(part);
+        if (ctl) (void)hipFree(ctl);
+        if (s) (void)hipStreamDestroy(s);
+        dbuf = part = nullptr;
+        ctl = nullptr;
+        s = nullptr;
+    }
+
+    ~Stepper() override { release(); }
+
+    hipStream_t stream() override { return s; }
+
+    // one stage: dM/dt at stage input Ys (null = y[cur])
+    void launch_stage(const T *Ys, const StageDev<T> &G)
+    {
+        if (conv) {
+            if (Ys) conv->forward_x(Ys, s);
+            else conv->forward_x(yb[0], s, yb[1], &ctl->cur);
+            conv->spectral(s);
+        }
+        hipLaunchKernelGGL(k_stage<T>, dim3(grid_rows), dim3(threads), stage_lds, s, sys, G, ctl,
+                           conv ? conv->S1 : nullptr, conv ? conv->px.template dev<T>() : FftPlanDev(),
+                           conv ? conv->g.Hx : 0, rpw, conv ? 1 : 0);
+        MM_HIP(hipGetLastError());
+    }
+
+    StageDev<T> blank_stage(const T *Ys)
+    {
+        StageDev<T> G;
+        std::memset(&G, 0, sizeof(G));
+        G.Ys = Ys;
+        G.yb0 = yb[0];
+        G.yb1 = yb[1];
+        G.norm_mode = d.normalize_mode;
+        return G;
+    }
+
+    // one attempt of the tableau from y[cur] with h from the control block
+    void enqueue_attempt()
+    {
+        const int ns = tab.num_steps;
+        const bool six = ns == 6;
+        for (int st = 0; st < ns; ++st) {
+            const T *Ys = st == 0 ? nullptr : tmp[(st - 1) & 1];
+            StageDev<T> G = blank_stage(Ys);
+            G.kout = (st < ns - 1) ? k[st] : nullptr;
+            G.want_dydt = (st == 0);
+            G.form = six ? FORM_SUM : FORM_SEQ;
+            if (st < ns - 1) {
+                G.prep = 1;
+                G.prep_out = tmp[st & 1];
+                G.nk = 0;
+                for (int q = 0; q < st; ++q) {
+                    G.kin[G.nk] = k[q];
+                    G.b[G.nk++] = tab.b[st + 1][q];
+                }
+                G.bself = tab.b[st + 1][st];
+                if (!six) {
+                    // sequential form of runge_kutta.py:107-111 / runge_kutta_4.py keeps
+                    // zero coefficients out (RK4 only adds k_s with h*b)
+                    int m2 = 0;
+                    for (int q = 0; q < G.nk; ++q)
+                        if (G.b[q] != 0.0) {
+                            G.kin[m2] = G.kin[q];
+                            G.b[m2++] = G.b[q];
+                        }
+                    G.nk = m2;
+                }
+            } else {
+                G.prep = 2;
+                G.nk = 0;
+                const bool skip1 = six && tab.c[1] == 0.0 && tab.ec[1] == 0.0;
+                for (int q = 0; q < st; ++q) {
+                    if (q == 1 && skip1) continue;
+                    G.kin[G.nk] = k[q];
+                    G.b[G.nk] = tab.c[q];
+                    G.ec[G.nk++] = tab.ec[q];
+                }
+                G.bself = tab.c[st];
+                G.ecself = tab.ec[st];
+            }
+            launch_stage(Ys, G);
+        }
+        const bool adaptive = d.method != MM_RK4;
+        hipLaunchKernelGGL(k_controller, dim3(1), dim3(1), 0, s, ctl, tab.order, d.eps_abs, d.eps_rel,
+                           sys.Ms_uniform, sys.Ms_u, adaptive ? 1 : 0);
+        MM_HIP(hipGetLastError());
+    }
+
+    void set_m(const double *M_host) override
+    {
+        StepCtl c;
+        MM_HIP(hipMemcpy(&c, ctl, sizeof(c), hipMemcpyDeviceToHost));
+        upload(yb[c.cur], M_host, 3 * n);
+    }
+
+    void get_m(double *M_host) override
+    {
+        StepCtl c;
+        MM_HIP(hipStreamSynchronize(s));
+        MM_HIP(hipMemcpy(&c, ctl, sizeof(c), hipMemcpyDeviceToHost));
+        download(M_host, yb[c.cur], 3 * n);
+    }
+
+    void dmdt(double *dM_host, double *H_host) override
+    {
+        StageDev<T> G = blank_stage(nullptr);
+        G.kout = tmp[0];
+        G.Hout = Hbuf;
+        launch_stage(nullptr, G);
+        if (dM_host) download(dM_host, tmp[0], 3 * n);
+        if (H_host) download(H_host, Hbuf, 3 * n);
+    }
+
+    void run(long long steps) override
+    {
+        if (steps <= 0) return;
+        StepCtl c;
+        MM_HIP(hipStreamSynchronize(s));
+        MM_HIP(hipMemcpy(&c, ctl, sizeof(c), hipMemcpyDeviceToHost));
+        c.target = c.accepted + steps;
+        c.stop = 0;
+        MM_HIP(hipMemcpy(ctl, &c, sizeof(c), hipMemcpyHostToDevice));
+        long long remaining = steps;
+        while (remaining > 0) {
+            for (long long q = 0; q < remaining; ++q) enqueue_attempt();
+            MM_HIP(hipStreamSynchronize(s));
+            MM_HIP(hipMemcpy(&c, ctl, sizeof(c), hipMemcpyDeviceToHost));
+            remaining = c.target - c.accepted;
+        }
+    }
+
+    void sync() override { MM_HIP(hipStreamSynchronize(s)); }
+
+    void stats(double *s5) override
+    {
+        StepCtl c;
+        MM_HIP(hipStreamSynchronize(s));
+        MM_HIP(hipMemcpy(&c, ctl, sizeof(c), hipMemcpyDeviceToHost));
+        s5[0] = c.t;
+        s5[1] = c.h_last;
+        s5[2] = c.h;
+        s5[3] = (double)c.attempts;
+        s5[4] = (double)c.rejections;
+    }
+
+    void average(double *a3) override
+    {
+        const int grid = (int)std::max<size_t>(1, std::min<size_t>(MM_RED_BLOCKS, (n + 255) / 256));
+        hipLaunchKernelGGL(k_sum3_sel<T>, dim3(grid), dim3(MM_RED_THREADS), 0, s, n, yb[0], yb[1], ctl, part);
+        MM_HIP(hipGetLastError());
+        std::vector<double> h((size_t)3 * MM_RED_BLOCKS);
+        MM_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * 3 * MM_RED_BLOCKS, hipMemcpyDeviceToHost, s));
+        MM_HIP(hipStreamSynchronize(s));
+        for (int c = 0; c < 3; ++c) {
+            double acc = 0.0;
+            for (int b = 0; b < grid; ++b) acc += h[(size_t)c * MM_RED_BLOCKS + b];
+            a3[c] = acc / (double)n;
+        }
+    }
+};
+
+struct mm_stepper {
+    StepperBase *impl;
+};
+
+extern "C" int mm_stepper_create(const mm_system_desc_t *desc, mm_stepper_t **out)
+{
+    MM_CAPI_BEGIN
+    MM_REQUIRE(desc && out, "null argument");
+    MM_REQUIRE(desc->precision == MM_FP64 || desc->precision == MM_FP32, "bad precision");
+    mm_stepper *st = new mm_stepper();
+    st->impl = nullptr;
+    try {
+        if (desc->precision == MM_FP64) {
+            auto *p = new Stepper<double>();
+            st->impl = p;
+            p->init(*desc);
+        } else {
+            auto *p = new Stepper<float>();
+            st->impl = p;
+            p->init(*desc);
+        }
+    } catch (...) {
+        delete st->impl;
+        delete st;
+        throw;
+    }
+    *out = st;
+    MM_CAPI_END
+}
+
+extern "C" int mm_stepper_set_m(mm_stepper_t *st, const double *M_host)
+{
+    MM_CAPI_BEGIN
+    MM_REQUIRE(st && M_host, "null argument");
+    st->impl->set_m(M_host);
+    MM_CAPI_END
+}
+
+extern "C" int mm_stepper_get_m(mm_stepper_t *st, double *M_host)
+{
+    MM_CAPI_BEGIN
+    MM_REQUIRE(st && M_host, "null argument");
+    st->impl->get_m(M_host);
+    MM_CAPI_END
+}
+
+extern "C" int mm_stepper_dmdt(mm_stepper_t *st, double *dMdt_host, double *Htot_host)
+{
+    MM_CAPI_BEGIN
+    MM_REQUIRE(st, "null argument");
+    st->impl->dmdt(dMdt_host, Htot_host);
+    MM_CAPI_END
+}
+
+extern "C" int mm_stepper_run(mm_stepper_t *st, int steps)
+{
+    MM_CAPI_BEGIN
+    MM_REQUIRE(st && steps >= 0, "bad argument");
+    st->impl->run(steps);
+    MM_CAPI_END
+}
+
+extern "C" int mm_stepper_sync(mm_stepper_t *st)
+{
+    MM_CAPI_BEGIN
+    MM_REQUIRE(st, "null argument");
+    st->impl->sync();
+    MM_CAPI_END
+}
+
+extern "C" int mm_stepper_stats(mm_stepper_t *st, double *stats5)
+{
+    MM_CAPI_BEGIN
+    MM_REQUIRE(st && stats5, "null argument");
+    st->impl->stats(stats5);
+    MM_CAPI_END
+}
+
+extern "C" int mm_stepper_average_m(mm_stepper_t *st, double *avg3)
+{
+    MM_CAPI_BEGIN
+    MM_REQUIRE(st && avg3, "null argument");
+    st->impl->average(avg3);
+    MM_CAPI_END
+}
+
+extern "C" void *mm_stepper_stream(mm_stepper_t *st) { return st ? (void *)st->impl->stream() : nullptr; }
+
+extern "C" int mm_stepper_destroy(mm_stepper_t *st)
+{
+    MM_CAPI_BEGIN
+    if (!st) return MM_OK;
+    delete st->impl;
+    delete st;
+    MM_CAPI_END
+}

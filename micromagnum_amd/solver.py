@@ -1,0 +1,130 @@
+"""Fused micromagnetic stepper (device-resident LLG + Runge-Kutta).
+
+Counterpart of the reference's create_solver/Solver.solve loop
+(src/magnum/micromagnetics/create_solver.py:27-79, src/magnum/solver/solver.py:76-142)
+for the modules StrayField, ExchangeField, AnisotropyField, ExternalField and
+SpinTorque, in that order, driven by rkf45/cc45 (MicroMagneticsStepSizeController)
+or classical rk4 with a fixed step.  The whole attempt loop runs on the device;
+see include/mm_hotpath.h (mm_stepper_*).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .magneto import GenerateDemagTensor, PADDING_ROUND_4
+
+
+def _field(a, shape, comps=1):
+    nx, ny, nz = shape
+    if a is None:
+        return None
+    full = ((comps,) if comps > 1 else ()) + (nz, ny, nx)
+    return np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=np.float64), full))
+
+
+class Stepper:
+    def __init__(self, shape, delta, Ms, alpha, A=None, demag=True, N=None, k_uni=None, k_cub=None,
+                 axis1=None, axis2=None, h_ext=None, P=None, xi=None, j=None, do_precess=True,
+                 method="rkf45", eps_abs=1e-4, eps_rel=1e-4, h0=1e-13, precision="fp64",
+                 normalize_mode=0, padding=PADDING_ROUND_4):
+        L = _lib.device_lib()
+        nx, ny, nz = shape
+        self.shape = shape
+        self.delta = tuple(float(v) for v in delta)
+        self.n = nx * ny * nz
+        keep = []
+
+        def f(a, comps=1):
+            arr = _field(a, shape, comps)
+            if arr is None:
+                return None
+            keep.append(arr)
+            return arr.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+        d = _lib.SystemDesc()
+        d.nx, d.ny, d.nz = nx, ny, nz
+        d.dx, d.dy, d.dz = self.delta
+        d.precision = _lib.MM_FP64 if precision == "fp64" else _lib.MM_FP32
+        d.method = _lib.METHODS[method]
+        d.eps_abs, d.eps_rel, d.h0 = float(eps_abs), float(eps_rel), float(h0)
+        d.normalize_mode = int(normalize_mode)
+        d.do_precess = int(bool(do_precess))
+        d.Ms_host = f(Ms)
+        d.alpha_host = f(alpha)
+        d.A_host = f(A)
+        d.k_uni_host = f(k_uni)
+        d.k_cub_host = f(k_cub)
+        d.axis1_host = f(axis1, 3)
+        d.axis2_host = f(axis2, 3)
+        d.P_host = f(P)
+        d.xi_host = f(xi)
+        d.j_host = f(j, 3)
+        if h_ext is not None:
+            d.use_zeeman = 1
+            for c in range(3):
+                d.h_ext[c] = float(h_ext[c])
+        if demag:
+            if N is None:
+                N = GenerateDemagTensor(nx, ny, nz, *self.delta, padding=padding)
+            N = np.ascontiguousarray(N, dtype=np.float64)
+            keep.append(N)
+            d.N_host = N.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+            d.ez, d.ey, d.ex = N.shape[:3]
+        self.N = N if demag else None
+        h = ctypes.c_void_p()
+        _lib.check(L.mm_stepper_create(ctypes.byref(d), ctypes.byref(h)))
+        self._h = h
+        self.precision = precision
+        self.method = method
+
+    def _dp(self, a):
+        return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+    def set_m(self, M):
+        M = _field(M, self.shape, 3).copy()
+        _lib.check(_lib.load().mm_stepper_set_m(self._h, self._dp(M)))
+
+    def get_m(self):
+        nx, ny, nz = self.shape
+        M = np.empty((3, nz, ny, nx))
+        _lib.check(_lib.load().mm_stepper_get_m(self._h, self._dp(M)))
+        return M
+
+    def dmdt(self):
+        nx, ny, nz = self.shape
+        dM = np.empty((3, nz, ny, nx))
+        H = np.empty((3, nz, ny, nx))
+        _lib.check(_lib.load().mm_stepper_dmdt(self._h, self._dp(dM), self._dp(H)))
+        return dM, H
+
+    def run(self, steps):
+        _lib.check(_lib.load().mm_stepper_run(self._h, int(steps)))
+
+    def sync(self):
+        _lib.check(_lib.load().mm_stepper_sync(self._h))
+
+    def stats(self):
+        s = np.zeros(5)
+        _lib.check(_lib.load().mm_stepper_stats(self._h, self._dp(s)))
+        return dict(t=s[0], h=s[1], h_next=s[2], attempts=int(s[3]), rejections=int(s[4]))
+
+    def average_m(self):
+        a = np.zeros(3)
+        _lib.check(_lib.load().mm_stepper_average_m(self._h, self._dp(a)))
+        return a
+
+    @property
+    def stream(self):
+        return _lib.load().mm_stepper_stream(self._h)
+
+    def close(self):
+        if getattr(self, "_h", None) and _lib._LIB is not None:
+            _lib._LIB.mm_stepper_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

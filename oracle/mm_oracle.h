@@ -1,0 +1,109 @@
+/*
+ * mm_oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * FP64 CPU restatement of MicroMagnum's effective-field + Runge-Kutta hot
+ * path.  It is the checker for the HIP library: only tests/, the smoke check
+ * and bench.py's CPU-baseline leg load it.  Nothing in the product links it.
+ *
+ * Layout conventions (same as the reference's matty storage,
+ * src/magneto/matrix/Shape.cpp:64-75, VectorMatrix_accessor.cpp:28-41):
+ *   scalar field   : double[n],      n = nx*ny*nz, index x + nx*(y + ny*z)
+ *   vector field   : double[3*n],    SoA: [x-comp n | y-comp n | z-comp n]
+ *   demag tensor   : double[6*Px*Py*Pz], component fastest
+ *                    (xx,xy,xz,yy,yz,zz), as GenerateDemagTensor returns it
+ *                    (src/magneto/mmm/demag/tensor.cpp:40).
+ */
+#ifndef MM_ORACLE_H
+#define MM_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- constants (src/magneto/mmm/constants.h:26-40) ---- */
+double orc_mu0(void);
+
+/* ---- 1-D complex DFT, unnormalised; sign -1 forward, +1 inverse ---- */
+void orc_fft(double *re_im /* interleaved, 2*n */, int n, int sign);
+
+/* ---- FFT convolution (src/magneto/math/conv/MatrixVectorConvolution_FFT.cpp:110-240) ---- */
+typedef struct orc_conv orc_conv;
+orc_conv *orc_conv_create(const double *N, int nx, int ny, int nz, int Px, int Py, int Pz);
+void orc_conv_execute(orc_conv *c, const double *M, double *H);
+void orc_conv_destroy(orc_conv *c);
+/* Fourier kernel as stored by the reference (re/im planes, x < Px/2+1), natural
+ * xyz order: out_re/out_im each 6*Hx*Py*Pz, index e + 6*(kx + Hx*(ky + Py*kz)). */
+void orc_conv_get_kernel(orc_conv *c, double *out_re, double *out_im);
+
+/* ---- direct-sum convolution (SymmetricMatrixVectorConvolution_Simple.cpp:36-80) ---- */
+void orc_conv_simple(const double *N, int nx, int ny, int nz, int Px, int Py, int Pz,
+                     const double *M, double *H);
+
+/* ---- local field terms ---- */
+double orc_exchange(int nx, int ny, int nz, double dx, double dy, double dz,
+                    int px, int py, int pz, const double *Ms, const double *A,
+                    const double *M, double *H);
+double orc_uniaxial_anisotropy(int n, const double *axis, const double *k,
+                               const double *Ms, const double *M, double *H);
+double orc_cubic_anisotropy(int n, const double *axis1, const double *axis2,
+                            const double *k, const double *Ms, const double *M, double *H);
+void orc_llge(int n, const double *f1, const double *f2, const double *M,
+              const double *H, double *dM);
+void orc_zhangli(int nx, int ny, int nz, double dx, double dy, double dz, int do_precess,
+                 const double *P, const double *xi, const double *Ms, const double *alpha,
+                 const double *j, const double *M, double *dM);
+
+/* ---- reductions / normalisation ---- */
+double orc_scaled_abs_max(int n, const double *M, const double *s, int s_uniform);
+double orc_abs_max(int n, const double *M);
+double orc_dot_sum(int n, const double *A, const double *B);
+void orc_average(int n, const double *M, double out[3]);
+/* mode 0: reference CPU semantics (norm from float-cast components, zero
+ * vector kept); mode 1: plain FP64 norm. */
+void orc_normalize(int n, double *M, const double *len, int mode);
+
+/* ---- Runge-Kutta tableaus and stage algebra ---- */
+enum { ORC_RKF45 = 0, ORC_CC45 = 1, ORC_DP54 = 2, ORC_RK23 = 3, ORC_RK4 = 4 };
+typedef struct {
+    int num_steps, order, fsal;
+    double a[7], b[7][7], c[7], ec[7];
+} orc_tableau;
+void orc_tableau_get(int method, orc_tableau *t);
+void orc_rk_prepare_step(int step, double h, const orc_tableau *t, double *const *k,
+                         const double *y, double *ytmp, int n);
+void orc_rk_combine_result(double h, const orc_tableau *t, double *const *k,
+                           double *y, double *y_err, int n);
+
+/* ---- micromagnetic system: H_tot and dM/dt in module order ---- */
+typedef struct {
+    int nx, ny, nz;
+    double dx, dy, dz;
+    int px, py, pz;               /* periodic flags (exchange only)                 */
+    const double *Ms, *A, *alpha; /* per-cell                                      */
+    int Ms_uniform;               /* reference Field.isUniform() of Ms              */
+    orc_conv *conv;               /* NULL: no stray field                           */
+    int use_exchange;
+    const double *k_uni, *k_cub, *axis1, *axis2; /* NULL k: term absent             */
+    int use_zeeman;
+    double h_ext[3];
+    const double *P, *xi, *j;     /* NULL P: no spin torque                         */
+    int do_precess;
+    int normalize_mode;           /* 0 reference float-norm, 1 fp64                 */
+} orc_system;
+
+void orc_llge_factors(const orc_system *s, double *f1, double *f2);
+void orc_h_tot(const orc_system *s, const double *M, double *H_tot);
+void orc_dmdt(const orc_system *s, const double *f1, const double *f2,
+              const double *M, double *dMdt, double *H_tot_out /* may be NULL */);
+
+/* Adaptive RK (rkf45/cc45/dp54/rk23) driven by MicroMagneticsStepSizeController,
+ * or fixed-step classical RK4.  Performs `steps` accepted steps in place on M.
+ * stats: [0]=t, [1]=h (last accepted), [2]=h_next, [3]=attempts, [4]=rejections. */
+void orc_run(const orc_system *s, int method, double eps_abs, double eps_rel,
+             double h0, int steps, double *M, double *stats,
+             double *avg_out /* 3*steps or NULL: <M> after each accepted step */);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

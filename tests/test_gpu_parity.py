@@ -1,0 +1,381 @@
+"""GPU parity: every hot-path entry point of libmmhip.so against the CPU
+oracle on the same seeded inputs.  FP64 bar: <= 1e-12 normwise relative
+(rounding-order differences only); FP32 bar stated per test."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import GOLDEN, gpu_available, rel_err, max_rel_err, random_unit_field, vortex_field
+
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not gpu_available(), reason="needs a HIP GPU")]
+torch = pytest.importorskip("torch")
+
+TOL64 = 1e-12
+
+
+def dev(a, dtype=None):
+    dtype = dtype or torch.float64
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(device="cuda", dtype=dtype)
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.double().cpu().numpy()
+
+
+@pytest.fixture(scope="module")
+def mg():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from micromagnum_amd import magneto
+    return magneto
+
+
+def _omf(name):
+    from micromagnum_amd.omf import read_omf
+    return read_omf(os.path.join(GOLDEN, name))
+
+
+# ---------------------------------------------------------------- exchange
+def test_exchange_H1_fixture(mg):
+    M = _omf("M1.omf")
+    Href = _omf("H1_exch.omf")
+    nx, ny, nz = M.num_nodes
+    Ms = np.full(M.data.shape[1:], 8e5)
+    A = np.full(M.data.shape[1:], 13e-12)
+    H = torch.empty_like(dev(M.data))
+    E = mg.exchange(nx, ny, nz, *M.delta, False, False, False, dev(Ms), dev(A), dev(M.data), H)
+    Ho, Eo = O.exchange(M.delta, Ms, A, M.data)
+    assert np.abs(host(H) - Href.data).max() < 2.0
+    assert rel_err(host(H), Ho) < TOL64
+    assert abs(E - Eo) <= TOL64 * abs(Eo)
+
+
+@pytest.mark.parametrize("pbc", [(0, 0, 0), (1, 0, 0), (0, 1, 1), (1, 1, 1)])
+@pytest.mark.parametrize("shape", [(13, 7, 5), (16, 16, 1), (1, 9, 4)])
+def test_exchange_random_vs_oracle(mg, shape, pbc):
+    nx, ny, nz = shape
+    rng = np.random.default_rng(11)
+    Ms = rng.uniform(4e5, 1.4e6, (nz, ny, nx))
+    Ms[rng.random((nz, ny, nx)) < 0.1] = 0.0  # vacuum cells
+    A = rng.uniform(5e-12, 2e-11, (nz, ny, nx))
+    M = random_unit_field(shape, 12) * Ms
+    delta = (2e-9, 3e-9, 2.5e-9)
+    H = torch.empty_like(dev(M))
+    E = mg.exchange(nx, ny, nz, *delta, *pbc, dev(Ms), dev(A), dev(M), H)
+    Ho, Eo = O.exchange(delta, Ms, A, M, pbc)
+    assert rel_err(host(H), Ho) < TOL64
+    assert abs(E - Eo) <= 1e-11 * abs(Eo)
+
+
+def test_exchange_f32_vs_oracle(mg):
+    shape = (32, 24, 4)
+    nx, ny, nz = shape
+    Ms = np.full((nz, ny, nx), 8e5)
+    A = np.full((nz, ny, nx), 1.3e-11)
+    M = random_unit_field(shape, 3) * 8e5
+    H = torch.empty_like(dev(M, torch.float32))
+    mg.exchange_f32(nx, ny, nz, 2e-9, 2e-9, 2e-9, 0, 0, 0, dev(Ms, torch.float32), dev(A, torch.float32),
+                    dev(M, torch.float32), H)
+    Ho, _ = O.exchange((2e-9,) * 3, Ms, A, M)
+    assert rel_err(host(H), Ho) < 1e-5  # FP32 bulk
+
+
+# ---------------------------------------------------------------- anisotropy / llge / stt
+def test_uniaxial_H1_fixture_and_random(mg):
+    M = _omf("M1.omf")
+    Href = _omf("H1_unianiso.omf")
+    n = M.data[0].size
+    axis = np.full(M.data.shape, 1.0 / np.sqrt(3.0))
+    k = np.full(M.data.shape[1:], 520e3)
+    Ms = np.full(M.data.shape[1:], 8e5)
+    H = torch.empty_like(dev(M.data))
+    E = mg.uniaxial_anisotropy(dev(axis), dev(k), dev(Ms), dev(M.data), H)
+    Ho, Eo = O.uniaxial_anisotropy(axis, k, Ms, M.data)
+    assert np.abs(host(H) - Href.data).max() < 10.0
+    assert rel_err(host(H), Ho) < TOL64 and abs(E - Eo) <= TOL64 * abs(Eo)
+    assert n == 10000
+
+
+def test_anisotropy_random_vs_oracle(mg):
+    shape = (17, 9, 3)
+    nx, ny, nz = shape
+    rng = np.random.default_rng(5)
+    Ms = rng.uniform(4e5, 1.4e6, (nz, ny, nx))
+    Ms[0, 0, :3] = 0.0
+    k = rng.uniform(-1e6, 1e6, (nz, ny, nx))
+    k[1, 2, :2] = 0.0
+    M = random_unit_field(shape, 6) * Ms
+    a1 = random_unit_field(shape, 7)
+    tmp = random_unit_field(shape, 8)
+    a2 = np.cross(a1, tmp, axis=0)
+    a2 /= np.sqrt((a2 ** 2).sum(axis=0))
+    H = torch.empty_like(dev(M))
+    E = mg.uniaxial_anisotropy(dev(a1), dev(k), dev(Ms), dev(M), H)
+    Ho, Eo = O.uniaxial_anisotropy(a1, k, Ms, M)
+    assert rel_err(host(H), Ho) < TOL64 and abs(E - Eo) <= 1e-11 * abs(Eo)
+    E = mg.cubic_anisotropy(dev(a1), dev(a2), dev(k), dev(Ms), dev(M), H)
+    Ho, Eo = O.cubic_anisotropy(a1, a2, k, Ms, M)
+    assert rel_err(host(H), Ho) < TOL64 and abs(E - Eo) <= 1e-11 * abs(Eo)
+
+
+def test_llge_known_answer_and_random(mg):
+    sh = (10, 10, 10)
+    M = np.broadcast_to(np.array([5.0, 10.0, 15.0])[:, None, None, None], (3,) + sh)
+    H = np.broadcast_to(np.array([20.0, 25.0, 30.0])[:, None, None, None], (3,) + sh)
+    dM = torch.empty_like(dev(M))
+    mg.llge(dev(np.full(sh, 10.0)), dev(np.full(sh, 20.0)), dev(M), dev(H), dM)
+    d = host(dM)
+    assert np.all(d[0] == -60750.0) and np.all(d[1] == -13500.0) and np.all(d[2] == 29250.0)
+    rng = np.random.default_rng(9)
+    f1, f2 = rng.standard_normal(sh), rng.standard_normal(sh)
+    M, H = rng.standard_normal((3,) + sh), rng.standard_normal((3,) + sh)
+    mg.llge(dev(f1), dev(f2), dev(M), dev(H), dM)
+    assert rel_err(host(dM), O.llge(f1, f2, M, H)) < TOL64
+
+
+@pytest.mark.parametrize("nz", [1, 20])
+def test_zhangli_vortex_known_answer(mg, nz):
+    shape = (100, 100, nz)
+    delta = (3e-9, 3e-9, 3e-9)
+    sh = (nz, 100, 100)
+    M = vortex_field(shape, delta, (150e-9, 150e-9))
+    j = np.broadcast_to(np.array([1e9, -1e9, 0.0])[:, None, None, None], (3,) + sh)
+    args = [np.full(sh, 1.0), np.full(sh, 0.02), np.full(sh, 8e5), np.full(sh, 0.01), j, M]
+    dM = torch.empty_like(dev(M))
+    mg.fdm_zhangli(100, 100, nz, *delta, True, *[dev(a) for a in args], dM)
+    avg = host(dM).reshape(3, -1).mean(axis=1)
+    ref = (340104691300.73352, 337831145947.55255, 7.4024200439453129e-06)
+    assert all(abs(a - b) < 1e6 for a, b in zip(avg, ref))
+    assert rel_err(host(dM), O.zhangli(delta, True, *args)) < TOL64
+
+
+def test_zhangli_random_vs_oracle(mg):
+    shape = (12, 9, 4)
+    nx, ny, nz = shape
+    rng = np.random.default_rng(21)
+    sh = (nz, ny, nx)
+    Ms = rng.uniform(4e5, 1.4e6, sh)
+    args = [rng.uniform(0.3, 1.0, sh), rng.uniform(0, 0.1, sh), Ms, rng.uniform(0.01, 0.5, sh),
+            rng.standard_normal((3,) + sh) * 1e12, random_unit_field(shape, 22) * Ms]
+    for prec in (True, False):
+        dM = torch.empty_like(dev(args[-1]))
+        mg.fdm_zhangli(nx, ny, nz, 2e-9, 3e-9, 4e-9, prec, *[dev(a) for a in args], dM)
+        assert rel_err(host(dM), O.zhangli((2e-9, 3e-9, 4e-9), prec, *args)) < TOL64
+
+
+# ---------------------------------------------------------------- reductions / normalise / RK
+def test_reductions_vs_oracle(mg):
+    shape = (40, 30, 7)
+    nx, ny, nz = shape
+    rng = np.random.default_rng(31)
+    M = rng.standard_normal((3, nz, ny, nx)) * 1e5
+    Ms = rng.uniform(4e5, 1.4e6, (nz, ny, nx))
+    Ms[0, 0, 0] = 0.0
+    assert abs(mg.scaled_abs_max(dev(M), dev(Ms)) - O.scaled_abs_max(M, Ms, uniform=False)) <= 1e-15
+    Mu = np.full((nz, ny, nx), 8e5)
+    assert mg.scaled_abs_max(dev(M), dev(Mu)) == O.scaled_abs_max(M, Mu, uniform=True)
+    Mk = np.zeros((3, 10, 10, 10))
+    Mk[0] = -8e5
+    assert mg.scaled_abs_max(dev(Mk), dev(np.full((10, 10, 10), 8e5))) == 1.0
+    B = rng.standard_normal(M.shape)
+    ref = float(np.sum(M * B))
+    assert abs(mg.dot_sum(dev(M), dev(B)) - ref) <= 1e-12 * np.sum(np.abs(M * B))
+    avg = mg.average(dev(M))
+    assert np.allclose(avg, M.reshape(3, -1).mean(axis=1), rtol=1e-12, atol=1e-9)
+    assert abs(mg.abs_max(dev(M)) - np.sqrt((M ** 2).sum(axis=0)).max()) <= 1e-12 * 1e6
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_normalize_vs_oracle(mg, mode):
+    shape = (33, 17, 3)
+    rng = np.random.default_rng(41 + mode)
+    M = rng.standard_normal((3, 3, 17, 33)) * 7e5
+    M[:, 0, 0, 0] = 0.0  # zero vector kept
+    Ms = rng.uniform(4e5, 1.4e6, (3, 17, 33))
+    t = dev(M)
+    mg.normalize(t, dev(Ms), mode)
+    ref = O.normalize(M, Ms, mode)
+    assert np.abs(host(t) - ref).max() <= 1e-15 * 1.4e6
+    assert np.all(host(t)[:, 0, 0, 0] == 0.0)
+    assert shape
+
+
+@pytest.mark.parametrize("method", ["rkf45", "cc45", "dp54", "rk23", "rk4"])
+def test_rk_algebra_vs_oracle(mg, method):
+    tab = mg.ButcherTableau(method)
+    ns = tab.getNumSteps()
+    n = 300
+    rng = np.random.default_rng(51)
+    ks = [rng.standard_normal((3, 1, 1, n)) for _ in range(ns)]
+    y = rng.standard_normal((3, 1, 1, n))
+    h = 1.7e-13
+    kd = [dev(k) for k in ks]
+    ot = O.tableau(method)
+    import ctypes
+    karr = (ctypes.POINTER(ctypes.c_double) * 7)()
+    kc = [np.ascontiguousarray(k.ravel()) for k in ks]
+    for q in range(ns):
+        karr[q] = kc[q].ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    t = O._Tableau()
+    O.lib().orc_tableau_get(O.METHODS[method], ctypes.byref(t))
+    yc = np.ascontiguousarray(y.ravel())
+    for step in range(1, ns):
+        yt = torch.empty_like(kd[0])
+        mg.rk_prepare_step(step, h, tab, kd[:step] + [None] * (ns - step), dev(y), yt)
+        ref = np.empty(3 * n)
+        O.lib().orc_rk_prepare_step(step, ctypes.c_double(h), ctypes.byref(t), karr, O._p(yc), O._p(ref), n)
+        assert rel_err(host(yt).ravel(), ref) < 1e-15
+    yd = dev(y)
+    ye = torch.empty_like(yd)
+    mg.rk_combine_result(h, tab, kd, yd, ye)
+    yref = yc.copy()
+    eref = np.empty(3 * n)
+    O.lib().orc_rk_combine_result(ctypes.c_double(h), ctypes.byref(t), karr, O._p(yref), O._p(eref), n)
+    assert rel_err(host(yd).ravel(), yref) < 1e-15
+    assert rel_err(host(ye).ravel(), eref) < 1e-12
+    assert ot["num_steps"] == ns
+
+
+# ---------------------------------------------------------------- demag convolution
+def _conv_case(mg, shape, delta, precision, seed=1):
+    nx, ny, nz = shape
+    N = mg.GenerateDemagTensor(nx, ny, nz, *delta)
+    M = random_unit_field(shape, seed) * 8e5
+    conv = mg.SymmetricMatrixVectorConvolution_FFT(N, nx, ny, nz, precision=precision)
+    dt = torch.float64 if precision == "fp64" else torch.float32
+    H = torch.empty_like(dev(M, dt))
+    conv.execute(dev(M, dt), H)
+    Ho = O.Conv(N, nx, ny, nz).execute(M)
+    return host(H), Ho, conv
+
+
+def test_conv_H1_stray_fixture(mg):
+    M = _omf("M1.omf")
+    Hs = _omf("H1_stray.omf")
+    nx, ny, nz = M.num_nodes
+    N = mg.GenerateDemagTensor(nx, ny, nz, *M.delta)
+    conv = mg.SymmetricMatrixVectorConvolution_FFT(N, nx, ny, nz)
+    H = torch.empty_like(dev(M.data))
+    conv.execute(dev(M.data), H)
+    Ho = O.Conv(N, nx, ny, nz).execute(M.data)
+    assert np.abs(host(H) - Hs.data).max() < 1.0  # stray_field_test.py:46-47
+    assert rel_err(host(H), Ho) < TOL64
+
+
+@pytest.mark.parametrize("shape,delta", [((24, 20, 6), (2e-9, 2e-9, 2e-9)), ((13, 11, 3), (3e-9, 2e-9, 4e-9)),
+                                         ((64, 16, 1), (4e-9, 4e-9, 3e-9)), ((5, 40, 2), (2e-9, 2e-9, 2e-9)),
+                                         ((1, 12, 7), (2e-9, 3e-9, 2e-9)), ((1, 1, 1), (2e-9, 2e-9, 2e-9))])
+def test_conv_fp64_vs_oracle(mg, shape, delta):
+    H, Ho, _ = _conv_case(mg, shape, delta, "fp64")
+    assert rel_err(H, Ho) < TOL64, rel_err(H, Ho)
+    assert max_rel_err(H, Ho) < 1e-11
+
+
+@pytest.mark.parametrize("shape", [(24, 20, 6), (64, 16, 1), (13, 11, 3)])
+def test_conv_fp32_vs_oracle(mg, shape):
+    H, Ho, _ = _conv_case(mg, shape, (2e-9, 2e-9, 2e-9), "fp32")
+    assert rel_err(H, Ho) < 1e-5, rel_err(H, Ho)  # FP32 FFT, FP32 accumulation
+
+
+def test_conv_non_smooth_padding_reembedded(mg):
+    # round4(2*13-1) = 28 = 4*7 is not radix-friendly: the library re-embeds the
+    # tensor into 30; the linear convolution is unchanged.
+    H, Ho, conv = _conv_case(mg, (13, 4, 1), (2e-9, 2e-9, 2e-9), "fp64")
+    assert conv.fft_size[0] == 30
+    assert rel_err(H, Ho) < TOL64
+
+
+# ---------------------------------------------------------------- fused stepper
+def _system(shape, with_stt=False, with_cubic=False):
+    nx, ny, nz = shape
+    kw = dict(Ms=1.1e6, alpha=0.02, A=1.5e-11, k_uni=5e5, axis1=(0.0, 0.0, 1.0), h_ext=(15915.49, 0.0, 0.0))
+    if with_cubic:
+        kw.update(k_cub=-5e3, axis2=(0.0, 1.0, 0.0), axis1=(1.0, 0.0, 0.0))
+    if with_stt:
+        kw.update(P=0.7, xi=0.05, j=(1e12, 0.0, 0.0))
+    return kw
+
+
+@pytest.mark.parametrize("shape,stt,cub", [((24, 20, 4), False, False), ((32, 32, 1), True, False),
+                                           ((16, 12, 5), True, True)])
+def test_stepper_dmdt_fp64_vs_oracle(shape, stt, cub):
+    from micromagnum_amd.solver import Stepper
+    kw = _system(shape, stt, cub)
+    delta = (2e-9, 2e-9, 2e-9)
+    st = Stepper(shape, delta, precision="fp64", **kw)
+    M = random_unit_field(shape, 5) * 1.1e6
+    st.set_m(M)
+    dM, H = st.dmdt()
+    sysO = O.System(shape, delta, N=st.N, **kw)
+    dMo, Ho = sysO.dmdt(M)
+    assert rel_err(H, Ho) < TOL64 and rel_err(dM, dMo) < TOL64
+    st.close()
+
+
+def test_stepper_dmdt_fp32_field_error():
+    from micromagnum_amd.solver import Stepper
+    shape = (48, 40, 8)
+    kw = _system(shape)
+    st = Stepper(shape, (2e-9,) * 3, precision="fp32", **kw)
+    M = random_unit_field(shape, 6) * 1.1e6
+    st.set_m(M)
+    _, H = st.dmdt()
+    Ho = O.System(shape, (2e-9,) * 3, N=st.N, **kw).h_tot(M)
+    assert rel_err(H, Ho) < 1e-5, rel_err(H, Ho)
+    st.close()
+
+
+@pytest.mark.parametrize("method", ["rkf45", "cc45"])
+def test_stepper_adaptive_trajectory_fp64_vs_oracle(method):
+    from micromagnum_amd.solver import Stepper
+    shape = (20, 16, 2)
+    kw = _system(shape)
+    st = Stepper(shape, (2e-9,) * 3, precision="fp64", method=method, eps_abs=1e-4, eps_rel=1e-4, **kw)
+    M = random_unit_field(shape, 8) * 1.1e6
+    st.set_m(M)
+    st.run(25)
+    Mg = st.get_m()
+    s = st.stats()
+    Mo, so = O.System(shape, (2e-9,) * 3, N=st.N, **kw).run(M, method=method, steps=25)
+    assert s["attempts"] == so["attempts"] and s["rejections"] == so["rejections"]
+    assert abs(s["t"] - so["t"]) <= 1e-12 * so["t"]
+    assert np.abs(Mg - Mo).max() / 1.1e6 < 1e-10
+    st.close()
+
+
+def test_stepper_rk4_thin_film_vs_oracle():
+    # the thin-film RK4 workload (128x32x1, 4x4x3 nm, Ms 8e5, A 1.3e-11, alpha 0.5)
+    # for 100 of its 1000 steps; <M> recorded every 10 steps
+    from micromagnum_amd.solver import Stepper
+    shape, delta = (128, 32, 1), (4e-9, 4e-9, 3e-9)
+    kw = dict(Ms=8e5, alpha=0.5, A=1.3e-11)
+    st = Stepper(shape, delta, precision="fp64", method="rk4", h0=1e-13, **kw)
+    M = random_unit_field(shape, 0x5EED) * 8e5
+    st.set_m(M)
+    avg = []
+    for _ in range(10):
+        st.run(10)
+        avg.append(st.average_m())
+    sysO = O.System(shape, delta, N=st.N, **kw)
+    Mo, so = sysO.run(M, method="rk4", steps=100, h0=1e-13, record_avg=True)
+    assert np.abs(st.get_m() - Mo).max() / 8e5 < 1e-10
+    assert np.abs(np.array(avg) - so["avg"][9::10]).max() / 8e5 < 1e-10
+    st.close()
+
+
+def test_stepper_fp32_tracks_fp64():
+    from micromagnum_amd.solver import Stepper
+    shape = (32, 32, 4)
+    kw = _system(shape)
+    M = random_unit_field(shape, 10) * 1.1e6
+    out = {}
+    for prec in ("fp64", "fp32"):
+        st = Stepper(shape, (2e-9,) * 3, precision=prec, method="rkf45", **kw)
+        st.set_m(M)
+        st.run(30)
+        out[prec] = (st.get_m(), st.stats())
+        st.close()
+    assert np.abs(out["fp32"][0] - out["fp64"][0]).max() / 1.1e6 < 1e-3

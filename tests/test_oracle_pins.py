@@ -1,0 +1,237 @@
+"""Pins of the CPU oracle against what the reference itself fixes:
+its OMF fixtures (src/magnum_tests/ref/*.omf), the known-answer tests of
+src/magnum_tests/*_test.py, closed forms and brute force.  CPU only."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import GOLDEN, vortex_field
+from micromagnum_amd.omf import read_omf
+
+PY_MS, PY_A = 8e5, 13e-12  # Material.Py (src/magnum/micromagnetics/world/material.py:111-126)
+
+
+# ---------------------------------------------------------------- FFT
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 7, 8, 12, 16, 30, 60, 64, 97, 100, 128, 250, 256, 400, 512])
+def test_fft_matches_numpy(n):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    ref = np.fft.fft(x)
+    assert np.abs(O.fft(x, -1) - ref).max() <= 1e-14 * max(1.0, np.abs(ref).max())
+    refi = np.fft.ifft(x) * n
+    assert np.abs(O.fft(x, +1) - refi).max() <= 1e-14 * max(1.0, np.abs(refi).max())
+
+
+@pytest.mark.parametrize("n", [3, 8, 15, 20])
+def test_fft_matches_brute_force_dft(n):
+    rng = np.random.default_rng(100 + n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    k = np.arange(n)
+    W = np.exp(-2j * np.pi * np.outer(k, k) / n)
+    assert np.abs(O.fft(x) - W @ x).max() < 1e-12
+
+
+# ---------------------------------------------------------------- fixtures
+def test_exchange_matches_H1_exch_fixture():
+    # exchange_test.py:31-50 (abs tol 2 A/m)
+    M = read_omf(os.path.join(GOLDEN, "M1.omf"))
+    Href = read_omf(os.path.join(GOLDEN, "H1_exch.omf"))
+    Ms = np.full(M.data.shape[1:], PY_MS)
+    A = np.full(M.data.shape[1:], PY_A)
+    H, _ = O.exchange(M.delta, Ms, A, M.data)
+    assert np.abs(H - Href.data).max() < 2.0
+    # the fixture is reproduced far below the test tolerance
+    assert np.abs(H - Href.data).max() < 1e-6
+
+
+def test_uniaxial_matches_H1_unianiso_fixture():
+    # anisotropy_test.py:30-47 (axis (1,1,1)/sqrt3, k=520e3, Ms=8e5, abs tol 10 A/m)
+    M = read_omf(os.path.join(GOLDEN, "M1.omf"))
+    Href = read_omf(os.path.join(GOLDEN, "H1_unianiso.omf"))
+    axis = np.full(M.data.shape, 1.0 / np.sqrt(3.0))
+    H, _ = O.uniaxial_anisotropy(axis, np.full(M.data.shape[1:], 520e3), np.full(M.data.shape[1:], 8e5), M.data)
+    assert np.abs(H - Href.data).max() < 10.0
+    assert np.abs(H - Href.data).max() < 1e-6
+
+
+# ---------------------------------------------------------------- known answers
+def test_llge_known_answer():
+    # llge_test.py:26-38
+    sh = (10, 10, 10)
+    M = np.broadcast_to(np.array([5.0, 10.0, 15.0])[:, None, None, None], (3,) + sh)
+    H = np.broadcast_to(np.array([20.0, 25.0, 30.0])[:, None, None, None], (3,) + sh)
+    dM = O.llge(np.full(sh, 10.0), np.full(sh, 20.0), M, H)
+    assert np.all(dM[0] == -60750.0) and np.all(dM[1] == -13500.0) and np.all(dM[2] == 29250.0)
+
+
+@pytest.mark.parametrize("uniform", [True, False])
+def test_scaled_abs_max_known_answer(uniform):
+    # scaled_abs_max_test.py:27-46
+    sh = (10, 10, 10)
+    M = np.zeros((3,) + sh)
+    M[0] = -8e5
+    Ms = np.full(sh, 8e5)
+    assert O.scaled_abs_max(M, Ms, uniform=uniform) == 1.0
+
+
+def test_vector_matrix_reductions_known_answers():
+    # vector_matrix_test.py:73-93
+    idx = np.arange(1000, dtype=np.float64).reshape(10, 10, 10)
+    A = np.stack([0.1 * idx, 0.2 * idx, -0.3 * idx])
+    B = np.stack([-0.4 * idx, 0.5 * idx, -0.6 * idx])
+    assert abs(O.lib().orc_dot_sum(1000, O._p(O._c(A)), O._p(O._c(B))) - 79880040.0) < 0.1
+    C = np.full((3, 10, 10, 10), 10.0)
+    C[:, 5, 0, 0] = (10.0, 20.0, 30.0)
+    assert abs(O.lib().orc_abs_max(1000, O._p(O._c(C))) - 37.41657387) < 1e-4
+    D = np.stack([idx, 2 * idx, 3 * idx])
+    out = (O.ctypes.c_double * 3)()
+    O.lib().orc_average(1000, O._p(O._c(D)), out)
+    assert tuple(out) == (999.0 / 2, 2 * 999.0 / 2, 3 * 999.0 / 2)
+
+
+def _uniform(sh, v):
+    return np.broadcast_to(np.asarray(v, dtype=np.float64)[:, None, None, None], (3,) + sh).copy()
+
+
+def test_uniaxial_closed_forms():
+    # anisotropy_test.py:52-89: parallel -> E = 0; orthogonal -> E = k V, H = 0
+    sh = (8, 8, 8)
+    V = 1e-27 * 8 ** 3
+    k = np.full(sh, 520e3)
+    Ms = np.full(sh, 8e5)
+    for ax, m in [((1, 0, 0), (8e5, 0, 0)), ((0, 1, 0), (0, -8e5, 0)), ((0, 0, -1), (0, 0, -1))]:
+        H, E = O.uniaxial_anisotropy(_uniform(sh, ax), k, Ms, _uniform(sh, m))
+        assert abs(E * 1e-27) < 1e-30
+    # (the reference's third case has |M| = 1, i.e. |m| = 1/Ms: E = k V / Ms^2)
+    for ax, m, frac in [((0, 0, 1), (8e5, 0, 0), 1.0), ((1, 0, 0), (0, -8e5, 0), 1.0),
+                        ((0, -1, 0), (0, 0, 1), 1.0 / 8e5 ** 2)]:
+        H, E = O.uniaxial_anisotropy(_uniform(sh, ax), k, Ms, _uniform(sh, m))
+        assert abs(E * 1e-27 - 520e3 * V * frac) < 1e-9 * 520e3 * V * frac
+        assert np.abs(H).max() == 0.0
+
+
+def test_cubic_closed_forms():
+    # anisotropy_test.py:93-119: m along a cubic axis -> E = 0, H = 0
+    sh = (8, 8, 8)
+    k = np.full(sh, 520e3)
+    Ms = np.full(sh, 8e5)
+    for a1, a2, m in [((1, 0, 0), (0, 1, 0), (8e5, 0, 0)), ((1, 0, 0), (0, -1, 0), (0, -8e5, 0)),
+                      ((0, -1, 0), (0, 0, -1), (0, 0, 8e5))]:
+        H, E = O.cubic_anisotropy(_uniform(sh, a1), _uniform(sh, a2), k, Ms, _uniform(sh, m))
+        assert abs(E) < 1e-9 and np.abs(H).max() < 1e-9
+
+
+@pytest.mark.parametrize("shape", [(16, 16, 1), (11, 19, 1), (12, 10, 9)])
+def test_exchange_uniform_is_zero_for_every_pbc(shape):
+    # exchange_test.py:55-77
+    nx, ny, nz = shape
+    rng = np.random.default_rng(7)
+    Ms = np.full((nz, ny, nx), PY_MS)
+    A = np.full((nz, ny, nx), PY_A)
+    for px in (0, 1):
+        for py in (0, 1):
+            for pz in (0, 1):
+                M = _uniform((nz, ny, nx), (rng.random(3) - 0.5) * 2e5)
+                H, _ = O.exchange((1e-9, 1e-9, 1e-9), Ms, A, M, (px, py, pz))
+                assert np.all(H == 0.0)
+
+
+@pytest.mark.parametrize("nz", [1, 20])
+def test_zhangli_vortex_regression_value(nz):
+    # spintorque_test.py:74-94: <dM> = (340104691300.73352, 337831145947.55255, 7.4e-06) +- 1e6
+    shape = (100, 100, nz)
+    delta = (3e-9, 3e-9, 3e-9)
+    M = vortex_field(shape, delta, (150e-9, 150e-9))
+    sh = (nz, 100, 100)
+    j = _uniform(sh, (1e9, -1e9, 0.0))
+    dM = O.zhangli(delta, True, np.full(sh, 1.0), np.full(sh, 0.02), np.full(sh, 8e5), np.full(sh, 0.01), j, M)
+    avg = dM.reshape(3, -1).mean(axis=1)
+    ref = (340104691300.73352, 337831145947.55255, 7.4024200439453129e-06)
+    assert all(abs(a - b) < 1e6 for a, b in zip(avg, ref))
+
+
+def test_zhangli_zero_gradient():
+    # spintorque_test.py:61-72
+    sh = (16, 20, 20)
+    M = np.zeros((3,) + sh)
+    j = _uniform(sh, (1e9, -1e9, 0.0))
+    dM = O.zhangli((3e-9,) * 3, True, np.full(sh, 1.0), np.full(sh, 0.02), np.full(sh, 8e5), np.full(sh, 0.01), j, M)
+    assert np.abs(dM.reshape(3, -1).mean(axis=1)).max() < 1e-3
+
+
+# ---------------------------------------------------------------- convolution
+def _mirrored_random_tensor(shape, P, seed):
+    """random entries mirrored like GenerateDemagTensor (tensor.cpp:167-180)"""
+    nx, ny, nz = shape
+    Px, Py, Pz = P
+    rng = np.random.default_rng(seed)
+    t = rng.standard_normal((nz, ny, nx, 6))
+    N = np.zeros((Pz, Py, Px, 6))
+    for o in (-1, 1):
+        for p in (-1, 1):
+            for q in (-1, 1):
+                s = np.array([1, o * p, o * q, 1, p * q, 1], dtype=np.float64)
+                for k in range(nz):
+                    for j in range(ny):
+                        for i in range(nx):
+                            N[(q * k) % Pz, (p * j) % Py, (o * i) % Px] = s * t[k, j, i]
+    return N
+
+
+@pytest.mark.parametrize("shape,P", [((5, 6, 3), (12, 12, 8)), ((7, 4, 1), (16, 8, 1)), ((1, 5, 4), (1, 12, 8)),
+                                     ((6, 1, 1), (12, 1, 1))])
+def test_fft_convolution_equals_direct_sum(shape, P):
+    N = _mirrored_random_tensor(shape, P, 3)
+    rng = np.random.default_rng(4)
+    M = rng.standard_normal((3, shape[2], shape[1], shape[0]))
+    H = O.Conv(N, *shape).execute(M)
+    Hs = O.conv_simple(N, M)
+    assert np.abs(H - Hs).max() <= 1e-13 * np.abs(Hs).max()
+
+
+# ---------------------------------------------------------------- Runge-Kutta
+@pytest.mark.parametrize("method", ["rkf45", "cc45", "dp54", "rk23", "rk4"])
+def test_tableau_order_conditions(method):
+    t = O.tableau(method)
+    a, b, c, ec = t["a"], t["b"], t["c"], t["ec"]
+    assert np.allclose(b.sum(axis=1), a, atol=1e-14)
+    assert abs(c.sum() - 1.0) < 1e-14 and abs(ec.sum()) < 1e-14
+    # order conditions of the advancing weights up to order 4
+    assert abs(c @ a - 1 / 2) < 1e-14
+    assert abs(c @ a ** 2 - 1 / 3) < 1e-14
+    assert abs(c @ (b @ a) - 1 / 6) < 1e-14
+    if t["order"] >= 4 or method == "rkf45":
+        assert abs(c @ a ** 3 - 1 / 4) < 1e-14
+        assert abs(c @ (a * (b @ a)) - 1 / 8) < 1e-14
+        assert abs(c @ (b @ a ** 2) - 1 / 12) < 1e-14
+        assert abs(c @ (b @ (b @ a)) - 1 / 24) < 1e-14
+
+
+def _single_spin(theta0, phi0, alpha, H0, method, steps, h0, eps=1e-10):
+    s = O.System((1, 1, 1), (2e-9,) * 3, Ms=8e5, alpha=alpha, h_ext=(0.0, 0.0, H0), normalize_mode=1)
+    M0 = 8e5 * np.array([np.sin(theta0) * np.cos(phi0), np.sin(theta0) * np.sin(phi0), np.cos(theta0)])
+    M, st = s.run(M0.reshape(3, 1, 1, 1), method=method, steps=steps, eps_abs=eps, eps_rel=eps, h0=h0)
+    g = 2.211e5 / (1 + alpha ** 2)
+    t = st["t"]
+    th = 2 * np.arctan(np.tan(theta0 / 2) * np.exp(-alpha * g * H0 * t))
+    ph = phi0 + g * H0 * t
+    Mref = 8e5 * np.array([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)])
+    return M.ravel(), Mref, st
+
+
+@pytest.mark.parametrize("method", ["rkf45", "cc45", "rk4"])
+def test_single_spin_matches_analytic_llg(method):
+    M, Mref, st = _single_spin(1.0, 0.3, 0.1, 1e5, method, 400, 2e-13)
+    assert np.abs(M - Mref).max() / 8e5 < 1e-7, (M, Mref, st)
+
+
+def test_rk4_converges_with_order_4():
+    errs = []
+    for h in (8e-13, 4e-13):
+        steps = int(round(2e-10 / h))
+        M, Mref, _ = _single_spin(1.0, 0.0, 0.05, 1e5, "rk4", steps, h)
+        errs.append(np.abs(M - Mref).max())
+    order = np.log2(errs[0] / errs[1])
+    assert 3.5 < order < 4.6, errs
