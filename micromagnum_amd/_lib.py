@@ -2,7 +2,8 @@
 
 The shared library is built in-tree by ``micromagnum_amd.build.build()``.
 Nothing here falls back to another implementation: a missing library or a
-missing GPU raises.
+missing GPU raises.  Device memory comes from the library itself
+(micromagnum_amd.device), so a process holds a single HIP runtime.
 """
 import ctypes
 import os
@@ -115,15 +116,13 @@ def check(rc):
 
 
 def device_lib():
-    """The library with a HIP device initialised; raises without a GPU."""
+    """The library with its HIP device initialised (MM_DEVICE, default 0);
+    raises when no device is visible.  torch is never touched here: the
+    library brings its own HIP runtime."""
     global _INIT
     L = load()
     if not _INIT:
-        import torch
-        if not torch.cuda.is_available():
-            raise RuntimeError("micromagnum_amd needs a HIP GPU (torch.cuda.is_available() is False)")
-        dev = torch.cuda.current_device()
-        torch.cuda.init()
+        dev = int(os.environ.get("MM_DEVICE", "0"))
         check(L.mm_init(dev))
         _INIT = True
     return L

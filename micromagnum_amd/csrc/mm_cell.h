@@ -40,6 +40,17 @@ template <typename T> __device__ __forceinline__ void store3(T *__restrict__ a, 
     a[2 * n + i] = v.z;
 }
 
+// |v| from the float-cast components with every product and sum rounded to
+// float and no contraction: the norm CPUDevice::normalize3 computes
+// (src/magneto/matrix/device/cpu/CPUDevice.cpp:255-259).
+__device__ __forceinline__ double float_cast_norm(double a, double b, double c)
+{
+#pragma clang fp contract(off)
+    const float e0 = (float)a, e1 = (float)b, e2 = (float)c;
+    const float s = e0 * e0 + e1 * e1 + e2 * e2;
+    return sqrt((double)s);
+}
+
 struct MeshDev {
     int nx, ny, nz;
     int px, py, pz;

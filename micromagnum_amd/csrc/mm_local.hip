@@ -158,9 +158,7 @@ __global__ void k_normalize(size_t n, T *__restrict__ M, const T *__restrict__ l
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         double norm;
         if (mode == 0) {
-            const float e0 = (float)M[i], e1 = (float)M[n + i], e2 = (float)M[2 * n + i];
-            const float s = __fadd_rn(__fadd_rn(__fmul_rn(e0, e0), __fmul_rn(e1, e1)), __fmul_rn(e2, e2));
-            norm = sqrt((double)s);
+            norm = float_cast_norm((double)M[i], (double)M[n + i], (double)M[2 * n + i]);
         } else {
             const double a = M[i], b = M[n + i], c = M[2 * n + i];
             norm = sqrt(a * a + b * b + c * c);

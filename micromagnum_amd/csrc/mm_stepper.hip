@@ -218,9 +218,7 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
     T *yn = ctl->cur ? G.yb0 : G.yb1;
     double norm;
     if (G.norm_mode == 0) {
-        const float e0 = (float)out[0], e1 = (float)out[1], e2 = (float)out[2];
-        const float s = __fadd_rn(__fadd_rn(__fmul_rn(e0, e0), __fmul_rn(e1, e1)), __fmul_rn(e2, e2));
-        norm = sqrt((double)s);
+        norm = float_cast_norm((double)out[0], (double)out[1], (double)out[2]);
     } else {
         norm = sqrt((double)out[0] * out[0] + (double)out[1] * out[1] + (double)out[2] * out[2]);
     }

@@ -1,10 +1,10 @@
 """Drop-in counterparts of the reference's ``magnum.magneto`` functions on the
 hot path (src/magneto/bindings/mmm/*.i, math.i:27-66, evolver.i:34-95).
 
-Names, argument order and meaning follow the reference; arrays are torch
-tensors on the HIP device in the reference's layout (scalar fields shape
-(nz, ny, nx), vector fields (3, nz, ny, nx), float64 unless a ``_f32`` entry
-point is used).  Errors raise RuntimeError like the reference's SWIG layer
+Names, argument order and meaning follow the reference; arrays are
+micromagnum_amd.device.DeviceArray objects in the reference's layout (scalar
+fields shape (nz, ny, nx), vector fields (3, nz, ny, nx), float64 unless a
+``_f32`` entry point is used).  Errors raise RuntimeError like the reference's SWIG layer
 (src/magneto/bindings/magneto.i:31-42).
 """
 import ctypes
@@ -23,29 +23,24 @@ __all__ = ["exchange", "uniaxial_anisotropy", "cubic_anisotropy", "llge", "fdm_z
            "PADDING_ROUND_POT", "PADDING_SMALL_PRIME_FACTORS"]
 
 
-def _torch():
-    import torch
-    return torch
+_F64 = np.dtype(np.float64)
+_F32 = np.dtype(np.float32)
 
 
 def _ptr(t, dtype=None):
-    torch = _torch()
-    if not isinstance(t, torch.Tensor) or not t.is_cuda:
-        raise TypeError("expected a torch tensor on the HIP device")
+    if not hasattr(t, "ptr") or not hasattr(t, "dtype"):
+        raise TypeError("expected a micromagnum_amd.device.DeviceArray")
     if dtype is not None and t.dtype != dtype:
         raise TypeError("expected dtype %s, got %s" % (dtype, t.dtype))
-    if not t.is_contiguous():
-        raise ValueError("expected a contiguous tensor")
-    return ctypes.c_void_p(t.data_ptr())
+    return t.ptr
 
 
 def _stream():
-    torch = _torch()
-    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    return None
 
 
 def _f64():
-    return _torch().float64
+    return _F64
 
 
 def _n(t):
@@ -63,10 +58,9 @@ def exchange(nx, ny, nz, dx, dy, dz, px, py, pz, Ms, A, M, H):
 
 
 def exchange_f32(nx, ny, nz, dx, dy, dz, px, py, pz, Ms, A, M, H):
-    torch = _torch()
     L = _lib.device_lib()
-    _lib.check(L.mm_exchange_f32(nx, ny, nz, dx, dy, dz, int(px), int(py), int(pz), _ptr(Ms, torch.float32),
-                                 _ptr(A, torch.float32), _ptr(M, torch.float32), _ptr(H, torch.float32), _stream()))
+    _lib.check(L.mm_exchange_f32(nx, ny, nz, dx, dy, dz, int(px), int(py), int(pz), _ptr(Ms, _F32),
+                                 _ptr(A, _F32), _ptr(M, _F32), _ptr(H, _F32), _stream()))
 
 
 def uniaxial_anisotropy(axis, k, Ms, M, H):
@@ -104,7 +98,8 @@ def fdm_zhangli(nx, ny, nz, dx, dy, dz, do_precess, P, xi, Ms, alpha, j, M, dM):
 
 
 def _is_uniform(t):
-    return bool((t == t.reshape(-1)[0]).all().item())
+    a = t.numpy().reshape(-1)
+    return bool(np.all(a == a[0]))
 
 
 def scaled_abs_max(M, scale, uniform=None):
@@ -190,12 +185,11 @@ class SymmetricMatrixVectorConvolution_FFT:
         return tuple(out)
 
     def execute(self, M, H):
-        torch = _torch()
         L = _lib.device_lib()
         if self.precision == _lib.MM_FP64:
-            _lib.check(L.mm_conv_execute(self._h, _ptr(M, torch.float64), _ptr(H, torch.float64), _stream()))
+            _lib.check(L.mm_conv_execute(self._h, _ptr(M, _F64), _ptr(H, _F64), _stream()))
         else:
-            _lib.check(L.mm_conv_execute_f32(self._h, _ptr(M, torch.float32), _ptr(H, torch.float32), _stream()))
+            _lib.check(L.mm_conv_execute_f32(self._h, _ptr(M, _F32), _ptr(H, _F32), _stream()))
 
     def __del__(self):
         try:
@@ -240,7 +234,7 @@ class ButcherTableau:
 def _kptrs(ks):
     arr = (ctypes.c_void_p * 8)()
     for q, k in enumerate(ks):
-        arr[q] = None if k is None else _ptr(k, _f64()).value
+        arr[q] = None if k is None else _ptr(k, _F64).value
     return arr
 
 

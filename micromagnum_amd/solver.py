@@ -20,7 +20,10 @@ def _field(a, shape, comps=1):
     if a is None:
         return None
     full = ((comps,) if comps > 1 else ()) + (nz, ny, nx)
-    return np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=np.float64), full))
+    a = np.asarray(a, dtype=np.float64)
+    if comps > 1 and a.ndim == 1:
+        a = a.reshape(comps, 1, 1, 1)
+    return np.ascontiguousarray(np.broadcast_to(a, full))
 
 
 class Stepper:

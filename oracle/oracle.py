@@ -212,6 +212,9 @@ class System:
         def field(a, comps=1):
             if a is None:
                 return None
+            a = np.asarray(a, dtype=np.float64)
+            if comps > 1 and a.ndim == 1:
+                a = a.reshape(comps, 1, 1, 1)
             a = _c(np.broadcast_to(a, ((comps,) if comps > 1 else ()) + (nz, ny, nx)))
             self._keep.append(a)
             return a

@@ -65,8 +65,10 @@ def vortex_field(shape, delta, core, polarization=1.0, chirality=1.0, core_radiu
 
 
 def gpu_available():
+    """libmmhip.so built and its HIP runtime sees a device (torch is not
+    initialised: the library brings its own HIP runtime)."""
     try:
-        import torch
-        return torch.cuda.is_available()
+        from micromagnum_amd.device import gpu_available as g
+        return g()
     except Exception:
         return False

@@ -10,25 +10,27 @@ import oracle as O
 from conftest import GOLDEN, gpu_available, rel_err, max_rel_err, random_unit_field, vortex_field
 
 pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not gpu_available(), reason="needs a HIP GPU")]
-torch = pytest.importorskip("torch")
 
 TOL64 = 1e-12
+F64, F32 = np.float64, np.float32
 
 
-def dev(a, dtype=None):
-    dtype = dtype or torch.float64
-    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(device="cuda", dtype=dtype)
+def dev(a, dtype=F64):
+    from micromagnum_amd.device import to_device
+    return to_device(np.ascontiguousarray(a, dtype=np.float64), dtype)
 
 
 def host(t):
-    torch.cuda.synchronize()
-    return t.double().cpu().numpy()
+    return t.numpy().astype(np.float64)
+
+
+def like(t):
+    from micromagnum_amd.device import empty_like
+    return empty_like(t)
 
 
 @pytest.fixture(scope="module")
 def mg():
-    if not torch.cuda.is_available():
-        pytest.skip("no GPU")
     from micromagnum_amd import magneto
     return magneto
 
@@ -45,7 +47,7 @@ def test_exchange_H1_fixture(mg):
     nx, ny, nz = M.num_nodes
     Ms = np.full(M.data.shape[1:], 8e5)
     A = np.full(M.data.shape[1:], 13e-12)
-    H = torch.empty_like(dev(M.data))
+    H = like(dev(M.data))
     E = mg.exchange(nx, ny, nz, *M.delta, False, False, False, dev(Ms), dev(A), dev(M.data), H)
     Ho, Eo = O.exchange(M.delta, Ms, A, M.data)
     assert np.abs(host(H) - Href.data).max() < 2.0
@@ -63,7 +65,7 @@ def test_exchange_random_vs_oracle(mg, shape, pbc):
     A = rng.uniform(5e-12, 2e-11, (nz, ny, nx))
     M = random_unit_field(shape, 12) * Ms
     delta = (2e-9, 3e-9, 2.5e-9)
-    H = torch.empty_like(dev(M))
+    H = like(dev(M))
     E = mg.exchange(nx, ny, nz, *delta, *pbc, dev(Ms), dev(A), dev(M), H)
     Ho, Eo = O.exchange(delta, Ms, A, M, pbc)
     assert rel_err(host(H), Ho) < TOL64
@@ -76,9 +78,9 @@ def test_exchange_f32_vs_oracle(mg):
     Ms = np.full((nz, ny, nx), 8e5)
     A = np.full((nz, ny, nx), 1.3e-11)
     M = random_unit_field(shape, 3) * 8e5
-    H = torch.empty_like(dev(M, torch.float32))
-    mg.exchange_f32(nx, ny, nz, 2e-9, 2e-9, 2e-9, 0, 0, 0, dev(Ms, torch.float32), dev(A, torch.float32),
-                    dev(M, torch.float32), H)
+    H = like(dev(M, F32))
+    mg.exchange_f32(nx, ny, nz, 2e-9, 2e-9, 2e-9, 0, 0, 0, dev(Ms, F32), dev(A, F32),
+                    dev(M, F32), H)
     Ho, _ = O.exchange((2e-9,) * 3, Ms, A, M)
     assert rel_err(host(H), Ho) < 1e-5  # FP32 bulk
 
@@ -91,7 +93,7 @@ def test_uniaxial_H1_fixture_and_random(mg):
     axis = np.full(M.data.shape, 1.0 / np.sqrt(3.0))
     k = np.full(M.data.shape[1:], 520e3)
     Ms = np.full(M.data.shape[1:], 8e5)
-    H = torch.empty_like(dev(M.data))
+    H = like(dev(M.data))
     E = mg.uniaxial_anisotropy(dev(axis), dev(k), dev(Ms), dev(M.data), H)
     Ho, Eo = O.uniaxial_anisotropy(axis, k, Ms, M.data)
     assert np.abs(host(H) - Href.data).max() < 10.0
@@ -112,7 +114,7 @@ def test_anisotropy_random_vs_oracle(mg):
     tmp = random_unit_field(shape, 8)
     a2 = np.cross(a1, tmp, axis=0)
     a2 /= np.sqrt((a2 ** 2).sum(axis=0))
-    H = torch.empty_like(dev(M))
+    H = like(dev(M))
     E = mg.uniaxial_anisotropy(dev(a1), dev(k), dev(Ms), dev(M), H)
     Ho, Eo = O.uniaxial_anisotropy(a1, k, Ms, M)
     assert rel_err(host(H), Ho) < TOL64 and abs(E - Eo) <= 1e-11 * abs(Eo)
@@ -125,7 +127,7 @@ def test_llge_known_answer_and_random(mg):
     sh = (10, 10, 10)
     M = np.broadcast_to(np.array([5.0, 10.0, 15.0])[:, None, None, None], (3,) + sh)
     H = np.broadcast_to(np.array([20.0, 25.0, 30.0])[:, None, None, None], (3,) + sh)
-    dM = torch.empty_like(dev(M))
+    dM = like(dev(M))
     mg.llge(dev(np.full(sh, 10.0)), dev(np.full(sh, 20.0)), dev(M), dev(H), dM)
     d = host(dM)
     assert np.all(d[0] == -60750.0) and np.all(d[1] == -13500.0) and np.all(d[2] == 29250.0)
@@ -144,7 +146,7 @@ def test_zhangli_vortex_known_answer(mg, nz):
     M = vortex_field(shape, delta, (150e-9, 150e-9))
     j = np.broadcast_to(np.array([1e9, -1e9, 0.0])[:, None, None, None], (3,) + sh)
     args = [np.full(sh, 1.0), np.full(sh, 0.02), np.full(sh, 8e5), np.full(sh, 0.01), j, M]
-    dM = torch.empty_like(dev(M))
+    dM = like(dev(M))
     mg.fdm_zhangli(100, 100, nz, *delta, True, *[dev(a) for a in args], dM)
     avg = host(dM).reshape(3, -1).mean(axis=1)
     ref = (340104691300.73352, 337831145947.55255, 7.4024200439453129e-06)
@@ -161,7 +163,7 @@ def test_zhangli_random_vs_oracle(mg):
     args = [rng.uniform(0.3, 1.0, sh), rng.uniform(0, 0.1, sh), Ms, rng.uniform(0.01, 0.5, sh),
             rng.standard_normal((3,) + sh) * 1e12, random_unit_field(shape, 22) * Ms]
     for prec in (True, False):
-        dM = torch.empty_like(dev(args[-1]))
+        dM = like(dev(args[-1]))
         mg.fdm_zhangli(nx, ny, nz, 2e-9, 3e-9, 4e-9, prec, *[dev(a) for a in args], dM)
         assert rel_err(host(dM), O.zhangli((2e-9, 3e-9, 4e-9), prec, *args)) < TOL64
 
@@ -223,13 +225,13 @@ def test_rk_algebra_vs_oracle(mg, method):
     O.lib().orc_tableau_get(O.METHODS[method], ctypes.byref(t))
     yc = np.ascontiguousarray(y.ravel())
     for step in range(1, ns):
-        yt = torch.empty_like(kd[0])
+        yt = like(kd[0])
         mg.rk_prepare_step(step, h, tab, kd[:step] + [None] * (ns - step), dev(y), yt)
         ref = np.empty(3 * n)
         O.lib().orc_rk_prepare_step(step, ctypes.c_double(h), ctypes.byref(t), karr, O._p(yc), O._p(ref), n)
         assert rel_err(host(yt).ravel(), ref) < 1e-15
     yd = dev(y)
-    ye = torch.empty_like(yd)
+    ye = like(yd)
     mg.rk_combine_result(h, tab, kd, yd, ye)
     yref = yc.copy()
     eref = np.empty(3 * n)
@@ -245,8 +247,8 @@ def _conv_case(mg, shape, delta, precision, seed=1):
     N = mg.GenerateDemagTensor(nx, ny, nz, *delta)
     M = random_unit_field(shape, seed) * 8e5
     conv = mg.SymmetricMatrixVectorConvolution_FFT(N, nx, ny, nz, precision=precision)
-    dt = torch.float64 if precision == "fp64" else torch.float32
-    H = torch.empty_like(dev(M, dt))
+    dt = F64 if precision == "fp64" else F32
+    H = like(dev(M, dt))
     conv.execute(dev(M, dt), H)
     Ho = O.Conv(N, nx, ny, nz).execute(M)
     return host(H), Ho, conv
@@ -258,7 +260,7 @@ def test_conv_H1_stray_fixture(mg):
     nx, ny, nz = M.num_nodes
     N = mg.GenerateDemagTensor(nx, ny, nz, *M.delta)
     conv = mg.SymmetricMatrixVectorConvolution_FFT(N, nx, ny, nz)
-    H = torch.empty_like(dev(M.data))
+    H = like(dev(M.data))
     conv.execute(dev(M.data), H)
     Ho = O.Conv(N, nx, ny, nz).execute(M.data)
     assert np.abs(host(H) - Hs.data).max() < 1.0  # stray_field_test.py:46-47
