@@ -63,6 +63,8 @@ int mm_device_free(void *ptr);
 int mm_memcpy_htod(void *dst, const void *src_host, size_t bytes, void *stream);
 int mm_memcpy_dtoh(void *dst_host, const void *src, size_t bytes, void *stream);
 int mm_synchronize(void *stream);
+/* Device-to-device streaming copy (bytes a multiple of 16). */
+int mm_device_copy(void *dst, const void *src, size_t bytes, void *stream);
 
 /* ------------------------------------------------------- demag tensor setup */
 /* round_tensor_dimension (src/magneto/mmm/demag/tensor_round.cpp:90-106) */
@@ -190,6 +192,13 @@ int mm_stepper_stats(mm_stepper_t *st, double *stats5);
 /* <M> per component (FP64 sums). */
 int mm_stepper_average_m(mm_stepper_t *st, double *avg3);
 void *mm_stepper_stream(mm_stepper_t *st);
+/* Measurement: enqueue `attempts` attempts with a HIP event after every
+ * kernel on the stepper's stream.  out[3*k+0..2] for kernel class k =
+ * 0 x forward, 1 y forward, 2 z (forward, multiply, inverse), 3 y inverse,
+ * 4 2-D y pass, 5 fused x inverse + stage epilogue, 6 step controller:
+ * average ms per launch, launches, average algorithmic HBM bytes per launch. */
+enum { MM_KCLASS_COUNT = 7 };
+int mm_stepper_kernel_profile(mm_stepper_t *st, int attempts, double *out);
 int mm_stepper_destroy(mm_stepper_t *st);
 
 #ifdef __cplusplus

@@ -57,6 +57,7 @@ SIGNATURES = {
     "mm_memcpy_htod": (_i, [_vp, _vp, ctypes.c_size_t, _vp]),
     "mm_memcpy_dtoh": (_i, [_vp, _vp, ctypes.c_size_t, _vp]),
     "mm_synchronize": (_i, [_vp]),
+    "mm_device_copy": (_i, [_vp, _vp, ctypes.c_size_t, _vp]),
     "mm_round_tensor_dimension": (_i, [_i, _i, _i, _ip]),
     "mm_generate_demag_tensor": (_i, [_i, _i, _i, _d, _d, _d, _i, _i, _i, _i, _i, _i, _dp, _ip]),
     "mm_conv_create": (_i, [_dp, _i, _i, _i, _i, _i, _i, _i, ctypes.POINTER(_vp)]),
@@ -87,6 +88,7 @@ SIGNATURES = {
     "mm_stepper_stats": (_i, [_vp, _dp]),
     "mm_stepper_average_m": (_i, [_vp, _dp]),
     "mm_stepper_stream": (_vp, [_vp]),
+    "mm_stepper_kernel_profile": (_i, [_vp, _i, _dp]),
     "mm_stepper_destroy": (_i, [_vp]),
 }
 

@@ -31,7 +31,8 @@ struct FftPlanHost {
         d.L = L;
         d.tline = tline;
         d.npass = (int)rad.size();
-        for (int i = 0; i < d.npass; ++i) d.rad[i] = rad[i];
+        d.radices = 0u;
+        for (int i = 0; i < d.npass; ++i) d.radices |= (unsigned)rad[i] << (4 * i);
         d.tw = sizeof(T) == 4 ? tw32 : tw64;
         return d;
     }
@@ -53,8 +54,43 @@ struct ConvLaunch {
     int tk_z;   // kx columns per WG, z pass (3 components each)
 };
 
+// Optional per-launch event log used by the measurement entry point: after
+// every kernel the owner records an event tagged with the kernel class.
+enum {
+    KC_XFWD = 0, KC_YFWD = 1, KC_ZMI = 2, KC_YINV = 3, KC_YMI2D = 4, KC_STAGE = 5, KC_CTRL = 6, KC_NCLASS = 7
+};
+struct KernelLog {
+    std::vector<hipEvent_t> ev;
+    std::vector<int> cls;
+    std::vector<double> bytes;
+    size_t used = 0;
+    void reserve(size_t n)
+    {
+        while (ev.size() < n) {
+            hipEvent_t e;
+            MM_HIP(hipEventCreate(&e));
+            ev.push_back(e);
+        }
+        cls.resize(n);
+        bytes.resize(n);
+    }
+    void mark(hipStream_t s, int c, double b)
+    {
+        MM_REQUIRE(used < ev.size(), "kernel log full");
+        MM_HIP(hipEventRecord(ev[used], s));
+        cls[used] = c;
+        bytes[used] = b;
+        ++used;
+    }
+    ~KernelLog()
+    {
+        for (auto e : ev) (void)hipEventDestroy(e);
+    }
+};
+
 template <typename T>
 struct ConvImpl {
+    KernelLog *klog = nullptr;
     ConvGeom g;
     ConvLaunch ln;
     FftPlanHost px, py, pz;

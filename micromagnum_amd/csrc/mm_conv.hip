@@ -27,6 +27,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 
@@ -48,8 +49,8 @@ void FftPlanHost::build(int len)
     while (n % 3 == 0) { rad.push_back(3); n /= 3; }
     if (n != 1) throw mm_error(MM_ERR_UNSUPPORTED, "FFT length " + std::to_string(len) + " has a prime factor > 5");
     MM_REQUIRE(rad.size() <= MM_FFT_MAX_PASSES, "FFT length needs too many passes");
-    const int ept = L > 2048 ? 16 : 8;
-    tline = std::max(1, (L + ept - 1) / ept);
+    tline = std::max(1, (L + MM_FFT_EPT - 1) / MM_FFT_EPT);
+    if (tline > 1024) throw mm_error(MM_ERR_UNSUPPORTED, "FFT length " + std::to_string(len) + " above 8192");
     std::vector<float2> t32(L);
     std::vector<double2> t64(L);
     for (int m = 0; m < L; ++m) {
@@ -109,9 +110,12 @@ int mm_max_lds_bytes()
     return g_max_lds;
 }
 
-template <typename K> static void set_lds(K kernel, size_t bytes)
+// Dynamic LDS above 64 KB needs no opt-in on gfx950 (the device reports its
+// per-workgroup maximum); launches are error-checked.  Kept as the single
+// place that validates a request against that maximum.
+template <typename K> static void set_lds(K, size_t bytes)
 {
-    if (bytes > 65536) MM_HIP(hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    MM_REQUIRE(bytes <= (size_t)mm_max_lds_bytes(), "LDS request exceeds the device maximum");
 }
 
 // ---------------------------------------------------------------------------
@@ -373,21 +377,31 @@ static int grid_1d(size_t n, int bs)
     return (int)std::min<size_t>(b, 65535 * 64);
 }
 
+static int env_int(const char *name, int dflt)
+{
+    const char *v = std::getenv(name);
+    return (v && *v) ? std::atoi(v) : dflt;
+}
+
+// Launch shapes: x passes take several rows per workgroup; the strided y and
+// z passes take tiles of adjacent kx columns (row segments of tile*16 or
+// tile*8 bytes).  Tiles are sized for several resident workgroups per CU so
+// that global loads of one overlap the FFT of another.  MM_LPW_X, MM_TK_Y and
+// MM_TK_Z override the choice (launch-shape sweeps).
 static ConvLaunch choose_launch(const ConvGeom &g, const FftPlanHost &px, const FftPlanHost &py,
                                 const FftPlanHost &pz, size_t csize)
 {
-    const int lds = mm_max_lds_bytes();
+    const int lds = std::min(mm_max_lds_bytes(), 65536);
     ConvLaunch ln;
-    ln.lpw_x = std::max(1, 256 / px.tline);
-    while (ln.lpw_x > 1 && (size_t)ln.lpw_x * (px.L + 1) * csize > (size_t)lds) --ln.lpw_x;
-    // y: 128-byte row segments, <= 1024 threads, fits LDS; 2-D needs 3 lines per column
+    ln.lpw_x = env_int("MM_LPW_X", std::max(1, 256 / px.tline));
+    while (ln.lpw_x > 1 && ((size_t)ln.lpw_x * (px.L + 1) * csize > (size_t)lds || ln.lpw_x * px.tline > 1024)) --ln.lpw_x;
     const int nl_y = g.is2d ? 3 : 1;
-    int tk = std::max(1, (int)(128 / csize));
+    int tk = env_int("MM_TK_Y", std::max(1, (int)(64 / csize)));
     while (tk > 1 && ((size_t)nl_y * tk * py.tline > 1024 || (size_t)nl_y * tk * (py.L + 1) * csize > (size_t)lds)) tk /= 2;
     ln.tk_y = tk;
     MM_REQUIRE((size_t)nl_y * tk * py.tline <= 1024 && (size_t)nl_y * tk * (py.L + 1) * csize <= (size_t)lds,
                "y transform length too large for one workgroup");
-    int tz = std::max(1, (int)(512 / csize));
+    int tz = env_int("MM_TK_Z", std::max(1, (int)(128 / csize)));
     while (tz > 1 && ((size_t)3 * tz * pz.tline > 1024 || (size_t)3 * tz * (pz.L + 1) * csize > (size_t)lds)) tz /= 2;
     ln.tk_z = tz;
     return ln;
@@ -492,10 +506,13 @@ void ConvImpl<T>::init(const double *N_host, int nx, int ny, int nz, int ex, int
     const size_t cs = sizeof(C2<T>);
     set_lds(k_xfwd<T, T>, (size_t)ln.lpw_x * (px.L + 1) * cs);
     set_lds(k_xinv<T, T>, (size_t)ln.lpw_x * (px.L + 1) * cs);
-    set_lds(k_yfwd<T>, (size_t)ln.tk_y * (py.L + 1) * cs);
-    set_lds(k_yinv<T>, (size_t)ln.tk_y * (py.L + 1) * cs);
-    set_lds(k_ymi2d<T>, (size_t)3 * ln.tk_y * (py.L + 1) * cs);
-    set_lds(k_zmi<T>, (size_t)3 * ln.tk_z * (pz.L + 1) * cs);
+    if (g.is2d) {
+        set_lds(k_ymi2d<T>, (size_t)3 * ln.tk_y * (py.L + 1) * cs);
+    } else {
+        set_lds(k_yfwd<T>, (size_t)ln.tk_y * (py.L + 1) * cs);
+        set_lds(k_yinv<T>, (size_t)ln.tk_y * (py.L + 1) * cs);
+        set_lds(k_zmi<T>, (size_t)3 * ln.tk_z * (pz.L + 1) * cs);
+    }
 }
 
 template <typename T> void ConvImpl<T>::release()
@@ -510,6 +527,24 @@ template <typename T> void ConvImpl<T>::release()
     pz.release();
 }
 
+// algorithmic HBM bytes per launch (each array read or written once)
+template <typename T> static double conv_bytes(const ConvGeom &g, int cls)
+{
+    const double b = sizeof(T), c = 2.0 * sizeof(T);
+    const double N = (double)g.nx * g.ny * g.nz;
+    const double s1 = 3.0 * c * g.Hx * g.ny * g.nz;  // S1 planes
+    const double s2 = 3.0 * c * g.Hx * g.Py * g.nz;  // S2 planes
+    const double K = 6.0 * b * g.Hx * g.Hy2 * g.Hz2;  // real octant kernel
+    switch (cls) {
+    case KC_XFWD: return 3.0 * b * N + s1;
+    case KC_YFWD: return s1 + s2;
+    case KC_ZMI: return 2.0 * s2 + K;
+    case KC_YINV: return s2 + s1;
+    case KC_YMI2D: return 2.0 * s1 + K;
+    default: return 0.0;
+    }
+}
+
 template <typename T> void ConvImpl<T>::forward_x(const T *M, hipStream_t s, const T *M_alt, const int *sel)
 {
     const size_t n = (size_t)g.nx * g.ny * g.nz;
@@ -519,6 +554,7 @@ template <typename T> void ConvImpl<T>::forward_x(const T *M, hipStream_t s, con
                        M, n, S1, (size_t)g.ny * g.nz * g.Hx, g.Hx, g.ny, g.nz, nrows, g.nx, px.dev<T>(), ln.lpw_x,
                        M_alt, sel);
     MM_HIP(hipGetLastError());
+    if (klog) klog->mark(s, KC_XFWD, conv_bytes<T>(g, KC_XFWD));
 }
 
 template <typename T> void ConvImpl<T>::spectral(hipStream_t s)
@@ -531,6 +567,7 @@ template <typename T> void ConvImpl<T>::spectral(hipStream_t s)
         hipLaunchKernelGGL((k_ymi2d<T>), dim3((g.Hx + tk - 1) / tk), dim3(3 * tk * py.tline), sh, s, S1, kern, plane,
                            g.Hx, g.Hy2, g.ny, py.dev<T>(), tk);
         MM_HIP(hipGetLastError());
+        if (klog) klog->mark(s, KC_YMI2D, conv_bytes<T>(g, KC_YMI2D));
         return;
     }
     const int tk = ln.tk_y;
@@ -538,14 +575,17 @@ template <typename T> void ConvImpl<T>::spectral(hipStream_t s)
     hipLaunchKernelGGL((k_yfwd<T>), dim3((g.Hx + tk - 1) / tk, g.nz, 3), dim3(tk * py.tline), shy, s, S1, S2, g.Hx,
                        g.ny, g.nz, py.dev<T>(), tk);
     MM_HIP(hipGetLastError());
+    if (klog) klog->mark(s, KC_YFWD, conv_bytes<T>(g, KC_YFWD));
     const int tz = ln.tk_z;
     const size_t shz = (size_t)3 * tz * (pz.L + 1) * cs;
     hipLaunchKernelGGL((k_zmi<T>), dim3((g.Hx + tz - 1) / tz, g.Py), dim3(3 * tz * pz.tline), shz, s, S2, kern, plane,
                        g.Hx, g.Hy2, g.Py, g.nz, pz.dev<T>(), tz);
     MM_HIP(hipGetLastError());
+    if (klog) klog->mark(s, KC_ZMI, conv_bytes<T>(g, KC_ZMI));
     hipLaunchKernelGGL((k_yinv<T>), dim3((g.Hx + tk - 1) / tk, g.nz, 3), dim3(tk * py.tline), shy, s, S2, S1, g.Hx,
                        g.ny, g.nz, py.dev<T>(), tk);
     MM_HIP(hipGetLastError());
+    if (klog) klog->mark(s, KC_YINV, conv_bytes<T>(g, KC_YINV));
 }
 
 template <typename T> void ConvImpl<T>::inverse_x(T *H, hipStream_t s)

@@ -13,7 +13,7 @@
 
 #include <hip/hip_runtime.h>
 
-#define MM_FFT_MAX_PASSES 16
+#define MM_FFT_MAX_PASSES 8
 
 template <typename T> struct cvec;
 template <> struct cvec<float> { typedef float2 type; };
@@ -41,12 +41,13 @@ template <typename T, bool INV> __device__ __forceinline__ C2<T> cmul_mi(C2<T> a
     return INV ? cmk<T>(-a.y, a.x) : cmk<T>(a.y, -a.x);
 }
 
-// device-side plan of one transform length (passed by value)
+// device-side plan of one transform length (passed by value).  The radix of
+// pass p is (radices >> 4p) & 15, so no array is indexed at run time.
 struct FftPlanDev {
     int L;
     int tline;             // threads per line
     int npass;
-    int rad[MM_FFT_MAX_PASSES];
+    unsigned radices;
     const void *tw;        // L entries, tw[m] = exp(-2 pi i m / L), precision T
 };
 
@@ -135,15 +136,16 @@ template <int R, typename T, bool INV> __device__ __forceinline__ void dftR(C2<T
     else dft8<T, INV>(v);
 }
 
-// maximum butterflies per thread and pass for radix R (16 elements / R)
-template <int R> struct MaxB { static const int value = (16 + R - 1) / R; };
+// Maximum butterflies per thread and pass for radix R when a line of length
+// L is spread over tline = ceil(L/EPT) threads: (L/R)/tline <= EPT/R.
+template <int R, int EPT> struct MaxB { static const int value = (EPT + R - 1) / R; };
 
 // One Stockham pass over every line of the workgroup.  `active` is false for
 // threads with no line; they still take part in both barriers.
-template <int R, typename T, bool INV>
+template <int R, int EPT, typename T, bool INV>
 __device__ __forceinline__ void fft_pass(C2<T> *line, const FftPlanDev &p, int t, bool active, int Ns)
 {
-    constexpr int MB = MaxB<R>::value;
+    constexpr int MB = MaxB<R, EPT>::value;
     const int L = p.L;
     const int nb = L / R;
     C2<T> v[MB][R];
@@ -181,22 +183,32 @@ __device__ __forceinline__ void fft_pass(C2<T> *line, const FftPlanDev &p, int t
 }
 
 // Full transform of one line per thread group.  All threads of the workgroup
-// must call this with the same plan (barriers inside).
-template <typename T, bool INV>
-__device__ __forceinline__ void lds_fft(C2<T> *line, const FftPlanDev &p, int t, bool active)
+// must call this with the same plan (barriers inside).  EPT is the number of
+// elements per thread the plan was built for.
+template <typename T, bool INV, int EPT>
+__device__ __forceinline__ void lds_fft_ept(C2<T> *line, const FftPlanDev &p, int t, bool active)
 {
     int Ns = 1;
-    for (int ps = 0; ps < p.npass; ++ps) {
-        const int R = p.rad[ps];
+    unsigned rs = p.radices;
+    for (int ps = 0; ps < p.npass; ++ps, rs >>= 4) {
+        const int R = (int)(rs & 15u);
         switch (R) {
-        case 8: fft_pass<8, T, INV>(line, p, t, active, Ns); break;
-        case 5: fft_pass<5, T, INV>(line, p, t, active, Ns); break;
-        case 4: fft_pass<4, T, INV>(line, p, t, active, Ns); break;
-        case 3: fft_pass<3, T, INV>(line, p, t, active, Ns); break;
-        default: fft_pass<2, T, INV>(line, p, t, active, Ns); break;
+        case 8: fft_pass<8, EPT, T, INV>(line, p, t, active, Ns); break;
+        case 5: fft_pass<5, EPT, T, INV>(line, p, t, active, Ns); break;
+        case 4: fft_pass<4, EPT, T, INV>(line, p, t, active, Ns); break;
+        case 3: fft_pass<3, EPT, T, INV>(line, p, t, active, Ns); break;
+        default: fft_pass<2, EPT, T, INV>(line, p, t, active, Ns); break;
         }
         Ns *= R;
     }
+}
+
+#define MM_FFT_EPT 8
+
+template <typename T, bool INV>
+__device__ __forceinline__ void lds_fft(C2<T> *line, const FftPlanDev &p, int t, bool active)
+{
+    lds_fft_ept<T, INV, MM_FFT_EPT>(line, p, t, active);
 }
 
 #endif

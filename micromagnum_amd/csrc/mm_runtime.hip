@@ -2,6 +2,7 @@
 // selection, memory helpers, demag-tensor setup entry points, tableaus.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -56,6 +57,26 @@ extern "C" int mm_memcpy_dtoh(void *dst_host, const void *src, size_t bytes, voi
     MM_CAPI_BEGIN
     MM_HIP(hipMemcpyAsync(dst_host, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream));
     MM_HIP(hipStreamSynchronize((hipStream_t)stream));
+    MM_CAPI_END
+}
+
+// plain streaming copy (16-byte vectors); also the reference kernel for
+// converting hardware byte counters into bytes of a known transfer
+__global__ void k_copy_calib(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n16)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+extern "C" int mm_device_copy(void *dst, const void *src, size_t bytes, void *stream)
+{
+    MM_CAPI_BEGIN
+    MM_REQUIRE(dst && src && bytes % 16 == 0, "copy needs 16-byte multiples");
+    const size_t n16 = bytes / 16;
+    const int grid = (int)std::min<size_t>((n16 + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_copy_calib, dim3(grid > 0 ? grid : 1), dim3(256), 0, (hipStream_t)stream,
+                       (const uint4 *)src, (uint4 *)dst, n16);
+    MM_HIP(hipGetLastError());
     MM_CAPI_END
 }
 

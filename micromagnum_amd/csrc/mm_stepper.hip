@@ -386,6 +386,7 @@ struct StepperBase {
     virtual void stats(double *s5) = 0;
     virtual void average(double *a3) = 0;
     virtual hipStream_t stream() = 0;
+    virtual void profile(int attempts, double *out) = 0;
 };
 
 static bool all_equal(const double *a, size_t n)
@@ -562,14 +563,13 @@ template <typename T> struct Stepper : public StepperBase {
             conv = new ConvImpl<T>();
             conv->init(d.N_host, d.nx, d.ny, d.nz, d.ex, d.ey, d.ez);
             const int tl = conv->px.tline;
-            rpw = std::max(1, 512 / (3 * tl));
+            rpw = std::max(1, 384 / (3 * tl));
+            if (const char *e = std::getenv("MM_RPW")) rpw = std::max(1, std::atoi(e));
             while (rpw > 1 && (size_t)3 * rpw * (conv->px.L + 1) * sizeof(C2<T>) > (size_t)mm_max_lds_bytes()) --rpw;
             MM_REQUIRE(3 * tl <= 1024, "x transform too long for the fused stage kernel");
             threads = 3 * rpw * tl;
             stage_lds = (size_t)3 * rpw * (conv->px.L + 1) * sizeof(C2<T>);
-            if (stage_lds > 65536)
-                MM_HIP(hipFuncSetAttribute((const void *)k_stage<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)stage_lds));
+            MM_REQUIRE(stage_lds <= (size_t)mm_max_lds_bytes(), "fused stage kernel exceeds the LDS maximum");
         } else {
             rpw = std::max(1, 256 / d.nx);
             threads = 256;
@@ -613,10 +613,28 @@ template <typename T> struct Stepper : public StepperBase {
 
     hipStream_t stream() override { return s; }
 
+    KernelLog *klog = nullptr;
+
+    // algorithmic bytes of one fused stage launch: demag rows (S1) plus every
+    // state array the epilogue reads or writes once (parameters excluded)
+    double stage_bytes(const StageDev<T> &G) const
+    {
+        const double V = 3.0 * sizeof(T) * (double)n;
+        double b = conv ? 3.0 * 2.0 * sizeof(T) * conv->g.Hx * d.ny * d.nz : 0.0;
+        b += V;                                    // stage input
+        if (G.prep && G.Ys) b += V;                // y (same array as the input on stage 0)
+        b += V * G.nk;                             // stored stage vectors
+        if (G.kout) b += V;
+        if (G.Hout) b += V;
+        if (G.prep) b += V;                        // next stage input or new state
+        return b;
+    }
+
     // one stage: dM/dt at stage input Ys (null = y[cur])
     void launch_stage(const T *Ys, const StageDev<T> &G)
     {
         if (conv) {
+            conv->klog = klog;
             if (Ys) conv->forward_x(Ys, s);
             else conv->forward_x(yb[0], s, yb[1], &ctl->cur);
             conv->spectral(s);
@@ -625,6 +643,7 @@ template <typename T> struct Stepper : public StepperBase {
                            conv ? conv->S1 : nullptr, conv ? conv->px.template dev<T>() : FftPlanDev(),
                            conv ? conv->g.Hx : 0, rpw, conv ? 1 : 0);
         MM_HIP(hipGetLastError());
+        if (klog) klog->mark(s, KC_STAGE, stage_bytes(G));
     }
 
     StageDev<T> blank_stage(const T *Ys)
@@ -688,6 +707,52 @@ template <typename T> struct Stepper : public StepperBase {
         hipLaunchKernelGGL(k_controller, dim3(1), dim3(1), 0, s, ctl, tab.order, d.eps_abs, d.eps_rel,
                            sys.Ms_uniform, sys.Ms_u, adaptive ? 1 : 0);
         MM_HIP(hipGetLastError());
+        if (klog) klog->mark(s, KC_CTRL, 0.0);
+    }
+
+    // Enqueue `attempts` attempts with an event after every kernel; per
+    // kernel class: [avg ms per launch, launches, avg algorithmic bytes].
+    void profile(int attempts, double *out) override
+    {
+        KernelLog log;
+        const int per = tab.num_steps * 5 + 2;
+        log.reserve((size_t)attempts * per + 1);
+        StepCtl c;
+        MM_HIP(hipStreamSynchronize(s));
+        MM_HIP(hipMemcpy(&c, ctl, sizeof(c), hipMemcpyDeviceToHost));
+        c.target = c.accepted + (1ll << 40);
+        c.stop = 0;
+        MM_HIP(hipMemcpy(ctl, &c, sizeof(c), hipMemcpyHostToDevice));
+        klog = &log;
+        log.mark(s, -1, 0.0);
+        try {
+            for (int q = 0; q < attempts; ++q) enqueue_attempt();
+        } catch (...) {
+            klog = nullptr;
+            if (conv) conv->klog = nullptr;
+            throw;
+        }
+        klog = nullptr;
+        if (conv) conv->klog = nullptr;
+        MM_HIP(hipStreamSynchronize(s));
+        for (int k = 0; k < 3 * KC_NCLASS; ++k) out[k] = 0.0;
+        for (size_t i = 1; i < log.used; ++i) {
+            float ms = 0.f;
+            MM_HIP(hipEventElapsedTime(&ms, log.ev[i - 1], log.ev[i]));
+            const int k = log.cls[i];
+            out[3 * k + 0] += ms;
+            out[3 * k + 1] += 1.0;
+            out[3 * k + 2] += log.bytes[i];
+        }
+        for (int k = 0; k < KC_NCLASS; ++k)
+            if (out[3 * k + 1] > 0) {
+                out[3 * k + 0] /= out[3 * k + 1];
+                out[3 * k + 2] /= out[3 * k + 1];
+            }
+        // leave the control block as run() expects it
+        MM_HIP(hipMemcpy(&c, ctl, sizeof(c), hipMemcpyDeviceToHost));
+        c.target = c.accepted;
+        MM_HIP(hipMemcpy(ctl, &c, sizeof(c), hipMemcpyHostToDevice));
     }
 
     void set_m(const double *M_host) override
@@ -850,6 +915,14 @@ extern "C" int mm_stepper_average_m(mm_stepper_t *st, double *avg3)
 }
 
 extern "C" void *mm_stepper_stream(mm_stepper_t *st) { return st ? (void *)st->impl->stream() : nullptr; }
+
+extern "C" int mm_stepper_kernel_profile(mm_stepper_t *st, int attempts, double *out)
+{
+    MM_CAPI_BEGIN
+    MM_REQUIRE(st && out && attempts > 0, "bad argument");
+    st->impl->profile(attempts, out);
+    MM_CAPI_END
+}
 
 extern "C" int mm_stepper_destroy(mm_stepper_t *st)
 {

@@ -117,6 +117,19 @@ class Stepper:
         _lib.check(_lib.load().mm_stepper_average_m(self._h, self._dp(a)))
         return a
 
+    KERNEL_CLASSES = ("x_fwd", "y_fwd", "z_mult", "y_inv", "y_mult_2d", "x_inv_stage", "controller")
+
+    def kernel_profile(self, attempts=5):
+        """Per kernel class: average ms per launch, launches, algorithmic bytes
+        per launch, from HIP events recorded between the kernels."""
+        out = np.zeros(3 * len(self.KERNEL_CLASSES))
+        _lib.check(_lib.load().mm_stepper_kernel_profile(self._h, int(attempts), self._dp(out)))
+        res = {}
+        for k, name in enumerate(self.KERNEL_CLASSES):
+            if out[3 * k + 1] > 0:
+                res[name] = dict(ms=out[3 * k], launches=int(out[3 * k + 1]), bytes=out[3 * k + 2])
+        return res
+
     @property
     def stream(self):
         return _lib.load().mm_stepper_stream(self._h)

@@ -41,6 +41,16 @@ static const double ORC_GAMMA = 2.211e5; /* src/magnum/micromagnetics/constants.
 
 double orc_mu0(void) { return ORC_MU0; }
 
+/* number of OpenMP threads used by the parallel loops (results do not depend on it) */
+void orc_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
 /* ------------------------------------------------------------------------ */
 /* 1-D complex DFT: mixed-radix Cooley-Tukey (decimation in time), radices  */
 /* 4, 2, 3, 5 and a direct O(p^2) DFT for any other prime factor.  Stands   */

@@ -22,6 +22,7 @@ extern "C" {
 
 /* ---- constants (src/magneto/mmm/constants.h:26-40) ---- */
 double orc_mu0(void);
+void orc_set_threads(int n);
 
 /* ---- 1-D complex DFT, unnormalised; sign -1 forward, +1 inverse ---- */
 void orc_fft(double *re_im /* interleaved, 2*n */, int n, int sign);

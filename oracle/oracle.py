@@ -64,6 +64,10 @@ def mu0():
     return lib().orc_mu0()
 
 
+def set_threads(n):
+    lib().orc_set_threads(int(n))
+
+
 def fft(x, sign=-1):
     """Unnormalised complex DFT of a 1-D complex array (sign -1 forward)."""
     x = np.ascontiguousarray(x, dtype=np.complex128).copy()
