@@ -1,0 +1,180 @@
+"""Pins of the demag tensor generator (micromagnum_amd/csrc/demag_tensor.cpp):
+closed forms, the reference's stray-field fixture, and high-precision
+(mpmath, 40 digits) evaluation of Newell's exact averaged-cell integrals.
+CPU only: the generator is host code of libmmhip.so."""
+import os
+
+import mpmath as mp
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import GOLDEN
+from micromagnum_amd import magneto as mg
+from micromagnum_amd.omf import read_omf
+
+mp.mp.dps = 40
+
+
+# Newell's f (Nzz form) and g (Nyz form), each summand guarded at its limit
+def _f002(x, y, z):
+    R = mp.sqrt(x * x + y * y + z * z)
+    s = -R * (x * x + y * y - 2 * z * z) / 6
+    if x != 0 and y != 0 and z != 0:
+        s -= x * y * z * mp.atan(x * y / (R * z))
+    if x != 0 and (y != 0 or z != 0):
+        s += x * (y * y - z * z) / 2 * mp.atanh(x / R)
+    if y != 0 and (x != 0 or z != 0):
+        s += y * (x * x - z * z) / 2 * mp.atanh(y / R)
+    return s
+
+
+def _f011(x, y, z):
+    R = mp.sqrt(x * x + y * y + z * z)
+    s = -2 * R * y * z
+    if x != 0 and z != 0:
+        s -= 3 * x * z * z * mp.atan(x * y / (R * z))
+    if x != 0 and y != 0:
+        s -= 3 * x * y * y * mp.atan(x * z / (R * y))
+    if x != 0:
+        s -= x ** 3 * mp.atan(y * z / (R * x))
+    if x != 0 and y != 0 and z != 0:
+        s += 6 * x * y * z * mp.atanh(x / R)
+    if z != 0:
+        s += (3 * x * x * z - z ** 3) * mp.atanh(y / R)
+    if y != 0:
+        s -= y * (y * y - 3 * x * x) * mp.atanh(z / R)
+    return s / 6
+
+
+def exact_entry(comp, i, j, k, lx, ly, lz):
+    """stored tensor entry (sign as returned by GenerateDemagTensor, H = N M)
+    at cell offset (i,j,k) > 0, by the 27-point second difference in mpmath."""
+    lx, ly, lz = mp.mpf(lx), mp.mpf(ly), mp.mpf(lz)
+    w = {-1: 1, 0: -2, 1: 1}
+    tot = mp.mpf(0)
+    for a in (-1, 0, 1):
+        for b in (-1, 0, 1):
+            for c in (-1, 0, 1):
+                x, y, z = (i + a) * lx, (j + b) * ly, (k + c) * lz
+                if comp == "zz":
+                    v = _f002(x, y, z)
+                elif comp == "xx":
+                    v = _f002(z, y, x)
+                elif comp == "yy":
+                    v = _f002(x, z, y)
+                elif comp == "yz":
+                    v = _f011(x, y, z)
+                elif comp == "xy":
+                    v = _f011(z, x, y)
+                else:  # xz
+                    v = _f011(y, x, z)
+                tot += w[a] * w[b] * w[c] * v
+    return float(tot / (4 * mp.pi * lx * ly * lz))
+
+
+IDX = {"xx": 0, "xy": 1, "xz": 2, "yy": 3, "yz": 4, "zz": 5}
+
+
+def test_cube_self_term_is_minus_one_third():
+    N = mg.GenerateDemagTensor(1, 1, 1, 1e-9, 1e-9, 1e-9)
+    assert N.shape == (1, 1, 1, 6)
+    assert np.allclose(N[0, 0, 0, [0, 3, 5]], -1.0 / 3.0, rtol=0, atol=1e-15)
+    assert np.all(N[0, 0, 0, [1, 2, 4]] == 0.0)
+
+
+@pytest.mark.parametrize("cell", [(2e-9, 3e-9, 5e-9), (4e-9, 4e-9, 3e-9), (1e-9, 7e-9, 2e-9)])
+def test_self_term_trace_is_minus_one(cell):
+    N = mg.GenerateDemagTensor(1, 1, 1, *cell)
+    assert abs(N[0, 0, 0, [0, 3, 5]].sum() + 1.0) < 1e-14
+
+
+def test_padding_policies():
+    # tensor_round.cpp:27-106
+    assert mg.round_tensor_dimension(1, False, mg.PADDING_ROUND_4) == 1
+    assert mg.round_tensor_dimension(256, False, mg.PADDING_ROUND_4) == 512
+    assert mg.round_tensor_dimension(200, False, mg.PADDING_ROUND_4) == 400
+    assert mg.round_tensor_dimension(13, False, mg.PADDING_ROUND_4) == 28
+    assert mg.round_tensor_dimension(13, False, mg.PADDING_DISABLE) == 25
+    assert mg.round_tensor_dimension(13, False, mg.PADDING_ROUND_8) == 32
+    assert mg.round_tensor_dimension(13, False, mg.PADDING_ROUND_POT) == 32
+    assert mg.round_tensor_dimension(13, False, mg.PADDING_SMALL_PRIME_FACTORS) == 27
+    assert mg.round_tensor_dimension(13, True, mg.PADDING_ROUND_4) == 13
+
+
+def test_H1_stray_fixture_through_oracle_convolution():
+    # stray_field_test.py:29-47 (abs tol 1 A/m); the fixture comes from OOMMF
+    M = read_omf(os.path.join(GOLDEN, "M1.omf"))
+    Hs = read_omf(os.path.join(GOLDEN, "H1_stray.omf"))
+    nx, ny, nz = M.num_nodes
+    N = mg.GenerateDemagTensor(nx, ny, nz, *M.delta)
+    H = O.Conv(N, nx, ny, nz).execute(M.data)
+    assert np.abs(H - Hs.data).max() < 1.0
+
+
+@pytest.fixture(scope="module")
+def tensor_block():
+    # 34 x 12 x 6 cells of 2 x 3 x 2.5 nm: near field, series and large-x cones
+    return mg.GenerateDemagTensor(34, 12, 6, 2e-9, 3e-9, 2.5e-9), (2e-9, 3e-9, 2.5e-9)
+
+
+NEAR = [(1, 1, 1), (2, 3, 1), (5, 2, 4), (3, 7, 2), (9, 5, 3), (4, 11, 5)]
+
+
+@pytest.mark.parametrize("ijk", NEAR)
+@pytest.mark.parametrize("comp", ["xx", "yy", "zz", "xy", "xz", "yz"])
+def test_near_field_matches_exact_newell(tensor_block, ijk, comp):
+    N, cell = tensor_block
+    i, j, k = ijk
+    ref = exact_entry(comp, i, j, k, *cell)
+    got = N[k, j, i, IDX[comp]]
+    # long double cancellation of the 27-point difference grows like r^6:
+    # ~2e-12 relative at 18 cells, the same for the reference's generator
+    assert abs(got - ref) <= 2e-11 * abs(ref) + 1e-18, (got, ref)
+
+
+@pytest.mark.parametrize("ijk,tol", [((20, 0, 0), 1e-6), ((33, 1, 0), 1e-6), ((14, 1, 1), 1e-5)])
+def test_large_distance_branches_match_exact_within_truncation(tensor_block, ijk, tol):
+    # points reached by the asymptotic large-x sum; the exact integrals in
+    # 40-digit arithmetic bound its truncation error, a wrong series term
+    # would show at the (l/x)^2 ~ 1e-2..1e-3 level
+    N, cell = tensor_block
+    i, j, k = ijk
+    for comp in ("yy", "zz", "yz"):
+        ref = exact_entry(comp, i, j, k, *cell)
+        got = N[k, j, i, IDX[comp]]
+        if abs(ref) < 1e-12:
+            continue
+        assert abs(got - ref) <= tol * abs(ref), (comp, got, ref)
+
+
+def test_far_field_series_matches_exact():
+    # 70 x 70 x 1 of cubic 1 nm cells: (40, 30, 0) is beyond the expansion
+    # radius (Taylor series of the second differences)
+    N = mg.GenerateDemagTensor(70, 70, 1, 1e-9, 1e-9, 1e-9)
+    for (i, j) in [(40, 30), (55, 45), (35, 34)]:
+        for comp in ("xx", "yy", "zz", "xy"):
+            ref = exact_entry(comp, i, j, 0, 1e-9, 1e-9, 1e-9)
+            got = N[0, j, i, IDX[comp]]
+            assert abs(got - ref) <= 1e-6 * abs(ref), (i, j, comp, got, ref)
+
+
+def test_point_dipole_limit():
+    # far along x: N_xx -> 2V/(4 pi r^3), N_yy = N_zz -> -V/(4 pi r^3)
+    N = mg.GenerateDemagTensor(201, 1, 1, 1e-9, 1e-9, 1e-9)
+    r = 200e-9
+    V = 1e-27
+    assert abs(N[0, 0, 200, 0] / (2 * V / (4 * np.pi * r ** 3)) - 1) < 1e-4
+    assert abs(N[0, 0, 200, 3] / (-V / (4 * np.pi * r ** 3)) - 1) < 1e-4
+
+
+def test_octant_symmetry(tensor_block):
+    N, _ = tensor_block
+    Pz, Py, Px, _ = N.shape
+    i, j, k = 5, 3, 2
+    e = N[k, j, i]
+    m = N[(-k) % Pz, (-j) % Py, (-i) % Px]
+    # xx,yy,zz even; xy odd in x,y; xz odd in x,z; yz odd in y,z -> all flips cancel pairwise
+    assert np.allclose(m, e * np.array([1, 1, 1, 1, 1, 1]))
+    mx = N[k, j, (-i) % Px]
+    assert np.allclose(mx, e * np.array([1, -1, -1, 1, 1, 1]))
