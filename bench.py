@@ -95,6 +95,20 @@ def cpu_baseline(N, M0, steps, threads):
     return dict(seconds=dt, steps=steps, attempts=st["attempts"])
 
 
+def calibration_copies(nbytes=256 << 20, reps=5):
+    """k_copy_calib over nbytes, reps times: reads and writes exactly nbytes each,
+    the known transfer against which FETCH_SIZE / WRITE_SIZE are scaled."""
+    import ctypes
+    from micromagnum_amd import _lib
+    from micromagnum_amd.device import empty
+    a = empty((nbytes // 8,))
+    b = empty((nbytes // 8,))
+    L = _lib.device_lib()
+    for _ in range(reps):
+        _lib.check(L.mm_device_copy(b.ptr, a.ptr, ctypes.c_size_t(nbytes), None))
+    _lib.check(L.mm_synchronize(None))
+
+
 def load_traffic(kernel_class, precision):
     """HBM bytes per launch from the committed rocprofv3 counter summary, if any."""
     path = os.path.join(ROOT, "profiles", "traffic_%s_block256x256x16_%s.json" % (kernel_class, precision))
@@ -113,6 +127,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: all host cores (capped at 16)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--calib", action="store_true",
+                    help="also run 5 copies of 256 MiB (k_copy_calib) for hardware-counter calibration")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -126,6 +142,9 @@ def main():
     if not gpu_available():
         print("bench.py needs a HIP GPU and the built libmmhip.so", file=sys.stderr)
         sys.exit(1)
+
+    if args.calib:
+        calibration_copies()
 
     N = GenerateDemagTensor(*SHAPE, *DELTA)
     P = (N.shape[2], N.shape[1], N.shape[0])

@@ -137,19 +137,19 @@ __global__ void k_xfwd(const TIN *__restrict__ in, size_t in_cstride, C2<T> *__r
     const int l = threadIdx.x / p.tline, t = threadIdx.x % p.tline;
     const int g = blockIdx.x * lpw + l;
     const bool active = g < nrows;
-    C2<T> *line = sm + (size_t)l * (L + 1);
+    C2<T> *line = sm + (size_t)l * lds_len(L);
     int c = 0, r = 0;
     if (active) {
         c = g / (ny * nz);
         r = g % (ny * nz);
         const TIN *src = in + c * in_cstride + (size_t)r * nx_in;
-        for (int x = t; x < L; x += p.tline) line[x] = cmk<T>(x < nx_in ? (T)src[x] : (T)0, (T)0);
+        for (int x = t; x < L; x += p.tline) line[sk(x)] = cmk<T>(x < nx_in ? (T)src[x] : (T)0, (T)0);
     }
     __syncthreads();
     lds_fft<T, false>(line, p, t, active);
     if (active) {
         C2<T> *dst = out + c * out_cstride + (size_t)r * Hx;
-        for (int kx = t; kx < Hx; kx += p.tline) dst[kx] = line[kx];
+        for (int kx = t; kx < Hx; kx += p.tline) dst[kx] = line[sk(kx)];
     }
 }
 
@@ -160,21 +160,21 @@ __global__ void k_yfwd(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, in
 {
     extern __shared__ char smem_raw[];
     C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
-    const int L = p.L, LP = L + 1;
+    const int L = p.L, LP = lds_len(L);
     const int kx0 = blockIdx.x * tk;
     const int z = blockIdx.y, c = blockIdx.z;
     const C2<T> *src = in + ((size_t)c * nzl + z) * (size_t)ny_in * Hx;
     C2<T> *dst = out + ((size_t)c * nzl + z) * (size_t)L * Hx;
     for (int e = threadIdx.x; e < tk * L; e += blockDim.x) {
         const int col = e % tk, y = e / tk, kx = kx0 + col;
-        sm[col * LP + y] = (y < ny_in && kx < Hx) ? src[(size_t)y * Hx + kx] : cmk<T>((T)0, (T)0);
+        sm[col * LP + sk(y)] = (y < ny_in && kx < Hx) ? src[(size_t)y * Hx + kx] : cmk<T>((T)0, (T)0);
     }
     __syncthreads();
     const int l = threadIdx.x / p.tline, t = threadIdx.x % p.tline;
     lds_fft<T, false>(sm + (size_t)l * LP, p, t, l < tk);
     for (int e = threadIdx.x; e < tk * L; e += blockDim.x) {
         const int col = e % tk, y = e / tk, kx = kx0 + col;
-        if (kx < Hx) dst[(size_t)y * Hx + kx] = sm[col * LP + y];
+        if (kx < Hx) dst[(size_t)y * Hx + kx] = sm[col * LP + sk(y)];
     }
 }
 
@@ -185,21 +185,21 @@ __global__ void k_yinv(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, in
 {
     extern __shared__ char smem_raw[];
     C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
-    const int L = p.L, LP = L + 1;
+    const int L = p.L, LP = lds_len(L);
     const int kx0 = blockIdx.x * tk;
     const int z = blockIdx.y, c = blockIdx.z;
     const C2<T> *src = in + ((size_t)c * nzl + z) * (size_t)L * Hx;
     C2<T> *dst = out + ((size_t)c * nzl + z) * (size_t)ny_out * Hx;
     for (int e = threadIdx.x; e < tk * L; e += blockDim.x) {
         const int col = e % tk, y = e / tk, kx = kx0 + col;
-        sm[col * LP + y] = (kx < Hx) ? src[(size_t)y * Hx + kx] : cmk<T>((T)0, (T)0);
+        sm[col * LP + sk(y)] = (kx < Hx) ? src[(size_t)y * Hx + kx] : cmk<T>((T)0, (T)0);
     }
     __syncthreads();
     const int l = threadIdx.x / p.tline, t = threadIdx.x % p.tline;
     lds_fft<T, true>(sm + (size_t)l * LP, p, t, l < tk);
     for (int e = threadIdx.x; e < tk * ny_out; e += blockDim.x) {
         const int col = e % tk, y = e / tk, kx = kx0 + col;
-        if (kx < Hx) dst[(size_t)y * Hx + kx] = sm[col * LP + y];
+        if (kx < Hx) dst[(size_t)y * Hx + kx] = sm[col * LP + sk(y)];
     }
 }
 
@@ -238,7 +238,7 @@ __global__ void k_zmi(C2<T> *__restrict__ S2, const T *__restrict__ K, size_t pl
 {
     extern __shared__ char smem_raw[];
     C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
-    const int L = p.L, LP = L + 1;
+    const int L = p.L, LP = lds_len(L);
     const int kx0 = blockIdx.x * tk;
     const int ky = blockIdx.y;
     const size_t cstride = (size_t)nz * Py * Hx;
@@ -246,7 +246,7 @@ __global__ void k_zmi(C2<T> *__restrict__ S2, const T *__restrict__ K, size_t pl
     C2<T> *base = S2 + (size_t)ky * Hx;
     for (int e = threadIdx.x; e < 3 * tk * L; e += blockDim.x) {
         const int col = e % tk, rest = e / tk, z = rest % L, c = rest / L, kx = kx0 + col;
-        sm[(c * tk + col) * LP + z] = (z < nz && kx < Hx) ? base[c * cstride + z * zstride + kx] : cmk<T>((T)0, (T)0);
+        sm[(c * tk + col) * LP + sk(z)] = (z < nz && kx < Hx) ? base[c * cstride + z * zstride + kx] : cmk<T>((T)0, (T)0);
     }
     __syncthreads();
     const int l = threadIdx.x / p.tline, t = threadIdx.x % p.tline;
@@ -258,14 +258,14 @@ __global__ void k_zmi(C2<T> *__restrict__ S2, const T *__restrict__ K, size_t pl
         if (kx < Hx) {
             T k[6];
             kernel_at<T>(K, plane, Hx, Hy2, Py, L, kx, ky, kz, k);
-            mult3<T>(sm[(0 * tk + col) * LP + kz], sm[(1 * tk + col) * LP + kz], sm[(2 * tk + col) * LP + kz], k);
+            mult3<T>(sm[(0 * tk + col) * LP + sk(kz)], sm[(1 * tk + col) * LP + sk(kz)], sm[(2 * tk + col) * LP + sk(kz)], k);
         }
     }
     __syncthreads();
     lds_fft<T, true>(line, p, t, active);
     for (int e = threadIdx.x; e < 3 * tk * nz; e += blockDim.x) {
         const int col = e % tk, rest = e / tk, z = rest % nz, c = rest / nz, kx = kx0 + col;
-        if (kx < Hx) base[c * cstride + z * zstride + kx] = sm[(c * tk + col) * LP + z];
+        if (kx < Hx) base[c * cstride + z * zstride + kx] = sm[(c * tk + col) * LP + sk(z)];
     }
 }
 
@@ -277,12 +277,12 @@ __global__ void k_ymi2d(C2<T> *__restrict__ S1, const T *__restrict__ K, size_t 
 {
     extern __shared__ char smem_raw[];
     C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
-    const int L = p.L, LP = L + 1;
+    const int L = p.L, LP = lds_len(L);
     const int kx0 = blockIdx.x * tk;
     const size_t cstride = (size_t)ny * Hx;
     for (int e = threadIdx.x; e < 3 * tk * L; e += blockDim.x) {
         const int col = e % tk, rest = e / tk, y = rest % L, c = rest / L, kx = kx0 + col;
-        sm[(c * tk + col) * LP + y] = (y < ny && kx < Hx) ? S1[c * cstride + (size_t)y * Hx + kx] : cmk<T>((T)0, (T)0);
+        sm[(c * tk + col) * LP + sk(y)] = (y < ny && kx < Hx) ? S1[c * cstride + (size_t)y * Hx + kx] : cmk<T>((T)0, (T)0);
     }
     __syncthreads();
     const int l = threadIdx.x / p.tline, t = threadIdx.x % p.tline;
@@ -294,14 +294,14 @@ __global__ void k_ymi2d(C2<T> *__restrict__ S1, const T *__restrict__ K, size_t 
         if (kx < Hx) {
             T k[6];
             kernel_at<T>(K, plane, Hx, Hy2, L, 1, kx, ky, 0, k);
-            mult3<T>(sm[(0 * tk + col) * LP + ky], sm[(1 * tk + col) * LP + ky], sm[(2 * tk + col) * LP + ky], k);
+            mult3<T>(sm[(0 * tk + col) * LP + sk(ky)], sm[(1 * tk + col) * LP + sk(ky)], sm[(2 * tk + col) * LP + sk(ky)], k);
         }
     }
     __syncthreads();
     lds_fft<T, true>(line, p, t, active);
     for (int e = threadIdx.x; e < 3 * tk * ny; e += blockDim.x) {
         const int col = e % tk, rest = e / tk, y = rest % ny, c = rest / ny, kx = kx0 + col;
-        if (kx < Hx) S1[c * cstride + (size_t)y * Hx + kx] = sm[(c * tk + col) * LP + y];
+        if (kx < Hx) S1[c * cstride + (size_t)y * Hx + kx] = sm[(c * tk + col) * LP + sk(y)];
     }
 }
 
@@ -316,19 +316,19 @@ __global__ void k_xinv(const C2<T> *__restrict__ in, TOUT *__restrict__ out, siz
     const int l = threadIdx.x / p.tline, t = threadIdx.x % p.tline;
     const int g = blockIdx.x * lpw + l;
     const bool active = g < nrows;
-    C2<T> *line = sm + (size_t)l * (L + 1);
+    C2<T> *line = sm + (size_t)l * lds_len(L);
     int c = 0, r = 0;
     if (active) {
         c = g / (ny * nz);
         r = g % (ny * nz);
         const C2<T> *src = in + (size_t)g * Hx;
-        for (int k = t; k < L; k += p.tline) line[k] = k < Hx ? src[k] : cconj<T>(src[L - k]);
+        for (int k = t; k < L; k += p.tline) line[sk(k)] = k < Hx ? src[k] : cconj<T>(src[L - k]);
     }
     __syncthreads();
     lds_fft<T, true>(line, p, t, active);
     if (active) {
         TOUT *dst = out + c * out_cstride + (size_t)r * nx;
-        for (int x = t; x < nx; x += p.tline) dst[x] = (TOUT)line[x].x;
+        for (int x = t; x < nx; x += p.tline) dst[x] = (TOUT)line[sk(x)].x;
     }
 }
 
@@ -337,20 +337,20 @@ __global__ void k_zfwd_setup(double2 *__restrict__ S, int Hx, int Py, FftPlanDev
 {
     extern __shared__ char smem_raw[];
     double2 *sm = reinterpret_cast<double2 *>(smem_raw);
-    const int L = p.L, LP = L + 1;
+    const int L = p.L, LP = lds_len(L);
     const int kx0 = blockIdx.x * tk, ky = blockIdx.y;
     const size_t zstride = (size_t)Py * Hx;
     double2 *base = S + (size_t)ky * Hx;
     for (int e = threadIdx.x; e < tk * L; e += blockDim.x) {
         const int col = e % tk, z = e / tk, kx = kx0 + col;
-        sm[col * LP + z] = kx < Hx ? base[z * zstride + kx] : cmk<double>(0.0, 0.0);
+        sm[col * LP + sk(z)] = kx < Hx ? base[z * zstride + kx] : cmk<double>(0.0, 0.0);
     }
     __syncthreads();
     const int l = threadIdx.x / p.tline, t = threadIdx.x % p.tline;
     lds_fft<double, false>(sm + (size_t)l * LP, p, t, l < tk);
     for (int e = threadIdx.x; e < tk * L; e += blockDim.x) {
         const int col = e % tk, z = e / tk, kx = kx0 + col;
-        if (kx < Hx) base[z * zstride + kx] = sm[col * LP + z];
+        if (kx < Hx) base[z * zstride + kx] = sm[col * LP + sk(z)];
     }
 }
 
@@ -394,15 +394,15 @@ static ConvLaunch choose_launch(const ConvGeom &g, const FftPlanHost &px, const 
     const int lds = std::min(mm_max_lds_bytes(), 65536);
     ConvLaunch ln;
     ln.lpw_x = env_int("MM_LPW_X", std::max(1, 256 / px.tline));
-    while (ln.lpw_x > 1 && ((size_t)ln.lpw_x * (px.L + 1) * csize > (size_t)lds || ln.lpw_x * px.tline > 1024)) --ln.lpw_x;
+    while (ln.lpw_x > 1 && ((size_t)ln.lpw_x * lds_len(px.L) * csize > (size_t)lds || ln.lpw_x * px.tline > 1024)) --ln.lpw_x;
     const int nl_y = g.is2d ? 3 : 1;
     int tk = env_int("MM_TK_Y", std::max(1, (int)(64 / csize)));
-    while (tk > 1 && ((size_t)nl_y * tk * py.tline > 1024 || (size_t)nl_y * tk * (py.L + 1) * csize > (size_t)lds)) tk /= 2;
+    while (tk > 1 && ((size_t)nl_y * tk * py.tline > 1024 || (size_t)nl_y * tk * lds_len(py.L) * csize > (size_t)lds)) tk /= 2;
     ln.tk_y = tk;
-    MM_REQUIRE((size_t)nl_y * tk * py.tline <= 1024 && (size_t)nl_y * tk * (py.L + 1) * csize <= (size_t)lds,
+    MM_REQUIRE((size_t)nl_y * tk * py.tline <= 1024 && (size_t)nl_y * tk * lds_len(py.L) * csize <= (size_t)lds,
                "y transform length too large for one workgroup");
     int tz = env_int("MM_TK_Z", std::max(1, (int)(128 / csize)));
-    while (tz > 1 && ((size_t)3 * tz * pz.tline > 1024 || (size_t)3 * tz * (pz.L + 1) * csize > (size_t)lds)) tz /= 2;
+    while (tz > 1 && ((size_t)3 * tz * pz.tline > 1024 || (size_t)3 * tz * lds_len(pz.L) * csize > (size_t)lds)) tz /= 2;
     ln.tk_z = tz;
     return ln;
 }
@@ -465,13 +465,13 @@ void ConvImpl<T>::init(const double *N_host, int nx, int ny, int nz, int ex, int
     const int lds = mm_max_lds_bytes();
     const FftPlanDev dx = px.dev<double>(), dy = py.dev<double>(), dz = pz.dev<double>();
     int lpw = std::max(1, 256 / px.tline);
-    while (lpw > 1 && (size_t)lpw * (px.L + 1) * 16 > (size_t)lds) --lpw;
+    while (lpw > 1 && (size_t)lpw * lds_len(px.L) * 16 > (size_t)lds) --lpw;
     int tky = 8;
-    while (tky > 1 && ((size_t)tky * py.tline > 1024 || (size_t)tky * (py.L + 1) * 16 > (size_t)lds)) tky /= 2;
+    while (tky > 1 && ((size_t)tky * py.tline > 1024 || (size_t)tky * lds_len(py.L) * 16 > (size_t)lds)) tky /= 2;
     int tkz = 8;
-    while (tkz > 1 && ((size_t)tkz * pz.tline > 1024 || (size_t)tkz * (pz.L + 1) * 16 > (size_t)lds)) tkz /= 2;
-    const size_t shx = (size_t)lpw * (px.L + 1) * 16, shy = (size_t)tky * (py.L + 1) * 16,
-                 shz = (size_t)tkz * (pz.L + 1) * 16;
+    while (tkz > 1 && ((size_t)tkz * pz.tline > 1024 || (size_t)tkz * lds_len(pz.L) * 16 > (size_t)lds)) tkz /= 2;
+    const size_t shx = (size_t)lpw * lds_len(px.L) * 16, shy = (size_t)tky * lds_len(py.L) * 16,
+                 shz = (size_t)tkz * lds_len(pz.L) * 16;
     set_lds(k_xfwd<double, double>, shx);
     set_lds(k_yfwd<double>, shy);
     set_lds(k_zfwd_setup, shz);
@@ -504,14 +504,14 @@ void ConvImpl<T>::init(const double *N_host, int nx, int ny, int nz, int ex, int
     if (!g.is2d) MM_HIP(hipMalloc(&S2, sizeof(C2<T>) * 3 * (size_t)nz * Py * g.Hx));
 
     const size_t cs = sizeof(C2<T>);
-    set_lds(k_xfwd<T, T>, (size_t)ln.lpw_x * (px.L + 1) * cs);
-    set_lds(k_xinv<T, T>, (size_t)ln.lpw_x * (px.L + 1) * cs);
+    set_lds(k_xfwd<T, T>, (size_t)ln.lpw_x * lds_len(px.L) * cs);
+    set_lds(k_xinv<T, T>, (size_t)ln.lpw_x * lds_len(px.L) * cs);
     if (g.is2d) {
-        set_lds(k_ymi2d<T>, (size_t)3 * ln.tk_y * (py.L + 1) * cs);
+        set_lds(k_ymi2d<T>, (size_t)3 * ln.tk_y * lds_len(py.L) * cs);
     } else {
-        set_lds(k_yfwd<T>, (size_t)ln.tk_y * (py.L + 1) * cs);
-        set_lds(k_yinv<T>, (size_t)ln.tk_y * (py.L + 1) * cs);
-        set_lds(k_zmi<T>, (size_t)3 * ln.tk_z * (pz.L + 1) * cs);
+        set_lds(k_yfwd<T>, (size_t)ln.tk_y * lds_len(py.L) * cs);
+        set_lds(k_yinv<T>, (size_t)ln.tk_y * lds_len(py.L) * cs);
+        set_lds(k_zmi<T>, (size_t)3 * ln.tk_z * lds_len(pz.L) * cs);
     }
 }
 
@@ -549,7 +549,7 @@ template <typename T> void ConvImpl<T>::forward_x(const T *M, hipStream_t s, con
 {
     const size_t n = (size_t)g.nx * g.ny * g.nz;
     const int nrows = 3 * g.ny * g.nz;
-    const size_t sh = (size_t)ln.lpw_x * (px.L + 1) * sizeof(C2<T>);
+    const size_t sh = (size_t)ln.lpw_x * lds_len(px.L) * sizeof(C2<T>);
     hipLaunchKernelGGL((k_xfwd<T, T>), dim3((nrows + ln.lpw_x - 1) / ln.lpw_x), dim3(ln.lpw_x * px.tline), sh, s,
                        M, n, S1, (size_t)g.ny * g.nz * g.Hx, g.Hx, g.ny, g.nz, nrows, g.nx, px.dev<T>(), ln.lpw_x,
                        M_alt, sel);
@@ -563,7 +563,7 @@ template <typename T> void ConvImpl<T>::spectral(hipStream_t s)
     const size_t plane = (size_t)g.Hx * g.Hy2 * g.Hz2;
     if (g.is2d) {
         const int tk = ln.tk_y;
-        const size_t sh = (size_t)3 * tk * (py.L + 1) * cs;
+        const size_t sh = (size_t)3 * tk * lds_len(py.L) * cs;
         hipLaunchKernelGGL((k_ymi2d<T>), dim3((g.Hx + tk - 1) / tk), dim3(3 * tk * py.tline), sh, s, S1, kern, plane,
                            g.Hx, g.Hy2, g.ny, py.dev<T>(), tk);
         MM_HIP(hipGetLastError());
@@ -571,13 +571,13 @@ template <typename T> void ConvImpl<T>::spectral(hipStream_t s)
         return;
     }
     const int tk = ln.tk_y;
-    const size_t shy = (size_t)tk * (py.L + 1) * cs;
+    const size_t shy = (size_t)tk * lds_len(py.L) * cs;
     hipLaunchKernelGGL((k_yfwd<T>), dim3((g.Hx + tk - 1) / tk, g.nz, 3), dim3(tk * py.tline), shy, s, S1, S2, g.Hx,
                        g.ny, g.nz, py.dev<T>(), tk);
     MM_HIP(hipGetLastError());
     if (klog) klog->mark(s, KC_YFWD, conv_bytes<T>(g, KC_YFWD));
     const int tz = ln.tk_z;
-    const size_t shz = (size_t)3 * tz * (pz.L + 1) * cs;
+    const size_t shz = (size_t)3 * tz * lds_len(pz.L) * cs;
     hipLaunchKernelGGL((k_zmi<T>), dim3((g.Hx + tz - 1) / tz, g.Py), dim3(3 * tz * pz.tline), shz, s, S2, kern, plane,
                        g.Hx, g.Hy2, g.Py, g.nz, pz.dev<T>(), tz);
     MM_HIP(hipGetLastError());
@@ -592,7 +592,7 @@ template <typename T> void ConvImpl<T>::inverse_x(T *H, hipStream_t s)
 {
     const size_t n = (size_t)g.nx * g.ny * g.nz;
     const int nrows = 3 * g.ny * g.nz;
-    const size_t sh = (size_t)ln.lpw_x * (px.L + 1) * sizeof(C2<T>);
+    const size_t sh = (size_t)ln.lpw_x * lds_len(px.L) * sizeof(C2<T>);
     hipLaunchKernelGGL((k_xinv<T, T>), dim3((nrows + ln.lpw_x - 1) / ln.lpw_x), dim3(ln.lpw_x * px.tline), sh, s, S1,
                        H, n, g.Hx, g.ny, g.nz, nrows, g.nx, px.dev<T>(), ln.lpw_x);
     MM_HIP(hipGetLastError());

@@ -41,6 +41,16 @@ template <typename T, bool INV> __device__ __forceinline__ C2<T> cmul_mi(C2<T> a
     return INV ? cmk<T>(-a.y, a.x) : cmk<T>(a.y, -a.x);
 }
 
+// LDS layout of a line: element j lives at j + (j >> MM_LDS_SKEW) (one pad
+// slot per 2^MM_LDS_SKEW elements) so that the strided Stockham writes of the
+// early passes and the transposing tile loads spread over the LDS banks; a
+// line of length L occupies lds_len(L) slots.  MM_LDS_SKEW = 0 disables it.
+#ifndef MM_LDS_SKEW
+#define MM_LDS_SKEW 3
+#endif
+__host__ __device__ __forceinline__ int sk(int j) { return MM_LDS_SKEW ? j + (j >> MM_LDS_SKEW) : j; }
+__host__ __device__ __forceinline__ int lds_len(int L) { return (MM_LDS_SKEW ? L + (L >> MM_LDS_SKEW) : L) + 1; }
+
 // device-side plan of one transform length (passed by value).  The radix of
 // pass p is (radices >> 4p) & 15, so no array is indexed at run time.
 struct FftPlanDev {
@@ -154,7 +164,7 @@ __device__ __forceinline__ void fft_pass(C2<T> *line, const FftPlanDev &p, int t
         const int b = t + q * p.tline;
         if (active && b < nb) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[q][r] = line[b + r * nb];
+            for (int r = 0; r < R; ++r) v[q][r] = line[sk(b + r * nb)];
         }
     }
     __syncthreads();
@@ -176,7 +186,7 @@ __device__ __forceinline__ void fft_pass(C2<T> *line, const FftPlanDev &p, int t
             dftR<R, T, INV>(v[q]);
             const int base = (b / Ns) * Ns * R + k;
 #pragma unroll
-            for (int r = 0; r < R; ++r) line[base + r * Ns] = v[q][r];
+            for (int r = 0; r < R; ++r) line[sk(base + r * Ns)] = v[q][r];
         }
     }
     __syncthreads();

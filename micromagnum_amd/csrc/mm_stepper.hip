@@ -88,6 +88,10 @@ template <typename T> struct StageDev {
     double ec[8], ecself; // error coefficients (final stage)
     int want_dydt;        // reduce max|k_s| into ctl->dydt_bits (stage 0)
     int norm_mode;
+    // FSAL (dp54, rk23): the last stage takes max|k_0| from kin[0] and then
+    // overwrites k_0 with its own k_s for the next attempt (runge_kutta.py:97-101,131-134)
+    int dydt_from_k0;
+    T *fsal_out;
 };
 
 template <typename T> __device__ __forceinline__ void atomic_max_bits(unsigned long long *a, double v)
@@ -207,6 +211,16 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
         return;
     }
     // final stage: error norm and normalised new state into the other buffer
+    if (G.dydt_from_k0) {
+        const double a = G.kin[0][i], b = G.kin[0][n + i], c = G.kin[0][2 * n + i];
+        double q = a * a + b * b + c * c;
+        if (!S.Ms_uniform) {
+            const double s = ms;
+            q = s == 0.0 ? 0.0 : q / (s * s);
+        }
+        dydt_max = fmax(dydt_max, q);
+    }
+    if (G.fsal_out) store3<T>(G.fsal_out, n, i, k);
     {
         double q = er[0] * er[0] + er[1] * er[1] + er[2] * er[2];
         if (!S.Ms_uniform) {
@@ -247,7 +261,7 @@ __global__ void k_stage(SysDev<T> S, StageDev<T> G, StepCtl *ctl, const C2<T> *_
     const int nrows = ny * nz;
     const int row0 = blockIdx.x * rpw;
     const T *Y = G.Ys ? G.Ys : (ctl->cur ? G.yb1 : G.yb0);
-    const int L = p.L, LP = L + 1;
+    const int L = p.L, LP = lds_len(L);
     if (has_demag) {
         const int l = threadIdx.x / p.tline, t = threadIdx.x % p.tline;
         const int c = l / rpw, r = l % rpw;
@@ -256,7 +270,7 @@ __global__ void k_stage(SysDev<T> S, StageDev<T> G, StepCtl *ctl, const C2<T> *_
         C2<T> *line = sm + (size_t)l * LP;
         if (active) {
             const C2<T> *src = S1 + ((size_t)c * nrows + gr) * Hx;
-            for (int k = t; k < L; k += p.tline) line[k] = k < Hx ? src[k] : cconj<T>(src[L - k]);
+            for (int k = t; k < L; k += p.tline) line[sk(k)] = k < Hx ? src[k] : cconj<T>(src[L - k]);
         }
         __syncthreads();
         lds_fft<T, true>(line, p, t, active);
@@ -267,10 +281,10 @@ __global__ void k_stage(SysDev<T> S, StageDev<T> G, StepCtl *ctl, const C2<T> *_
         const int gr = row0 + r;
         if (gr >= nrows) continue;
         V3<T> Hd = v3<T>(0, 0, 0);
-        if (has_demag) Hd = v3<T>(sm[(0 * rpw + r) * LP + x].x, sm[(1 * rpw + r) * LP + x].x, sm[(2 * rpw + r) * LP + x].x);
+        if (has_demag) Hd = v3<T>(sm[(0 * rpw + r) * LP + sk(x)].x, sm[(1 * rpw + r) * LP + sk(x)].x, sm[(2 * rpw + r) * LP + sk(x)].x);
         stage_cell<T>(S, G, ctl, Y, x, gr % ny, gr / ny, Hd, err_max, dydt_max);
     }
-    if (G.prep == 2 || G.want_dydt) {
+    if (G.prep == 2 || G.want_dydt || G.dydt_from_k0) {
         __shared__ double red[2][16];
         double a = wave_max(err_max), b = wave_max(dydt_max);
         const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -286,7 +300,7 @@ __global__ void k_stage(SysDev<T> S, StageDev<T> G, StepCtl *ctl, const C2<T> *_
                 eb = fmax(eb, red[1][q]);
             }
             if (G.prep == 2) atomic_max_bits<T>(&ctl->err_bits, ea);
-            if (G.want_dydt) atomic_max_bits<T>(&ctl->dydt_bits, eb);
+            if (G.want_dydt || G.dydt_from_k0) atomic_max_bits<T>(&ctl->dydt_bits, eb);
         }
     }
 }
@@ -347,6 +361,18 @@ __global__ void k_controller(StepCtl *ctl, int order, double eps_abs, double eps
         ctl->rejections += 1;
     }
     ctl->h = hn;
+}
+
+// ytmp_1 = y; ytmp_1 += k_0 * (h b10) for attempts that start from a FSAL k_0
+template <typename T>
+__global__ void k_prep_first(size_t m3, const T *__restrict__ y0, const T *__restrict__ y1, const T *__restrict__ k0,
+                             T *__restrict__ out, const StepCtl *ctl, double b10)
+{
+    if (ctl->stop) return;
+    const T *y = ctl->cur ? y1 : y0;
+    const T f = (T)(ctl->h * b10);
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < m3; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = y[i] + k0[i] * f;
 }
 
 template <typename T> __global__ void k_cast_in(size_t n, const double *__restrict__ a, T *__restrict__ b)
@@ -484,8 +510,6 @@ template <typename T> struct Stepper : public StepperBase {
         MM_HIP(hipMalloc(&ctl, sizeof(StepCtl)));
         int rc = mm_tableau(d.method, &tab);
         if (rc != MM_OK) throw mm_error(rc, mm_last_error());
-        if (d.method == MM_DP54 || d.method == MM_RK23)
-            throw mm_error(MM_ERR_UNSUPPORTED, "fused stepper: FSAL methods (dp54, rk23) are not implemented yet");
 
         // physical parameters; LLG factors as landau_lifshitz_gilbert.py:133-158
         const double mu0 = MM_MU0_VALUE;
@@ -565,10 +589,10 @@ template <typename T> struct Stepper : public StepperBase {
             const int tl = conv->px.tline;
             rpw = std::max(1, 384 / (3 * tl));
             if (const char *e = std::getenv("MM_RPW")) rpw = std::max(1, std::atoi(e));
-            while (rpw > 1 && (size_t)3 * rpw * (conv->px.L + 1) * sizeof(C2<T>) > (size_t)mm_max_lds_bytes()) --rpw;
+            while (rpw > 1 && (size_t)3 * rpw * lds_len(conv->px.L) * sizeof(C2<T>) > (size_t)mm_max_lds_bytes()) --rpw;
             MM_REQUIRE(3 * tl <= 1024, "x transform too long for the fused stage kernel");
             threads = 3 * rpw * tl;
-            stage_lds = (size_t)3 * rpw * (conv->px.L + 1) * sizeof(C2<T>);
+            stage_lds = (size_t)3 * rpw * lds_len(conv->px.L) * sizeof(C2<T>);
             MM_REQUIRE(stage_lds <= (size_t)mm_max_lds_bytes(), "fused stage kernel exceeds the LDS maximum");
         } else {
             rpw = std::max(1, 256 / d.nx);
@@ -657,16 +681,29 @@ template <typename T> struct Stepper : public StepperBase {
         return G;
     }
 
+    bool have_fsal = false;
+
     // one attempt of the tableau from y[cur] with h from the control block
     void enqueue_attempt()
     {
         const int ns = tab.num_steps;
         const bool six = ns == 6;
-        for (int st = 0; st < ns; ++st) {
+        const bool fsal = tab.fsal != 0;
+        int st0 = 0;
+        if (fsal && have_fsal) {
+            // k_0 is the last stage of the previous attempt: only ytmp_1 is needed
+            const size_t m3 = 3 * n;
+            hipLaunchKernelGGL(k_prep_first<T>, dim3((int)std::min<size_t>((m3 + 255) / 256, 16384)), dim3(256), 0, s,
+                               m3, yb[0], yb[1], k[0], tmp[0], ctl, tab.b[1][0]);
+            MM_HIP(hipGetLastError());
+            if (klog) klog->mark(s, KC_CTRL, 3.0 * 3.0 * sizeof(T) * n);
+            st0 = 1;
+        }
+        for (int st = st0; st < ns; ++st) {
             const T *Ys = st == 0 ? nullptr : tmp[(st - 1) & 1];
             StageDev<T> G = blank_stage(Ys);
             G.kout = (st < ns - 1) ? k[st] : nullptr;
-            G.want_dydt = (st == 0);
+            G.want_dydt = (st == 0) && !fsal;
             G.form = six ? FORM_SUM : FORM_SEQ;
             if (st < ns - 1) {
                 G.prep = 1;
@@ -700,9 +737,14 @@ template <typename T> struct Stepper : public StepperBase {
                 }
                 G.bself = tab.c[st];
                 G.ecself = tab.ec[st];
+                if (fsal) {
+                    G.dydt_from_k0 = 1;
+                    G.fsal_out = k[0];
+                }
             }
             launch_stage(Ys, G);
         }
+        if (fsal) have_fsal = true;
         const bool adaptive = d.method != MM_RK4;
         hipLaunchKernelGGL(k_controller, dim3(1), dim3(1), 0, s, ctl, tab.order, d.eps_abs, d.eps_rel,
                            sys.Ms_uniform, sys.Ms_u, adaptive ? 1 : 0);
@@ -760,6 +802,7 @@ template <typename T> struct Stepper : public StepperBase {
         StepCtl c;
         MM_HIP(hipMemcpy(&c, ctl, sizeof(c), hipMemcpyDeviceToHost));
         upload(yb[c.cur], M_host, 3 * n);
+        have_fsal = false;
     }
 
     void get_m(double *M_host) override

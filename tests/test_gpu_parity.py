@@ -381,3 +381,43 @@ def test_stepper_fp32_tracks_fp64():
         out[prec] = (st.get_m(), st.stats())
         st.close()
     assert np.abs(out["fp32"][0] - out["fp64"][0]).max() / 1.1e6 < 1e-3
+
+
+@pytest.mark.parametrize("method", ["dp54", "rk23"])
+def test_stepper_fsal_trajectory_fp64_vs_oracle(method):
+    # FSAL methods keep the reference's reuse of the last stage of the most
+    # recent attempt (runge_kutta.py:97-101,131-134); the oracle does the same
+    from micromagnum_amd.solver import Stepper
+    shape = (24, 20, 2)
+    kw = _system(shape, with_stt=True)
+    st = Stepper(shape, (3e-9, 3e-9, 2e-9), precision="fp64", method=method, eps_abs=1e-5, eps_rel=1e-5, **kw)
+    M = random_unit_field(shape, 14) * 1.1e6
+    st.set_m(M)
+    st.run(20)
+    Mg = st.get_m()
+    s = st.stats()
+    Mo, so = O.System(shape, (3e-9, 3e-9, 2e-9), N=st.N, **kw).run(M, method=method, steps=20, eps_abs=1e-5,
+                                                                  eps_rel=1e-5)
+    assert s["attempts"] == so["attempts"] and s["rejections"] == so["rejections"]
+    assert abs(s["t"] - so["t"]) <= 1e-12 * so["t"]
+    assert np.abs(Mg - Mo).max() / 1.1e6 < 1e-10
+    st.close()
+
+
+def test_stepper_vortex_spin_torque_dp54_fp32_tracks_fp64():
+    # the spin-torque vortex workload (3x3x2 nm cells, Ms 8.6e5, A 1.3e-11,
+    # alpha 0.1, j = 1e12 A/m^2 along x, P 0.7, beta 0.05, DP54 tol 1e-5) on a
+    # 64x64x2 film: FP32 bulk against the FP64 path
+    from micromagnum_amd.solver import Stepper
+    shape, delta = (64, 64, 2), (3e-9, 3e-9, 2e-9)
+    nx, ny, nz = shape
+    M = vortex_field(shape, delta, (nx * delta[0] / 2, ny * delta[1] / 2), Ms=8.6e5)
+    kw = dict(Ms=8.6e5, alpha=0.1, A=1.3e-11, P=0.7, xi=0.05, j=(1e12, 0.0, 0.0))
+    out = {}
+    for prec in ("fp64", "fp32"):
+        st = Stepper(shape, delta, precision=prec, method="dp54", eps_abs=1e-5, eps_rel=1e-5, **kw)
+        st.set_m(M)
+        st.run(40)
+        out[prec] = st.get_m()
+        st.close()
+    assert np.abs(out["fp32"] - out["fp64"]).max() / 8.6e5 < 1e-3
