@@ -75,6 +75,7 @@ def run_gpu(precision, N, M0, steps, warmup):
     t1 = time.perf_counter()
     s1 = st.stats()
     prof = st.kernel_profile(attempts=6)
+    st.set_m(M0)
     dM, H = st.dmdt()
     res = dict(seconds=t1 - t0, attempts=s1["attempts"] - s0["attempts"],
                rejections=s1["rejections"] - s0["rejections"], sim_time=s1["t"] - s0["t"],
@@ -203,6 +204,7 @@ def main():
                        "step_model_roofline_frac": round(Bstep64 * a64 / HBM_PEAK, 4)}
         out["fp32_over_fp64"] = round(steps_s / s64, 3)
         H32, H64 = r32["H"], r64["H"]
+        # H_tot of the initial state, FP32 path against the FP64 path
         out["field_err_fp32_vs_fp64"] = float(np.linalg.norm(H32 - H64) / np.linalg.norm(H64))
 
     if not args.no_cpu_baseline:

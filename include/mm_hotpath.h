@@ -184,7 +184,7 @@ int mm_stepper_get_m(mm_stepper_t *st, double *M_host);
 /* One evaluation of dM/dt and H_tot at the current M (host outputs, may be NULL). */
 int mm_stepper_dmdt(mm_stepper_t *st, double *dMdt_host, double *Htot_host);
 /* Run `steps` accepted steps (adaptive methods retry rejected attempts on
- * the device).  Returns when the work is enqueued; mm_stepper_sync waits. */
+ * the device).  Returns when they are done. */
 int mm_stepper_run(mm_stepper_t *st, int steps);
 int mm_stepper_sync(mm_stepper_t *st);
 /* stats: [0]=t [1]=h of last accepted step [2]=next h [3]=attempts [4]=rejections */
@@ -199,6 +199,36 @@ void *mm_stepper_stream(mm_stepper_t *st);
  * average ms per launch, launches, average algorithmic HBM bytes per launch. */
 enum { MM_KCLASS_COUNT = 7 };
 int mm_stepper_kernel_profile(mm_stepper_t *st, int attempts, double *out);
+/* Energies at the current M [J] (FP64 sums of FP64 or FP32 products):
+ * E5[0] E_stray = -mu0/2 V sum M.H_stray   (stray_field.py:51)
+ * E5[1] E_exch  = -mu0/2 V sum M.H_exch    (exchange_field.py:58)
+ * E5[2] E_aniso = V sum of the uniaxial (k |a x m|^2) and cubic energy
+ *                 densities                (anisotropy_field.py:84-87,
+ *                 anisotropy_cpu.cpp:64-65,114; with both terms present the
+ *                 reference's E_aniso is None, here it is their sum)
+ * E5[3] E_ext   = -mu0 V sum M.H_ext       (external_field.py:33-35)
+ * E5[4] E_tot   = sum of the present terms (landau_lifshitz_gilbert.py:103-106)
+ * with mu0 = 4 pi 1e-7 as the Python front-end uses it (constants.py:21). */
+int mm_stepper_energy(mm_stepper_t *st, double *E5);
+/* Relaxation: run until the rule holds on an accepted state, or until
+ * max_steps more accepted steps.  The rule is checked once per accepted
+ * state whose step count (accepted steps since creation, as state.step) is
+ * a multiple of check_every, from the k_0 = dM/dt of the attempt that
+ * starts there (for FSAL methods the carried-over k_0); a state that
+ * satisfies it is kept and that attempt is discarded.
+ *   MM_RELAX_DMDT:       scaled_abs_max(dM/dt, Ms) < threshold [1/s]
+ *                        (1e7 /s = 0.01 rad/ns);
+ *   MM_RELAX_DEG_PER_NS: the reference's Relaxed(threshold, check_every)
+ *                        (condition.py:169-200): deg_per_ns =
+ *                        1e-9 (180/pi) atan2(|dM/dt|max h, |M|max) / h
+ *                        (landau_lifshitz_gilbert.py:127-131, h = last
+ *                        accepted step) <= threshold and below the previous
+ *                        check's value.
+ * info3: [0] accepted steps made, [1] 1 if relaxed else 0, [2] value of the
+ * last check. */
+enum { MM_RELAX_DMDT = 1, MM_RELAX_DEG_PER_NS = 2 };
+int mm_stepper_relax(mm_stepper_t *st, int criterion, double threshold, int check_every,
+                     long long max_steps, double *info3);
 int mm_stepper_destroy(mm_stepper_t *st);
 
 #ifdef __cplusplus

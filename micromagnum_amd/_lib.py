@@ -15,6 +15,7 @@ MM_OK = 0
 MM_FP64, MM_FP32 = 0, 1
 MM_RKF45, MM_CC45, MM_DP54, MM_RK23, MM_RK4 = 0, 1, 2, 3, 4
 METHODS = {"rkf45": MM_RKF45, "cc45": MM_CC45, "dp54": MM_DP54, "rk23": MM_RK23, "rk4": MM_RK4}
+MM_RELAX_DMDT, MM_RELAX_DEG_PER_NS = 1, 2
 
 PADDING_DISABLE = 0
 PADDING_ROUND_2 = 1
@@ -89,6 +90,8 @@ SIGNATURES = {
     "mm_stepper_average_m": (_i, [_vp, _dp]),
     "mm_stepper_stream": (_vp, [_vp]),
     "mm_stepper_kernel_profile": (_i, [_vp, _i, _dp]),
+    "mm_stepper_energy": (_i, [_vp, _dp]),
+    "mm_stepper_relax": (_i, [_vp, _i, _d, _i, ctypes.c_longlong, _dp]),
     "mm_stepper_destroy": (_i, [_vp]),
 }
 

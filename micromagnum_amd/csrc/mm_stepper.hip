@@ -37,9 +37,19 @@ struct StepCtl {
     double t, h, h_last;
     unsigned long long err_bits;   // max of |err|^2 (or |err|^2/Ms^2) as bits
     unsigned long long dydt_bits;  // max of |k0|^2 (or scaled)
+    unsigned long long draw_bits;  // max of |k0|^2, never scaled (dMdt.absMax()^2)
+    unsigned long long mraw_bits;  // max of |y|^2 of the attempt's start state (M.absMax()^2)
     long long attempts, rejections, accepted, target;
     int cur;                       // index of the current y buffer
     int stop;                      // accepted == target: remaining launches exit
+    // relaxation stop rule, checked on the first attempt from every
+    // check_every-th accepted state (condition.py:169-200 checks state.step)
+    int relax_mode;                // MM_RELAX_NONE / _DMDT / _DEG_PER_NS
+    int relax_every;
+    int relaxed;                   // the current state satisfied the rule
+    int have_last;                 // relax_last holds a previous deg/ns value
+    double relax_thr, relax_value, relax_last;
+    long long last_check;          // accepted count of the last check
 };
 
 // a per-cell parameter that is either uniform (p == null) or an array
@@ -99,30 +109,51 @@ template <typename T> __device__ __forceinline__ void atomic_max_bits(unsigned l
     atomicMax(a, (unsigned long long)__double_as_longlong(v));
 }
 
+template <typename T> __device__ __forceinline__ double dotd(const V3<T> &a, const V3<T> &b)
+{
+    return (double)a.x * b.x + (double)a.y * b.y + (double)a.z * b.z;
+}
+
+// per-thread maxima of one stage launch (squared norms, FP64)
+struct StageRed {
+    double err;   // |err|^2 (/Ms^2 when Ms is not uniform)
+    double dydt;  // |k0|^2 (/Ms^2 when Ms is not uniform)
+    double draw;  // |k0|^2
+    double mraw;  // |y|^2 of the attempt's start state
+};
+
+// per-thread sums of M.H per term (energy launches only)
+struct EnergyAcc {
+    double demag, exch, aniso, zee;
+};
+
 // Per-cell epilogue: everything after the demag field of one cell.
-template <typename T>
+template <typename T, bool EN>
 __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T> &G, const StepCtl *ctl,
                                            const T *__restrict__ Y, int x, int y, int z, const V3<T> &Hd,
-                                           double &err_max, double &dydt_max)
+                                           StageRed &red, EnergyAcc &ea)
 {
     const size_t n = S.n;
     const size_t i = (size_t)x + (size_t)S.m.nx * ((size_t)y + (size_t)S.m.ny * z);
     const T ms = S.Ms.at(i);
     const V3<T> Mv = v3<T>(Y[i], Y[n + i], Y[2 * n + i]);
     V3<T> H = Hd;
+    if (EN) ea.demag += dotd<T>(Mv, Hd);
     if (S.has_exch && ms != (T)0) {
         const V3<T> he = exchange_cell<T>(S.m, x, y, z, i, n, S.wx, S.wy, S.wz, S.coef_ex, S.Ms.p ? S.Ms.p : nullptr,
                                           S.A.p ? S.A.p : nullptr, Y, ms);
         H.x += he.x;
         H.y += he.y;
         H.z += he.z;
+        if (EN) ea.exch += dotd<T>(Mv, he);
     }
     if (S.has_uni || S.has_cub) {
-        T e0;
+        T e0 = (T)0, e1 = (T)0;
         V3<T> ha = v3<T>(0, 0, 0);
-        if (S.has_cub) ha = cubic_cell<T>(Mv, S.axis1.at(n, i), S.axis2.at(n, i), S.kcub.at(i), ms, S.cm2mu0, &e0);
+        if (S.has_cub) ha = cubic_cell<T>(Mv, S.axis1.at(n, i), S.axis2.at(n, i), S.kcub.at(i), ms, S.cm2mu0, &e1);
         if (S.has_uni) {
             const V3<T> hu = uniaxial_cell<T>(Mv, S.axis1.at(n, i), S.kuni.at(i), ms, S.mu0, &e0);
+            if (EN) ea.aniso += (double)e0;
             if (S.has_cub) {
                 ha.x += hu.x;
                 ha.y += hu.y;
@@ -131,6 +162,7 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
                 ha = hu;
             }
         }
+        if (EN) ea.aniso += (double)e1;
         H.x += ha.x;
         H.y += ha.y;
         H.z += ha.z;
@@ -139,7 +171,9 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
         H.x += S.hext[0];
         H.y += S.hext[1];
         H.z += S.hext[2];
+        if (EN) ea.zee += (double)Mv.x * S.hext[0] + (double)Mv.y * S.hext[1] + (double)Mv.z * S.hext[2];
     }
+    if (EN) return;
     V3<T> k = llge_cell<T>(Mv, H, S.f1.at(i), S.f2.at(i));
     if (S.has_stt) {
         const V3<T> st = zhangli_cell<T>(S.m, x, y, z, i, n, S.dx, S.dy, S.dz, S.do_precess != 0, S.P.at(i), S.xi.at(i),
@@ -153,11 +187,13 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
     if (G.want_dydt) {
         const double a = k.x, b = k.y, c = k.z;
         double q = a * a + b * b + c * c;
+        red.draw = fmax(red.draw, q);
+        red.mraw = fmax(red.mraw, (double)Mv.x * Mv.x + (double)Mv.y * Mv.y + (double)Mv.z * Mv.z);
         if (!S.Ms_uniform) {
             const double s = ms;
             q = s == 0.0 ? 0.0 : q / (s * s);
         }
-        dydt_max = fmax(dydt_max, q);
+        red.dydt = fmax(red.dydt, q);
     }
     if (!G.prep) return;
     const double h = ctl->h;
@@ -214,11 +250,14 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
     if (G.dydt_from_k0) {
         const double a = G.kin[0][i], b = G.kin[0][n + i], c = G.kin[0][2 * n + i];
         double q = a * a + b * b + c * c;
+        red.draw = fmax(red.draw, q);
+        const double y0 = yc[i], y1 = yc[n + i], y2 = yc[2 * n + i];
+        red.mraw = fmax(red.mraw, y0 * y0 + y1 * y1 + y2 * y2);
         if (!S.Ms_uniform) {
             const double s = ms;
             q = s == 0.0 ? 0.0 : q / (s * s);
         }
-        dydt_max = fmax(dydt_max, q);
+        red.dydt = fmax(red.dydt, q);
     }
     if (G.fsal_out) store3<T>(G.fsal_out, n, i, k);
     {
@@ -227,7 +266,7 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
             const double s = ms;
             q = s == 0.0 ? 0.0 : q / (s * s);
         }
-        err_max = fmax(err_max, q);
+        red.err = fmax(red.err, q);
     }
     T *yn = ctl->cur ? G.yb0 : G.yb1;
     double norm;
@@ -249,12 +288,13 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
 
 // x inverse of the demag spectrum rows + epilogue.  rpw rows per block, all
 // three components of a row in the same block.  has_demag == 0: rows are
-// processed without the FFT part (H_demag = 0).
-template <typename T>
+// processed without the FFT part (H_demag = 0).  EN: instead of the stage
+// epilogue, per-block sums of M.H per term into epart[q * gridDim.x + block].
+template <typename T, bool EN>
 __global__ void k_stage(SysDev<T> S, StageDev<T> G, StepCtl *ctl, const C2<T> *__restrict__ S1, FftPlanDev p,
-                        int Hx, int rpw, int has_demag)
+                        int Hx, int rpw, int has_demag, double *__restrict__ epart)
 {
-    if (ctl->stop) return;
+    if (!EN && ctl->stop) return;
     extern __shared__ char smem_raw[];
     C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
     const int nx = S.m.nx, ny = S.m.ny, nz = S.m.nz;
@@ -275,34 +315,92 @@ __global__ void k_stage(SysDev<T> S, StageDev<T> G, StepCtl *ctl, const C2<T> *_
         __syncthreads();
         lds_fft<T, true>(line, p, t, active);
     }
-    double err_max = 0.0, dydt_max = 0.0;
+    StageRed red = {0.0, 0.0, 0.0, 0.0};
+    EnergyAcc ea = {0.0, 0.0, 0.0, 0.0};
     for (int e = threadIdx.x; e < rpw * nx; e += blockDim.x) {
         const int r = e / nx, x = e % nx;
         const int gr = row0 + r;
         if (gr >= nrows) continue;
         V3<T> Hd = v3<T>(0, 0, 0);
         if (has_demag) Hd = v3<T>(sm[(0 * rpw + r) * LP + sk(x)].x, sm[(1 * rpw + r) * LP + sk(x)].x, sm[(2 * rpw + r) * LP + sk(x)].x);
-        stage_cell<T>(S, G, ctl, Y, x, gr % ny, gr / ny, Hd, err_max, dydt_max);
+        stage_cell<T, EN>(S, G, ctl, Y, x, gr % ny, gr / ny, Hd, red, ea);
+    }
+    __shared__ double wred[4][16];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (int)((blockDim.x + 63) >> 6);
+    if (EN) {
+        const double a = wave_sum(ea.demag), b = wave_sum(ea.exch), c = wave_sum(ea.aniso), d = wave_sum(ea.zee);
+        if (lane == 0) {
+            wred[0][w] = a;
+            wred[1][w] = b;
+            wred[2][w] = c;
+            wred[3][w] = d;
+        }
+        __syncthreads();
+        if (threadIdx.x < 4) {
+            double acc = 0.0;
+            for (int q = 0; q < nw; ++q) acc += wred[threadIdx.x][q];
+            epart[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = acc;
+        }
+        return;
     }
     if (G.prep == 2 || G.want_dydt || G.dydt_from_k0) {
-        __shared__ double red[2][16];
-        double a = wave_max(err_max), b = wave_max(dydt_max);
-        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        const double a = wave_max(red.err), b = wave_max(red.dydt), c = wave_max(red.draw), d = wave_max(red.mraw);
         if (lane == 0) {
-            red[0][w] = a;
-            red[1][w] = b;
+            wred[0][w] = a;
+            wred[1][w] = b;
+            wred[2][w] = c;
+            wred[3][w] = d;
         }
         __syncthreads();
         if (threadIdx.x == 0) {
-            double ea = red[0][0], eb = red[1][0];
-            for (int q = 1; q < (int)((blockDim.x + 63) >> 6); ++q) {
-                ea = fmax(ea, red[0][q]);
-                eb = fmax(eb, red[1][q]);
+            double ea0 = wred[0][0], eb = wred[1][0], ec = wred[2][0], ed = wred[3][0];
+            for (int q = 1; q < nw; ++q) {
+                ea0 = fmax(ea0, wred[0][q]);
+                eb = fmax(eb, wred[1][q]);
+                ec = fmax(ec, wred[2][q]);
+                ed = fmax(ed, wred[3][q]);
             }
-            if (G.prep == 2) atomic_max_bits<T>(&ctl->err_bits, ea);
-            if (G.want_dydt || G.dydt_from_k0) atomic_max_bits<T>(&ctl->dydt_bits, eb);
+            if (G.prep == 2) atomic_max_bits<T>(&ctl->err_bits, ea0);
+            if (G.want_dydt || G.dydt_from_k0) {
+                atomic_max_bits<T>(&ctl->dydt_bits, eb);
+                atomic_max_bits<T>(&ctl->draw_bits, ec);
+                atomic_max_bits<T>(&ctl->mraw_bits, ed);
+            }
         }
     }
+}
+
+// Relaxation rule on the state the attempt started from; true: stop there
+// (the attempt is discarded).  MM_RELAX_DMDT compares scaled_abs_max(dM/dt,
+// Ms) with the threshold; MM_RELAX_DEG_PER_NS is the reference's Relaxed
+// condition (condition.py:169-200) on deg_per_ns
+// (landau_lifshitz_gilbert.py:127-131) with state.h = the last accepted h.
+__device__ bool relax_check(StepCtl *ctl, double dmax_scaled)
+{
+    if (!ctl->relax_mode || ctl->last_check == ctl->accepted) return false;
+    if (ctl->accepted % ctl->relax_every != 0) return false;
+    ctl->last_check = ctl->accepted;
+    bool relaxed;
+    double v;
+    if (ctl->relax_mode == 1) {
+        v = dmax_scaled;
+        relaxed = v < ctl->relax_thr;
+    } else {
+        const double dm = sqrt(__longlong_as_double((long long)ctl->draw_bits));
+        const double mm = sqrt(__longlong_as_double((long long)ctl->mraw_bits));
+        const double h = ctl->h_last;
+        const double deg_per_step = (180.0 / 3.14159265358979323846) * atan2(dm * h, mm);
+        v = 1e-9 * deg_per_step / h;
+        relaxed = ctl->have_last && v < ctl->relax_last && v <= ctl->relax_thr;
+        ctl->relax_last = v;
+        ctl->have_last = 1;
+    }
+    ctl->relax_value = v;
+    if (relaxed) {
+        ctl->relaxed = 1;
+        ctl->stop = 1;
+    }
+    return relaxed;
 }
 
 // MicroMagneticsStepSizeController.adjust_stepsize
@@ -312,11 +410,17 @@ __global__ void k_controller(StepCtl *ctl, int order, double eps_abs, double eps
                              int adaptive)
 {
     if (ctl->stop) return;
-    ctl->attempts += 1;
     const double h = ctl->h;
     const unsigned long long ebits = ctl->err_bits, dbits = ctl->dydt_bits;
+    double dmax = sqrt(__longlong_as_double((long long)dbits));
+    if (Ms_uniform) dmax /= Ms_u;
+    const bool relaxed = relax_check(ctl, dmax);
     ctl->err_bits = 0ull;
     ctl->dydt_bits = 0ull;
+    ctl->draw_bits = 0ull;
+    ctl->mraw_bits = 0ull;
+    if (relaxed) return;
+    ctl->attempts += 1;
     if (!adaptive) {
         ctl->t += h;
         ctl->h_last = h;
@@ -326,11 +430,7 @@ __global__ void k_controller(StepCtl *ctl, int order, double eps_abs, double eps
         return;
     }
     double e_abs = sqrt(__longlong_as_double((long long)ebits));
-    double dmax = sqrt(__longlong_as_double((long long)dbits));
-    if (Ms_uniform) {
-        e_abs /= Ms_u;
-        dmax /= Ms_u;
-    }
+    if (Ms_uniform) e_abs /= Ms_u;
     const double HEADROOM = 0.85, MINS = 0.2, MAXS = 5.0, MINH = 1e-15, MAXH = 1e-10;
     const double den = h * dmax;
     int accept;
@@ -413,6 +513,8 @@ struct StepperBase {
     virtual void average(double *a3) = 0;
     virtual hipStream_t stream() = 0;
     virtual void profile(int attempts, double *out) = 0;
+    virtual void energy(double *E5) = 0;
+    virtual void relax(int criterion, double thr, int every, long long max_steps, double *info3) = 0;
 };
 
 static bool all_equal(const double *a, size_t n)
@@ -436,6 +538,7 @@ template <typename T> struct Stepper : public StepperBase {
     T *Hbuf = nullptr;
     double *dbuf = nullptr;
     double *part = nullptr;
+    double *epart = nullptr;
     StepCtl *ctl = nullptr;
     int rpw = 1;
     size_t stage_lds = 0;
@@ -601,6 +704,7 @@ template <typename T> struct Stepper : public StepperBase {
         }
         threads = std::max(64, ((threads + 63) / 64) * 64);
         grid_rows = (d.ny * d.nz + rpw - 1) / rpw;
+        MM_HIP(hipMalloc(&epart, sizeof(double) * 4 * grid_rows));
 
         for (int b = 0; b < 2; ++b) yb[b] = dalloc(3 * n);
         for (int q = 0; q < tab.num_steps; ++q) k[q] = dalloc(3 * n);
@@ -626,9 +730,10 @@ template <typename T> struct Stepper : public StepperBase {
         allocs.clear();
         if (dbuf) (void)hipFree(dbuf);
         if (part) (void)hipFree(part);
+        if (epart) (void)hipFree(epart);
         if (ctl) (void)hipFree(ctl);
         if (s) (void)hipStreamDestroy(s);
-        dbuf = part = nullptr;
+        dbuf = part = epart = nullptr;
         ctl = nullptr;
         s = nullptr;
     }
@@ -663,9 +768,9 @@ template <typename T> struct Stepper : public StepperBase {
             else conv->forward_x(yb[0], s, yb[1], &ctl->cur);
             conv->spectral(s);
         }
-        hipLaunchKernelGGL(k_stage<T>, dim3(grid_rows), dim3(threads), stage_lds, s, sys, G, ctl,
+        hipLaunchKernelGGL((k_stage<T, false>), dim3(grid_rows), dim3(threads), stage_lds, s, sys, G, ctl,
                            conv ? conv->S1 : nullptr, conv ? conv->px.template dev<T>() : FftPlanDev(),
-                           conv ? conv->g.Hx : 0, rpw, conv ? 1 : 0);
+                           conv ? conv->g.Hx : 0, rpw, conv ? 1 : 0, nullptr);
         MM_HIP(hipGetLastError());
         if (klog) klog->mark(s, KC_STAGE, stage_bytes(G));
     }
@@ -823,6 +928,79 @@ template <typename T> struct Stepper : public StepperBase {
         if (H_host) download(H_host, Hbuf, 3 * n);
     }
 
+    // E_stray, E_exch, E_aniso, E_ext, E_tot at the current state
+    void energy(double *E5) override
+    {
+        StageDev<T> G = blank_stage(nullptr);
+        if (conv) {
+            conv->klog = nullptr;
+            conv->forward_x(yb[0], s, yb[1], &ctl->cur);
+            conv->spectral(s);
+        }
+        hipLaunchKernelGGL((k_stage<T, true>), dim3(grid_rows), dim3(threads), stage_lds, s, sys, G, ctl,
+                           conv ? conv->S1 : nullptr, conv ? conv->px.template dev<T>() : FftPlanDev(),
+                           conv ? conv->g.Hx : 0, rpw, conv ? 1 : 0, epart);
+        MM_HIP(hipGetLastError());
+        std::vector<double> h((size_t)4 * grid_rows);
+        MM_HIP(hipMemcpyAsync(h.data(), epart, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
+        MM_HIP(hipStreamSynchronize(s));
+        double sum[4];
+        for (int q = 0; q < 4; ++q) {
+            double acc = 0.0;
+            for (int b = 0; b < grid_rows; ++b) acc += h[(size_t)q * grid_rows + b];
+            sum[q] = acc;
+        }
+        // the reference's front-end evaluates these with MU0 = 4 pi 1e-7 in
+        // Python doubles (constants.py:21)
+        const double mu0 = 4.0 * 3.14159265358979323846 * 1e-7;
+        const double V = d.dx * d.dy * d.dz;
+        E5[0] = conv ? -mu0 / 2.0 * V * sum[0] : 0.0;
+        E5[1] = sys.has_exch ? -mu0 / 2.0 * V * sum[1] : 0.0;
+        E5[2] = (sys.has_uni || sys.has_cub) ? sum[2] * V : 0.0;
+        E5[3] = sys.has_zee ? -mu0 * V * sum[3] : 0.0;
+        E5[4] = E5[0] + E5[1] + E5[2] + E5[3];
+    }
+
+    // step until the relaxation rule holds on an accepted state or max_steps
+    // accepted steps were made; attempts are enqueued in batches between
+    // reads of the control block
+    void relax(int criterion, double thr, int every, long long max_steps, double *info3) override
+    {
+        MM_REQUIRE(criterion == MM_RELAX_DMDT || criterion == MM_RELAX_DEG_PER_NS, "unknown relaxation criterion");
+        MM_REQUIRE(every >= 1 && max_steps >= 0, "bad relaxation arguments");
+        StepCtl c;
+        MM_HIP(hipStreamSynchronize(s));
+        MM_HIP(hipMemcpy(&c, ctl, sizeof(c), hipMemcpyDeviceToHost));
+        const long long a0 = c.accepted;
+        c.target = c.accepted + max_steps;
+        c.stop = max_steps == 0 ? 1 : 0;
+        c.relax_mode = criterion;
+        c.relax_every = every;
+        c.relax_thr = thr;
+        c.relaxed = 0;
+        c.have_last = 0;
+        c.relax_value = 0.0;
+        c.last_check = -1;
+        MM_HIP(hipMemcpy(ctl, &c, sizeof(c), hipMemcpyHostToDevice));
+        const long long batch = 32;
+        while (!c.stop && !c.relaxed && c.accepted < c.target) {
+            const long long q = std::min(batch, c.target - c.accepted);
+            for (long long r = 0; r < q; ++r) enqueue_attempt();
+            MM_HIP(hipStreamSynchronize(s));
+            MM_HIP(hipMemcpy(&c, ctl, sizeof(c), hipMemcpyDeviceToHost));
+        }
+        info3[0] = (double)(c.accepted - a0);
+        info3[1] = c.relaxed ? 1.0 : 0.0;
+        info3[2] = c.relax_value;
+        // a discarded attempt may have overwritten the FSAL vector
+        if (c.relaxed) have_fsal = false;
+        c.relax_mode = 0;
+        c.relaxed = 0;
+        c.stop = 0;
+        c.target = c.accepted;
+        MM_HIP(hipMemcpy(ctl, &c, sizeof(c), hipMemcpyHostToDevice));
+    }
+
     void run(long long steps) override
     {
         if (steps <= 0) return;
@@ -964,6 +1142,23 @@ extern "C" int mm_stepper_kernel_profile(mm_stepper_t *st, int attempts, double 
     MM_CAPI_BEGIN
     MM_REQUIRE(st && out && attempts > 0, "bad argument");
     st->impl->profile(attempts, out);
+    MM_CAPI_END
+}
+
+extern "C" int mm_stepper_energy(mm_stepper_t *st, double *E5)
+{
+    MM_CAPI_BEGIN
+    MM_REQUIRE(st && E5, "null argument");
+    st->impl->energy(E5);
+    MM_CAPI_END
+}
+
+extern "C" int mm_stepper_relax(mm_stepper_t *st, int criterion, double threshold, int check_every,
+                                long long max_steps, double *info3)
+{
+    MM_CAPI_BEGIN
+    MM_REQUIRE(st && info3, "null argument");
+    st->impl->relax(criterion, threshold, check_every, max_steps, info3);
     MM_CAPI_END
 }
 

@@ -117,6 +117,25 @@ class Stepper:
         _lib.check(_lib.load().mm_stepper_average_m(self._h, self._dp(a)))
         return a
 
+    ENERGY_TERMS = ("E_stray", "E_exch", "E_aniso", "E_ext", "E_tot")
+
+    def energy(self):
+        """Energies of the terms at the current M, in joule (the reference's
+        E_stray, E_exch, E_aniso, E_ext and their sum E_tot)."""
+        e = np.zeros(5)
+        _lib.check(_lib.load().mm_stepper_energy(self._h, self._dp(e)))
+        return dict(zip(self.ENERGY_TERMS, e.tolist()))
+
+    def relax(self, threshold=1e7, criterion="dmdt", check_every=1, max_steps=1000000):
+        """Step until relaxed (see mm_stepper_relax).  criterion "dmdt":
+        scaled_abs_max(dM/dt, Ms) < threshold [1/s]; "deg_per_ns": the
+        reference's Relaxed(threshold, check_every) condition."""
+        crit = {"dmdt": _lib.MM_RELAX_DMDT, "deg_per_ns": _lib.MM_RELAX_DEG_PER_NS}[criterion]
+        info = np.zeros(3)
+        _lib.check(_lib.load().mm_stepper_relax(self._h, crit, float(threshold), int(check_every),
+                                                int(max_steps), self._dp(info)))
+        return dict(steps=int(info[0]), relaxed=bool(info[1]), value=float(info[2]))
+
     KERNEL_CLASSES = ("x_fwd", "y_fwd", "z_mult", "y_inv", "y_mult_2d", "x_inv_stage", "controller")
 
     def kernel_profile(self, attempts=5):

@@ -858,8 +858,42 @@ static int controller(double eps_abs, double eps_rel, double h, int order,
  * for method ORC_RK4 (runge_kutta_4.py:28-59 with stage 4 at y + h k3).
  * FSAL (dp54, rk23) keeps the reference behaviour of reusing the last stage
  * of the most recent attempt, accepted or not (runge_kutta.py:97-101,131-134). */
-void orc_run(const orc_system *s, int method, double eps_abs, double eps_rel,
-             double h0, int steps, double *M, double *stats, double *avg_out)
+/* Relaxation rule (mm_stepper_relax semantics): checked before a step whose
+ * accepted-step count is a multiple of `every`, from the k_0 the step's first
+ * attempt uses (dM/dt at M, or the carried-over FSAL vector).
+ * criterion 1: scaled_abs_max(k_0, Ms) < thr; criterion 2: Relaxed(thr)
+ * (condition.py:169-200) on deg_per_ns (landau_lifshitz_gilbert.py:127-131). */
+typedef struct {
+    int criterion, every;
+    double thr;
+    int have_last, relaxed;
+    double last, value;
+} relax_state;
+
+static int relax_test(relax_state *r, const orc_system *s, int n, const double *k0,
+                      const double *M, double h_last)
+{
+    double v;
+    int relaxed;
+    if (r->criterion == 1) {
+        v = orc_scaled_abs_max(n, k0, s->Ms, s->Ms_uniform);
+        relaxed = v < r->thr;
+    } else {
+        const double dm = orc_abs_max(n, k0), mm = orc_abs_max(n, M);
+        const double deg_per_step = (180.0 / 3.14159265358979323846) * atan2(dm * h_last, mm);
+        v = 1e-9 * deg_per_step / h_last;
+        relaxed = r->have_last && v < r->last && v <= r->thr;
+        r->last = v;
+        r->have_last = 1;
+    }
+    r->value = v;
+    if (relaxed) r->relaxed = 1;
+    return relaxed;
+}
+
+static long run_impl(const orc_system *s, int method, double eps_abs, double eps_rel,
+                     double h0, long steps, double *M, double *stats, double *avg_out,
+                     relax_state *rx)
 {
     const int n = s->nx * s->ny * s->nz;
     const long m = 3L * n;
@@ -878,10 +912,18 @@ void orc_run(const orc_system *s, int method, double eps_abs, double eps_rel,
     double tnow = 0.0, h_try = h0, h_last = h0, h_next = h0;
     long attempts = 0, rejections = 0;
 
-    for (int st = 0; st < steps; ++st) {
+    long st;
+    for (st = 0; st < steps; ++st) {
+        int k0_ready = 0;
+        if (rx && st % rx->every == 0) {
+            if (method != ORC_RK4 && t.fsal && have_fsal) memcpy(k[0], fsal, sizeof(double) * m);
+            else orc_dmdt(s, f1, f2, M, k[0], NULL);
+            k0_ready = 1;
+            if (relax_test(rx, s, n, k[0], M, h_last)) break;
+        }
         if (method == ORC_RK4) {
             const double h = h0;
-            orc_dmdt(s, f1, f2, M, k[0], NULL);
+            if (!k0_ready) orc_dmdt(s, f1, f2, M, k[0], NULL);
             for (int sg = 1; sg < 4; ++sg) {
                 orc_rk_prepare_step(sg, h, &t, k, M, ytmp, n);
                 orc_dmdt(s, f1, f2, ytmp, k[sg], NULL);
@@ -897,7 +939,8 @@ void orc_run(const orc_system *s, int method, double eps_abs, double eps_rel,
             for (;;) {
                 attempts++;
                 /* apply(state, h_try) */
-                if (t.fsal && have_fsal) memcpy(k[0], fsal, sizeof(double) * m);
+                if (k0_ready) k0_ready = 0;  /* k[0] of this state already computed */
+                else if (t.fsal && have_fsal) memcpy(k[0], fsal, sizeof(double) * m);
                 else orc_dmdt(s, f1, f2, M, k[0], NULL);
                 for (int sg = 1; sg < t.num_steps; ++sg) {
                     orc_rk_prepare_step(sg, h_try, &t, k, M, ytmp, n);
@@ -929,4 +972,64 @@ void orc_run(const orc_system *s, int method, double eps_abs, double eps_rel,
     }
     for (int i = 0; i < 7; ++i) free(k[i]);
     free(f1); free(f2); free(y0); free(ytmp); free(yerr); free(fsal);
+    return st;
+}
+
+void orc_run(const orc_system *s, int method, double eps_abs, double eps_rel,
+             double h0, int steps, double *M, double *stats, double *avg_out)
+{
+    run_impl(s, method, eps_abs, eps_rel, h0, steps, M, stats, avg_out, NULL);
+}
+
+long orc_relax(const orc_system *s, int method, double eps_abs, double eps_rel, double h0,
+               int criterion, double threshold, int check_every, long max_steps,
+               double *M, double *stats, double *info3)
+{
+    relax_state r;
+    memset(&r, 0, sizeof(r));
+    r.criterion = criterion;
+    r.every = check_every < 1 ? 1 : check_every;
+    r.thr = threshold;
+    const long done = run_impl(s, method, eps_abs, eps_rel, h0, max_steps, M, stats, NULL, &r);
+    if (info3) {
+        info3[0] = (double)done;
+        info3[1] = (double)r.relaxed;
+        info3[2] = r.value;
+    }
+    return done;
+}
+
+/* Term energies at M: stray_field.py:51, exchange_field.py:58,
+ * anisotropy_field.py:84-87 (uniaxial + cubic summed), external_field.py:33-35,
+ * E_tot landau_lifshitz_gilbert.py:103-106; MU0 as the Python front-end's
+ * 4.0*math.pi*1e-7 (constants.py:21). */
+void orc_energy(const orc_system *s, const double *M, double *E5)
+{
+    const int n = s->nx * s->ny * s->nz;
+    const long m = 3L * n;
+    const double mu0 = 4.0 * 3.14159265358979323846 * 1e-7;
+    const double V = s->dx * s->dy * s->dz;
+    double *tmp = (double *)malloc(sizeof(double) * m);
+    for (int q = 0; q < 5; ++q) E5[q] = 0.0;
+    if (s->conv) {
+        orc_conv_execute(s->conv, M, tmp);
+        E5[0] = -mu0 / 2.0 * V * orc_dot_sum(n, M, tmp);
+    }
+    if (s->use_exchange) {
+        orc_exchange(s->nx, s->ny, s->nz, s->dx, s->dy, s->dz, s->px, s->py, s->pz, s->Ms, s->A, M, tmp);
+        E5[1] = -mu0 / 2.0 * V * orc_dot_sum(n, M, tmp);
+    }
+    if (s->k_uni || s->k_cub) {
+        double e = 0.0;
+        if (s->k_uni) e += orc_uniaxial_anisotropy(n, s->axis1, s->k_uni, s->Ms, M, tmp);
+        if (s->k_cub) e += orc_cubic_anisotropy(n, s->axis1, s->axis2, s->k_cub, s->Ms, M, tmp);
+        E5[2] = e * V;
+    }
+    if (s->use_zeeman) {
+        for (int c = 0; c < 3; ++c)
+            for (int i = 0; i < n; ++i) tmp[c * (long)n + i] = s->h_ext[c];
+        E5[3] = -mu0 * V * orc_dot_sum(n, M, tmp);
+    }
+    E5[4] = E5[0] + E5[1] + E5[2] + E5[3];
+    free(tmp);
 }

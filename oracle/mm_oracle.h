@@ -104,6 +104,17 @@ void orc_run(const orc_system *s, int method, double eps_abs, double eps_rel,
              double h0, int steps, double *M, double *stats,
              double *avg_out /* 3*steps or NULL: <M> after each accepted step */);
 
+/* Run until the relaxation rule holds or max_steps accepted steps (the rule
+ * of mm_stepper_relax: criterion 1 scaled_abs_max(dM/dt, Ms) < threshold,
+ * 2 the reference's Relaxed condition on deg_per_ns).  info3: [0] steps
+ * made, [1] relaxed, [2] last checked value.  Returns the steps made. */
+long orc_relax(const orc_system *s, int method, double eps_abs, double eps_rel, double h0,
+               int criterion, double threshold, int check_every, long max_steps,
+               double *M, double *stats, double *info3);
+
+/* E_stray, E_exch, E_aniso, E_ext, E_tot at M [J]. */
+void orc_energy(const orc_system *s, const double *M, double *E5);
+
 #ifdef __cplusplus
 }
 #endif

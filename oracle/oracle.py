@@ -287,3 +287,30 @@ class System:
         if avg is not None:
             out["avg"] = avg
         return M, out
+
+    ENERGY_TERMS = ("E_stray", "E_exch", "E_aniso", "E_ext", "E_tot")
+
+    def energy(self, M):
+        """E_stray, E_exch, E_aniso, E_ext, E_tot at M (joule)."""
+        M = _c(M)
+        e = np.zeros(5)
+        lib().orc_energy(ctypes.byref(self.s), _p(M), _p(e))
+        return dict(zip(self.ENERGY_TERMS, e.tolist()))
+
+    def relax(self, M, method="rkf45", threshold=1e7, criterion="dmdt", check_every=1,
+              max_steps=100000, eps_abs=1e-4, eps_rel=1e-4, h0=1e-13):
+        """Run until relaxed (criterion "dmdt": scaled_abs_max(dM/dt, Ms) <
+        threshold; "deg_per_ns": the reference's Relaxed condition)."""
+        M = _c(M).copy()
+        stats = np.zeros(5)
+        info = np.zeros(3)
+        crit = {"dmdt": 1, "deg_per_ns": 2}[criterion]
+        L = lib()
+        L.orc_relax.restype = ctypes.c_long
+        L.orc_relax(ctypes.byref(self.s), METHODS[method], ctypes.c_double(eps_abs),
+                    ctypes.c_double(eps_rel), ctypes.c_double(h0), crit, ctypes.c_double(threshold),
+                    int(check_every), ctypes.c_long(int(max_steps)), _p(M), _p(stats), _p(info))
+        out = dict(t=stats[0], h=stats[1], h_next=stats[2], attempts=int(stats[3]),
+                   rejections=int(stats[4]), steps=int(info[0]), relaxed=bool(info[1]),
+                   value=float(info[2]))
+        return M, out

@@ -421,3 +421,46 @@ def test_stepper_vortex_spin_torque_dp54_fp32_tracks_fp64():
         out[prec] = st.get_m()
         st.close()
     assert np.abs(out["fp32"] - out["fp64"]).max() / 8.6e5 < 1e-3
+
+
+# ---------------------------------------------------------------- energies and relaxation
+@pytest.mark.parametrize("shape,stt,cub", [((24, 20, 4), False, False), ((16, 12, 5), True, True)])
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 1e-5)])
+def test_stepper_energy_vs_oracle(shape, stt, cub, prec, tol):
+    from micromagnum_amd.solver import Stepper
+    kw = _system(shape, stt, cub)
+    delta = (2e-9, 2e-9, 2e-9)
+    st = Stepper(shape, delta, precision=prec, **kw)
+    M = random_unit_field(shape, 21) * 1.1e6
+    st.set_m(M)
+    E = st.energy()
+    Eo = O.System(shape, delta, N=st.N, **kw).energy(M)
+    for key in ("E_stray", "E_exch", "E_aniso", "E_ext", "E_tot"):
+        assert abs(E[key] - Eo[key]) <= tol * max(abs(Eo[key]), 1e-30), (key, E[key], Eo[key])
+    st.close()
+
+
+@pytest.mark.parametrize("criterion,thr,every", [("dmdt", 3e9, 1), ("deg_per_ns", 2e4, 3)])
+@pytest.mark.parametrize("method", ["rkf45", "dp54"])
+def test_stepper_relax_vs_oracle(criterion, thr, every, method):
+    # cubic film (Ms 8e5, A 1.3e-11, K1 -5e3 along x,y, alpha 1) relaxed from a
+    # random state: same stop step, same state, same checked value
+    from micromagnum_amd.solver import Stepper
+    shape, delta = (24, 16, 2), (3e-9, 3e-9, 3e-9)
+    kw = dict(Ms=8e5, alpha=1.0, A=1.3e-11, k_cub=-5e3, axis1=(1.0, 0.0, 0.0), axis2=(0.0, 1.0, 0.0))
+    st = Stepper(shape, delta, precision="fp64", method=method, **kw)
+    M = random_unit_field(shape, 31) * 8e5
+    st.set_m(M)
+    info = st.relax(threshold=thr, criterion=criterion, check_every=every, max_steps=400)
+    Mg = st.get_m()
+    Mo, io = O.System(shape, delta, N=st.N, **kw).relax(M, method=method, threshold=thr, criterion=criterion,
+                                                         check_every=every, max_steps=400)
+    assert io["relaxed"] and info["relaxed"]
+    assert info["steps"] == io["steps"]
+    assert abs(info["value"] - io["value"]) <= 1e-10 * io["value"]
+    assert np.abs(Mg - Mo).max() / 8e5 < 1e-10
+    # stepping on after a relaxation continues from the kept state
+    a0 = st.stats()["attempts"]
+    st.run(2)
+    assert st.stats()["attempts"] >= a0 + 2 and np.isfinite(st.get_m()).all()
+    st.close()
