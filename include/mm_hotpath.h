@@ -69,11 +69,15 @@ int mm_device_copy(void *dst, const void *src, size_t bytes, void *stream);
 /* ------------------------------------------------------- demag tensor setup */
 /* round_tensor_dimension (src/magneto/mmm/demag/tensor_round.cpp:90-106) */
 int mm_round_tensor_dimension(int dim, int periodic, int padding, int *out);
-/* GenerateDemagTensor (src/magneto/mmm/demag/demag_tensor.cpp:116-186), open
- * boundaries.  exp_out receives (ex, ey, ez).  N_host (may be NULL to query the
- * size) receives 6*ex*ey*ez doubles, component fastest (xx,xy,xz,yy,yz,zz),
- * already sign-flipped so that H = N (*) M.  Periodic axes return
- * MM_ERR_UNSUPPORTED.  nthreads <= 0: one per host core. */
+/* GenerateDemagTensor (src/magneto/mmm/demag/demag_tensor.cpp:116-186).
+ * exp_out receives (ex, ey, ez): padded per `padding` along open axes, the
+ * mesh size along periodic ones.  N_host (may be NULL to query the size)
+ * receives 6*ex*ey*ez doubles, component fastest (xx,xy,xz,yy,yz,zz),
+ * already sign-flipped so that H = N (*) M.  Periodic axes sum `repeat`
+ * images per axis cell by cell plus the far-image (infinity) correction of
+ * tensor.cpp:66-160, which the environment variable
+ * MAGNUM_DEMAG_NO_INFINITY_CORRECTION switches off as in the reference.
+ * nthreads <= 0: one per host core. */
 int mm_generate_demag_tensor(int nx, int ny, int nz, double dx, double dy, double dz,
                              int px, int py, int pz, int repeat, int padding,
                              int nthreads, double *N_host, int *exp_out);
@@ -175,6 +179,8 @@ typedef struct {
     double h_ext[3];        /* uniform external field [A/m]                       */
     const double *N_host;   /* demag tensor (6*ex*ey*ez) or NULL = no stray field */
     int ex, ey, ez;
+    int px, py, pz;         /* periodic axes (exchange neighbours wrap; the tensor
+                               must be generated with the same flags)           */
 } mm_system_desc_t;
 
 typedef struct mm_stepper mm_stepper_t;

@@ -46,7 +46,8 @@ class SystemDesc(ctypes.Structure):
                 ("k_uni_host", _dp), ("k_cub_host", _dp), ("axis1_host", _dp), ("axis2_host", _dp),
                 ("P_host", _dp), ("xi_host", _dp), ("j_host", _dp),
                 ("use_zeeman", _i), ("h_ext", _d * 3),
-                ("N_host", _dp), ("ex", _i), ("ey", _i), ("ez", _i)]
+                ("N_host", _dp), ("ex", _i), ("ey", _i), ("ez", _i),
+                ("px", _i), ("py", _i), ("pz", _i)]
 
 
 # name -> (restype, argtypes); every symbol of include/mm_hotpath.h

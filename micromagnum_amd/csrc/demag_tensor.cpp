@@ -28,6 +28,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <functional>
 #include <stdexcept>
 #include <thread>
 #include <vector>
@@ -36,11 +37,14 @@ namespace {
 
 typedef long double ld;
 
-static const ld PI_L = 3.141592653589793238462643383279502884197169399L;
+// pi as the reference declares it (src/magneto/mmm/constants.h:26): a double
+// literal stored in a long double, i.e. pi rounded to 53 bits
+static const ld PI_L = (ld)3.141592653589793238462643383279502884197169399;
 static const double EXPANSION_RADIUS = 30.0;
 static const double CONE_RATIO = 6.0;
 
 #include "newell_table.inc"
+#include "pbc_table.inc"
 
 // ---------------------------------------------------------------------------
 // padding policy (tensor_round.cpp:27-106)
@@ -286,43 +290,132 @@ int mm_round_tensor_dimension_impl(int dim, bool periodic, int padding)
     }
 }
 
-void mm_demag_tensor_impl(int nx, int ny, int nz, double dx, double dy, double dz,
-                          int ex, int ey, int ez, double *N, int nthreads)
+namespace {
+
+// Far-image sums of the point-dipole tensor (pbc_table.inc) for one periodic
+// axis p (1-D) or two periodic axes p, s (2-D), the reference's PBC_Demag_1D /
+// PBC_Demag_2D (tensor_integrals.h:464-519) called as in tensor.cpp:66-160:
+// the periodic axes come first, `ord` holds the derivative order of the
+// component along each of the three axes of that order.
+static ld pbc1(const int ord[3], const ld l[3], const long idx[3], int np)
 {
-    if (ex < 2 * nx - 1 || ey < 2 * ny - 1 || ez < 2 * nz - 1)
-        throw std::runtime_error("demag tensor: padded size too small for open boundaries");
+    const ld L = l[0] * (ld)np;
+    const ld x = (ld)idx[0] * l[0] - 0.5L * L, y = (ld)idx[1] * l[1], z = (ld)idx[2] * l[2];
+    return pbc1_sorted(ord[0], ord[1], ord[2], x, y, z, L);
+}
+
+static ld pbc2(const int ord[3], const ld l[3], const long idx[3], int n0, int n1)
+{
+    const ld L0 = l[0] * (ld)n0, L1 = l[1] * (ld)n1;
+    const ld x = (ld)idx[0] * l[0] - 0.5L * L0, y = (ld)idx[1] * l[1] - 0.5L * L1, z = (ld)idx[2] * l[2];
+    return pbc2_sorted(ord[0], ord[1], ord[2], x, y, z, L0, L1);
+}
+
+} // namespace
+
+void mm_demag_tensor_impl(int nx, int ny, int nz, double dx, double dy, double dz,
+                          int ex, int ey, int ez, double *N, int nthreads,
+                          int px, int py, int pz, int repeat)
+{
+    const int n3[3] = {nx, ny, nz}, e3[3] = {ex, ey, ez}, per[3] = {px != 0, py != 0, pz != 0};
+    for (int a = 0; a < 3; ++a) {
+        if (per[a] ? e3[a] != n3[a] : e3[a] < 2 * n3[a] - 1)
+            throw std::runtime_error(per[a] ? "demag tensor: a periodic axis needs the unpadded size"
+                                            : "demag tensor: padded size too small for open boundaries");
+    }
+    if ((px || py || pz) && repeat < 1) throw std::runtime_error("demag tensor: periodic repeat must be >= 1");
+    // images summed cell by cell per axis (demag_tensor.cpp:155-157)
+    const int rep3[3] = {px ? repeat : 1, py ? repeat : 1, pz ? repeat : 1};
+    const bool no_inf = std::getenv("MAGNUM_DEMAG_NO_INFINITY_CORRECTION") != nullptr;
     const size_t tot = (size_t)6 * ex * ey * ez;
     std::fill(N, N + tot, 0.0);
-    const ld lx = dx, ly = dy, lz = dz;
+    const ld l3[3] = {(ld)dx, (ld)dy, (ld)dz};
+    const ld lx = l3[0], ly = l3[1], lz = l3[2];
+    // component e = (xx, xy, xz, yy, yz, zz) -> its two axes
+    static const int CA[6] = {0, 0, 0, 1, 1, 2}, CB[6] = {0, 1, 2, 1, 2, 2};
+    const double scale = (double)(-lx * ly * lz / (4.0L * PI_L));
 
-    auto work = [&](int i) {
-        for (int j = 0; j < ny; ++j)
-            for (int k = 0; k < nz; ++k) {
-                int mcs = 1;
-                if (i == 0) mcs *= 2;
-                if (j == 0) mcs *= 2;
-                if (k == 0) mcs *= 2;
-                // the reference accumulates -I_T/mcs and flips the sign of the
-                // whole field afterwards; both negations are exact.
-                const double Nxx = (double)(-cell_integral(2, 0, 0, lx, ly, lz, i, j, k) / (ld)mcs);
-                const double Nyy = (double)(-cell_integral(0, 2, 0, lx, ly, lz, i, j, k) / (ld)mcs);
-                const double Nzz = (double)(-cell_integral(0, 0, 2, lx, ly, lz, i, j, k) / (ld)mcs);
-                const double Nxy = (double)(-cell_integral(1, 1, 0, lx, ly, lz, i, j, k) / (ld)mcs);
-                const double Nyz = (double)(-cell_integral(0, 1, 1, lx, ly, lz, i, j, k) / (ld)mcs);
-                const double Nxz = (double)(-cell_integral(1, 0, 1, lx, ly, lz, i, j, k) / (ld)mcs);
-                for (int o = -1; o <= 1; o += 2)
-                    for (int p = -1; p <= 1; p += 2)
-                        for (int q = -1; q <= 1; q += 2) {
-                            const int I = (o * i + ex) % ex, J = (p * j + ey) % ey, K = (q * k + ez) % ez;
-                            double *e = N + 6 * ((size_t)I + (size_t)ex * ((size_t)J + (size_t)ey * K));
-                            e[0] += Nxx;
-                            e[1] += o * p * Nxy;
-                            e[2] += o * q * Nxz;
-                            e[3] += Nyy;
-                            e[4] += p * q * Nyz;
-                            e[5] += Nzz;
+    // the six entries for offset (i,j,k) before mirroring, as the reference
+    // accumulates them: double sums, each term rounded once
+    auto entry = [&](int i, int j, int k, double *v) {
+        double acc[6] = {0, 0, 0, 0, 0, 0};
+        for (int rx = 0; rx < rep3[0]; ++rx)
+            for (int ry = 0; ry < rep3[1]; ++ry)
+                for (int rz = 0; rz < rep3[2]; ++rz) {
+                    const long I = i + (long)rx * nx, J = j + (long)ry * ny, K = k + (long)rz * nz;
+                    int mcs = 1;
+                    if (I == 0) mcs *= 2;
+                    if (J == 0) mcs *= 2;
+                    if (K == 0) mcs *= 2;
+                    for (int e = 0; e < 6; ++e) {
+                        int o[3] = {0, 0, 0};
+                        ++o[CA[e]];
+                        ++o[CB[e]];
+                        acc[e] = (double)((ld)acc[e] - cell_integral(o[0], o[1], o[2], lx, ly, lz, I, J, K) / (ld)mcs);
+                    }
+                }
+        if (!no_inf) {
+            const int idx0[3] = {i, j, k};
+            // one periodic axis p: tails beyond the repeat zone for every image
+            // of the other axes inside theirs (tensor.cpp:66-118)
+            for (int p = 0; p < 3; ++p) {
+                if (rep3[p] <= 1) continue;
+                const int s = p == 0 ? 1 : 0, t = p == 2 ? 1 : 2;
+                const ld l[3] = {l3[p], l3[s], l3[t]};
+                for (int rs = 0; rs < rep3[s]; ++rs)
+                    for (int rt = 0; rt < rep3[t]; ++rt) {
+                        const long idx[3] = {idx0[p] + (long)rep3[p] * n3[p], idx0[s] + (long)rs * n3[s],
+                                             idx0[t] + (long)rt * n3[t]};
+                        int mcs = 1;
+                        if (idx[1] == 0) mcs *= 2;
+                        if (idx[2] == 0) mcs *= 2;
+                        for (int e = 0; e < 6; ++e) {
+                            int oa[3] = {0, 0, 0};
+                            ++oa[CA[e]];
+                            ++oa[CB[e]];
+                            const int ord[3] = {oa[p], oa[s], oa[t]};
+                            acc[e] += scale * (double)pbc1(ord, l, idx, n3[p]) / mcs;
                         }
+                    }
             }
+            // two periodic axes p < s: quadrant tails (tensor.cpp:120-160)
+            for (int p = 0; p < 3; ++p)
+                for (int s = p + 1; s < 3; ++s) {
+                    if (rep3[p] <= 1 || rep3[s] <= 1) continue;
+                    const int t = 3 - p - s;
+                    const ld l[3] = {l3[p], l3[s], l3[t]};
+                    for (int rt = 0; rt < rep3[t]; ++rt) {
+                        const long idx[3] = {idx0[p] + (long)rep3[p] * n3[p], idx0[s] + (long)rep3[s] * n3[s],
+                                             idx0[t] + (long)rt * n3[t]};
+                        const int mcs = idx[2] == 0 ? 2 : 1;
+                        for (int e = 0; e < 6; ++e) {
+                            int oa[3] = {0, 0, 0};
+                            ++oa[CA[e]];
+                            ++oa[CB[e]];
+                            const int ord[3] = {oa[p], oa[s], oa[t]};
+                            acc[e] += scale * (double)pbc2(ord, l, idx, n3[p], n3[s]) / mcs;
+                        }
+                    }
+                }
+        }
+        for (int e = 0; e < 6; ++e) v[e] = acc[e];
+    };
+
+    // mirror into the 8 octants with the sign pattern of each component
+    // (tensor.cpp:167-180); along a periodic axis -i wraps to n - i
+    auto fold = [&](int i, int j, int k, const double *v) {
+        for (int o = -1; o <= 1; o += 2)
+            for (int p = -1; p <= 1; p += 2)
+                for (int q = -1; q <= 1; q += 2) {
+                    const int I = (o * i + ex) % ex, J = (p * j + ey) % ey, K = (q * k + ez) % ez;
+                    double *e = N + 6 * ((size_t)I + (size_t)ex * ((size_t)J + (size_t)ey * K));
+                    e[0] += v[0];
+                    e[1] += o * p * v[1];
+                    e[2] += o * q * v[2];
+                    e[3] += v[3];
+                    e[4] += p * q * v[4];
+                    e[5] += v[5];
+                }
     };
 
     if (nthreads <= 0) {
@@ -330,18 +423,44 @@ void mm_demag_tensor_impl(int nx, int ny, int nz, double dx, double dy, double d
         nthreads = env ? std::atoi(env) : (int)std::thread::hardware_concurrency();
         if (nthreads <= 0) nthreads = 1;
     }
-    nthreads = std::min(nthreads, nx);
-    if (nthreads <= 1) {
-        for (int i = 0; i < nx; ++i) work(i);
-    } else {
-        // distinct i map to distinct x-planes of the padded grid (ex >= 2nx-1),
-        // so threads never write the same entry.
+    nthreads = std::max(1, std::min(nthreads, nx));
+    auto parallel_over_i = [&](const std::function<void(int)> &f) {
+        if (nthreads <= 1) {
+            for (int i = 0; i < nx; ++i) f(i);
+            return;
+        }
         std::vector<std::thread> pool;
         for (int t = 0; t < nthreads; ++t)
             pool.emplace_back([&, t]() {
-                for (int i = t; i < nx; i += nthreads) work(i);
+                for (int i = t; i < nx; i += nthreads) f(i);
             });
         for (auto &th : pool) th.join();
+    };
+
+    if (!px) {
+        // distinct i map to distinct x-planes of the padded grid (ex >= 2nx-1),
+        // so threads never write the same entry and every entry sees its
+        // terms in the reference's order
+        parallel_over_i([&](int i) {
+            double v[6];
+            for (int j = 0; j < ny; ++j)
+                for (int k = 0; k < nz; ++k) {
+                    entry(i, j, k, v);
+                    fold(i, j, k, v);
+                }
+        });
+    } else {
+        // i and nx - i share a plane: compute the entries in parallel, then
+        // mirror them in the reference's loop order
+        const size_t n = (size_t)nx * ny * nz;
+        std::vector<double> vals(6 * n);
+        parallel_over_i([&](int i) {
+            for (int j = 0; j < ny; ++j)
+                for (int k = 0; k < nz; ++k) entry(i, j, k, &vals[6 * ((size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k))]);
+        });
+        for (int i = 0; i < nx; ++i)
+            for (int j = 0; j < ny; ++j)
+                for (int k = 0; k < nz; ++k) fold(i, j, k, &vals[6 * ((size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k))]);
     }
     for (size_t q = 0; q < tot; ++q) N[q] = -N[q];
 }

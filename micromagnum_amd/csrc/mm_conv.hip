@@ -85,10 +85,17 @@ bool mm_is_smooth(int n)
 // Any padded size P >= 2n-1 gives the same linear convolution for open
 // boundaries (the tensor is zero between the +/- offsets), so the given size
 // is kept if it is radix-friendly and even, else the next such size is used.
+// A periodic axis (given == n) is a circular convolution of period n and
+// keeps its size, which must then be radix-friendly itself.
 int mm_fft_size_for(int n, int given)
 {
     if (n == 1) return 1;
-    if (given >= 2 * n - 1 && given % 2 == 0 && mm_is_smooth(given)) return given;
+    if (given < 2 * n - 1) {
+        MM_REQUIRE(given == n, "tensor size along an axis must be the mesh size (periodic) or >= 2n-1 (open)");
+        MM_REQUIRE(mm_is_smooth(n), "a periodic axis needs a mesh size of the form 2^a 3^b 5^c");
+        return n;
+    }
+    if (given % 2 == 0 && mm_is_smooth(given)) return given;
     int p = 2 * n - 1;
     while (!(p % 2 == 0 && mm_is_smooth(p))) ++p;
     return p;
@@ -411,8 +418,6 @@ template <typename T>
 void ConvImpl<T>::init(const double *N_host, int nx, int ny, int nz, int ex, int ey, int ez)
 {
     MM_REQUIRE(nx >= 1 && ny >= 1 && nz >= 1, "bad mesh size");
-    MM_REQUIRE(ex >= 2 * nx - 1 && ey >= 2 * ny - 1 && ez >= 2 * nz - 1,
-               "tensor padding too small for open boundaries (periodic axes are not supported)");
     g.nx = nx; g.ny = ny; g.nz = nz;
     g.Px = mm_fft_size_for(nx, ex);
     g.Py = mm_fft_size_for(ny, ey);

@@ -43,7 +43,8 @@ void mm_set_error(const std::string &msg);
 // host-side tensor generator (demag_tensor.cpp)
 int mm_round_tensor_dimension_impl(int dim, bool periodic, int padding);
 void mm_demag_tensor_impl(int nx, int ny, int nz, double dx, double dy, double dz,
-                          int ex, int ey, int ez, double *N, int nthreads);
+                          int ex, int ey, int ez, double *N, int nthreads,
+                          int px, int py, int pz, int repeat);
 
 // constants (src/magneto/mmm/constants.h:26-40)
 static const long double MM_PI_L = 3.141592653589793238462643383279502884197169399L;

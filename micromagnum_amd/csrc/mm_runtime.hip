@@ -101,18 +101,16 @@ extern "C" int mm_generate_demag_tensor(int nx, int ny, int nz, double dx, doubl
     MM_CAPI_BEGIN
     MM_REQUIRE(nx >= 1 && ny >= 1 && nz >= 1, "bad mesh size");
     MM_REQUIRE(dx > 0 && dy > 0 && dz > 0, "cell sizes must be positive");
-    (void)repeat;
-    if (px || py || pz)
-        throw mm_error(MM_ERR_UNSUPPORTED, "periodic demag tensor fields are not implemented");
-    const int ex = mm_round_tensor_dimension_impl(nx, false, padding);
-    const int ey = mm_round_tensor_dimension_impl(ny, false, padding);
-    const int ez = mm_round_tensor_dimension_impl(nz, false, padding);
+    MM_REQUIRE(!(px || py || pz) || repeat >= 1, "periodic_repeat must be >= 1");
+    const int ex = mm_round_tensor_dimension_impl(nx, px != 0, padding);
+    const int ey = mm_round_tensor_dimension_impl(ny, py != 0, padding);
+    const int ez = mm_round_tensor_dimension_impl(nz, pz != 0, padding);
     if (exp_out) {
         exp_out[0] = ex;
         exp_out[1] = ey;
         exp_out[2] = ez;
     }
-    if (N_host) mm_demag_tensor_impl(nx, ny, nz, dx, dy, dz, ex, ey, ez, N_host, nthreads);
+    if (N_host) mm_demag_tensor_impl(nx, ny, nz, dx, dy, dz, ex, ey, ez, N_host, nthreads, px, py, pz, repeat);
     MM_CAPI_END
 }
 

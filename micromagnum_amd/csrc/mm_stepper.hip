@@ -632,6 +632,9 @@ template <typename T> struct Stepper : public StepperBase {
         sys.m.nx = d.nx;
         sys.m.ny = d.ny;
         sys.m.nz = d.nz;
+        sys.m.px = d.px != 0;
+        sys.m.py = d.py != 0;
+        sys.m.pz = d.pz != 0;
         sys.n = n;
         sys.dx = (T)d.dx;
         sys.dy = (T)d.dy;

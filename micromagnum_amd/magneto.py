@@ -149,8 +149,9 @@ def round_tensor_dimension(dim, periodic, padding):
 
 def GenerateDemagTensor(nx, ny, nz, dx, dy, dz, px=False, py=False, pz=False, repeat=1,
                         padding=PADDING_ROUND_4, cache_dir=None, nthreads=0):
-    """Demag tensor field (demag.i:36-42; demag_tensor.cpp:116-186), open
-    boundaries.  Returns a float64 numpy array of shape (ez, ey, ex, 6) whose
+    """Demag tensor field (demag.i:36-42; demag_tensor.cpp:116-186).  Periodic
+    axes keep the mesh size and sum `repeat` images plus the far-image
+    correction.  Returns a float64 numpy array of shape (ez, ey, ex, 6) whose
     memory order is the reference's Shape(6, ex, ey, ez)."""
     L = _lib.load()
     ex = (ctypes.c_int * 3)()

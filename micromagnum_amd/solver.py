@@ -30,7 +30,7 @@ class Stepper:
     def __init__(self, shape, delta, Ms, alpha, A=None, demag=True, N=None, k_uni=None, k_cub=None,
                  axis1=None, axis2=None, h_ext=None, P=None, xi=None, j=None, do_precess=True,
                  method="rkf45", eps_abs=1e-4, eps_rel=1e-4, h0=1e-13, precision="fp64",
-                 normalize_mode=0, padding=PADDING_ROUND_4):
+                 normalize_mode=0, padding=PADDING_ROUND_4, pbc=(False, False, False), periodic_repeat=None):
         L = _lib.device_lib()
         nx, ny, nz = shape
         self.shape = shape
@@ -67,9 +67,15 @@ class Stepper:
             d.use_zeeman = 1
             for c in range(3):
                 d.h_ext[c] = float(h_ext[c])
+        d.px, d.py, d.pz = (int(bool(b)) for b in pbc)
+        if periodic_repeat is None:
+            # RectangularMesh defaults (src/magnum/mesh/rectangular_mesh.py:27-31)
+            periodic_repeat = {0: 1, 1: 15, 2: 3, 3: 2}[sum(1 for b in pbc if b)]
+        self.pbc = tuple(bool(b) for b in pbc)
         if demag:
             if N is None:
-                N = GenerateDemagTensor(nx, ny, nz, *self.delta, padding=padding)
+                N = GenerateDemagTensor(nx, ny, nz, *self.delta, *self.pbc, repeat=periodic_repeat,
+                                        padding=padding)
             N = np.ascontiguousarray(N, dtype=np.float64)
             keep.append(N)
             d.N_host = N.ctypes.data_as(ctypes.POINTER(ctypes.c_double))

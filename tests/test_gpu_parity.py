@@ -464,3 +464,52 @@ def test_stepper_relax_vs_oracle(criterion, thr, every, method):
     st.run(2)
     assert st.stats()["attempts"] >= a0 + 2 and np.isfinite(st.get_m()).all()
     st.close()
+
+
+# ---------------------------------------------------------------- periodic boundaries
+@pytest.mark.parametrize("shape,pbc", [((16, 12, 3), (True, False, False)), ((20, 15, 1), (True, True, False)),
+                                       ((8, 9, 10), (True, True, True)), ((15, 8, 4), (False, False, True)),
+                                       ((6, 5, 1), (False, True, False))])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_conv_periodic_vs_oracle(mg, shape, pbc, precision):
+    # periodic axes: circular convolution of period n (odd and even n)
+    nx, ny, nz = shape
+    N = mg.GenerateDemagTensor(nx, ny, nz, 2e-9, 3e-9, 2.5e-9, *pbc, repeat=2)
+    M = random_unit_field(shape, 41) * 8e5
+    conv = mg.SymmetricMatrixVectorConvolution_FFT(N, nx, ny, nz, precision=precision)
+    assert tuple(conv.fft_size[a] for a in range(3) if pbc[a]) == tuple(shape[a] for a in range(3) if pbc[a])
+    dt = F64 if precision == "fp64" else F32
+    H = like(dev(M, dt))
+    conv.execute(dev(M, dt), H)
+    Ho = O.Conv(N, nx, ny, nz).execute(M)
+    if nx * ny * nz <= 800:
+        assert rel_err(Ho, O.conv_simple(N, M)) < 1e-13
+    tol = TOL64 if precision == "fp64" else 1e-5
+    assert rel_err(host(H), Ho) < tol, rel_err(host(H), Ho)
+
+
+def test_conv_periodic_rejects_non_smooth_axis(mg):
+    N = mg.GenerateDemagTensor(7, 4, 1, 2e-9, 2e-9, 2e-9, True, False, False, repeat=1)
+    with pytest.raises(RuntimeError):
+        mg.SymmetricMatrixVectorConvolution_FFT(N, 7, 4, 1)
+
+
+@pytest.mark.parametrize("pbc,method", [((True, False, False), "rkf45"), ((True, True, False), "dp54")])
+def test_stepper_periodic_vs_oracle(pbc, method):
+    # periodic exchange neighbours + periodic demag tensor in the fused stepper
+    from micromagnum_amd.solver import Stepper
+    shape, delta = (16, 10, 2), (3e-9, 3e-9, 2e-9)
+    kw = dict(Ms=8e5, alpha=0.3, A=1.3e-11, k_uni=2e5, axis1=(0.0, 0.0, 1.0), h_ext=(1e4, 0.0, 0.0))
+    st = Stepper(shape, delta, precision="fp64", method=method, pbc=pbc, periodic_repeat=2, **kw)
+    M = random_unit_field(shape, 43) * 8e5
+    st.set_m(M)
+    dM, H = st.dmdt()
+    sysO = O.System(shape, delta, N=st.N, pbc=pbc, **kw)
+    dMo, Ho = sysO.dmdt(M)
+    assert rel_err(H, Ho) < TOL64 and rel_err(dM, dMo) < TOL64
+    st.run(15)
+    Mo, so = sysO.run(M, method=method, steps=15)
+    s = st.stats()
+    assert s["attempts"] == so["attempts"] and s["rejections"] == so["rejections"]
+    assert np.abs(st.get_m() - Mo).max() / 8e5 < 1e-10
+    st.close()
