@@ -192,6 +192,12 @@ int mm_stepper_dmdt(mm_stepper_t *st, double *dMdt_host, double *Htot_host);
 /* Run `steps` accepted steps (adaptive methods retry rejected attempts on
  * the device).  Returns when they are done. */
 int mm_stepper_run(mm_stepper_t *st, int steps);
+/* Run until t reaches t_end or max_steps accepted steps were made, with the
+ * reference's overshoot rule (runge_kutta.py:74-78): an accepted step that
+ * would pass t_end is redone from the same state with h = t_end - t and
+ * taken without error check; the controller's step size resumes after it.
+ * This is what solve(Time(t_end)) does step by step. */
+int mm_stepper_run_until(mm_stepper_t *st, double t_end, long long max_steps);
 int mm_stepper_sync(mm_stepper_t *st);
 /* stats: [0]=t [1]=h of last accepted step [2]=next h [3]=attempts [4]=rejections */
 int mm_stepper_stats(mm_stepper_t *st, double *stats5);

@@ -50,6 +50,11 @@ struct StepCtl {
     int have_last;                 // relax_last holds a previous deg/ns value
     double relax_thr, relax_value, relax_last;
     long long last_check;          // accepted count of the last check
+    // end time (RungeKutta.evolve's t_max, runge_kutta.py:74-78): an accepted
+    // step past t_max is redone from the same state with h = t_max - t and
+    // taken without error check (forced); the controller's h resumes after it
+    int tmax_on, forced;
+    double t_max, h_saved;
 };
 
 // a per-cell parameter that is either uniform (p == null) or an array
@@ -403,9 +408,19 @@ __device__ bool relax_check(StepCtl *ctl, double dmax_scaled)
     return relaxed;
 }
 
+__device__ __forceinline__ void accept_step(StepCtl *ctl, double h)
+{
+    ctl->t += h;
+    ctl->h_last = h;
+    ctl->cur = 1 - ctl->cur;
+    ctl->accepted += 1;
+    if (ctl->accepted >= ctl->target || (ctl->tmax_on && ctl->t >= ctl->t_max)) ctl->stop = 1;
+}
+
 // MicroMagneticsStepSizeController.adjust_stepsize
 // (micro_magnetics_stepsize_controller.py:35-73) + RungeKutta.evolve retry
-// (runge_kutta.py:56-72) + state update (:80-86).
+// and overshoot handling (runge_kutta.py:56-86).  Non-adaptive (rk4): every
+// attempt is accepted with the fixed h.
 __global__ void k_controller(StepCtl *ctl, int order, double eps_abs, double eps_rel, int Ms_uniform, double Ms_u,
                              int adaptive)
 {
@@ -421,42 +436,43 @@ __global__ void k_controller(StepCtl *ctl, int order, double eps_abs, double eps
     ctl->mraw_bits = 0ull;
     if (relaxed) return;
     ctl->attempts += 1;
-    if (!adaptive) {
-        ctl->t += h;
-        ctl->h_last = h;
-        ctl->cur = 1 - ctl->cur;
-        ctl->accepted += 1;
-        if (ctl->accepted >= ctl->target) ctl->stop = 1;
+    if (ctl->forced) {
+        ctl->forced = 0;
+        accept_step(ctl, h);
+        ctl->h = ctl->h_saved;
         return;
     }
-    double e_abs = sqrt(__longlong_as_double((long long)ebits));
-    if (Ms_uniform) e_abs /= Ms_u;
-    const double HEADROOM = 0.85, MINS = 0.2, MAXS = 5.0, MINH = 1e-15, MAXH = 1e-10;
-    const double den = h * dmax;
-    int accept;
-    double hn;
-    if (den == 0.0) {
-        accept = 1;
-        hn = 5e-14;
-    } else {
-        const double e_rel = e_abs / den;
-        accept = (e_abs <= eps_abs) && (e_rel <= eps_rel);
-        const double sa = HEADROOM * pow(eps_abs / e_abs, 1.0 / (order + 1.0));
-        const double sr = HEADROOM * pow(eps_rel / e_rel, 1.0 / order);
-        double sc = sa < sr ? sa : sr;
-        if (sc < MINS) sc = MINS;
-        if (sc > MAXS) sc = MAXS;
-        hn = h * sc;
-        if (hn < MINH) hn = MINH;
-        if (hn > MAXH) hn = MAXH;
-        if (hn == MINH) accept = 1;
+    int accept = 1;
+    double hn = h;
+    if (adaptive) {
+        double e_abs = sqrt(__longlong_as_double((long long)ebits));
+        if (Ms_uniform) e_abs /= Ms_u;
+        const double HEADROOM = 0.85, MINS = 0.2, MAXS = 5.0, MINH = 1e-15, MAXH = 1e-10;
+        const double den = h * dmax;
+        if (den == 0.0) {
+            hn = 5e-14;
+        } else {
+            const double e_rel = e_abs / den;
+            accept = (e_abs <= eps_abs) && (e_rel <= eps_rel);
+            const double sa = HEADROOM * pow(eps_abs / e_abs, 1.0 / (order + 1.0));
+            const double sr = HEADROOM * pow(eps_rel / e_rel, 1.0 / order);
+            double sc = sa < sr ? sa : sr;
+            if (sc < MINS) sc = MINS;
+            if (sc > MAXS) sc = MAXS;
+            hn = h * sc;
+            if (hn < MINH) hn = MINH;
+            if (hn > MAXH) hn = MAXH;
+            if (hn == MINH) accept = 1;
+        }
     }
     if (accept) {
-        ctl->t += h;
-        ctl->h_last = h;
-        ctl->cur = 1 - ctl->cur;
-        ctl->accepted += 1;
-        if (ctl->accepted >= ctl->target) ctl->stop = 1;
+        if (ctl->tmax_on && ctl->t + h > ctl->t_max) {
+            ctl->forced = 1;
+            ctl->h_saved = hn;
+            ctl->h = ctl->t_max - ctl->t;
+            return;
+        }
+        accept_step(ctl, h);
     } else {
         ctl->rejections += 1;
     }
@@ -515,6 +531,7 @@ struct StepperBase {
     virtual void profile(int attempts, double *out) = 0;
     virtual void energy(double *E5) = 0;
     virtual void relax(int criterion, double thr, int every, long long max_steps, double *info3) = 0;
+    virtual void run_until(double t_end, long long max_steps) = 0;
 };
 
 static bool all_equal(const double *a, size_t n)
@@ -977,6 +994,7 @@ template <typename T> struct Stepper : public StepperBase {
         const long long a0 = c.accepted;
         c.target = c.accepted + max_steps;
         c.stop = max_steps == 0 ? 1 : 0;
+        c.tmax_on = 0;
         c.relax_mode = criterion;
         c.relax_every = every;
         c.relax_thr = thr;
@@ -985,13 +1003,7 @@ template <typename T> struct Stepper : public StepperBase {
         c.relax_value = 0.0;
         c.last_check = -1;
         MM_HIP(hipMemcpy(ctl, &c, sizeof(c), hipMemcpyHostToDevice));
-        const long long batch = 32;
-        while (!c.stop && !c.relaxed && c.accepted < c.target) {
-            const long long q = std::min(batch, c.target - c.accepted);
-            for (long long r = 0; r < q; ++r) enqueue_attempt();
-            MM_HIP(hipStreamSynchronize(s));
-            MM_HIP(hipMemcpy(&c, ctl, sizeof(c), hipMemcpyDeviceToHost));
-        }
+        drive(c, 32);
         info3[0] = (double)(c.accepted - a0);
         info3[1] = c.relaxed ? 1.0 : 0.0;
         info3[2] = c.relax_value;
@@ -1004,6 +1016,18 @@ template <typename T> struct Stepper : public StepperBase {
         MM_HIP(hipMemcpy(ctl, &c, sizeof(c), hipMemcpyHostToDevice));
     }
 
+    // enqueue attempts in batches until the device sets stop; c holds the
+    // control block on entry (already on the device) and on return
+    void drive(StepCtl &c, long long batch)
+    {
+        while (!c.stop) {
+            const long long q = std::max(1ll, std::min(batch, c.target - c.accepted));
+            for (long long r = 0; r < q; ++r) enqueue_attempt();
+            MM_HIP(hipStreamSynchronize(s));
+            MM_HIP(hipMemcpy(&c, ctl, sizeof(c), hipMemcpyDeviceToHost));
+        }
+    }
+
     void run(long long steps) override
     {
         if (steps <= 0) return;
@@ -1012,14 +1036,29 @@ template <typename T> struct Stepper : public StepperBase {
         MM_HIP(hipMemcpy(&c, ctl, sizeof(c), hipMemcpyDeviceToHost));
         c.target = c.accepted + steps;
         c.stop = 0;
+        c.tmax_on = 0;
         MM_HIP(hipMemcpy(ctl, &c, sizeof(c), hipMemcpyHostToDevice));
-        long long remaining = steps;
-        while (remaining > 0) {
-            for (long long q = 0; q < remaining; ++q) enqueue_attempt();
-            MM_HIP(hipStreamSynchronize(s));
-            MM_HIP(hipMemcpy(&c, ctl, sizeof(c), hipMemcpyDeviceToHost));
-            remaining = c.target - c.accepted;
-        }
+        drive(c, 1ll << 40);
+    }
+
+    // accepted steps until t reaches t_end (last step clipped) or max_steps
+    void run_until(double t_end, long long max_steps) override
+    {
+        StepCtl c;
+        MM_HIP(hipStreamSynchronize(s));
+        MM_HIP(hipMemcpy(&c, ctl, sizeof(c), hipMemcpyDeviceToHost));
+        c.target = c.accepted + max_steps;
+        c.stop = (max_steps <= 0 || c.t >= t_end) ? 1 : 0;
+        c.tmax_on = 1;
+        c.t_max = t_end;
+        c.forced = 0;
+        MM_HIP(hipMemcpy(ctl, &c, sizeof(c), hipMemcpyHostToDevice));
+        drive(c, 64);
+        c.tmax_on = 0;
+        c.forced = 0;
+        c.stop = 0;
+        c.target = c.accepted;
+        MM_HIP(hipMemcpy(ctl, &c, sizeof(c), hipMemcpyHostToDevice));
     }
 
     void sync() override { MM_HIP(hipStreamSynchronize(s)); }
@@ -1111,6 +1150,14 @@ extern "C" int mm_stepper_run(mm_stepper_t *st, int steps)
     MM_CAPI_BEGIN
     MM_REQUIRE(st && steps >= 0, "bad argument");
     st->impl->run(steps);
+    MM_CAPI_END
+}
+
+extern "C" int mm_stepper_run_until(mm_stepper_t *st, double t_end, long long max_steps)
+{
+    MM_CAPI_BEGIN
+    MM_REQUIRE(st, "null argument");
+    st->impl->run_until(t_end, max_steps);
     MM_CAPI_END
 }
 

@@ -110,6 +110,11 @@ class Stepper:
     def run(self, steps):
         _lib.check(_lib.load().mm_stepper_run(self._h, int(steps)))
 
+    def run_until(self, t_end, max_steps=10000000):
+        """Step until t == t_end (the last step clipped as in the reference's
+        RungeKutta.evolve) or max_steps accepted steps."""
+        _lib.check(_lib.load().mm_stepper_run_until(self._h, float(t_end), int(max_steps)))
+
     def sync(self):
         _lib.check(_lib.load().mm_stepper_sync(self._h))
 

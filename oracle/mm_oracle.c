@@ -853,11 +853,6 @@ static int controller(double eps_abs, double eps_rel, double h, int order,
     return accept;
 }
 
-/* RungeKutta.evolve / apply (runge_kutta.py:47-137) with MicroMagnetics
- * finish_step = normalize(Ms) (micro_magnetics.py:49-50); classical RK4
- * for method ORC_RK4 (runge_kutta_4.py:28-59 with stage 4 at y + h k3).
- * FSAL (dp54, rk23) keeps the reference behaviour of reusing the last stage
- * of the most recent attempt, accepted or not (runge_kutta.py:97-101,131-134). */
 /* Relaxation rule (mm_stepper_relax semantics): checked before a step whose
  * accepted-step count is a multiple of `every`, from the k_0 the step's first
  * attempt uses (dM/dt at M, or the carried-over FSAL vector).
@@ -891,9 +886,50 @@ static int relax_test(relax_state *r, const orc_system *s, int n, const double *
     return relaxed;
 }
 
+/* RungeKutta.apply (runge_kutta.py:89-137), or one classical RK4 step
+ * (runge_kutta_4.py:28-59 with stage 4 at y + h k3): one attempt from M with
+ * step h.  k0_given: k[0] already holds this attempt's k_0.  FSAL methods
+ * (dp54, rk23) take k_0 from the last stage of the most recent attempt,
+ * accepted or not (runge_kutta.py:97-101,131-134). */
+static void apply_attempt(const orc_system *s, int method, const orc_tableau *t, const double *f1,
+                          const double *f2, double **k, double *M, double *ytmp, double *yerr,
+                          double *fsal, int *have_fsal, double h, int k0_given, int n)
+{
+    const long m = 3L * n;
+    if (!k0_given) {
+        if (method != ORC_RK4 && t->fsal && *have_fsal) memcpy(k[0], fsal, sizeof(double) * m);
+        else orc_dmdt(s, f1, f2, M, k[0], NULL);
+    }
+    if (method == ORC_RK4) {
+        for (int sg = 1; sg < 4; ++sg) {
+            orc_rk_prepare_step(sg, h, t, k, M, ytmp, n);
+            orc_dmdt(s, f1, f2, ytmp, k[sg], NULL);
+        }
+        for (int sg = 0; sg < 4; ++sg) {
+            const double f = h * (sg == 0 || sg == 3 ? 1.0 : 2.0) / 6.0;
+            for (long i = 0; i < m; ++i) M[i] += k[sg][i] * f;
+        }
+        return;
+    }
+    for (int sg = 1; sg < t->num_steps; ++sg) {
+        orc_rk_prepare_step(sg, h, t, k, M, ytmp, n);
+        orc_dmdt(s, f1, f2, ytmp, k[sg], NULL);
+    }
+    orc_rk_combine_result(h, t, k, M, yerr, n);
+    if (t->fsal) {
+        memcpy(fsal, k[t->num_steps - 1], sizeof(double) * m);
+        *have_fsal = 1;
+    }
+}
+
+/* RungeKutta.evolve (runge_kutta.py:47-87) per accepted step, then
+ * finish_step = normalize(Ms) (micro_magnetics.py:49-50); classical RK4 with
+ * the fixed h0 for ORC_RK4.  t_max (tmax_on): an accepted step past it is
+ * redone from the same state with h = t_max - t (runge_kutta.py:74-78) and
+ * stepping ends once t >= t_max. */
 static long run_impl(const orc_system *s, int method, double eps_abs, double eps_rel,
                      double h0, long steps, double *M, double *stats, double *avg_out,
-                     relax_state *rx)
+                     relax_state *rx, int tmax_on, double t_max)
 {
     const int n = s->nx * s->ny * s->nz;
     const long m = 3L * n;
@@ -914,6 +950,7 @@ static long run_impl(const orc_system *s, int method, double eps_abs, double eps
 
     long st;
     for (st = 0; st < steps; ++st) {
+        if (tmax_on && tnow >= t_max) break;
         int k0_ready = 0;
         if (rx && st % rx->every == 0) {
             if (method != ORC_RK4 && t.fsal && have_fsal) memcpy(k[0], fsal, sizeof(double) * m);
@@ -921,47 +958,34 @@ static long run_impl(const orc_system *s, int method, double eps_abs, double eps
             k0_ready = 1;
             if (relax_test(rx, s, n, k[0], M, h_last)) break;
         }
-        if (method == ORC_RK4) {
-            const double h = h0;
-            if (!k0_ready) orc_dmdt(s, f1, f2, M, k[0], NULL);
-            for (int sg = 1; sg < 4; ++sg) {
-                orc_rk_prepare_step(sg, h, &t, k, M, ytmp, n);
-                orc_dmdt(s, f1, f2, ytmp, k[sg], NULL);
-            }
-            for (int sg = 0; sg < 4; ++sg) {
-                const double f = h * (sg == 0 || sg == 3 ? 1.0 : 2.0) / 6.0;
-                for (long i = 0; i < m; ++i) M[i] += k[sg][i] * f;
-            }
+        memcpy(y0, M, sizeof(double) * m);
+        for (;;) {
+            double h = method == ORC_RK4 ? h0 : h_try;
             attempts++;
-            tnow += h; h_last = h; h_next = h;
-        } else {
-            memcpy(y0, M, sizeof(double) * m);
-            for (;;) {
-                attempts++;
-                /* apply(state, h_try) */
-                if (k0_ready) k0_ready = 0;  /* k[0] of this state already computed */
-                else if (t.fsal && have_fsal) memcpy(k[0], fsal, sizeof(double) * m);
-                else orc_dmdt(s, f1, f2, M, k[0], NULL);
-                for (int sg = 1; sg < t.num_steps; ++sg) {
-                    orc_rk_prepare_step(sg, h_try, &t, k, M, ytmp, n);
-                    orc_dmdt(s, f1, f2, ytmp, k[sg], NULL);
-                }
-                orc_rk_combine_result(h_try, &t, k, M, yerr, n);
-                if (t.fsal) { memcpy(fsal, k[t.num_steps - 1], sizeof(double) * m); have_fsal = 1; }
+            apply_attempt(s, method, &t, f1, f2, k, M, ytmp, yerr, fsal, &have_fsal, h, k0_ready, n);
+            k0_ready = 0;
+            double hn = h;
+            int acc = 1;
+            if (method != ORC_RK4) {
                 const double max_err = orc_scaled_abs_max(n, yerr, s->Ms, s->Ms_uniform);
                 const double max_dydt = orc_scaled_abs_max(n, k[0], s->Ms, s->Ms_uniform);
-                double hn;
-                const int acc = controller(eps_abs, eps_rel, h_try, t.order, max_err, max_dydt, &hn);
-                if (!acc) {
-                    rejections++;
-                    memcpy(M, y0, sizeof(double) * m);
-                    h_try = hn;
-                    continue;
-                }
-                tnow += h_try; h_last = h_try; h_next = hn;
-                h_try = hn;
-                break;
+                acc = controller(eps_abs, eps_rel, h, t.order, max_err, max_dydt, &hn);
             }
+            if (!acc) {
+                rejections++;
+                memcpy(M, y0, sizeof(double) * m);
+                h_try = hn;
+                continue;
+            }
+            if (tmax_on && tnow + h > t_max) {
+                h = t_max - tnow;
+                memcpy(M, y0, sizeof(double) * m);
+                attempts++;
+                apply_attempt(s, method, &t, f1, f2, k, M, ytmp, yerr, fsal, &have_fsal, h, 0, n);
+            }
+            tnow += h; h_last = h; h_next = hn;
+            h_try = hn;
+            break;
         }
         orc_normalize(n, M, s->Ms, s->normalize_mode);
         if (avg_out) orc_average(n, M, avg_out + 3 * st);
@@ -978,7 +1002,13 @@ static long run_impl(const orc_system *s, int method, double eps_abs, double eps
 void orc_run(const orc_system *s, int method, double eps_abs, double eps_rel,
              double h0, int steps, double *M, double *stats, double *avg_out)
 {
-    run_impl(s, method, eps_abs, eps_rel, h0, steps, M, stats, avg_out, NULL);
+    run_impl(s, method, eps_abs, eps_rel, h0, steps, M, stats, avg_out, NULL, 0, 0.0);
+}
+
+long orc_run_until(const orc_system *s, int method, double eps_abs, double eps_rel, double h0,
+                   double t_end, long max_steps, double *M, double *stats)
+{
+    return run_impl(s, method, eps_abs, eps_rel, h0, max_steps, M, stats, NULL, NULL, 1, t_end);
 }
 
 long orc_relax(const orc_system *s, int method, double eps_abs, double eps_rel, double h0,
@@ -990,7 +1020,7 @@ long orc_relax(const orc_system *s, int method, double eps_abs, double eps_rel, 
     r.criterion = criterion;
     r.every = check_every < 1 ? 1 : check_every;
     r.thr = threshold;
-    const long done = run_impl(s, method, eps_abs, eps_rel, h0, max_steps, M, stats, NULL, &r);
+    const long done = run_impl(s, method, eps_abs, eps_rel, h0, max_steps, M, stats, NULL, &r, 0, 0.0);
     if (info3) {
         info3[0] = (double)done;
         info3[1] = (double)r.relaxed;

@@ -112,6 +112,11 @@ long orc_relax(const orc_system *s, int method, double eps_abs, double eps_rel, 
                int criterion, double threshold, int check_every, long max_steps,
                double *M, double *stats, double *info3);
 
+/* Run until t_end (the last accepted step redone with h = t_end - t as in
+ * runge_kutta.py:74-78) or max_steps accepted steps.  Returns the steps made. */
+long orc_run_until(const orc_system *s, int method, double eps_abs, double eps_rel, double h0,
+                   double t_end, long max_steps, double *M, double *stats);
+
 /* E_stray, E_exch, E_aniso, E_ext, E_tot at M [J]. */
 void orc_energy(const orc_system *s, const double *M, double *E5);
 

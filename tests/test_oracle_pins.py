@@ -235,3 +235,40 @@ def test_rk4_converges_with_order_4():
         errs.append(np.abs(M - Mref).max())
     order = np.log2(errs[0] / errs[1])
     assert 3.5 < order < 4.6, errs
+
+
+@pytest.mark.parametrize("method", ["rkf45", "dp54", "rk4"])
+def test_run_until_lands_on_t_end_and_matches_analytic_llg(method):
+    # the last step is clipped to t_end (runge_kutta.py:74-78); the state at
+    # t_end follows the analytic single-spin LLG solution
+    alpha, H0, theta0, phi0 = 0.1, 1e5, 1.0, 0.3
+    s = O.System((1, 1, 1), (2e-9,) * 3, Ms=8e5, alpha=alpha, h_ext=(0.0, 0.0, H0), normalize_mode=1)
+    M0 = 8e5 * np.array([np.sin(theta0) * np.cos(phi0), np.sin(theta0) * np.sin(phi0), np.cos(theta0)])
+    t_end = 3.3e-11
+    M, st = s.run_until(M0.reshape(3, 1, 1, 1), t_end, method=method, eps_abs=1e-10, eps_rel=1e-10, h0=2e-13)
+    assert abs(st["t"] - t_end) <= 1e-15 * t_end
+    assert st["h"] < st["h_next"] or method == "rk4"  # the clipped step is the short one
+    g = 2.211e5 / (1 + alpha ** 2)
+    th = 2 * np.arctan(np.tan(theta0 / 2) * np.exp(-alpha * g * H0 * t_end))
+    ph = phi0 + g * H0 * t_end
+    Mref = 8e5 * np.array([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)])
+    # dp54 redoes the clipped step with the carried-over k_0 of the full step
+    # (the reference's FSAL reuse, runge_kutta.py:97-101), which costs accuracy
+    tol = 1e-4 if method == "dp54" else 1e-7
+    assert np.abs(M.ravel() - Mref).max() / 8e5 < tol
+
+
+def test_run_until_takes_the_unclipped_steps_of_run():
+    # up to the clipped step, run_until makes exactly the steps run() makes
+    shape = (6, 5, 2)
+    s = O.System(shape, (2e-9,) * 3, Ms=8e5, alpha=0.2, A=1.3e-11, k_uni=1e5, axis1=(0.0, 0.0, 1.0))
+    M0 = np.ascontiguousarray(np.broadcast_to(np.array([0.6, 0.0, 0.8]).reshape(3, 1, 1, 1) * 8e5, (3, 2, 5, 6)))
+    M5, s5 = s.run(M0, steps=5)
+    M6, s6 = s.run(M0, steps=6)
+    t_end = 0.5 * (s5["t"] + s6["t"])
+    Mu, su = s.run_until(M0, t_end)
+    assert su["steps"] == 6 and su["t"] == pytest.approx(t_end, rel=1e-15)
+    assert su["attempts"] == s6["attempts"] + 1  # the redone final step
+    # a run ending exactly on a step boundary needs no clipping
+    Mb, sb = s.run_until(M0, s5["t"])
+    assert sb["steps"] == 5 and np.array_equal(Mb, M5)
