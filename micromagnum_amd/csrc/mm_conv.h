@@ -44,6 +44,7 @@ int mm_fft_size_for(int n, int given);     // radix-friendly padded size for n c
 // Geometry of a convolution: physical (nx,ny,nz), padded FFT sizes (Px,Py,Pz).
 struct ConvGeom {
     int nx, ny, nz, Px, Py, Pz, Hx, Hy2, Hz2;
+    int Hp;  // row pitch of the spectra and the kernel planes (>= Hx)
     bool is2d;
 };
 
@@ -94,9 +95,9 @@ struct ConvImpl {
     ConvGeom g;
     ConvLaunch ln;
     FftPlanHost px, py, pz;
-    T *kern = nullptr;      // 6 real planes [e][kz<Hz2][ky<Hy2][kx<Hx], scaled 1/(PxPyPz)
-    C2<T> *S1 = nullptr;    // [c][z<nz][y<ny][kx<Hx]
-    C2<T> *S2 = nullptr;    // [c][z<nz][ky<Py][kx<Hx]   (3-D only)
+    T *kern = nullptr;      // 6 real planes [e][kz<Hz2][ky<Hy2][kx<Hx], pitch Hp, scaled 1/(PxPyPz)
+    C2<T> *S1 = nullptr;    // [c][z<nz][y<ny][kx<Hx], pitch Hp
+    C2<T> *S2 = nullptr;    // [c][z<nz][ky<Py][kx<Hx], pitch Hp   (3-D only)
     size_t kern_elems = 0;
 
     void init(const double *N_host, int nx, int ny, int nz, int ex, int ey, int ez);

@@ -133,7 +133,7 @@ template <typename K> static void set_lds(K, size_t bytes)
 // sel (may be null): device index choosing in (0) or in_alt (1)
 template <typename T, typename TIN>
 __global__ void k_xfwd(const TIN *__restrict__ in, size_t in_cstride, C2<T> *__restrict__ out,
-                       size_t out_cstride, int Hx, int ny, int nz, int nrows, int nx_in,
+                       size_t out_cstride, int Hx, int Hp, int ny, int nz, int nrows, int nx_in,
                        FftPlanDev p, int lpw, const TIN *__restrict__ in_alt = nullptr,
                        const int *__restrict__ sel = nullptr)
 {
@@ -155,14 +155,14 @@ __global__ void k_xfwd(const TIN *__restrict__ in, size_t in_cstride, C2<T> *__r
     __syncthreads();
     lds_fft<T, false>(line, p, t, active);
     if (active) {
-        C2<T> *dst = out + c * out_cstride + (size_t)r * Hx;
+        C2<T> *dst = out + c * out_cstride + (size_t)r * Hp;
         for (int kx = t; kx < Hx; kx += p.tline) dst[kx] = line[sk(kx)];
     }
 }
 
 // y forward: columns kx of planes (c, z); rows y < ny_in read, Py written
 template <typename T>
-__global__ void k_yfwd(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, int Hx, int ny_in,
+__global__ void k_yfwd(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, int Hx, int Hp, int ny_in,
                        int nzl, FftPlanDev p, int tk)
 {
     extern __shared__ char smem_raw[];
@@ -170,24 +170,24 @@ __global__ void k_yfwd(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, in
     const int L = p.L, LP = lds_len(L);
     const int kx0 = blockIdx.x * tk;
     const int z = blockIdx.y, c = blockIdx.z;
-    const C2<T> *src = in + ((size_t)c * nzl + z) * (size_t)ny_in * Hx;
-    C2<T> *dst = out + ((size_t)c * nzl + z) * (size_t)L * Hx;
+    const C2<T> *src = in + ((size_t)c * nzl + z) * (size_t)ny_in * Hp;
+    C2<T> *dst = out + ((size_t)c * nzl + z) * (size_t)L * Hp;
     for (int e = threadIdx.x; e < tk * L; e += blockDim.x) {
         const int col = e % tk, y = e / tk, kx = kx0 + col;
-        sm[col * LP + sk(y)] = (y < ny_in && kx < Hx) ? src[(size_t)y * Hx + kx] : cmk<T>((T)0, (T)0);
+        sm[col * LP + sk(y)] = (y < ny_in && kx < Hx) ? src[(size_t)y * Hp + kx] : cmk<T>((T)0, (T)0);
     }
     __syncthreads();
     const int l = threadIdx.x / p.tline, t = threadIdx.x % p.tline;
     lds_fft<T, false>(sm + (size_t)l * LP, p, t, l < tk);
     for (int e = threadIdx.x; e < tk * L; e += blockDim.x) {
         const int col = e % tk, y = e / tk, kx = kx0 + col;
-        if (kx < Hx) dst[(size_t)y * Hx + kx] = sm[col * LP + sk(y)];
+        if (kx < Hx) dst[(size_t)y * Hp + kx] = sm[col * LP + sk(y)];
     }
 }
 
 // y inverse: columns of S2 planes (c, z), all Py rows read, y < ny_out written
 template <typename T>
-__global__ void k_yinv(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, int Hx, int ny_out,
+__global__ void k_yinv(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, int Hx, int Hp, int ny_out,
                        int nzl, FftPlanDev p, int tk)
 {
     extern __shared__ char smem_raw[];
@@ -195,30 +195,30 @@ __global__ void k_yinv(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, in
     const int L = p.L, LP = lds_len(L);
     const int kx0 = blockIdx.x * tk;
     const int z = blockIdx.y, c = blockIdx.z;
-    const C2<T> *src = in + ((size_t)c * nzl + z) * (size_t)L * Hx;
-    C2<T> *dst = out + ((size_t)c * nzl + z) * (size_t)ny_out * Hx;
+    const C2<T> *src = in + ((size_t)c * nzl + z) * (size_t)L * Hp;
+    C2<T> *dst = out + ((size_t)c * nzl + z) * (size_t)ny_out * Hp;
     for (int e = threadIdx.x; e < tk * L; e += blockDim.x) {
         const int col = e % tk, y = e / tk, kx = kx0 + col;
-        sm[col * LP + sk(y)] = (kx < Hx) ? src[(size_t)y * Hx + kx] : cmk<T>((T)0, (T)0);
+        sm[col * LP + sk(y)] = (kx < Hx) ? src[(size_t)y * Hp + kx] : cmk<T>((T)0, (T)0);
     }
     __syncthreads();
     const int l = threadIdx.x / p.tline, t = threadIdx.x % p.tline;
     lds_fft<T, true>(sm + (size_t)l * LP, p, t, l < tk);
     for (int e = threadIdx.x; e < tk * ny_out; e += blockDim.x) {
         const int col = e % tk, y = e / tk, kx = kx0 + col;
-        if (kx < Hx) dst[(size_t)y * Hx + kx] = sm[col * LP + sk(y)];
+        if (kx < Hx) dst[(size_t)y * Hp + kx] = sm[col * LP + sk(y)];
     }
 }
 
 // real octant-stored kernel entry e at (kx, ky, kz) of the full spectrum
 template <typename T>
-__device__ __forceinline__ void kernel_at(const T *__restrict__ K, size_t plane, int Hx, int Hy2,
+__device__ __forceinline__ void kernel_at(const T *__restrict__ K, size_t plane, int Hp, int Hy2,
                                           int Py, int Pz, int kx, int ky, int kz, T k[6])
 {
     const bool fy = ky > Py / 2, fz = kz > Pz / 2;
     const int kyf = fy ? Py - ky : ky;
     const int kzf = fz ? Pz - kz : kz;
-    const size_t o = ((size_t)kzf * Hy2 + kyf) * Hx + kx;
+    const size_t o = ((size_t)kzf * Hy2 + kyf) * Hp + kx;
     const T sy = fy ? (T)-1 : (T)1, sz = fz ? (T)-1 : (T)1;
     k[0] = K[o];
     k[1] = sy * K[plane + o];
@@ -240,7 +240,7 @@ __device__ __forceinline__ void mult3(C2<T> &mx, C2<T> &my, C2<T> &mz, const T k
 // z pass (3-D): forward over z (nz rows, zero-extended), multiply, inverse,
 // keep z < nz.  Lines: 3 components x tk columns for one ky.
 template <typename T>
-__global__ void k_zmi(C2<T> *__restrict__ S2, const T *__restrict__ K, size_t plane, int Hx, int Hy2,
+__global__ void k_zmi(C2<T> *__restrict__ S2, const T *__restrict__ K, size_t plane, int Hx, int Hp, int Hy2,
                       int Py, int nz, FftPlanDev p, int tk)
 {
     extern __shared__ char smem_raw[];
@@ -248,9 +248,9 @@ __global__ void k_zmi(C2<T> *__restrict__ S2, const T *__restrict__ K, size_t pl
     const int L = p.L, LP = lds_len(L);
     const int kx0 = blockIdx.x * tk;
     const int ky = blockIdx.y;
-    const size_t cstride = (size_t)nz * Py * Hx;
-    const size_t zstride = (size_t)Py * Hx;
-    C2<T> *base = S2 + (size_t)ky * Hx;
+    const size_t cstride = (size_t)nz * Py * Hp;
+    const size_t zstride = (size_t)Py * Hp;
+    C2<T> *base = S2 + (size_t)ky * Hp;
     for (int e = threadIdx.x; e < 3 * tk * L; e += blockDim.x) {
         const int col = e % tk, rest = e / tk, z = rest % L, c = rest / L, kx = kx0 + col;
         sm[(c * tk + col) * LP + sk(z)] = (z < nz && kx < Hx) ? base[c * cstride + z * zstride + kx] : cmk<T>((T)0, (T)0);
@@ -264,7 +264,7 @@ __global__ void k_zmi(C2<T> *__restrict__ S2, const T *__restrict__ K, size_t pl
         const int col = e % tk, kz = e / tk, kx = kx0 + col;
         if (kx < Hx) {
             T k[6];
-            kernel_at<T>(K, plane, Hx, Hy2, Py, L, kx, ky, kz, k);
+            kernel_at<T>(K, plane, Hp, Hy2, Py, L, kx, ky, kz, k);
             mult3<T>(sm[(0 * tk + col) * LP + sk(kz)], sm[(1 * tk + col) * LP + sk(kz)], sm[(2 * tk + col) * LP + sk(kz)], k);
         }
     }
@@ -279,17 +279,17 @@ __global__ void k_zmi(C2<T> *__restrict__ S2, const T *__restrict__ K, size_t pl
 // 2-D: y forward (ny rows, zero-extended) + multiply (kz = 0) + y inverse,
 // keep y < ny, in place on S1 planes [c][y][kx].
 template <typename T>
-__global__ void k_ymi2d(C2<T> *__restrict__ S1, const T *__restrict__ K, size_t plane, int Hx, int Hy2,
+__global__ void k_ymi2d(C2<T> *__restrict__ S1, const T *__restrict__ K, size_t plane, int Hx, int Hp, int Hy2,
                         int ny, FftPlanDev p, int tk)
 {
     extern __shared__ char smem_raw[];
     C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
     const int L = p.L, LP = lds_len(L);
     const int kx0 = blockIdx.x * tk;
-    const size_t cstride = (size_t)ny * Hx;
+    const size_t cstride = (size_t)ny * Hp;
     for (int e = threadIdx.x; e < 3 * tk * L; e += blockDim.x) {
         const int col = e % tk, rest = e / tk, y = rest % L, c = rest / L, kx = kx0 + col;
-        sm[(c * tk + col) * LP + sk(y)] = (y < ny && kx < Hx) ? S1[c * cstride + (size_t)y * Hx + kx] : cmk<T>((T)0, (T)0);
+        sm[(c * tk + col) * LP + sk(y)] = (y < ny && kx < Hx) ? S1[c * cstride + (size_t)y * Hp + kx] : cmk<T>((T)0, (T)0);
     }
     __syncthreads();
     const int l = threadIdx.x / p.tline, t = threadIdx.x % p.tline;
@@ -300,7 +300,7 @@ __global__ void k_ymi2d(C2<T> *__restrict__ S1, const T *__restrict__ K, size_t 
         const int col = e % tk, ky = e / tk, kx = kx0 + col;
         if (kx < Hx) {
             T k[6];
-            kernel_at<T>(K, plane, Hx, Hy2, L, 1, kx, ky, 0, k);
+            kernel_at<T>(K, plane, Hp, Hy2, L, 1, kx, ky, 0, k);
             mult3<T>(sm[(0 * tk + col) * LP + sk(ky)], sm[(1 * tk + col) * LP + sk(ky)], sm[(2 * tk + col) * LP + sk(ky)], k);
         }
     }
@@ -308,14 +308,14 @@ __global__ void k_ymi2d(C2<T> *__restrict__ S1, const T *__restrict__ K, size_t 
     lds_fft<T, true>(line, p, t, active);
     for (int e = threadIdx.x; e < 3 * tk * ny; e += blockDim.x) {
         const int col = e % tk, rest = e / tk, y = rest % ny, c = rest / ny, kx = kx0 + col;
-        if (kx < Hx) S1[c * cstride + (size_t)y * Hx + kx] = sm[(c * tk + col) * LP + sk(y)];
+        if (kx < Hx) S1[c * cstride + (size_t)y * Hp + kx] = sm[(c * tk + col) * LP + sk(y)];
     }
 }
 
 // x inverse (C2R): rows (c, z, y) of S1, Hermitian extension, keep x < nx
 template <typename T, typename TOUT>
 __global__ void k_xinv(const C2<T> *__restrict__ in, TOUT *__restrict__ out, size_t out_cstride, int Hx,
-                       int ny, int nz, int nrows, int nx, FftPlanDev p, int lpw)
+                       int Hp, int ny, int nz, int nrows, int nx, FftPlanDev p, int lpw)
 {
     extern __shared__ char smem_raw[];
     C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
@@ -328,7 +328,7 @@ __global__ void k_xinv(const C2<T> *__restrict__ in, TOUT *__restrict__ out, siz
     if (active) {
         c = g / (ny * nz);
         r = g % (ny * nz);
-        const C2<T> *src = in + (size_t)g * Hx;
+        const C2<T> *src = in + (size_t)g * Hp;
         for (int k = t; k < L; k += p.tline) line[sk(k)] = k < Hx ? src[k] : cconj<T>(src[L - k]);
     }
     __syncthreads();
@@ -340,14 +340,14 @@ __global__ void k_xinv(const C2<T> *__restrict__ in, TOUT *__restrict__ out, siz
 }
 
 // setup: forward z over full lines (no multiply), all Pz outputs stored
-__global__ void k_zfwd_setup(double2 *__restrict__ S, int Hx, int Py, FftPlanDev p, int tk)
+__global__ void k_zfwd_setup(double2 *__restrict__ S, int Hx, int Hp, int Py, FftPlanDev p, int tk)
 {
     extern __shared__ char smem_raw[];
     double2 *sm = reinterpret_cast<double2 *>(smem_raw);
     const int L = p.L, LP = lds_len(L);
     const int kx0 = blockIdx.x * tk, ky = blockIdx.y;
-    const size_t zstride = (size_t)Py * Hx;
-    double2 *base = S + (size_t)ky * Hx;
+    const size_t zstride = (size_t)Py * Hp;
+    double2 *base = S + (size_t)ky * Hp;
     for (int e = threadIdx.x; e < tk * L; e += blockDim.x) {
         const int col = e % tk, z = e / tk, kx = kx0 + col;
         sm[col * LP + sk(z)] = kx < Hx ? base[z * zstride + kx] : cmk<double>(0.0, 0.0);
@@ -363,7 +363,7 @@ __global__ void k_zfwd_setup(double2 *__restrict__ S, int Hx, int Py, FftPlanDev
 
 // setup: real part of the octant ky <= Py/2, kz <= Pz/2, scaled
 template <typename T>
-__global__ void k_extract_kernel(const double2 *__restrict__ S, T *__restrict__ K, int Hx, int Py,
+__global__ void k_extract_kernel(const double2 *__restrict__ S, T *__restrict__ K, int Hx, int Hp, int Py,
                                  int Hy2, int Hz2, double scale)
 {
     const size_t n = (size_t)Hx * Hy2 * Hz2;
@@ -371,7 +371,7 @@ __global__ void k_extract_kernel(const double2 *__restrict__ S, T *__restrict__ 
         const int kx = (int)(q % Hx);
         const size_t r = q / Hx;
         const int ky = (int)(r % Hy2), kz = (int)(r / Hy2);
-        K[q] = (T)(scale * S[((size_t)kz * Py + ky) * Hx + kx].x);
+        K[((size_t)kz * Hy2 + ky) * Hp + kx] = (T)(scale * S[((size_t)kz * Py + ky) * Hp + kx].x);
     }
 }
 
@@ -403,12 +403,12 @@ static ConvLaunch choose_launch(const ConvGeom &g, const FftPlanHost &px, const 
     ln.lpw_x = env_int("MM_LPW_X", std::max(1, 256 / px.tline));
     while (ln.lpw_x > 1 && ((size_t)ln.lpw_x * lds_len(px.L) * csize > (size_t)lds || ln.lpw_x * px.tline > 1024)) --ln.lpw_x;
     const int nl_y = g.is2d ? 3 : 1;
-    int tk = env_int("MM_TK_Y", std::max(1, (int)(64 / csize)));
+    int tk = env_int("MM_TK_Y", 8);
     while (tk > 1 && ((size_t)nl_y * tk * py.tline > 1024 || (size_t)nl_y * tk * lds_len(py.L) * csize > (size_t)lds)) tk /= 2;
     ln.tk_y = tk;
     MM_REQUIRE((size_t)nl_y * tk * py.tline <= 1024 && (size_t)nl_y * tk * lds_len(py.L) * csize <= (size_t)lds,
                "y transform length too large for one workgroup");
-    int tz = env_int("MM_TK_Z", std::max(1, (int)(128 / csize)));
+    int tz = env_int("MM_TK_Z", 16);
     while (tz > 1 && ((size_t)3 * tz * pz.tline > 1024 || (size_t)3 * tz * lds_len(pz.L) * csize > (size_t)lds)) tz /= 2;
     ln.tk_z = tz;
     return ln;
@@ -423,6 +423,9 @@ void ConvImpl<T>::init(const double *N_host, int nx, int ny, int nz, int ex, int
     g.Py = mm_fft_size_for(ny, ey);
     g.Pz = mm_fft_size_for(nz, ez);
     g.Hx = g.Px / 2 + 1;
+    // spectral row pitch: a multiple of 16 complex values, so that rows and
+    // the column tiles of the y/z passes start on 128-byte boundaries
+    g.Hp = (g.Hx + 15) / 16 * 16;
     g.Hy2 = g.Py / 2 + 1;
     g.Hz2 = g.Pz / 2 + 1;
     g.is2d = (nz == 1 && g.Pz == 1);
@@ -458,12 +461,13 @@ void ConvImpl<T>::init(const double *N_host, int nx, int ny, int nz, int ex, int
     }
 
     // Fourier transform of each component in FP64 on the device
-    const size_t plane = (size_t)g.Hx * g.Hy2 * g.Hz2;
+    const size_t plane = (size_t)g.Hp * g.Hy2 * g.Hz2;
     kern_elems = 6 * plane;
     MM_HIP(hipMalloc(&kern, sizeof(T) * kern_elems));
+    MM_HIP(hipMemset(kern, 0, sizeof(T) * kern_elems));
     double *dN = nullptr;
     double2 *T1 = nullptr, *T2 = nullptr;
-    const size_t nspec = (size_t)g.Hx * Py * Pz;
+    const size_t nspec = (size_t)g.Hp * Py * Pz;
     MM_HIP(hipMalloc(&dN, sizeof(double) * plane_full));
     MM_HIP(hipMalloc(&T1, sizeof(double2) * nspec));
     MM_HIP(hipMalloc(&T2, sizeof(double2) * nspec));
@@ -485,19 +489,19 @@ void ConvImpl<T>::init(const double *N_host, int nx, int ny, int nz, int ex, int
         MM_HIP(hipMemcpy(dN, Nr.data() + e * plane_full, sizeof(double) * plane_full, hipMemcpyHostToDevice));
         const int nrows = Py * Pz;
         hipLaunchKernelGGL((k_xfwd<double, double>), dim3((nrows + lpw - 1) / lpw), dim3(lpw * px.tline), shx, 0,
-                           dN, 0, T1, 0, g.Hx, Py, Pz, nrows, Px, dx, lpw, (const double *)nullptr,
+                           dN, 0, T1, 0, g.Hx, g.Hp, Py, Pz, nrows, Px, dx, lpw, (const double *)nullptr,
                            (const int *)nullptr);
         MM_HIP(hipGetLastError());
         hipLaunchKernelGGL((k_yfwd<double>), dim3((g.Hx + tky - 1) / tky, Pz, 1), dim3(tky * py.tline), shy, 0,
-                           T1, T2, g.Hx, Py, Pz, dy, tky);
+                           T1, T2, g.Hx, g.Hp, Py, Pz, dy, tky);
         MM_HIP(hipGetLastError());
         if (Pz > 1) {
             hipLaunchKernelGGL(k_zfwd_setup, dim3((g.Hx + tkz - 1) / tkz, Py, 1), dim3(tkz * pz.tline), shz, 0,
-                               T2, g.Hx, Py, dz, tkz);
+                               T2, g.Hx, g.Hp, Py, dz, tkz);
             MM_HIP(hipGetLastError());
         }
         hipLaunchKernelGGL((k_extract_kernel<T>), dim3(grid_1d(plane, 256)), dim3(256), 0, 0, T2, kern + e * plane,
-                           g.Hx, Py, g.Hy2, g.Hz2, scale);
+                           g.Hx, g.Hp, Py, g.Hy2, g.Hz2, scale);
         MM_HIP(hipGetLastError());
     }
     MM_HIP(hipDeviceSynchronize());
@@ -505,8 +509,8 @@ void ConvImpl<T>::init(const double *N_host, int nx, int ny, int nz, int ex, int
     (void)hipFree(T1);
     (void)hipFree(T2);
 
-    MM_HIP(hipMalloc(&S1, sizeof(C2<T>) * 3 * (size_t)nz * ny * g.Hx));
-    if (!g.is2d) MM_HIP(hipMalloc(&S2, sizeof(C2<T>) * 3 * (size_t)nz * Py * g.Hx));
+    MM_HIP(hipMalloc(&S1, sizeof(C2<T>) * 3 * (size_t)nz * ny * g.Hp));
+    if (!g.is2d) MM_HIP(hipMalloc(&S2, sizeof(C2<T>) * 3 * (size_t)nz * Py * g.Hp));
 
     const size_t cs = sizeof(C2<T>);
     set_lds(k_xfwd<T, T>, (size_t)ln.lpw_x * lds_len(px.L) * cs);
@@ -556,7 +560,7 @@ template <typename T> void ConvImpl<T>::forward_x(const T *M, hipStream_t s, con
     const int nrows = 3 * g.ny * g.nz;
     const size_t sh = (size_t)ln.lpw_x * lds_len(px.L) * sizeof(C2<T>);
     hipLaunchKernelGGL((k_xfwd<T, T>), dim3((nrows + ln.lpw_x - 1) / ln.lpw_x), dim3(ln.lpw_x * px.tline), sh, s,
-                       M, n, S1, (size_t)g.ny * g.nz * g.Hx, g.Hx, g.ny, g.nz, nrows, g.nx, px.dev<T>(), ln.lpw_x,
+                       M, n, S1, (size_t)g.ny * g.nz * g.Hp, g.Hx, g.Hp, g.ny, g.nz, nrows, g.nx, px.dev<T>(), ln.lpw_x,
                        M_alt, sel);
     MM_HIP(hipGetLastError());
     if (klog) klog->mark(s, KC_XFWD, conv_bytes<T>(g, KC_XFWD));
@@ -565,29 +569,29 @@ template <typename T> void ConvImpl<T>::forward_x(const T *M, hipStream_t s, con
 template <typename T> void ConvImpl<T>::spectral(hipStream_t s)
 {
     const size_t cs = sizeof(C2<T>);
-    const size_t plane = (size_t)g.Hx * g.Hy2 * g.Hz2;
+    const size_t plane = (size_t)g.Hp * g.Hy2 * g.Hz2;
     if (g.is2d) {
         const int tk = ln.tk_y;
         const size_t sh = (size_t)3 * tk * lds_len(py.L) * cs;
         hipLaunchKernelGGL((k_ymi2d<T>), dim3((g.Hx + tk - 1) / tk), dim3(3 * tk * py.tline), sh, s, S1, kern, plane,
-                           g.Hx, g.Hy2, g.ny, py.dev<T>(), tk);
+                           g.Hx, g.Hp, g.Hy2, g.ny, py.dev<T>(), tk);
         MM_HIP(hipGetLastError());
         if (klog) klog->mark(s, KC_YMI2D, conv_bytes<T>(g, KC_YMI2D));
         return;
     }
     const int tk = ln.tk_y;
     const size_t shy = (size_t)tk * lds_len(py.L) * cs;
-    hipLaunchKernelGGL((k_yfwd<T>), dim3((g.Hx + tk - 1) / tk, g.nz, 3), dim3(tk * py.tline), shy, s, S1, S2, g.Hx,
+    hipLaunchKernelGGL((k_yfwd<T>), dim3((g.Hx + tk - 1) / tk, g.nz, 3), dim3(tk * py.tline), shy, s, S1, S2, g.Hx, g.Hp,
                        g.ny, g.nz, py.dev<T>(), tk);
     MM_HIP(hipGetLastError());
     if (klog) klog->mark(s, KC_YFWD, conv_bytes<T>(g, KC_YFWD));
     const int tz = ln.tk_z;
     const size_t shz = (size_t)3 * tz * lds_len(pz.L) * cs;
     hipLaunchKernelGGL((k_zmi<T>), dim3((g.Hx + tz - 1) / tz, g.Py), dim3(3 * tz * pz.tline), shz, s, S2, kern, plane,
-                       g.Hx, g.Hy2, g.Py, g.nz, pz.dev<T>(), tz);
+                       g.Hx, g.Hp, g.Hy2, g.Py, g.nz, pz.dev<T>(), tz);
     MM_HIP(hipGetLastError());
     if (klog) klog->mark(s, KC_ZMI, conv_bytes<T>(g, KC_ZMI));
-    hipLaunchKernelGGL((k_yinv<T>), dim3((g.Hx + tk - 1) / tk, g.nz, 3), dim3(tk * py.tline), shy, s, S2, S1, g.Hx,
+    hipLaunchKernelGGL((k_yinv<T>), dim3((g.Hx + tk - 1) / tk, g.nz, 3), dim3(tk * py.tline), shy, s, S2, S1, g.Hx, g.Hp,
                        g.ny, g.nz, py.dev<T>(), tk);
     MM_HIP(hipGetLastError());
     if (klog) klog->mark(s, KC_YINV, conv_bytes<T>(g, KC_YINV));
@@ -599,7 +603,7 @@ template <typename T> void ConvImpl<T>::inverse_x(T *H, hipStream_t s)
     const int nrows = 3 * g.ny * g.nz;
     const size_t sh = (size_t)ln.lpw_x * lds_len(px.L) * sizeof(C2<T>);
     hipLaunchKernelGGL((k_xinv<T, T>), dim3((nrows + ln.lpw_x - 1) / ln.lpw_x), dim3(ln.lpw_x * px.tline), sh, s, S1,
-                       H, n, g.Hx, g.ny, g.nz, nrows, g.nx, px.dev<T>(), ln.lpw_x);
+                       H, n, g.Hx, g.Hp, g.ny, g.nz, nrows, g.nx, px.dev<T>(), ln.lpw_x);
     MM_HIP(hipGetLastError());
 }
 

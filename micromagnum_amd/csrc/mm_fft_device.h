@@ -41,12 +41,14 @@ template <typename T, bool INV> __device__ __forceinline__ C2<T> cmul_mi(C2<T> a
     return INV ? cmk<T>(-a.y, a.x) : cmk<T>(a.y, -a.x);
 }
 
-// LDS layout of a line: element j lives at j + (j >> MM_LDS_SKEW) (one pad
-// slot per 2^MM_LDS_SKEW elements) so that the strided Stockham writes of the
-// early passes and the transposing tile loads spread over the LDS banks; a
-// line of length L occupies lds_len(L) slots.  MM_LDS_SKEW = 0 disables it.
+// LDS layout of a line: with MM_LDS_SKEW = s > 0 element j lives at
+// j + (j >> s) (one pad slot per 2^s elements, spreading strided Stockham
+// writes over the LDS banks); a line of length L occupies lds_len(L) slots.
+// Measured on MI355X (256x256x16 cells, RKF45) the skew costs more in index
+// arithmetic than it saves in bank conflicts (544 vs 599 steps/s FP32), so
+// it is off by default.
 #ifndef MM_LDS_SKEW
-#define MM_LDS_SKEW 3
+#define MM_LDS_SKEW 0
 #endif
 __host__ __device__ __forceinline__ int sk(int j) { return MM_LDS_SKEW ? j + (j >> MM_LDS_SKEW) : j; }
 __host__ __device__ __forceinline__ int lds_len(int L) { return (MM_LDS_SKEW ? L + (L >> MM_LDS_SKEW) : L) + 1; }

@@ -297,7 +297,7 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
 // epilogue, per-block sums of M.H per term into epart[q * gridDim.x + block].
 template <typename T, bool EN>
 __global__ void k_stage(SysDev<T> S, StageDev<T> G, StepCtl *ctl, const C2<T> *__restrict__ S1, FftPlanDev p,
-                        int Hx, int rpw, int has_demag, double *__restrict__ epart)
+                        int Hx, int Hp, int rpw, int has_demag, double *__restrict__ epart)
 {
     if (!EN && ctl->stop) return;
     extern __shared__ char smem_raw[];
@@ -314,7 +314,7 @@ __global__ void k_stage(SysDev<T> S, StageDev<T> G, StepCtl *ctl, const C2<T> *_
         const bool active = (l < 3 * rpw) && gr < nrows;
         C2<T> *line = sm + (size_t)l * LP;
         if (active) {
-            const C2<T> *src = S1 + ((size_t)c * nrows + gr) * Hx;
+            const C2<T> *src = S1 + ((size_t)c * nrows + gr) * Hp;
             for (int k = t; k < L; k += p.tline) line[sk(k)] = k < Hx ? src[k] : cconj<T>(src[L - k]);
         }
         __syncthreads();
@@ -710,7 +710,7 @@ template <typename T> struct Stepper : public StepperBase {
             conv = new ConvImpl<T>();
             conv->init(d.N_host, d.nx, d.ny, d.nz, d.ex, d.ey, d.ez);
             const int tl = conv->px.tline;
-            rpw = std::max(1, 384 / (3 * tl));
+            rpw = std::max(1, 192 / (3 * tl));
             if (const char *e = std::getenv("MM_RPW")) rpw = std::max(1, std::atoi(e));
             while (rpw > 1 && (size_t)3 * rpw * lds_len(conv->px.L) * sizeof(C2<T>) > (size_t)mm_max_lds_bytes()) --rpw;
             MM_REQUIRE(3 * tl <= 1024, "x transform too long for the fused stage kernel");
@@ -790,7 +790,7 @@ template <typename T> struct Stepper : public StepperBase {
         }
         hipLaunchKernelGGL((k_stage<T, false>), dim3(grid_rows), dim3(threads), stage_lds, s, sys, G, ctl,
                            conv ? conv->S1 : nullptr, conv ? conv->px.template dev<T>() : FftPlanDev(),
-                           conv ? conv->g.Hx : 0, rpw, conv ? 1 : 0, nullptr);
+                           conv ? conv->g.Hx : 0, conv ? conv->g.Hp : 0, rpw, conv ? 1 : 0, nullptr);
         MM_HIP(hipGetLastError());
         if (klog) klog->mark(s, KC_STAGE, stage_bytes(G));
     }
@@ -959,7 +959,7 @@ template <typename T> struct Stepper : public StepperBase {
         }
         hipLaunchKernelGGL((k_stage<T, true>), dim3(grid_rows), dim3(threads), stage_lds, s, sys, G, ctl,
                            conv ? conv->S1 : nullptr, conv ? conv->px.template dev<T>() : FftPlanDev(),
-                           conv ? conv->g.Hx : 0, rpw, conv ? 1 : 0, epart);
+                           conv ? conv->g.Hx : 0, conv ? conv->g.Hp : 0, rpw, conv ? 1 : 0, epart);
         MM_HIP(hipGetLastError());
         std::vector<double> h((size_t)4 * grid_rows);
         MM_HIP(hipMemcpyAsync(h.data(), epart, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
