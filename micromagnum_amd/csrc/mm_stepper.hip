@@ -35,10 +35,6 @@
 // device control block
 struct StepCtl {
     double t, h, h_last;
-    unsigned long long err_bits;   // max of |err|^2 (or |err|^2/Ms^2) as bits
-    unsigned long long dydt_bits;  // max of |k0|^2 (or scaled)
-    unsigned long long draw_bits;  // max of |k0|^2, never scaled (dMdt.absMax()^2)
-    unsigned long long mraw_bits;  // max of |y|^2 of the attempt's start state (M.absMax()^2)
     long long attempts, rejections, accepted, target;
     int cur;                       // index of the current y buffer
     int stop;                      // accepted == target: remaining launches exit
@@ -101,18 +97,18 @@ template <typename T> struct StageDev {
     double b[8];          // coefficients for kin
     double bself;         // coefficient for k_s
     double ec[8], ecself; // error coefficients (final stage)
-    int want_dydt;        // reduce max|k_s| into ctl->dydt_bits (stage 0)
+    int want_dydt;        // reduce max|k_s| (stage 0)
     int norm_mode;
     // FSAL (dp54, rk23): the last stage takes max|k_0| from kin[0] and then
     // overwrites k_0 with its own k_s for the next attempt (runge_kutta.py:97-101,131-134)
     int dydt_from_k0;
     T *fsal_out;
+    // per-block maxima, rows [0] |err|^2 (final stage), [1] |k0|^2 scaled,
+    // [2] |k0|^2, [3] |y|^2, one entry per block of the launch; the
+    // controller reduces them (no same-address atomics across the grid)
+    double *mpart;
 };
 
-template <typename T> __device__ __forceinline__ void atomic_max_bits(unsigned long long *a, double v)
-{
-    atomicMax(a, (unsigned long long)__double_as_longlong(v));
-}
 
 template <typename T> __device__ __forceinline__ double dotd(const V3<T> &a, const V3<T> &b)
 {
@@ -365,11 +361,12 @@ __global__ void k_stage(SysDev<T> S, StageDev<T> G, StepCtl *ctl, const C2<T> *_
                 ec = fmax(ec, wred[2][q]);
                 ed = fmax(ed, wred[3][q]);
             }
-            if (G.prep == 2) atomic_max_bits<T>(&ctl->err_bits, ea0);
+            const size_t nb = gridDim.x, b = blockIdx.x;
+            if (G.prep == 2) G.mpart[b] = ea0;
             if (G.want_dydt || G.dydt_from_k0) {
-                atomic_max_bits<T>(&ctl->dydt_bits, eb);
-                atomic_max_bits<T>(&ctl->draw_bits, ec);
-                atomic_max_bits<T>(&ctl->mraw_bits, ed);
+                G.mpart[nb + b] = eb;
+                G.mpart[2 * nb + b] = ec;
+                G.mpart[3 * nb + b] = ed;
             }
         }
     }
@@ -380,7 +377,7 @@ __global__ void k_stage(SysDev<T> S, StageDev<T> G, StepCtl *ctl, const C2<T> *_
 // Ms) with the threshold; MM_RELAX_DEG_PER_NS is the reference's Relaxed
 // condition (condition.py:169-200) on deg_per_ns
 // (landau_lifshitz_gilbert.py:127-131) with state.h = the last accepted h.
-__device__ bool relax_check(StepCtl *ctl, double dmax_scaled)
+__device__ bool relax_check(StepCtl *ctl, double dmax_scaled, double draw2, double mraw2)
 {
     if (!ctl->relax_mode || ctl->last_check == ctl->accepted) return false;
     if (ctl->accepted % ctl->relax_every != 0) return false;
@@ -391,8 +388,8 @@ __device__ bool relax_check(StepCtl *ctl, double dmax_scaled)
         v = dmax_scaled;
         relaxed = v < ctl->relax_thr;
     } else {
-        const double dm = sqrt(__longlong_as_double((long long)ctl->draw_bits));
-        const double mm = sqrt(__longlong_as_double((long long)ctl->mraw_bits));
+        const double dm = sqrt(draw2);
+        const double mm = sqrt(mraw2);
         const double h = ctl->h_last;
         const double deg_per_step = (180.0 / 3.14159265358979323846) * atan2(dm * h, mm);
         v = 1e-9 * deg_per_step / h;
@@ -421,19 +418,31 @@ __device__ __forceinline__ void accept_step(StepCtl *ctl, double h)
 // (micro_magnetics_stepsize_controller.py:35-73) + RungeKutta.evolve retry
 // and overshoot handling (runge_kutta.py:56-86).  Non-adaptive (rk4): every
 // attempt is accepted with the fixed h.
-__global__ void k_controller(StepCtl *ctl, int order, double eps_abs, double eps_rel, int Ms_uniform, double Ms_u,
-                             int adaptive)
+__global__ void __launch_bounds__(256) k_controller(StepCtl *ctl, const double *__restrict__ mpart, int nb,
+                                                   int order, double eps_abs, double eps_rel, int Ms_uniform,
+                                                   double Ms_u, int adaptive)
 {
     if (ctl->stop) return;
+    // maxima of the per-block partials of this attempt's stage launches
+    double m[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+        for (int q = 0; q < 4; ++q) m[q] = fmax(m[q], mpart[(size_t)q * nb + b]);
+    __shared__ double red[4][4];
+    for (int q = 0; q < 4; ++q) {
+        const double w = wave_max(m[q]);
+        if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = w;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    for (int q = 0; q < 4; ++q) {
+        double v = red[q][0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) v = fmax(v, red[q][w]);
+        m[q] = v;
+    }
     const double h = ctl->h;
-    const unsigned long long ebits = ctl->err_bits, dbits = ctl->dydt_bits;
-    double dmax = sqrt(__longlong_as_double((long long)dbits));
+    double dmax = sqrt(m[1]);
     if (Ms_uniform) dmax /= Ms_u;
-    const bool relaxed = relax_check(ctl, dmax);
-    ctl->err_bits = 0ull;
-    ctl->dydt_bits = 0ull;
-    ctl->draw_bits = 0ull;
-    ctl->mraw_bits = 0ull;
+    const bool relaxed = relax_check(ctl, dmax, m[2], m[3]);
     if (relaxed) return;
     ctl->attempts += 1;
     if (ctl->forced) {
@@ -445,7 +454,7 @@ __global__ void k_controller(StepCtl *ctl, int order, double eps_abs, double eps
     int accept = 1;
     double hn = h;
     if (adaptive) {
-        double e_abs = sqrt(__longlong_as_double((long long)ebits));
+        double e_abs = sqrt(m[0]);
         if (Ms_uniform) e_abs /= Ms_u;
         const double HEADROOM = 0.85, MINS = 0.2, MAXS = 5.0, MINH = 1e-15, MAXH = 1e-10;
         const double den = h * dmax;
@@ -556,6 +565,7 @@ template <typename T> struct Stepper : public StepperBase {
     double *dbuf = nullptr;
     double *part = nullptr;
     double *epart = nullptr;
+    double *mpart = nullptr;
     StepCtl *ctl = nullptr;
     int rpw = 1;
     size_t stage_lds = 0;
@@ -725,6 +735,8 @@ template <typename T> struct Stepper : public StepperBase {
         threads = std::max(64, ((threads + 63) / 64) * 64);
         grid_rows = (d.ny * d.nz + rpw - 1) / rpw;
         MM_HIP(hipMalloc(&epart, sizeof(double) * 4 * grid_rows));
+        MM_HIP(hipMalloc(&mpart, sizeof(double) * 4 * grid_rows));
+        MM_HIP(hipMemset(mpart, 0, sizeof(double) * 4 * grid_rows));
 
         for (int b = 0; b < 2; ++b) yb[b] = dalloc(3 * n);
         for (int q = 0; q < tab.num_steps; ++q) k[q] = dalloc(3 * n);
@@ -751,9 +763,10 @@ template <typename T> struct Stepper : public StepperBase {
         if (dbuf) (void)hipFree(dbuf);
         if (part) (void)hipFree(part);
         if (epart) (void)hipFree(epart);
+        if (mpart) (void)hipFree(mpart);
         if (ctl) (void)hipFree(ctl);
         if (s) (void)hipStreamDestroy(s);
-        dbuf = part = epart = nullptr;
+        dbuf = part = epart = mpart = nullptr;
         ctl = nullptr;
         s = nullptr;
     }
@@ -803,6 +816,7 @@ template <typename T> struct Stepper : public StepperBase {
         G.yb0 = yb[0];
         G.yb1 = yb[1];
         G.norm_mode = d.normalize_mode;
+        G.mpart = mpart;
         return G;
     }
 
@@ -871,8 +885,8 @@ template <typename T> struct Stepper : public StepperBase {
         }
         if (fsal) have_fsal = true;
         const bool adaptive = d.method != MM_RK4;
-        hipLaunchKernelGGL(k_controller, dim3(1), dim3(1), 0, s, ctl, tab.order, d.eps_abs, d.eps_rel,
-                           sys.Ms_uniform, sys.Ms_u, adaptive ? 1 : 0);
+        hipLaunchKernelGGL(k_controller, dim3(1), dim3(256), 0, s, ctl, mpart, grid_rows, tab.order, d.eps_abs,
+                           d.eps_rel, sys.Ms_uniform, sys.Ms_u, adaptive ? 1 : 0);
         MM_HIP(hipGetLastError());
         if (klog) klog->mark(s, KC_CTRL, 0.0);
     }
