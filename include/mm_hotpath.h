@@ -77,10 +77,14 @@ int mm_round_tensor_dimension(int dim, int periodic, int padding, int *out);
  * images per axis cell by cell plus the far-image (infinity) correction of
  * tensor.cpp:66-160, which the environment variable
  * MAGNUM_DEMAG_NO_INFINITY_CORRECTION switches off as in the reference.
+ * cache_dir (may be NULL): a field cached there under the reference's file
+ * name (demag_tensor.cpp:52-112, raw doubles) is loaded instead of computed;
+ * a computation longer than 30 s (or any, with MM_DEMAG_CACHE_ALWAYS set) is
+ * saved there.  MAGNUM_DEMAG_GARBAGE skips the computation (zeros).
  * nthreads <= 0: one per host core. */
 int mm_generate_demag_tensor(int nx, int ny, int nz, double dx, double dy, double dz,
                              int px, int py, int pz, int repeat, int padding,
-                             int nthreads, double *N_host, int *exp_out);
+                             const char *cache_dir, int nthreads, double *N_host, int *exp_out);
 
 /* ------------------------------------------------------ FFT convolution */
 /* SymmetricMatrixVectorConvolution_FFT(N, nx, ny, nz)

@@ -3,7 +3,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -95,8 +100,24 @@ extern "C" int mm_round_tensor_dimension(int dim, int periodic, int padding, int
     MM_CAPI_END
 }
 
+// Cache file of a tensor field, named as the reference names it
+// (demag_tensor.cpp:52-71) so that caches can be shared: raw doubles in the
+// returned layout, cell sizes in integer picometres.
+static std::string tensor_cache_file(const char *dir, int nx, int ny, int nz, int ex, int ey, int ez, double dx,
+                                     double dy, double dz, int px, int py, int pz, int repeat)
+{
+    std::ostringstream ss;
+    ss << dir << "/Demag--" << nx << "-" << ny << "-" << nz << "--" << ex << "-" << ey << "-" << ez << "--"
+       << int(dx * 1e12) << "-" << int(dy * 1e12) << "-" << int(dz * 1e12);
+    if (px || py || pz)
+        ss << "--p-" << (px ? "x" : "") << (py ? "y" : "") << (pz ? "z" : "") << "-" << repeat;
+    ss << ".dat";
+    return ss.str();
+}
+
 extern "C" int mm_generate_demag_tensor(int nx, int ny, int nz, double dx, double dy, double dz, int px, int py,
-                                        int pz, int repeat, int padding, int nthreads, double *N_host, int *exp_out)
+                                        int pz, int repeat, int padding, const char *cache_dir, int nthreads,
+                                        double *N_host, int *exp_out)
 {
     MM_CAPI_BEGIN
     MM_REQUIRE(nx >= 1 && ny >= 1 && nz >= 1, "bad mesh size");
@@ -110,7 +131,35 @@ extern "C" int mm_generate_demag_tensor(int nx, int ny, int nz, double dx, doubl
         exp_out[1] = ey;
         exp_out[2] = ez;
     }
-    if (N_host) mm_demag_tensor_impl(nx, ny, nz, dx, dy, dz, ex, ey, ez, N_host, nthreads, px, py, pz, repeat);
+    if (!N_host) return MM_OK;
+    const size_t count = (size_t)6 * ex * ey * ez;
+    // demag_tensor.cpp:146-152: skip the computation (contents undefined there, zeros here)
+    if (std::getenv("MAGNUM_DEMAG_GARBAGE")) {
+        std::fill(N_host, N_host + count, 0.0);
+        return MM_OK;
+    }
+    std::string path;
+    if (cache_dir && cache_dir[0]) {
+        path = tensor_cache_file(cache_dir, nx, ny, nz, ex, ey, ez, dx, dy, dz, px, py, pz, repeat);
+        std::ifstream in(path, std::ios::binary);
+        if (in) {
+            in.read(reinterpret_cast<char *>(N_host), (std::streamsize)(sizeof(double) * count));
+            if ((size_t)in.gcount() == sizeof(double) * count) return MM_OK;
+        }
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    mm_demag_tensor_impl(nx, ny, nz, dx, dy, dz, ex, ey, ez, N_host, nthreads, px, py, pz, repeat);
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    // saved when the computation took longer than 30 s (demag_tensor.cpp:179-182),
+    // or always with MM_DEMAG_CACHE_ALWAYS set
+    if (!path.empty() && (secs > 30.0 || std::getenv("MM_DEMAG_CACHE_ALWAYS"))) {
+        const std::string tmp = path + ".tmp";
+        std::ofstream out(tmp, std::ios::binary);
+        out.write(reinterpret_cast<const char *>(N_host), (std::streamsize)(sizeof(double) * count));
+        out.close();
+        if (out) std::rename(tmp.c_str(), path.c_str());
+        else std::remove(tmp.c_str());
+    }
     MM_CAPI_END
 }
 

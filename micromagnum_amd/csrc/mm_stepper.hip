@@ -103,6 +103,10 @@ template <typename T> struct StageDev {
     // overwrites k_0 with its own k_s for the next attempt (runge_kutta.py:97-101,131-134)
     int dydt_from_k0;
     T *fsal_out;
+    // prep == 1 with the demag field: the epilogue also transforms the next
+    // stage input along x (rows into S1), so that stage needs no x pass
+    int fwd;
+    C2<T> *fwd_S1;
     // per-block maxima, rows [0] |err|^2 (final stage), [1] |k0|^2 scaled,
     // [2] |k0|^2, [3] |y|^2, one entry per block of the launch; the
     // controller reduces them (no same-address atomics across the grid)
@@ -132,7 +136,7 @@ struct EnergyAcc {
 template <typename T, bool EN>
 __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T> &G, const StepCtl *ctl,
                                            const T *__restrict__ Y, int x, int y, int z, const V3<T> &Hd,
-                                           StageRed &red, EnergyAcc &ea)
+                                           StageRed &red, EnergyAcc &ea, T *nxt)
 {
     const size_t n = S.n;
     const size_t i = (size_t)x + (size_t)S.m.nx * ((size_t)y + (size_t)S.m.ny * z);
@@ -245,6 +249,9 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
         G.prep_out[i] = out[0];
         G.prep_out[n + i] = out[1];
         G.prep_out[2 * n + i] = out[2];
+        nxt[0] = out[0];
+        nxt[1] = out[1];
+        nxt[2] = out[2];
         return;
     }
     // final stage: error norm and normalised new state into the other buffer
@@ -318,13 +325,36 @@ __global__ void k_stage(SysDev<T> S, StageDev<T> G, StepCtl *ctl, const C2<T> *_
     }
     StageRed red = {0.0, 0.0, 0.0, 0.0};
     EnergyAcc ea = {0.0, 0.0, 0.0, 0.0};
+    const bool fused = !EN && has_demag && G.fwd;
     for (int e = threadIdx.x; e < rpw * nx; e += blockDim.x) {
         const int r = e / nx, x = e % nx;
         const int gr = row0 + r;
         if (gr >= nrows) continue;
         V3<T> Hd = v3<T>(0, 0, 0);
         if (has_demag) Hd = v3<T>(sm[(0 * rpw + r) * LP + sk(x)].x, sm[(1 * rpw + r) * LP + sk(x)].x, sm[(2 * rpw + r) * LP + sk(x)].x);
-        stage_cell<T, EN>(S, G, ctl, Y, x, gr % ny, gr / ny, Hd, red, ea);
+        T nxt[3];
+        stage_cell<T, EN>(S, G, ctl, Y, x, gr % ny, gr / ny, Hd, red, ea, nxt);
+        // the slot (r, x) of the three lines is this thread's alone: the next
+        // stage input replaces H_demag in place
+        if (fused)
+            for (int c = 0; c < 3; ++c) sm[(c * rpw + r) * LP + sk(x)] = cmk<T>(nxt[c], (T)0);
+    }
+    if (fused) {
+        for (int e = threadIdx.x; e < 3 * rpw * (L - nx); e += blockDim.x) {
+            const int l = e / (L - nx), x = nx + e % (L - nx);
+            sm[(size_t)l * LP + sk(x)] = cmk<T>((T)0, (T)0);
+        }
+        __syncthreads();
+        const int l = threadIdx.x / p.tline, t = threadIdx.x % p.tline;
+        const int c = l / rpw, r = l % rpw;
+        const int gr = row0 + r;
+        const bool active = (l < 3 * rpw) && gr < nrows;
+        C2<T> *line = sm + (size_t)l * LP;
+        lds_fft<T, false>(line, p, t, active);
+        if (active) {
+            C2<T> *dst = G.fwd_S1 + ((size_t)c * nrows + gr) * Hp;
+            for (int k = t; k < Hx; k += p.tline) dst[k] = line[sk(k)];
+        }
     }
     __shared__ double wred[4][16];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (int)((blockDim.x + 63) >> 6);
@@ -498,6 +528,44 @@ __global__ void k_prep_first(size_t m3, const T *__restrict__ y0, const T *__res
     const T f = (T)(ctl->h * b10);
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < m3; i += (size_t)gridDim.x * blockDim.x)
         out[i] = y[i] + k0[i] * f;
+}
+
+// k_prep_first fused with the x forward transform of its result: rows
+// (c, z, y) of ytmp_1 = y + k_0 (h b10), written to `out` and, transformed
+// along x (zero-extended to L), to S1 rows (pitch Hp)
+template <typename T>
+__global__ void k_xfwd_prep(const T *__restrict__ y0, const T *__restrict__ y1, const T *__restrict__ k0,
+                            T *__restrict__ out, C2<T> *__restrict__ S1, const StepCtl *ctl, double b10, int nx,
+                            int ny, int nz, int Hx, int Hp, FftPlanDev p, int lpw)
+{
+    if (ctl->stop) return;
+    extern __shared__ char smem_raw[];
+    C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
+    const T *y = ctl->cur ? y1 : y0;
+    const T f = (T)(ctl->h * b10);
+    const int L = p.L;
+    const int nrows = 3 * ny * nz;
+    const int l = threadIdx.x / p.tline, t = threadIdx.x % p.tline;
+    const int g = blockIdx.x * lpw + l;
+    const bool active = g < nrows;
+    C2<T> *line = sm + (size_t)l * lds_len(L);
+    if (active) {
+        const size_t base = (size_t)g * nx;
+        for (int x = t; x < L; x += p.tline) {
+            T v = (T)0;
+            if (x < nx) {
+                v = y[base + x] + k0[base + x] * f;
+                out[base + x] = v;
+            }
+            line[sk(x)] = cmk<T>(v, (T)0);
+        }
+    }
+    __syncthreads();
+    lds_fft<T, false>(line, p, t, active);
+    if (active) {
+        C2<T> *dst = S1 + (size_t)g * Hp;
+        for (int kx = t; kx < Hx; kx += p.tline) dst[kx] = line[sk(kx)];
+    }
 }
 
 template <typename T> __global__ void k_cast_in(size_t n, const double *__restrict__ a, T *__restrict__ b)
@@ -789,16 +857,20 @@ template <typename T> struct Stepper : public StepperBase {
         if (G.kout) b += V;
         if (G.Hout) b += V;
         if (G.prep) b += V;                        // next stage input or new state
+        if (G.fwd) b += 3.0 * 2.0 * sizeof(T) * conv->g.Hx * d.ny * d.nz;  // its x transform
         return b;
     }
 
     // one stage: dM/dt at stage input Ys (null = y[cur])
-    void launch_stage(const T *Ys, const StageDev<T> &G)
+    // need_fwd: S1 does not yet hold the x transform of the stage input
+    void launch_stage(const T *Ys, const StageDev<T> &G, bool need_fwd = true)
     {
         if (conv) {
             conv->klog = klog;
-            if (Ys) conv->forward_x(Ys, s);
-            else conv->forward_x(yb[0], s, yb[1], &ctl->cur);
+            if (need_fwd) {
+                if (Ys) conv->forward_x(Ys, s);
+                else conv->forward_x(yb[0], s, yb[1], &ctl->cur);
+            }
             conv->spectral(s);
         }
         hipLaunchKernelGGL((k_stage<T, false>), dim3(grid_rows), dim3(threads), stage_lds, s, sys, G, ctl,
@@ -829,13 +901,28 @@ template <typename T> struct Stepper : public StepperBase {
         const bool six = ns == 6;
         const bool fsal = tab.fsal != 0;
         int st0 = 0;
+        bool s1_ready = false;  // S1 holds the x transform of the next stage input
         if (fsal && have_fsal) {
             // k_0 is the last stage of the previous attempt: only ytmp_1 is needed
-            const size_t m3 = 3 * n;
-            hipLaunchKernelGGL(k_prep_first<T>, dim3((int)std::min<size_t>((m3 + 255) / 256, 16384)), dim3(256), 0, s,
-                               m3, yb[0], yb[1], k[0], tmp[0], ctl, tab.b[1][0]);
-            MM_HIP(hipGetLastError());
-            if (klog) klog->mark(s, KC_CTRL, 3.0 * 3.0 * sizeof(T) * n);
+            if (conv) {
+                const FftPlanHost &px = conv->px;
+                const int lpw = conv->ln.lpw_x;
+                const int nrows = 3 * d.ny * d.nz;
+                const size_t sh = (size_t)lpw * lds_len(px.L) * sizeof(C2<T>);
+                hipLaunchKernelGGL(k_xfwd_prep<T>, dim3((nrows + lpw - 1) / lpw), dim3(lpw * px.tline), sh, s, yb[0],
+                                   yb[1], k[0], tmp[0], conv->S1, ctl, tab.b[1][0], d.nx, d.ny, d.nz, conv->g.Hx,
+                                   conv->g.Hp, px.template dev<T>(), lpw);
+                MM_HIP(hipGetLastError());
+                if (klog)
+                    klog->mark(s, KC_XFWD, 3.0 * 3.0 * sizeof(T) * n + 3.0 * 2.0 * sizeof(T) * conv->g.Hx * d.ny * d.nz);
+                s1_ready = true;
+            } else {
+                const size_t m3 = 3 * n;
+                hipLaunchKernelGGL(k_prep_first<T>, dim3((int)std::min<size_t>((m3 + 255) / 256, 16384)), dim3(256), 0,
+                                   s, m3, yb[0], yb[1], k[0], tmp[0], ctl, tab.b[1][0]);
+                MM_HIP(hipGetLastError());
+                if (klog) klog->mark(s, KC_CTRL, 3.0 * 3.0 * sizeof(T) * n);
+            }
             st0 = 1;
         }
         for (int st = st0; st < ns; ++st) {
@@ -843,6 +930,10 @@ template <typename T> struct Stepper : public StepperBase {
             StageDev<T> G = blank_stage(Ys);
             G.kout = (st < ns - 1) ? k[st] : nullptr;
             G.want_dydt = (st == 0) && !fsal;
+            if (conv && st < ns - 1) {
+                G.fwd = 1;
+                G.fwd_S1 = conv->S1;
+            }
             G.form = six ? FORM_SUM : FORM_SEQ;
             if (st < ns - 1) {
                 G.prep = 1;
@@ -881,7 +972,8 @@ template <typename T> struct Stepper : public StepperBase {
                     G.fsal_out = k[0];
                 }
             }
-            launch_stage(Ys, G);
+            launch_stage(Ys, G, !s1_ready);
+            s1_ready = G.fwd != 0;
         }
         if (fsal) have_fsal = true;
         const bool adaptive = d.method != MM_RK4;

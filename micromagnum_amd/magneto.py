@@ -155,11 +155,12 @@ def GenerateDemagTensor(nx, ny, nz, dx, dy, dz, px=False, py=False, pz=False, re
     memory order is the reference's Shape(6, ex, ey, ez)."""
     L = _lib.load()
     ex = (ctypes.c_int * 3)()
+    cd = cache_dir.encode() if cache_dir else None
     _lib.check(L.mm_generate_demag_tensor(nx, ny, nz, dx, dy, dz, int(px), int(py), int(pz), int(repeat),
-                                          int(padding), int(nthreads), None, ex))
+                                          int(padding), cd, int(nthreads), None, ex))
     N = np.empty((ex[2], ex[1], ex[0], 6), dtype=np.float64)
     _lib.check(L.mm_generate_demag_tensor(nx, ny, nz, dx, dy, dz, int(px), int(py), int(pz), int(repeat),
-                                          int(padding), int(nthreads),
+                                          int(padding), cd, int(nthreads),
                                           N.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ex))
     return N
 

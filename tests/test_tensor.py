@@ -178,3 +178,27 @@ def test_octant_symmetry(tensor_block):
     assert np.allclose(m, e * np.array([1, 1, 1, 1, 1, 1]))
     mx = N[k, j, (-i) % Px]
     assert np.allclose(mx, e * np.array([1, -1, -1, 1, 1, 1]))
+
+
+def test_tensor_cache_round_trip(tmp_path, monkeypatch):
+    # demag_tensor.cpp:52-112: file name from sizes, padded sizes, cell sizes
+    # in picometres and the periodic flags; raw doubles; loaded when present
+    monkeypatch.setenv("MM_DEMAG_CACHE_ALWAYS", "1")
+    N = mg.GenerateDemagTensor(6, 5, 2, 2e-9, 3e-9, 4e-9, True, False, False, repeat=3, cache_dir=str(tmp_path))
+    f = tmp_path / "Demag--6-5-2--6-12-4--2000-3000-4000--p-x-3.dat"
+    assert f.exists() and f.stat().st_size == N.nbytes
+    assert np.array_equal(np.fromfile(f, dtype=np.float64), N.ravel())
+    # a cached field is read back instead of computed: plant a marker value
+    planted = N.copy()
+    planted.flat[7] = 123.0
+    planted.tofile(f)
+    monkeypatch.delenv("MM_DEMAG_CACHE_ALWAYS")
+    N2 = mg.GenerateDemagTensor(6, 5, 2, 2e-9, 3e-9, 4e-9, True, False, False, repeat=3, cache_dir=str(tmp_path))
+    assert N2.flat[7] == 123.0
+    # a truncated file is ignored and the field recomputed
+    planted.ravel()[:10].tofile(f)
+    N3 = mg.GenerateDemagTensor(6, 5, 2, 2e-9, 3e-9, 4e-9, True, False, False, repeat=3, cache_dir=str(tmp_path))
+    assert np.array_equal(N3, N)
+    # open boundaries: no periodic suffix; fast computations are not saved
+    mg.GenerateDemagTensor(3, 3, 1, 2e-9, 2e-9, 2e-9, cache_dir=str(tmp_path))
+    assert not (tmp_path / "Demag--3-3-1--8-8-1--2000-2000-2000.dat").exists()
