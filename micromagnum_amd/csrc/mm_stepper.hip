@@ -448,16 +448,17 @@ __device__ __forceinline__ void accept_step(StepCtl *ctl, double h)
 // (micro_magnetics_stepsize_controller.py:35-73) + RungeKutta.evolve retry
 // and overshoot handling (runge_kutta.py:56-86).  Non-adaptive (rk4): every
 // attempt is accepted with the fixed h.
-__global__ void __launch_bounds__(256) k_controller(StepCtl *ctl, const double *__restrict__ mpart, int nb,
-                                                   int order, double eps_abs, double eps_rel, int Ms_uniform,
-                                                   double Ms_u, int adaptive)
+__global__ void __launch_bounds__(1024) k_controller(StepCtl *ctl, const double *__restrict__ mpart, int nb,
+                                                    int order, double eps_abs, double eps_rel, int Ms_uniform,
+                                                    double Ms_u, int adaptive)
 {
     if (ctl->stop) return;
     // maxima of the per-block partials of this attempt's stage launches
     double m[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
     for (int b = threadIdx.x; b < nb; b += blockDim.x)
         for (int q = 0; q < 4; ++q) m[q] = fmax(m[q], mpart[(size_t)q * nb + b]);
-    __shared__ double red[4][4];
+    __shared__ double red[4][16];
     for (int q = 0; q < 4; ++q) {
         const double w = wave_max(m[q]);
         if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = w;
@@ -977,7 +978,7 @@ template <typename T> struct Stepper : public StepperBase {
         }
         if (fsal) have_fsal = true;
         const bool adaptive = d.method != MM_RK4;
-        hipLaunchKernelGGL(k_controller, dim3(1), dim3(256), 0, s, ctl, mpart, grid_rows, tab.order, d.eps_abs,
+        hipLaunchKernelGGL(k_controller, dim3(1), dim3(1024), 0, s, ctl, mpart, grid_rows, tab.order, d.eps_abs,
                            d.eps_rel, sys.Ms_uniform, sys.Ms_u, adaptive ? 1 : 0);
         MM_HIP(hipGetLastError());
         if (klog) klog->mark(s, KC_CTRL, 0.0);
