@@ -247,6 +247,23 @@ int mm_stepper_relax(mm_stepper_t *st, int criterion, double threshold, int chec
                      long long max_steps, double *info3);
 int mm_stepper_destroy(mm_stepper_t *st);
 
+/* ------------------------------------------------------ ensembles */
+/* Many independent systems (each a full mm_system_desc_t: own mesh, material,
+ * method, precision) stepped together: the batched variant of the fused
+ * stepper.  The reference steps one system per process; an ensemble of its
+ * scripts maps to one mm_batch.  Systems are spread over `nstreams` shared
+ * streams (<= 0: 4), each keeps its own step control, and mm_batch_run makes
+ * `steps` accepted steps in every system.  Results per system equal those of
+ * mm_stepper_* on the same description. */
+typedef struct mm_batch mm_batch_t;
+int mm_batch_create(const mm_system_desc_t *descs, int count, int nstreams, mm_batch_t **out);
+int mm_batch_set_m(mm_batch_t *b, int index, const double *M_host);
+int mm_batch_get_m(mm_batch_t *b, int index, double *M_host);
+int mm_batch_run(mm_batch_t *b, int steps);
+int mm_batch_stats(mm_batch_t *b, int index, double *stats5);
+int mm_batch_average_m(mm_batch_t *b, int index, double *avg3);
+int mm_batch_destroy(mm_batch_t *b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -95,6 +95,13 @@ SIGNATURES = {
     "mm_stepper_energy": (_i, [_vp, _dp]),
     "mm_stepper_relax": (_i, [_vp, _i, _d, _i, ctypes.c_longlong, _dp]),
     "mm_stepper_destroy": (_i, [_vp]),
+    "mm_batch_create": (_i, [ctypes.POINTER(SystemDesc), _i, _i, ctypes.POINTER(_vp)]),
+    "mm_batch_set_m": (_i, [_vp, _i, _dp]),
+    "mm_batch_get_m": (_i, [_vp, _i, _dp]),
+    "mm_batch_run": (_i, [_vp, _i]),
+    "mm_batch_stats": (_i, [_vp, _i, _dp]),
+    "mm_batch_average_m": (_i, [_vp, _i, _dp]),
+    "mm_batch_destroy": (_i, [_vp]),
 }
 
 _LIB = None

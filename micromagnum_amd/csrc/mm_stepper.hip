@@ -610,6 +610,13 @@ struct StepperBase {
     virtual void energy(double *E5) = 0;
     virtual void relax(int criterion, double thr, int every, long long max_steps, double *info3) = 0;
     virtual void run_until(double t_end, long long max_steps) = 0;
+    // ensemble driving: arm(steps) sets the target, then rounds of
+    // enqueue_some() on the stepper's stream and refresh() after the stream
+    // is synchronised, while pending()
+    virtual void arm(long long steps) = 0;
+    virtual bool pending() const = 0;
+    virtual void enqueue_some(long long batch) = 0;
+    virtual void refresh() = 0;
 };
 
 static bool all_equal(const double *a, size_t n)
@@ -697,13 +704,21 @@ template <typename T> struct Stepper : public StepperBase {
         MM_HIP(hipStreamSynchronize(s));
     }
 
-    void init(const mm_system_desc_t &desc)
+    bool own_stream = true;
+
+    // ext: a stream shared with other steppers (ensembles), else an own one
+    void init(const mm_system_desc_t &desc, hipStream_t ext = nullptr)
     {
         d = desc;
         MM_REQUIRE(d.nx > 0 && d.ny > 0 && d.nz > 0, "bad mesh size");
         MM_REQUIRE(d.Ms_host && d.alpha_host, "Ms and alpha are required");
         n = (size_t)d.nx * d.ny * d.nz;
-        MM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        if (ext) {
+            s = ext;
+            own_stream = false;
+        } else {
+            MM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        }
         MM_HIP(hipMalloc(&dbuf, sizeof(double) * 3 * n));
         MM_HIP(hipMalloc(&part, sizeof(double) * 3 * MM_RED_BLOCKS));
         MM_HIP(hipMalloc(&ctl, sizeof(StepCtl)));
@@ -822,6 +837,11 @@ template <typename T> struct Stepper : public StepperBase {
 
     void release()
     {
+        for (int g = 0; g < 2; ++g)
+            if (gexec[g]) {
+                (void)hipGraphExecDestroy(gexec[g]);
+                gexec[g] = nullptr;
+            }
         if (conv) {
             conv->release();
             delete conv;
@@ -834,7 +854,7 @@ template <typename T> struct Stepper : public StepperBase {
         if (epart) (void)hipFree(epart);
         if (mpart) (void)hipFree(mpart);
         if (ctl) (void)hipFree(ctl);
-        if (s) (void)hipStreamDestroy(s);
+        if (s && own_stream) (void)hipStreamDestroy(s);
         dbuf = part = epart = mpart = nullptr;
         ctl = nullptr;
         s = nullptr;
@@ -1129,11 +1149,72 @@ template <typename T> struct Stepper : public StepperBase {
     {
         while (!c.stop) {
             const long long q = std::max(1ll, std::min(batch, c.target - c.accepted));
-            for (long long r = 0; r < q; ++r) enqueue_attempt();
+            for (long long r = 0; r < q; ++r) launch_attempt();
             MM_HIP(hipStreamSynchronize(s));
             MM_HIP(hipMemcpy(&c, ctl, sizeof(c), hipMemcpyDeviceToHost));
         }
     }
+
+    // One attempt as a replay of a captured HIP graph (one graph for attempts
+    // that compute k_0, one for FSAL attempts that reuse it): every kernel
+    // argument is fixed at set-up and all step state lives in device memory,
+    // so the recorded launches are valid for every attempt.  MM_STEPPER_GRAPHS=0
+    // launches the kernels one by one instead.
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    int use_graphs = -1;
+
+    void launch_attempt()
+    {
+        if (use_graphs < 0) {
+            const char *e = std::getenv("MM_STEPPER_GRAPHS");
+            use_graphs = (e && std::atoi(e) == 0) ? 0 : 1;
+        }
+        if (!use_graphs || klog) {
+            enqueue_attempt();
+            return;
+        }
+        const int g = (tab.fsal && have_fsal) ? 1 : 0;
+        if (!gexec[g]) {
+            hipGraph_t graph = nullptr;
+            MM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            try {
+                enqueue_attempt();
+            } catch (...) {
+                (void)hipStreamEndCapture(s, &graph);
+                if (graph) (void)hipGraphDestroy(graph);
+                throw;
+            }
+            MM_HIP(hipStreamEndCapture(s, &graph));
+            const hipError_t e = hipGraphInstantiate(&gexec[g], graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            MM_HIP(e);
+        } else if (tab.fsal) {
+            have_fsal = true;  // what enqueue_attempt records for the next attempt
+        }
+        MM_HIP(hipGraphLaunch(gexec[g], s));
+    }
+
+    StepCtl cached;
+
+    void arm(long long steps) override
+    {
+        MM_HIP(hipStreamSynchronize(s));
+        MM_HIP(hipMemcpy(&cached, ctl, sizeof(cached), hipMemcpyDeviceToHost));
+        cached.target = cached.accepted + std::max(0ll, steps);
+        cached.stop = steps > 0 ? 0 : 1;
+        cached.tmax_on = 0;
+        MM_HIP(hipMemcpy(ctl, &cached, sizeof(cached), hipMemcpyHostToDevice));
+    }
+
+    bool pending() const override { return !cached.stop; }
+
+    void enqueue_some(long long batch) override
+    {
+        const long long q = std::max(1ll, std::min(batch, cached.target - cached.accepted));
+        for (long long r = 0; r < q; ++r) launch_attempt();
+    }
+
+    void refresh() override { MM_HIP(hipMemcpy(&cached, ctl, sizeof(cached), hipMemcpyDeviceToHost)); }
 
     void run(long long steps) override
     {
@@ -1316,6 +1397,121 @@ extern "C" int mm_stepper_relax(mm_stepper_t *st, int criterion, double threshol
     MM_CAPI_BEGIN
     MM_REQUIRE(st && info3, "null argument");
     st->impl->relax(criterion, threshold, check_every, max_steps, info3);
+    MM_CAPI_END
+}
+
+// ---------------------------------------------------------------------------
+// ensembles: independent systems stepped together, spread over a few shared
+// streams so that the small launches of different systems overlap; every
+// system keeps its own control block (own h, accept/reject, t)
+// ---------------------------------------------------------------------------
+struct mm_batch {
+    std::vector<StepperBase *> sims;
+    std::vector<hipStream_t> streams;
+};
+
+static void batch_free(mm_batch *b)
+{
+    for (StepperBase *p : b->sims) delete p;
+    for (hipStream_t st : b->streams) (void)hipStreamDestroy(st);
+    delete b;
+}
+
+extern "C" int mm_batch_create(const mm_system_desc_t *descs, int count, int nstreams, mm_batch_t **out)
+{
+    MM_CAPI_BEGIN
+    MM_REQUIRE(descs && out && count > 0, "bad argument");
+    mm_batch *b = new mm_batch();
+    try {
+        const int ns = std::max(1, std::min(nstreams > 0 ? nstreams : 4, count));
+        for (int q = 0; q < ns; ++q) {
+            hipStream_t st = nullptr;
+            MM_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+            b->streams.push_back(st);
+        }
+        for (int i = 0; i < count; ++i) {
+            const mm_system_desc_t &d = descs[i];
+            MM_REQUIRE(d.precision == MM_FP64 || d.precision == MM_FP32, "bad precision");
+            hipStream_t st = b->streams[i % ns];
+            if (d.precision == MM_FP64) {
+                auto *p = new Stepper<double>();
+                b->sims.push_back(p);
+                p->init(d, st);
+            } else {
+                auto *p = new Stepper<float>();
+                b->sims.push_back(p);
+                p->init(d, st);
+            }
+        }
+    } catch (...) {
+        batch_free(b);
+        throw;
+    }
+    *out = b;
+    MM_CAPI_END
+}
+
+static StepperBase *batch_at(mm_batch *b, int index)
+{
+    MM_REQUIRE(b && index >= 0 && index < (int)b->sims.size(), "bad ensemble index");
+    return b->sims[index];
+}
+
+extern "C" int mm_batch_set_m(mm_batch_t *b, int index, const double *M_host)
+{
+    MM_CAPI_BEGIN
+    MM_REQUIRE(M_host, "null argument");
+    batch_at(b, index)->set_m(M_host);
+    MM_CAPI_END
+}
+
+extern "C" int mm_batch_get_m(mm_batch_t *b, int index, double *M_host)
+{
+    MM_CAPI_BEGIN
+    MM_REQUIRE(M_host, "null argument");
+    batch_at(b, index)->get_m(M_host);
+    MM_CAPI_END
+}
+
+extern "C" int mm_batch_stats(mm_batch_t *b, int index, double *stats5)
+{
+    MM_CAPI_BEGIN
+    MM_REQUIRE(stats5, "null argument");
+    batch_at(b, index)->stats(stats5);
+    MM_CAPI_END
+}
+
+extern "C" int mm_batch_average_m(mm_batch_t *b, int index, double *avg3)
+{
+    MM_CAPI_BEGIN
+    MM_REQUIRE(avg3, "null argument");
+    batch_at(b, index)->average(avg3);
+    MM_CAPI_END
+}
+
+extern "C" int mm_batch_run(mm_batch_t *b, int steps)
+{
+    MM_CAPI_BEGIN
+    MM_REQUIRE(b && steps >= 0, "bad argument");
+    for (StepperBase *p : b->sims) p->arm(steps);
+    for (;;) {
+        bool any = false;
+        for (StepperBase *p : b->sims)
+            if (p->pending()) {
+                p->enqueue_some(1ll << 40);
+                any = true;
+            }
+        if (!any) break;
+        for (hipStream_t st : b->streams) MM_HIP(hipStreamSynchronize(st));
+        for (StepperBase *p : b->sims) p->refresh();
+    }
+    MM_CAPI_END
+}
+
+extern "C" int mm_batch_destroy(mm_batch_t *b)
+{
+    MM_CAPI_BEGIN
+    if (b) batch_free(b);
     MM_CAPI_END
 }
 

@@ -26,17 +26,19 @@ def _field(a, shape, comps=1):
     return np.ascontiguousarray(np.broadcast_to(a, full))
 
 
-class Stepper:
+class SystemSpec:
+    """One system's mm_system_desc_t plus the host arrays it points to."""
+
     def __init__(self, shape, delta, Ms, alpha, A=None, demag=True, N=None, k_uni=None, k_cub=None,
                  axis1=None, axis2=None, h_ext=None, P=None, xi=None, j=None, do_precess=True,
                  method="rkf45", eps_abs=1e-4, eps_rel=1e-4, h0=1e-13, precision="fp64",
-                 normalize_mode=0, padding=PADDING_ROUND_4, pbc=(False, False, False), periodic_repeat=None):
-        L = _lib.device_lib()
+                 normalize_mode=0, padding=PADDING_ROUND_4, pbc=(False, False, False), periodic_repeat=None,
+                 cache_dir=None):
         nx, ny, nz = shape
-        self.shape = shape
+        self.shape = tuple(shape)
         self.delta = tuple(float(v) for v in delta)
         self.n = nx * ny * nz
-        keep = []
+        self.keep = keep = []
 
         def f(a, comps=1):
             arr = _field(a, shape, comps)
@@ -75,17 +77,26 @@ class Stepper:
         if demag:
             if N is None:
                 N = GenerateDemagTensor(nx, ny, nz, *self.delta, *self.pbc, repeat=periodic_repeat,
-                                        padding=padding)
+                                        padding=padding, cache_dir=cache_dir)
             N = np.ascontiguousarray(N, dtype=np.float64)
             keep.append(N)
             d.N_host = N.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
             d.ez, d.ey, d.ex = N.shape[:3]
         self.N = N if demag else None
-        h = ctypes.c_void_p()
-        _lib.check(L.mm_stepper_create(ctypes.byref(d), ctypes.byref(h)))
-        self._h = h
+        self.desc = d
         self.precision = precision
         self.method = method
+
+
+class Stepper:
+    def __init__(self, shape, delta, Ms, alpha, **kw):
+        L = _lib.device_lib()
+        spec = SystemSpec(shape, delta, Ms, alpha, **kw)
+        self.shape, self.delta, self.n, self.pbc = spec.shape, spec.delta, spec.n, spec.pbc
+        self.N, self.precision, self.method = spec.N, spec.precision, spec.method
+        h = ctypes.c_void_p()
+        _lib.check(L.mm_stepper_create(ctypes.byref(spec.desc), ctypes.byref(h)))
+        self._h = h
 
     def _dp(self, a):
         return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
@@ -167,6 +178,60 @@ class Stepper:
     def close(self):
         if getattr(self, "_h", None) and _lib._LIB is not None:
             _lib._LIB.mm_stepper_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Ensemble:
+    """Independent systems stepped together (mm_batch_*); `specs` is a list of
+    keyword dicts for SystemSpec (shape, delta, Ms, alpha, ...)."""
+
+    def __init__(self, specs, nstreams=4):
+        L = _lib.device_lib()
+        self.specs = [SystemSpec(**kw) for kw in specs]
+        arr = (_lib.SystemDesc * len(self.specs))(*[sp.desc for sp in self.specs])
+        h = ctypes.c_void_p()
+        _lib.check(L.mm_batch_create(arr, len(self.specs), int(nstreams), ctypes.byref(h)))
+        self._h = h
+
+    def __len__(self):
+        return len(self.specs)
+
+    @staticmethod
+    def _dp(a):
+        return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+    def set_m(self, i, M):
+        M = _field(M, self.specs[i].shape, 3).copy()
+        _lib.check(_lib.load().mm_batch_set_m(self._h, int(i), self._dp(M)))
+
+    def get_m(self, i):
+        nx, ny, nz = self.specs[i].shape
+        M = np.empty((3, nz, ny, nx))
+        _lib.check(_lib.load().mm_batch_get_m(self._h, int(i), self._dp(M)))
+        return M
+
+    def run(self, steps):
+        _lib.check(_lib.load().mm_batch_run(self._h, int(steps)))
+
+    def stats(self, i):
+        s = np.zeros(5)
+        _lib.check(_lib.load().mm_batch_stats(self._h, int(i), self._dp(s)))
+        return dict(t=s[0], h=s[1], h_next=s[2], attempts=int(s[3]), rejections=int(s[4]))
+
+    def average_m(self, i):
+        a = np.zeros(3)
+        _lib.check(_lib.load().mm_batch_average_m(self._h, int(i), self._dp(a)))
+        return a
+
+    def close(self):
+        if getattr(self, "_h", None) and _lib._LIB is not None:
+            _lib._LIB.mm_batch_destroy(self._h)
             self._h = None
 
     def __del__(self):

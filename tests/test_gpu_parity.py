@@ -567,3 +567,82 @@ def test_stepper_periodic_vs_oracle(pbc, method):
     assert s["attempts"] == so["attempts"] and s["rejections"] == so["rejections"]
     assert np.abs(st.get_m() - Mo).max() / 8e5 < 1e-10
     st.close()
+
+
+@pytest.mark.parametrize("method", ["rkf45", "dp54", "rk4"])
+def test_stepper_graph_replays_equal_eager_launches(method, monkeypatch):
+    # attempts replayed from captured HIP graphs against the same attempts
+    # launched kernel by kernel: bitwise equal after every call, with eager
+    # work (dM/dt evaluation, <M>) and synchronisation between the replays
+    from micromagnum_amd.solver import Stepper
+    shape, delta = (20, 16, 3), (3e-9, 3e-9, 2e-9)
+    kw = _system(shape, with_stt=True)
+    M = random_unit_field(shape, 51) * 1.1e6
+    out = {}
+    for graphs in ("1", "0"):
+        monkeypatch.setenv("MM_STEPPER_GRAPHS", graphs)
+        st = Stepper(shape, delta, precision="fp64", method=method, eps_abs=1e-4, eps_rel=1e-4, h0=5e-14, **kw)
+        st.set_m(M)
+        states = []
+        for rep in range(4):
+            st.run(3 + rep)
+            st.sync()
+            st.dmdt()
+            states.append((st.get_m(), st.stats(), st.average_m()))
+        st.set_m(M)  # back to a first attempt that computes k_0
+        st.run(2)
+        states.append((st.get_m(), st.stats(), st.average_m()))
+        out[graphs] = states
+        st.close()
+    for a, b in zip(out["1"], out["0"]):
+        assert np.array_equal(a[0], b[0])
+        assert a[1] == b[1]
+        assert np.array_equal(a[2], b[2])
+
+
+# ---------------------------------------------------------------- ensembles
+def _ensemble_specs(count, seed, precision="fp32", max_xy=96):
+    # the ensemble workload's distributions (sizes U[32,max_xy]^2 x U[1,4],
+    # Ms U[4e5,1.4e6], A U[5e-12,2e-11], K U[0,1e6] along a random axis,
+    # cells U[2,5] nm, alpha 0.5, RKF45), small sizes for a test
+    rng = np.random.default_rng(seed)
+    specs = []
+    for _ in range(count):
+        shape = (int(rng.integers(32, max_xy + 1)), int(rng.integers(32, max_xy + 1)), int(rng.integers(1, 5)))
+        delta = tuple(float(v) for v in rng.uniform(2e-9, 5e-9, 3))
+        ax = rng.standard_normal(3)
+        ax /= np.linalg.norm(ax)
+        specs.append(dict(shape=shape, delta=delta, Ms=float(rng.uniform(4e5, 1.4e6)), alpha=0.5,
+                          A=float(rng.uniform(5e-12, 2e-11)), k_uni=float(rng.uniform(0, 1e6)), axis1=tuple(ax),
+                          method="rkf45", precision=precision))
+    return specs
+
+
+def test_ensemble_equals_single_steppers_and_oracle():
+    from micromagnum_amd.solver import Ensemble, Stepper
+    specs = _ensemble_specs(6, 61)
+    specs[0]["precision"] = "fp64"
+    ens = Ensemble(specs, nstreams=3)
+    Ms = []
+    for i, sp in enumerate(specs):
+        M = random_unit_field(sp["shape"], 100 + i) * sp["Ms"]
+        Ms.append(M)
+        ens.set_m(i, M)
+    ens.run(12)
+    for i, sp in enumerate(specs):
+        kw = {k: v for k, v in sp.items() if k not in ("shape", "delta", "Ms", "alpha")}
+        st = Stepper(sp["shape"], sp["delta"], sp["Ms"], sp["alpha"], N=ens.specs[i].N, **kw)
+        st.set_m(Ms[i])
+        st.run(12)
+        assert np.array_equal(ens.get_m(i), st.get_m()), i
+        assert ens.stats(i) == st.stats(), i
+        st.close()
+    # the FP64 member against the oracle
+    sp = specs[0]
+    kw = {k: v for k, v in sp.items() if k in ("A", "k_uni", "axis1")}
+    Mo, so = O.System(sp["shape"], sp["delta"], Ms=sp["Ms"], alpha=sp["alpha"], N=ens.specs[0].N, **kw).run(
+        Ms[0], method="rkf45", steps=12)
+    s = ens.stats(0)
+    assert s["attempts"] == so["attempts"] and s["rejections"] == so["rejections"]
+    assert np.abs(ens.get_m(0) - Mo).max() / sp["Ms"] < 1e-10
+    ens.close()
