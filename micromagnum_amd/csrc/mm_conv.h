@@ -95,6 +95,9 @@ struct ConvImpl {
     ConvGeom g;
     ConvLaunch ln;
     FftPlanHost px, py, pz;
+    FftPlanHost pyh;        // length Py/2, for the pruned y transforms
+    bool prune_y = false;
+    int tk_yp = 1;          // kx columns per WG of the pruned y transforms
     T *kern = nullptr;      // 6 real planes [e][kz<Hz2][ky<Hy2][kx<Hx], pitch Hp, scaled 1/(PxPyPz)
     C2<T> *S1 = nullptr;    // [c][z<nz][y<ny][kx<Hx], pitch Hp
     C2<T> *S2 = nullptr;    // [c][z<nz][ky<Py][kx<Hx], pitch Hp   (3-D only)

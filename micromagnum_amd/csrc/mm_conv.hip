@@ -210,6 +210,101 @@ __global__ void k_yinv(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, in
     }
 }
 
+// Pruned y transforms for zero-padded lines (P = 2h even, data only in
+// y < ny <= h).  Forward: X[2m] = DFT_h(x)[m], X[2m+1] = DFT_h(x W_P^n)[m];
+// inverse: x[n] = IDFT_h(X_even)[n] + W_P^-n IDFT_h(X_odd)[n] for n < h.
+// Two half-length transforms per line halve the LDS per line, which lets a
+// workgroup take 128-byte column tiles.  twP: the length-P twiddle table.
+// Each thread keeps the elements e = tid + q*blockDim (col = e % tk,
+// n = e / tk) of its tile in registers between the two halves; blockDim =
+// tk * tline with tline >= h/8 gives at most 8 per thread.
+template <typename T>
+__global__ void k_yfwd_pr(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, int Hx, int Hp, int ny_in,
+                          int nzl, FftPlanDev ph, const C2<T> *__restrict__ twP, int tk)
+{
+    extern __shared__ char smem_raw[];
+    C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
+    const int h = ph.L, LP = lds_len(h);
+    const int kx0 = blockIdx.x * tk;
+    const int z = blockIdx.y, c = blockIdx.z;
+    const C2<T> *src = in + ((size_t)c * nzl + z) * (size_t)ny_in * Hp;
+    C2<T> *dst = out + ((size_t)c * nzl + z) * (size_t)(2 * h) * Hp;
+    constexpr int MAXE = 8;
+    C2<T> xr[MAXE];
+#pragma unroll
+    for (int q = 0; q < MAXE; ++q) {
+        const int e = threadIdx.x + q * blockDim.x;
+        const int col = e % tk, n = e / tk, kx = kx0 + col;
+        xr[q] = cmk<T>((T)0, (T)0);
+        if (e < tk * h) {
+            if (n < ny_in && kx < Hx) xr[q] = src[(size_t)n * Hp + kx];
+            sm[col * LP + sk(n)] = xr[q];
+        }
+    }
+    __syncthreads();
+    const int l = threadIdx.x / ph.tline, t = threadIdx.x % ph.tline;
+    lds_fft<T, false>(sm + (size_t)l * LP, ph, t, l < tk);
+    for (int e = threadIdx.x; e < tk * h; e += blockDim.x) {
+        const int col = e % tk, m = e / tk, kx = kx0 + col;
+        if (kx < Hx) dst[(size_t)(2 * m) * Hp + kx] = sm[col * LP + sk(m)];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < MAXE; ++q) {
+        const int e = threadIdx.x + q * blockDim.x;
+        const int col = e % tk, n = e / tk;
+        if (e < tk * h) sm[col * LP + sk(n)] = cmul<T>(xr[q], twP[n]);
+    }
+    __syncthreads();
+    lds_fft<T, false>(sm + (size_t)l * LP, ph, t, l < tk);
+    for (int e = threadIdx.x; e < tk * h; e += blockDim.x) {
+        const int col = e % tk, m = e / tk, kx = kx0 + col;
+        if (kx < Hx) dst[(size_t)(2 * m + 1) * Hp + kx] = sm[col * LP + sk(m)];
+    }
+}
+
+template <typename T>
+__global__ void k_yinv_pr(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, int Hx, int Hp, int ny_out,
+                          int nzl, FftPlanDev ph, const C2<T> *__restrict__ twP, int tk)
+{
+    extern __shared__ char smem_raw[];
+    C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
+    const int h = ph.L, LP = lds_len(h);
+    const int kx0 = blockIdx.x * tk;
+    const int z = blockIdx.y, c = blockIdx.z;
+    const C2<T> *src = in + ((size_t)c * nzl + z) * (size_t)(2 * h) * Hp;
+    C2<T> *dst = out + ((size_t)c * nzl + z) * (size_t)ny_out * Hp;
+    const int l = threadIdx.x / ph.tline, t = threadIdx.x % ph.tline;
+    constexpr int MAXE = 8;
+    C2<T> ar[MAXE];
+    for (int e = threadIdx.x; e < tk * h; e += blockDim.x) {
+        const int col = e % tk, m = e / tk, kx = kx0 + col;
+        sm[col * LP + sk(m)] = (kx < Hx) ? src[(size_t)(2 * m) * Hp + kx] : cmk<T>((T)0, (T)0);
+    }
+    __syncthreads();
+    lds_fft<T, true>(sm + (size_t)l * LP, ph, t, l < tk);
+#pragma unroll
+    for (int q = 0; q < MAXE; ++q) {
+        const int e = threadIdx.x + q * blockDim.x;
+        const int col = e % tk, n = e / tk;
+        ar[q] = (e < tk * h) ? sm[col * LP + sk(n)] : cmk<T>((T)0, (T)0);
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < tk * h; e += blockDim.x) {
+        const int col = e % tk, m = e / tk, kx = kx0 + col;
+        sm[col * LP + sk(m)] = (kx < Hx) ? src[(size_t)(2 * m + 1) * Hp + kx] : cmk<T>((T)0, (T)0);
+    }
+    __syncthreads();
+    lds_fft<T, true>(sm + (size_t)l * LP, ph, t, l < tk);
+#pragma unroll
+    for (int q = 0; q < MAXE; ++q) {
+        const int e = threadIdx.x + q * blockDim.x;
+        const int col = e % tk, n = e / tk, kx = kx0 + col;
+        if (e < tk * h && n < ny_out && kx < Hx)
+            dst[(size_t)n * Hp + kx] = cadd<T>(ar[q], cmul<T>(sm[col * LP + sk(n)], cconj<T>(twP[n])));
+    }
+}
+
 // real octant-stored kernel entry e at (kx, ky, kz) of the full spectrum
 template <typename T>
 __device__ __forceinline__ void kernel_at(const T *__restrict__ K, size_t plane, int Hp, int Hy2,
@@ -451,6 +546,20 @@ void ConvImpl<T>::init(const double *N_host, int nx, int ny, int nz, int ex, int
     py.build(g.Py);
     pz.build(g.Pz);
     ln = choose_launch(g, px, py, pz, sizeof(C2<T>));
+    // pruned y transforms when the padding is at least half of the line
+    {
+        const char *e = std::getenv("MM_PRUNE_Y");
+        const bool want = !(e && std::atoi(e) == 0);
+        prune_y = want && !g.is2d && g.Py % 2 == 0 && ny <= g.Py / 2 && g.Py / 2 >= 8;
+        if (prune_y) {
+            pyh.build(g.Py / 2);
+            const int lds = std::min(mm_max_lds_bytes(), 65536);
+            const size_t cs = sizeof(C2<T>);
+            int tk = env_int("MM_TK_YP", (int)(128 / cs));
+            while (tk > 1 && ((size_t)tk * pyh.tline > 1024 || (size_t)tk * lds_len(pyh.L) * cs > (size_t)lds)) tk /= 2;
+            tk_yp = tk;
+        }
+    }
     MM_REQUIRE((size_t)px.tline <= 1024, "x transform length too large");
 
     // re-embed the tensor into the radix-friendly padded grid
@@ -552,6 +661,7 @@ template <typename T> void ConvImpl<T>::release()
     px.release();
     py.release();
     pz.release();
+    pyh.release();
 }
 
 // algorithmic HBM bytes per launch (each array read or written once)
@@ -599,8 +709,15 @@ template <typename T> void ConvImpl<T>::spectral(hipStream_t s)
     }
     const int tk = ln.tk_y;
     const size_t shy = (size_t)tk * lds_len(py.L) * cs;
-    hipLaunchKernelGGL((k_yfwd<T>), dim3((g.Hx + tk - 1) / tk, g.nz, 3), dim3(tk * py.tline), shy, s, S1, S2, g.Hx, g.Hp,
-                       g.ny, g.nz, py.dev<T>(), tk);
+    const int tkp = tk_yp;
+    const size_t shyp = (size_t)tkp * lds_len(pyh.L) * cs;
+    if (prune_y) {
+        hipLaunchKernelGGL((k_yfwd_pr<T>), dim3((g.Hx + tkp - 1) / tkp, g.nz, 3), dim3(tkp * pyh.tline), shyp, s, S1, S2,
+                           g.Hx, g.Hp, g.ny, g.nz, pyh.dev<T>(), (const C2<T> *)py.dev<T>().tw, tkp);
+    } else {
+        hipLaunchKernelGGL((k_yfwd<T>), dim3((g.Hx + tk - 1) / tk, g.nz, 3), dim3(tk * py.tline), shy, s, S1, S2, g.Hx,
+                           g.Hp, g.ny, g.nz, py.dev<T>(), tk);
+    }
     MM_HIP(hipGetLastError());
     if (klog) klog->mark(s, KC_YFWD, conv_bytes<T>(g, KC_YFWD));
     const int tz = ln.tk_z;
@@ -609,8 +726,13 @@ template <typename T> void ConvImpl<T>::spectral(hipStream_t s)
                        g.Hx, g.Hp, g.Hy2, g.Py, g.nz, pz.dev<T>(), tz);
     MM_HIP(hipGetLastError());
     if (klog) klog->mark(s, KC_ZMI, conv_bytes<T>(g, KC_ZMI));
-    hipLaunchKernelGGL((k_yinv<T>), dim3((g.Hx + tk - 1) / tk, g.nz, 3), dim3(tk * py.tline), shy, s, S2, S1, g.Hx, g.Hp,
-                       g.ny, g.nz, py.dev<T>(), tk);
+    if (prune_y) {
+        hipLaunchKernelGGL((k_yinv_pr<T>), dim3((g.Hx + tkp - 1) / tkp, g.nz, 3), dim3(tkp * pyh.tline), shyp, s, S2, S1,
+                           g.Hx, g.Hp, g.ny, g.nz, pyh.dev<T>(), (const C2<T> *)py.dev<T>().tw, tkp);
+    } else {
+        hipLaunchKernelGGL((k_yinv<T>), dim3((g.Hx + tk - 1) / tk, g.nz, 3), dim3(tk * py.tline), shy, s, S2, S1, g.Hx,
+                           g.Hp, g.ny, g.nz, py.dev<T>(), tk);
+    }
     MM_HIP(hipGetLastError());
     if (klog) klog->mark(s, KC_YINV, conv_bytes<T>(g, KC_YINV));
 }
