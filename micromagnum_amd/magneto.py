@@ -28,8 +28,11 @@ _F32 = np.dtype(np.float32)
 
 
 def _ptr(t, dtype=None):
+    from .matty import _Abstract
+    if isinstance(t, _Abstract):  # Matrix / VectorMatrix / Field / VectorField
+        t = t.device()
     if not hasattr(t, "ptr") or not hasattr(t, "dtype"):
-        raise TypeError("expected a micromagnum_amd.device.DeviceArray")
+        raise TypeError("expected a micromagnum_amd.device.DeviceArray or a matty Matrix/VectorMatrix")
     if dtype is not None and t.dtype != dtype:
         raise TypeError("expected dtype %s, got %s" % (dtype, t.dtype))
     return t.ptr
@@ -44,16 +47,52 @@ def _f64():
 
 
 def _n(t):
+    from .matty import _Abstract
+    if isinstance(t, _Abstract):
+        return t.size() * t._comps
     return int(t.numel())
 
 
-def exchange(nx, ny, nz, dx, dy, dz, px, py, pz, Ms, A, M, H):
-    """exchange(nx,ny,nz,dx,dy,dz,px,py,pz,Ms,A,M,H) -> energy
+def _rwptr(t, dtype=None):
+    """pointer of an argument updated in place"""
+    from .matty import _Abstract
+    if isinstance(t, _Abstract):
+        d = t.device()
+        t.mark_device()
+        return _ptr(d, dtype)
+    return _ptr(t, dtype)
+
+
+def _wptr(t, dtype=None):
+    """pointer of an output argument (a matty object is then marked as
+    holding device data)"""
+    from .matty import _Abstract
+    if isinstance(t, _Abstract):
+        d = t.device_out()
+        t.mark_device()
+        return _ptr(d, dtype)
+    return _ptr(t, dtype)
+
+
+def exchange(*args):
+    """exchange(nx,ny,nz,dx,dy,dz,px,py,pz,Ms,A,M,H) -> energy, or
+    exchange(Ms, A, M, H) with Fields on a RectangularMesh
     (src/magneto/bindings/mmm/exchange.i:24-39; exchange_cpu.cpp:60-235)."""
+    if len(args) == 4:
+        Ms, A, M, H = args
+        mesh = M.mesh
+        nx, ny, nz = mesh.num_nodes
+        dx, dy, dz = mesh.delta
+        px, py, pz = mesh.pbc
+    elif len(args) == 13:
+        nx, ny, nz, dx, dy, dz, px, py, pz, Ms, A, M, H = args
+    else:
+        raise TypeError("exchange() takes 4 or 13 arguments")
     L = _lib.device_lib()
     e = ctypes.c_double(0.0)
-    _lib.check(L.mm_exchange(nx, ny, nz, dx, dy, dz, int(px), int(py), int(pz), _ptr(Ms, _f64()),
-                             _ptr(A, _f64()), _ptr(M, _f64()), _ptr(H, _f64()), ctypes.byref(e), _stream()))
+    pMs, pA, pM = _ptr(Ms, _f64()), _ptr(A, _f64()), _ptr(M, _f64())
+    _lib.check(L.mm_exchange(nx, ny, nz, dx, dy, dz, int(px), int(py), int(pz), pMs, pA, pM, _wptr(H, _f64()),
+                             ctypes.byref(e), _stream()))
     return e.value
 
 
@@ -68,7 +107,7 @@ def uniaxial_anisotropy(axis, k, Ms, M, H):
     L = _lib.device_lib()
     e = ctypes.c_double(0.0)
     _lib.check(L.mm_uniaxial_anisotropy(_n(Ms), _ptr(axis, _f64()), _ptr(k, _f64()), _ptr(Ms, _f64()),
-                                        _ptr(M, _f64()), _ptr(H, _f64()), ctypes.byref(e), _stream()))
+                                        _ptr(M, _f64()), _wptr(H, _f64()), ctypes.byref(e), _stream()))
     return e.value
 
 
@@ -77,7 +116,7 @@ def cubic_anisotropy(axis1, axis2, k, Ms, M, H):
     L = _lib.device_lib()
     e = ctypes.c_double(0.0)
     _lib.check(L.mm_cubic_anisotropy(_n(Ms), _ptr(axis1, _f64()), _ptr(axis2, _f64()), _ptr(k, _f64()),
-                                     _ptr(Ms, _f64()), _ptr(M, _f64()), _ptr(H, _f64()), ctypes.byref(e),
+                                     _ptr(Ms, _f64()), _ptr(M, _f64()), _wptr(H, _f64()), ctypes.byref(e),
                                      _stream()))
     return e.value
 
@@ -86,7 +125,7 @@ def llge(f1, f2, M, H, dM):
     """llge(f1,f2,M,H,dM): dM = f1 MxH + f2 Mx(MxH) (llge.i:24-29)."""
     L = _lib.device_lib()
     _lib.check(L.mm_llge(_n(f1), _ptr(f1, _f64()), _ptr(f2, _f64()), _ptr(M, _f64()), _ptr(H, _f64()),
-                         _ptr(dM, _f64()), _stream()))
+                         _wptr(dM, _f64()), _stream()))
 
 
 def fdm_zhangli(nx, ny, nz, dx, dy, dz, do_precess, P, xi, Ms, alpha, j, M, dM):
@@ -94,10 +133,13 @@ def fdm_zhangli(nx, ny, nz, dx, dy, dz, do_precess, P, xi, Ms, alpha, j, M, dM):
     L = _lib.device_lib()
     _lib.check(L.mm_fdm_zhangli(nx, ny, nz, dx, dy, dz, int(bool(do_precess)), _ptr(P, _f64()), _ptr(xi, _f64()),
                                 _ptr(Ms, _f64()), _ptr(alpha, _f64()), _ptr(j, _f64()), _ptr(M, _f64()),
-                                _ptr(dM, _f64()), _stream()))
+                                _wptr(dM, _f64()), _stream()))
 
 
 def _is_uniform(t):
+    from .matty import _Abstract
+    if isinstance(t, _Abstract):
+        return t.isUniform()
     a = t.numpy().reshape(-1)
     return bool(np.all(a == a[0]))
 
@@ -137,7 +179,9 @@ def average(M):
 def normalize(M, length, mode=0):
     """VectorMatrix.normalize(Matrix len) in place (VectorMatrix.cpp:218-230)."""
     L = _lib.device_lib()
-    _lib.check(L.mm_normalize(_n(length), _ptr(M, _f64()), _ptr(length, _f64()), int(mode), _stream()))
+    pl = _ptr(length, _f64())
+    pM = _rwptr(M, _f64())
+    _lib.check(L.mm_normalize(_n(length), pM, pl, int(mode), _stream()))
 
 
 def round_tensor_dimension(dim, periodic, padding):
@@ -189,7 +233,8 @@ class SymmetricMatrixVectorConvolution_FFT:
     def execute(self, M, H):
         L = _lib.device_lib()
         if self.precision == _lib.MM_FP64:
-            _lib.check(L.mm_conv_execute(self._h, _ptr(M, _F64), _ptr(H, _F64), _stream()))
+            pM = _ptr(M, _F64)
+            _lib.check(L.mm_conv_execute(self._h, pM, _wptr(H, _F64), _stream()))
         else:
             _lib.check(L.mm_conv_execute_f32(self._h, _ptr(M, _F32), _ptr(H, _F32), _stream()))
 
@@ -243,12 +288,14 @@ def _kptrs(ks):
 def rk_prepare_step(step, h, tab, k, y, ytmp):
     """rk_prepare_step(step, h, tab, k0..k5, y, ytmp) (evolver.i:34-95); k is a list."""
     L = _lib.device_lib()
-    _lib.check(L.mm_rk_prepare_step(int(step), float(h), ctypes.byref(tab._t), _n(y) // 3, _kptrs(k),
-                                    _ptr(y, _f64()), _ptr(ytmp, _f64()), _stream()))
+    ks, py = _kptrs(k), _ptr(y, _f64())
+    _lib.check(L.mm_rk_prepare_step(int(step), float(h), ctypes.byref(tab._t), _n(y) // 3, ks, py,
+                                    _wptr(ytmp, _f64()), _stream()))
 
 
 def rk_combine_result(h, tab, k, y, y_err):
     """rk_combine_result(h, tab, k0..k5, y, y_err) (evolver.i:34-95); k is a list."""
     L = _lib.device_lib()
-    _lib.check(L.mm_rk_combine_result(float(h), ctypes.byref(tab._t), _n(y) // 3, _kptrs(k), _ptr(y, _f64()),
-                                      _ptr(y_err, _f64()), _stream()))
+    ks = _kptrs(k)
+    _lib.check(L.mm_rk_combine_result(float(h), ctypes.byref(tab._t), _n(y) // 3, ks, _rwptr(y, _f64()),
+                                      _wptr(y_err, _f64()), _stream()))
