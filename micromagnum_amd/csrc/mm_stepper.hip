@@ -295,12 +295,14 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
 }
 
 // x inverse of the demag spectrum rows + epilogue.  rpw rows per block, all
-// three components of a row in the same block.  has_demag == 0: rows are
-// processed without the FFT part (H_demag = 0).  EN: instead of the stage
+// three components of a row in the same block.  has_demag: 0 no stray field,
+// 1 inverse x transform of S1 rows in LDS, 2 H_demag read from Hdem (lines
+// too long for three in one workgroup's LDS).  EN: instead of the stage
 // epilogue, per-block sums of M.H per term into epart[q * gridDim.x + block].
 template <typename T, bool EN>
 __global__ void k_stage(SysDev<T> S, StageDev<T> G, StepCtl *ctl, const C2<T> *__restrict__ S1, FftPlanDev p,
-                        int Hx, int Hp, int rpw, int has_demag, double *__restrict__ epart)
+                        int Hx, int Hp, int rpw, int has_demag, double *__restrict__ epart,
+                        const T *__restrict__ Hdem)
 {
     if (!EN && ctl->stop) return;
     extern __shared__ char smem_raw[];
@@ -310,7 +312,7 @@ __global__ void k_stage(SysDev<T> S, StageDev<T> G, StepCtl *ctl, const C2<T> *_
     const int row0 = blockIdx.x * rpw;
     const T *Y = G.Ys ? G.Ys : (ctl->cur ? G.yb1 : G.yb0);
     const int L = p.L, LP = lds_len(L);
-    if (has_demag) {
+    if (has_demag == 1) {
         const int l = threadIdx.x / p.tline, t = threadIdx.x % p.tline;
         const int c = l / rpw, r = l % rpw;
         const int gr = row0 + r;
@@ -325,13 +327,18 @@ __global__ void k_stage(SysDev<T> S, StageDev<T> G, StepCtl *ctl, const C2<T> *_
     }
     StageRed red = {0.0, 0.0, 0.0, 0.0};
     EnergyAcc ea = {0.0, 0.0, 0.0, 0.0};
-    const bool fused = !EN && has_demag && G.fwd;
+    const bool fused = !EN && has_demag == 1 && G.fwd;
     for (int e = threadIdx.x; e < rpw * nx; e += blockDim.x) {
         const int r = e / nx, x = e % nx;
         const int gr = row0 + r;
         if (gr >= nrows) continue;
         V3<T> Hd = v3<T>(0, 0, 0);
-        if (has_demag) Hd = v3<T>(sm[(0 * rpw + r) * LP + sk(x)].x, sm[(1 * rpw + r) * LP + sk(x)].x, sm[(2 * rpw + r) * LP + sk(x)].x);
+        if (has_demag == 1) {
+            Hd = v3<T>(sm[(0 * rpw + r) * LP + sk(x)].x, sm[(1 * rpw + r) * LP + sk(x)].x, sm[(2 * rpw + r) * LP + sk(x)].x);
+        } else if (has_demag == 2) {
+            const size_t i = (size_t)gr * nx + x;
+            Hd = v3<T>(Hdem[i], Hdem[S.n + i], Hdem[2 * S.n + i]);
+        }
         T nxt[3];
         stage_cell<T, EN>(S, G, ctl, Y, x, gr % ny, gr / ny, Hd, red, ea, nxt);
         // the slot (r, x) of the three lines is this thread's alone: the next
@@ -645,6 +652,8 @@ template <typename T> struct Stepper : public StepperBase {
     StepCtl *ctl = nullptr;
     int rpw = 1;
     size_t stage_lds = 0;
+    bool split_x = false;   // inverse x transform as a separate pass
+    T *Hdem = nullptr;      // H_demag of the split mode
     int grid_rows = 1;
     int threads = 256;
 
@@ -804,13 +813,23 @@ template <typename T> struct Stepper : public StepperBase {
             conv = new ConvImpl<T>();
             conv->init(d.N_host, d.nx, d.ny, d.nz, d.ex, d.ey, d.ez);
             const int tl = conv->px.tline;
-            rpw = std::max(1, 192 / (3 * tl));
-            if (const char *e = std::getenv("MM_RPW")) rpw = std::max(1, std::atoi(e));
-            while (rpw > 1 && (size_t)3 * rpw * lds_len(conv->px.L) * sizeof(C2<T>) > (size_t)mm_max_lds_bytes()) --rpw;
-            MM_REQUIRE(3 * tl <= 1024, "x transform too long for the fused stage kernel");
-            threads = 3 * rpw * tl;
-            stage_lds = (size_t)3 * rpw * lds_len(conv->px.L) * sizeof(C2<T>);
-            MM_REQUIRE(stage_lds <= (size_t)mm_max_lds_bytes(), "fused stage kernel exceeds the LDS maximum");
+            split_x = 3 * tl > 1024 ||
+                      (size_t)3 * lds_len(conv->px.L) * sizeof(C2<T>) > (size_t)mm_max_lds_bytes();
+            if (const char *e = std::getenv("MM_SPLIT_X")) split_x = split_x || std::atoi(e) != 0;
+            if (split_x) {
+                // the inverse x transform runs as its own pass into Hdem
+                rpw = std::max(1, 256 / d.nx);
+                threads = 256;
+                stage_lds = 0;
+                Hdem = dalloc(3 * n);
+            } else {
+                rpw = std::max(1, 192 / (3 * tl));
+                if (const char *e = std::getenv("MM_RPW")) rpw = std::max(1, std::atoi(e));
+                while (rpw > 1 && (size_t)3 * rpw * lds_len(conv->px.L) * sizeof(C2<T>) > (size_t)mm_max_lds_bytes())
+                    --rpw;
+                threads = 3 * rpw * tl;
+                stage_lds = (size_t)3 * rpw * lds_len(conv->px.L) * sizeof(C2<T>);
+            }
         } else {
             rpw = std::max(1, 256 / d.nx);
             threads = 256;
@@ -872,6 +891,7 @@ template <typename T> struct Stepper : public StepperBase {
     {
         const double V = 3.0 * sizeof(T) * (double)n;
         double b = conv ? 3.0 * 2.0 * sizeof(T) * conv->g.Hx * d.ny * d.nz : 0.0;
+        if (split_x) b += 2.0 * V;                 // Hdem written by the x pass, read here
         b += V;                                    // stage input
         if (G.prep && G.Ys) b += V;                // y (same array as the input on stage 0)
         b += V * G.nk;                             // stored stage vectors
@@ -883,6 +903,8 @@ template <typename T> struct Stepper : public StepperBase {
     }
 
     // one stage: dM/dt at stage input Ys (null = y[cur])
+    int demag_mode() const { return conv ? (split_x ? 2 : 1) : 0; }
+
     // need_fwd: S1 does not yet hold the x transform of the stage input
     void launch_stage(const T *Ys, const StageDev<T> &G, bool need_fwd = true)
     {
@@ -893,10 +915,11 @@ template <typename T> struct Stepper : public StepperBase {
                 else conv->forward_x(yb[0], s, yb[1], &ctl->cur);
             }
             conv->spectral(s);
+            if (split_x) conv->inverse_x(Hdem, s);
         }
         hipLaunchKernelGGL((k_stage<T, false>), dim3(grid_rows), dim3(threads), stage_lds, s, sys, G, ctl,
                            conv ? conv->S1 : nullptr, conv ? conv->px.template dev<T>() : FftPlanDev(),
-                           conv ? conv->g.Hx : 0, conv ? conv->g.Hp : 0, rpw, conv ? 1 : 0, nullptr);
+                           conv ? conv->g.Hx : 0, conv ? conv->g.Hp : 0, rpw, demag_mode(), nullptr, Hdem);
         MM_HIP(hipGetLastError());
         if (klog) klog->mark(s, KC_STAGE, stage_bytes(G));
     }
@@ -951,7 +974,7 @@ template <typename T> struct Stepper : public StepperBase {
             StageDev<T> G = blank_stage(Ys);
             G.kout = (st < ns - 1) ? k[st] : nullptr;
             G.want_dydt = (st == 0) && !fsal;
-            if (conv && st < ns - 1) {
+            if (conv && !split_x && st < ns - 1) {
                 G.fwd = 1;
                 G.fwd_S1 = conv->S1;
             }
@@ -1083,10 +1106,11 @@ template <typename T> struct Stepper : public StepperBase {
             conv->klog = nullptr;
             conv->forward_x(yb[0], s, yb[1], &ctl->cur);
             conv->spectral(s);
+            if (split_x) conv->inverse_x(Hdem, s);
         }
         hipLaunchKernelGGL((k_stage<T, true>), dim3(grid_rows), dim3(threads), stage_lds, s, sys, G, ctl,
                            conv ? conv->S1 : nullptr, conv ? conv->px.template dev<T>() : FftPlanDev(),
-                           conv ? conv->g.Hx : 0, conv ? conv->g.Hp : 0, rpw, conv ? 1 : 0, epart);
+                           conv ? conv->g.Hx : 0, conv ? conv->g.Hp : 0, rpw, demag_mode(), epart, Hdem);
         MM_HIP(hipGetLastError());
         std::vector<double> h((size_t)4 * grid_rows);
         MM_HIP(hipMemcpyAsync(h.data(), epart, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));

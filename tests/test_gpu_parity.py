@@ -646,3 +646,25 @@ def test_ensemble_equals_single_steppers_and_oracle():
     assert s["attempts"] == so["attempts"] and s["rejections"] == so["rejections"]
     assert np.abs(ens.get_m(0) - Mo).max() / sp["Ms"] < 1e-10
     ens.close()
+
+
+@pytest.mark.parametrize("method", ["rkf45", "dp54"])
+def test_stepper_split_x_equals_fused(method, monkeypatch):
+    # lines too long for three in one workgroup's LDS take a separate inverse
+    # x pass (forced here): the arithmetic is the same, so the trajectories
+    # are bitwise equal
+    from micromagnum_amd.solver import Stepper
+    shape, delta = (24, 20, 3), (3e-9, 3e-9, 2e-9)
+    kw = _system(shape, with_stt=True)
+    M = random_unit_field(shape, 71) * 1.1e6
+    out = {}
+    for split in ("0", "1"):
+        monkeypatch.setenv("MM_SPLIT_X", split)
+        st = Stepper(shape, delta, precision="fp64", method=method, **kw)
+        st.set_m(M)
+        st.run(8)
+        out[split] = (st.get_m(), st.stats(), st.energy())
+        st.close()
+    assert np.array_equal(out["0"][0], out["1"][0])
+    assert out["0"][1] == out["1"][1]
+    assert out["0"][2] == out["1"][2]
