@@ -346,17 +346,6 @@ __global__ void k_zmi(C2<T> *__restrict__ S2, const T *__restrict__ K, size_t pl
     const size_t cstride = (size_t)nz * Py * Hp;
     const size_t zstride = (size_t)Py * Hp;
     C2<T> *base = S2 + (size_t)ky * Hp;
-    // kernel entries of this thread's (kx, kz) points, loaded before the
-    // transform so that their latency overlaps it; blockDim = 3 tk tline with
-    // tline >= L/8 gives at most 3 points per thread
-    constexpr int MAXP = 3;
-    T kr[MAXP][6];
-#pragma unroll
-    for (int q = 0; q < MAXP; ++q) {
-        const int e = threadIdx.x + q * blockDim.x;
-        const int col = e % tk, kz = e / tk, kx = kx0 + col;
-        if (e < tk * L && kx < Hx) kernel_at<T>(K, plane, Hp, Hy2, Py, L, kx, ky, kz, kr[q]);
-    }
     for (int e = threadIdx.x; e < 3 * tk * L; e += blockDim.x) {
         const int col = e % tk, rest = e / tk, z = rest % L, c = rest / L, kx = kx0 + col;
         sm[(c * tk + col) * LP + sk(z)] = (z < nz && kx < Hx) ? base[c * cstride + z * zstride + kx] : cmk<T>((T)0, (T)0);
@@ -366,14 +355,7 @@ __global__ void k_zmi(C2<T> *__restrict__ S2, const T *__restrict__ K, size_t pl
     const bool active = l < 3 * tk;
     C2<T> *line = sm + (size_t)l * LP;
     lds_fft<T, false>(line, p, t, active);
-#pragma unroll
-    for (int q = 0; q < MAXP; ++q) {
-        const int e = threadIdx.x + q * blockDim.x;
-        const int col = e % tk, kz = e / tk, kx = kx0 + col;
-        if (e < tk * L && kx < Hx)
-            mult3<T>(sm[(0 * tk + col) * LP + sk(kz)], sm[(1 * tk + col) * LP + sk(kz)], sm[(2 * tk + col) * LP + sk(kz)], kr[q]);
-    }
-    for (int e = threadIdx.x + MAXP * blockDim.x; e < tk * L; e += blockDim.x) {
+    for (int e = threadIdx.x; e < tk * L; e += blockDim.x) {
         const int col = e % tk, kz = e / tk, kx = kx0 + col;
         if (kx < Hx) {
             T k[6];
@@ -548,8 +530,10 @@ void ConvImpl<T>::init(const double *N_host, int nx, int ny, int nz, int ex, int
     ln = choose_launch(g, px, py, pz, sizeof(C2<T>));
     // pruned y transforms when the padding is at least half of the line
     {
+        // measured on MI355X (256x256x16 cells): slower than the full-length
+        // y transforms in FP32 (48 vs 38 us per launch), equal in FP64; opt-in
         const char *e = std::getenv("MM_PRUNE_Y");
-        const bool want = !(e && std::atoi(e) == 0);
+        const bool want = e && std::atoi(e) != 0;
         prune_y = want && !g.is2d && g.Py % 2 == 0 && ny <= g.Py / 2 && g.Py / 2 >= 8;
         if (prune_y) {
             pyh.build(g.Py / 2);

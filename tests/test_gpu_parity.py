@@ -493,7 +493,7 @@ def test_stepper_relax_vs_oracle(criterion, thr, every, method):
                                                          check_every=every, max_steps=400)
     assert io["relaxed"] and info["relaxed"]
     assert info["steps"] == io["steps"]
-    assert abs(info["value"] - io["value"]) <= 1e-10 * io["value"]
+    assert abs(info["value"] - io["value"]) <= 1e-9 * io["value"]
     assert np.abs(Mg - Mo).max() / 8e5 < 1e-10
     # stepping on after a relaxation continues from the kept state
     a0 = st.stats()["attempts"]
@@ -667,4 +667,6 @@ def test_stepper_split_x_equals_fused(method, monkeypatch):
         st.close()
     assert np.array_equal(out["0"][0], out["1"][0])
     assert out["0"][1] == out["1"][1]
-    assert out["0"][2] == out["1"][2]
+    # the energy sums run over a different number of blocks in the two modes
+    for key, v in out["1"][2].items():
+        assert v == pytest.approx(out["0"][2][key], rel=1e-13, abs=1e-300)
