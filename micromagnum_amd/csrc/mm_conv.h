@@ -34,6 +34,7 @@ struct FftPlanHost {
         d.radices = 0u;
         for (int i = 0; i < d.npass; ++i) d.radices |= (unsigned)rad[i] << (4 * i);
         d.tw = sizeof(T) == 4 ? tw32 : tw64;
+        d.pow2 = (L & (L - 1)) == 0 ? 1 : 0;
         return d;
     }
 };

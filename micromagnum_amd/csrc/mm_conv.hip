@@ -145,6 +145,8 @@ __global__ void k_xfwd(const TIN *__restrict__ in, size_t in_cstride, C2<T> *__r
     const int g = blockIdx.x * lpw + l;
     const bool active = g < nrows;
     C2<T> *line = sm + (size_t)l * lds_len(L);
+    C2<T> *twl = sm + (size_t)lpw * lds_len(L);
+    load_twiddles<T>(twl, p);
     int c = 0, r = 0;
     if (active) {
         c = g / (ny * nz);
@@ -153,7 +155,7 @@ __global__ void k_xfwd(const TIN *__restrict__ in, size_t in_cstride, C2<T> *__r
         for (int x = t; x < L; x += p.tline) line[sk(x)] = cmk<T>(x < nx_in ? (T)src[x] : (T)0, (T)0);
     }
     __syncthreads();
-    lds_fft<T, false>(line, p, t, active);
+    lds_fft<T, false>(line, p, t, active, twl);
     if (active) {
         C2<T> *dst = out + c * out_cstride + (size_t)r * Hp;
         for (int kx = t; kx < Hx; kx += p.tline) dst[kx] = line[sk(kx)];
@@ -172,13 +174,15 @@ __global__ void k_yfwd(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, in
     const int z = blockIdx.y, c = blockIdx.z;
     const C2<T> *src = in + ((size_t)c * nzl + z) * (size_t)ny_in * Hp;
     C2<T> *dst = out + ((size_t)c * nzl + z) * (size_t)L * Hp;
+    C2<T> *twl = sm + (size_t)tk * LP;
+    load_twiddles<T>(twl, p);
     for (int e = threadIdx.x; e < tk * L; e += blockDim.x) {
         const int col = e % tk, y = e / tk, kx = kx0 + col;
         sm[col * LP + sk(y)] = (y < ny_in && kx < Hx) ? src[(size_t)y * Hp + kx] : cmk<T>((T)0, (T)0);
     }
     __syncthreads();
     const int l = threadIdx.x / p.tline, t = threadIdx.x % p.tline;
-    lds_fft<T, false>(sm + (size_t)l * LP, p, t, l < tk);
+    lds_fft<T, false>(sm + (size_t)l * LP, p, t, l < tk, twl);
     for (int e = threadIdx.x; e < tk * L; e += blockDim.x) {
         const int col = e % tk, y = e / tk, kx = kx0 + col;
         if (kx < Hx) dst[(size_t)y * Hp + kx] = sm[col * LP + sk(y)];
@@ -197,13 +201,15 @@ __global__ void k_yinv(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, in
     const int z = blockIdx.y, c = blockIdx.z;
     const C2<T> *src = in + ((size_t)c * nzl + z) * (size_t)L * Hp;
     C2<T> *dst = out + ((size_t)c * nzl + z) * (size_t)ny_out * Hp;
+    C2<T> *twl = sm + (size_t)tk * LP;
+    load_twiddles<T>(twl, p);
     for (int e = threadIdx.x; e < tk * L; e += blockDim.x) {
         const int col = e % tk, y = e / tk, kx = kx0 + col;
         sm[col * LP + sk(y)] = (kx < Hx) ? src[(size_t)y * Hp + kx] : cmk<T>((T)0, (T)0);
     }
     __syncthreads();
     const int l = threadIdx.x / p.tline, t = threadIdx.x % p.tline;
-    lds_fft<T, true>(sm + (size_t)l * LP, p, t, l < tk);
+    lds_fft<T, true>(sm + (size_t)l * LP, p, t, l < tk, twl);
     for (int e = threadIdx.x; e < tk * ny_out; e += blockDim.x) {
         const int col = e % tk, y = e / tk, kx = kx0 + col;
         if (kx < Hx) dst[(size_t)y * Hp + kx] = sm[col * LP + sk(y)];
@@ -229,6 +235,8 @@ __global__ void k_yfwd_pr(const C2<T> *__restrict__ in, C2<T> *__restrict__ out,
     const int z = blockIdx.y, c = blockIdx.z;
     const C2<T> *src = in + ((size_t)c * nzl + z) * (size_t)ny_in * Hp;
     C2<T> *dst = out + ((size_t)c * nzl + z) * (size_t)(2 * h) * Hp;
+    C2<T> *twl = sm + (size_t)tk * LP;
+    load_twiddles<T>(twl, ph);
     constexpr int MAXE = 8;
     C2<T> xr[MAXE];
 #pragma unroll
@@ -243,7 +251,7 @@ __global__ void k_yfwd_pr(const C2<T> *__restrict__ in, C2<T> *__restrict__ out,
     }
     __syncthreads();
     const int l = threadIdx.x / ph.tline, t = threadIdx.x % ph.tline;
-    lds_fft<T, false>(sm + (size_t)l * LP, ph, t, l < tk);
+    lds_fft<T, false>(sm + (size_t)l * LP, ph, t, l < tk, twl);
     for (int e = threadIdx.x; e < tk * h; e += blockDim.x) {
         const int col = e % tk, m = e / tk, kx = kx0 + col;
         if (kx < Hx) dst[(size_t)(2 * m) * Hp + kx] = sm[col * LP + sk(m)];
@@ -256,7 +264,7 @@ __global__ void k_yfwd_pr(const C2<T> *__restrict__ in, C2<T> *__restrict__ out,
         if (e < tk * h) sm[col * LP + sk(n)] = cmul<T>(xr[q], twP[n]);
     }
     __syncthreads();
-    lds_fft<T, false>(sm + (size_t)l * LP, ph, t, l < tk);
+    lds_fft<T, false>(sm + (size_t)l * LP, ph, t, l < tk, twl);
     for (int e = threadIdx.x; e < tk * h; e += blockDim.x) {
         const int col = e % tk, m = e / tk, kx = kx0 + col;
         if (kx < Hx) dst[(size_t)(2 * m + 1) * Hp + kx] = sm[col * LP + sk(m)];
@@ -275,6 +283,8 @@ __global__ void k_yinv_pr(const C2<T> *__restrict__ in, C2<T> *__restrict__ out,
     const C2<T> *src = in + ((size_t)c * nzl + z) * (size_t)(2 * h) * Hp;
     C2<T> *dst = out + ((size_t)c * nzl + z) * (size_t)ny_out * Hp;
     const int l = threadIdx.x / ph.tline, t = threadIdx.x % ph.tline;
+    C2<T> *twl = sm + (size_t)tk * LP;
+    load_twiddles<T>(twl, ph);
     constexpr int MAXE = 8;
     C2<T> ar[MAXE];
     for (int e = threadIdx.x; e < tk * h; e += blockDim.x) {
@@ -282,7 +292,7 @@ __global__ void k_yinv_pr(const C2<T> *__restrict__ in, C2<T> *__restrict__ out,
         sm[col * LP + sk(m)] = (kx < Hx) ? src[(size_t)(2 * m) * Hp + kx] : cmk<T>((T)0, (T)0);
     }
     __syncthreads();
-    lds_fft<T, true>(sm + (size_t)l * LP, ph, t, l < tk);
+    lds_fft<T, true>(sm + (size_t)l * LP, ph, t, l < tk, twl);
 #pragma unroll
     for (int q = 0; q < MAXE; ++q) {
         const int e = threadIdx.x + q * blockDim.x;
@@ -295,7 +305,7 @@ __global__ void k_yinv_pr(const C2<T> *__restrict__ in, C2<T> *__restrict__ out,
         sm[col * LP + sk(m)] = (kx < Hx) ? src[(size_t)(2 * m + 1) * Hp + kx] : cmk<T>((T)0, (T)0);
     }
     __syncthreads();
-    lds_fft<T, true>(sm + (size_t)l * LP, ph, t, l < tk);
+    lds_fft<T, true>(sm + (size_t)l * LP, ph, t, l < tk, twl);
 #pragma unroll
     for (int q = 0; q < MAXE; ++q) {
         const int e = threadIdx.x + q * blockDim.x;
@@ -346,6 +356,8 @@ __global__ void k_zmi(C2<T> *__restrict__ S2, const T *__restrict__ K, size_t pl
     const size_t cstride = (size_t)nz * Py * Hp;
     const size_t zstride = (size_t)Py * Hp;
     C2<T> *base = S2 + (size_t)ky * Hp;
+    C2<T> *twl = sm + (size_t)3 * tk * LP;
+    load_twiddles<T>(twl, p);
     for (int e = threadIdx.x; e < 3 * tk * L; e += blockDim.x) {
         const int col = e % tk, rest = e / tk, z = rest % L, c = rest / L, kx = kx0 + col;
         sm[(c * tk + col) * LP + sk(z)] = (z < nz && kx < Hx) ? base[c * cstride + z * zstride + kx] : cmk<T>((T)0, (T)0);
@@ -354,7 +366,7 @@ __global__ void k_zmi(C2<T> *__restrict__ S2, const T *__restrict__ K, size_t pl
     const int l = threadIdx.x / p.tline, t = threadIdx.x % p.tline;
     const bool active = l < 3 * tk;
     C2<T> *line = sm + (size_t)l * LP;
-    lds_fft<T, false>(line, p, t, active);
+    lds_fft<T, false>(line, p, t, active, twl);
     for (int e = threadIdx.x; e < tk * L; e += blockDim.x) {
         const int col = e % tk, kz = e / tk, kx = kx0 + col;
         if (kx < Hx) {
@@ -364,7 +376,7 @@ __global__ void k_zmi(C2<T> *__restrict__ S2, const T *__restrict__ K, size_t pl
         }
     }
     __syncthreads();
-    lds_fft<T, true>(line, p, t, active);
+    lds_fft<T, true>(line, p, t, active, twl);
     for (int e = threadIdx.x; e < 3 * tk * nz; e += blockDim.x) {
         const int col = e % tk, rest = e / tk, z = rest % nz, c = rest / nz, kx = kx0 + col;
         if (kx < Hx) base[c * cstride + z * zstride + kx] = sm[(c * tk + col) * LP + sk(z)];
@@ -382,6 +394,8 @@ __global__ void k_ymi2d(C2<T> *__restrict__ S1, const T *__restrict__ K, size_t 
     const int L = p.L, LP = lds_len(L);
     const int kx0 = blockIdx.x * tk;
     const size_t cstride = (size_t)ny * Hp;
+    C2<T> *twl = sm + (size_t)3 * tk * LP;
+    load_twiddles<T>(twl, p);
     for (int e = threadIdx.x; e < 3 * tk * L; e += blockDim.x) {
         const int col = e % tk, rest = e / tk, y = rest % L, c = rest / L, kx = kx0 + col;
         sm[(c * tk + col) * LP + sk(y)] = (y < ny && kx < Hx) ? S1[c * cstride + (size_t)y * Hp + kx] : cmk<T>((T)0, (T)0);
@@ -390,7 +404,7 @@ __global__ void k_ymi2d(C2<T> *__restrict__ S1, const T *__restrict__ K, size_t 
     const int l = threadIdx.x / p.tline, t = threadIdx.x % p.tline;
     const bool active = l < 3 * tk;
     C2<T> *line = sm + (size_t)l * LP;
-    lds_fft<T, false>(line, p, t, active);
+    lds_fft<T, false>(line, p, t, active, twl);
     for (int e = threadIdx.x; e < tk * L; e += blockDim.x) {
         const int col = e % tk, ky = e / tk, kx = kx0 + col;
         if (kx < Hx) {
@@ -400,7 +414,7 @@ __global__ void k_ymi2d(C2<T> *__restrict__ S1, const T *__restrict__ K, size_t 
         }
     }
     __syncthreads();
-    lds_fft<T, true>(line, p, t, active);
+    lds_fft<T, true>(line, p, t, active, twl);
     for (int e = threadIdx.x; e < 3 * tk * ny; e += blockDim.x) {
         const int col = e % tk, rest = e / tk, y = rest % ny, c = rest / ny, kx = kx0 + col;
         if (kx < Hx) S1[c * cstride + (size_t)y * Hp + kx] = sm[(c * tk + col) * LP + sk(y)];
@@ -419,6 +433,8 @@ __global__ void k_xinv(const C2<T> *__restrict__ in, TOUT *__restrict__ out, siz
     const int g = blockIdx.x * lpw + l;
     const bool active = g < nrows;
     C2<T> *line = sm + (size_t)l * lds_len(L);
+    C2<T> *twl = sm + (size_t)lpw * lds_len(L);
+    load_twiddles<T>(twl, p);
     int c = 0, r = 0;
     if (active) {
         c = g / (ny * nz);
@@ -427,7 +443,7 @@ __global__ void k_xinv(const C2<T> *__restrict__ in, TOUT *__restrict__ out, siz
         for (int k = t; k < L; k += p.tline) line[sk(k)] = k < Hx ? src[k] : cconj<T>(src[L - k]);
     }
     __syncthreads();
-    lds_fft<T, true>(line, p, t, active);
+    lds_fft<T, true>(line, p, t, active, twl);
     if (active) {
         TOUT *dst = out + c * out_cstride + (size_t)r * nx;
         for (int x = t; x < nx; x += p.tline) dst[x] = (TOUT)line[sk(x)].x;
@@ -443,13 +459,15 @@ __global__ void k_zfwd_setup(double2 *__restrict__ S, int Hx, int Hp, int Py, Ff
     const int kx0 = blockIdx.x * tk, ky = blockIdx.y;
     const size_t zstride = (size_t)Py * Hp;
     double2 *base = S + (size_t)ky * Hp;
+    double2 *twl = sm + (size_t)tk * LP;
+    load_twiddles<double>(twl, p);
     for (int e = threadIdx.x; e < tk * L; e += blockDim.x) {
         const int col = e % tk, z = e / tk, kx = kx0 + col;
         sm[col * LP + sk(z)] = kx < Hx ? base[z * zstride + kx] : cmk<double>(0.0, 0.0);
     }
     __syncthreads();
     const int l = threadIdx.x / p.tline, t = threadIdx.x % p.tline;
-    lds_fft<double, false>(sm + (size_t)l * LP, p, t, l < tk);
+    lds_fft<double, false>(sm + (size_t)l * LP, p, t, l < tk, twl);
     for (int e = threadIdx.x; e < tk * L; e += blockDim.x) {
         const int col = e % tk, z = e / tk, kx = kx0 + col;
         if (kx < Hx) base[z * zstride + kx] = sm[col * LP + sk(z)];
@@ -490,21 +508,27 @@ static int env_int(const char *name, int dflt)
 // tile*8 bytes).  Tiles are sized for several resident workgroups per CU so
 // that global loads of one overlap the FFT of another.  MM_LPW_X, MM_TK_Y and
 // MM_TK_Z override the choice (launch-shape sweeps).
+// LDS bytes of `nlines` transform lines of length L plus the plan's twiddle table
+static size_t lines_lds(int nlines, int L, size_t csize)
+{
+    return ((size_t)nlines * lds_len(L) + (size_t)L) * csize;
+}
+
 static ConvLaunch choose_launch(const ConvGeom &g, const FftPlanHost &px, const FftPlanHost &py,
                                 const FftPlanHost &pz, size_t csize)
 {
-    const int lds = std::min(mm_max_lds_bytes(), 65536);
+    const int lds = mm_max_lds_bytes();
     ConvLaunch ln;
     ln.lpw_x = env_int("MM_LPW_X", std::max(1, 256 / px.tline));
-    while (ln.lpw_x > 1 && ((size_t)ln.lpw_x * lds_len(px.L) * csize > (size_t)lds || ln.lpw_x * px.tline > 1024)) --ln.lpw_x;
+    while (ln.lpw_x > 1 && (lines_lds(ln.lpw_x, px.L, csize) > (size_t)lds || ln.lpw_x * px.tline > 1024)) --ln.lpw_x;
     const int nl_y = g.is2d ? 3 : 1;
     int tk = env_int("MM_TK_Y", 8);
-    while (tk > 1 && ((size_t)nl_y * tk * py.tline > 1024 || (size_t)nl_y * tk * lds_len(py.L) * csize > (size_t)lds)) tk /= 2;
+    while (tk > 1 && ((size_t)nl_y * tk * py.tline > 1024 || lines_lds(nl_y * tk, py.L, csize) > (size_t)lds)) tk /= 2;
     ln.tk_y = tk;
-    MM_REQUIRE((size_t)nl_y * tk * py.tline <= 1024 && (size_t)nl_y * tk * lds_len(py.L) * csize <= (size_t)lds,
+    MM_REQUIRE((size_t)nl_y * tk * py.tline <= 1024 && lines_lds(nl_y * tk, py.L, csize) <= (size_t)lds,
                "y transform length too large for one workgroup");
     int tz = env_int("MM_TK_Z", 16);
-    while (tz > 1 && ((size_t)3 * tz * pz.tline > 1024 || (size_t)3 * tz * lds_len(pz.L) * csize > (size_t)lds)) tz /= 2;
+    while (tz > 1 && ((size_t)3 * tz * pz.tline > 1024 || lines_lds(3 * tz, pz.L, csize) > (size_t)lds)) tz /= 2;
     ln.tk_z = tz;
     return ln;
 }
@@ -537,10 +561,10 @@ void ConvImpl<T>::init(const double *N_host, int nx, int ny, int nz, int ex, int
         prune_y = want && !g.is2d && g.Py % 2 == 0 && ny <= g.Py / 2 && g.Py / 2 >= 8;
         if (prune_y) {
             pyh.build(g.Py / 2);
-            const int lds = std::min(mm_max_lds_bytes(), 65536);
+            const int lds = mm_max_lds_bytes();
             const size_t cs = sizeof(C2<T>);
             int tk = env_int("MM_TK_YP", (int)(128 / cs));
-            while (tk > 1 && ((size_t)tk * pyh.tline > 1024 || (size_t)tk * lds_len(pyh.L) * cs > (size_t)lds)) tk /= 2;
+            while (tk > 1 && ((size_t)tk * pyh.tline > 1024 || lines_lds(tk, pyh.L, cs) > (size_t)lds)) tk /= 2;
             tk_yp = tk;
         }
     }
@@ -585,13 +609,13 @@ void ConvImpl<T>::init(const double *N_host, int nx, int ny, int nz, int ex, int
     const int lds = mm_max_lds_bytes();
     const FftPlanDev dx = px.dev<double>(), dy = py.dev<double>(), dz = pz.dev<double>();
     int lpw = std::max(1, 256 / px.tline);
-    while (lpw > 1 && (size_t)lpw * lds_len(px.L) * 16 > (size_t)lds) --lpw;
+    while (lpw > 1 && lines_lds(lpw, px.L, 16) > (size_t)lds) --lpw;
     int tky = 8;
-    while (tky > 1 && ((size_t)tky * py.tline > 1024 || (size_t)tky * lds_len(py.L) * 16 > (size_t)lds)) tky /= 2;
+    while (tky > 1 && ((size_t)tky * py.tline > 1024 || lines_lds(tky, py.L, 16) > (size_t)lds)) tky /= 2;
     int tkz = 8;
-    while (tkz > 1 && ((size_t)tkz * pz.tline > 1024 || (size_t)tkz * lds_len(pz.L) * 16 > (size_t)lds)) tkz /= 2;
-    const size_t shx = (size_t)lpw * lds_len(px.L) * 16, shy = (size_t)tky * lds_len(py.L) * 16,
-                 shz = (size_t)tkz * lds_len(pz.L) * 16;
+    while (tkz > 1 && ((size_t)tkz * pz.tline > 1024 || lines_lds(tkz, pz.L, 16) > (size_t)lds)) tkz /= 2;
+    const size_t shx = lines_lds(lpw, px.L, 16), shy = lines_lds(tky, py.L, 16),
+                 shz = lines_lds(tkz, pz.L, 16);
     set_lds(k_xfwd<double, double>, shx);
     set_lds(k_yfwd<double>, shy);
     set_lds(k_zfwd_setup, shz);
@@ -624,14 +648,14 @@ void ConvImpl<T>::init(const double *N_host, int nx, int ny, int nz, int ex, int
     if (!g.is2d) MM_HIP(hipMalloc(&S2, sizeof(C2<T>) * 3 * (size_t)nz * Py * g.Hp));
 
     const size_t cs = sizeof(C2<T>);
-    set_lds(k_xfwd<T, T>, (size_t)ln.lpw_x * lds_len(px.L) * cs);
-    set_lds(k_xinv<T, T>, (size_t)ln.lpw_x * lds_len(px.L) * cs);
+    set_lds(k_xfwd<T, T>, lines_lds(ln.lpw_x, px.L, cs));
+    set_lds(k_xinv<T, T>, lines_lds(ln.lpw_x, px.L, cs));
     if (g.is2d) {
-        set_lds(k_ymi2d<T>, (size_t)3 * ln.tk_y * lds_len(py.L) * cs);
+        set_lds(k_ymi2d<T>, lines_lds(3 * ln.tk_y, py.L, cs));
     } else {
-        set_lds(k_yfwd<T>, (size_t)ln.tk_y * lds_len(py.L) * cs);
-        set_lds(k_yinv<T>, (size_t)ln.tk_y * lds_len(py.L) * cs);
-        set_lds(k_zmi<T>, (size_t)3 * ln.tk_z * lds_len(pz.L) * cs);
+        set_lds(k_yfwd<T>, lines_lds(ln.tk_y, py.L, cs));
+        set_lds(k_yinv<T>, lines_lds(ln.tk_y, py.L, cs));
+        set_lds(k_zmi<T>, lines_lds(3 * ln.tk_z, pz.L, cs));
     }
 }
 
@@ -670,7 +694,7 @@ template <typename T> void ConvImpl<T>::forward_x(const T *M, hipStream_t s, con
 {
     const size_t n = (size_t)g.nx * g.ny * g.nz;
     const int nrows = 3 * g.ny * g.nz;
-    const size_t sh = (size_t)ln.lpw_x * lds_len(px.L) * sizeof(C2<T>);
+    const size_t sh = lines_lds(ln.lpw_x, px.L, sizeof(C2<T>));
     hipLaunchKernelGGL((k_xfwd<T, T>), dim3((nrows + ln.lpw_x - 1) / ln.lpw_x), dim3(ln.lpw_x * px.tline), sh, s,
                        M, n, S1, (size_t)g.ny * g.nz * g.Hp, g.Hx, g.Hp, g.ny, g.nz, nrows, g.nx, px.dev<T>(), ln.lpw_x,
                        M_alt, sel);
@@ -684,7 +708,7 @@ template <typename T> void ConvImpl<T>::spectral(hipStream_t s)
     const size_t plane = (size_t)g.Hp * g.Hy2 * g.Hz2;
     if (g.is2d) {
         const int tk = ln.tk_y;
-        const size_t sh = (size_t)3 * tk * lds_len(py.L) * cs;
+        const size_t sh = lines_lds(3 * tk, py.L, cs);
         hipLaunchKernelGGL((k_ymi2d<T>), dim3((g.Hx + tk - 1) / tk), dim3(3 * tk * py.tline), sh, s, S1, kern, plane,
                            g.Hx, g.Hp, g.Hy2, g.ny, py.dev<T>(), tk);
         MM_HIP(hipGetLastError());
@@ -692,9 +716,9 @@ template <typename T> void ConvImpl<T>::spectral(hipStream_t s)
         return;
     }
     const int tk = ln.tk_y;
-    const size_t shy = (size_t)tk * lds_len(py.L) * cs;
+    const size_t shy = lines_lds(tk, py.L, cs);
     const int tkp = tk_yp;
-    const size_t shyp = (size_t)tkp * lds_len(pyh.L) * cs;
+    const size_t shyp = lines_lds(tkp, pyh.L, cs);
     if (prune_y) {
         hipLaunchKernelGGL((k_yfwd_pr<T>), dim3((g.Hx + tkp - 1) / tkp, g.nz, 3), dim3(tkp * pyh.tline), shyp, s, S1, S2,
                            g.Hx, g.Hp, g.ny, g.nz, pyh.dev<T>(), (const C2<T> *)py.dev<T>().tw, tkp);
@@ -705,7 +729,7 @@ template <typename T> void ConvImpl<T>::spectral(hipStream_t s)
     MM_HIP(hipGetLastError());
     if (klog) klog->mark(s, KC_YFWD, conv_bytes<T>(g, KC_YFWD));
     const int tz = ln.tk_z;
-    const size_t shz = (size_t)3 * tz * lds_len(pz.L) * cs;
+    const size_t shz = lines_lds(3 * tz, pz.L, cs);
     hipLaunchKernelGGL((k_zmi<T>), dim3((g.Hx + tz - 1) / tz, g.Py), dim3(3 * tz * pz.tline), shz, s, S2, kern, plane,
                        g.Hx, g.Hp, g.Hy2, g.Py, g.nz, pz.dev<T>(), tz);
     MM_HIP(hipGetLastError());
@@ -725,7 +749,7 @@ template <typename T> void ConvImpl<T>::inverse_x(T *H, hipStream_t s)
 {
     const size_t n = (size_t)g.nx * g.ny * g.nz;
     const int nrows = 3 * g.ny * g.nz;
-    const size_t sh = (size_t)ln.lpw_x * lds_len(px.L) * sizeof(C2<T>);
+    const size_t sh = lines_lds(ln.lpw_x, px.L, sizeof(C2<T>));
     hipLaunchKernelGGL((k_xinv<T, T>), dim3((nrows + ln.lpw_x - 1) / ln.lpw_x), dim3(ln.lpw_x * px.tline), sh, s, S1,
                        H, n, g.Hx, g.Hp, g.ny, g.nz, nrows, g.nx, px.dev<T>(), ln.lpw_x);
     MM_HIP(hipGetLastError());

@@ -61,7 +61,19 @@ struct FftPlanDev {
     int npass;
     unsigned radices;
     const void *tw;        // L entries, tw[m] = exp(-2 pi i m / L), precision T
+    int pow2;              // L is a power of two (all radices 8, 4, 2)
 };
+
+// The kernels copy the plan's twiddle table into LDS next to their lines
+// (one read of L values per workgroup) before the barrier that precedes the
+// first transform, so no pass waits on a global load.
+template <typename T> __device__ __forceinline__ void load_twiddles(C2<T> *twl, const FftPlanDev &p)
+{
+    const C2<T> *g = (const C2<T> *)p.tw;
+    for (int i = threadIdx.x; i < p.L; i += blockDim.x) twl[i] = g[i];
+}
+
+template <int R> struct Log2R { static const int value = R == 8 ? 3 : (R == 4 ? 2 : 1); };
 
 // ---- size-R DFTs on registers --------------------------------------------
 template <typename T, bool INV> __device__ __forceinline__ void dft2(C2<T> *v)
@@ -153,30 +165,31 @@ template <int R, typename T, bool INV> __device__ __forceinline__ void dftR(C2<T
 template <int R, int EPT> struct MaxB { static const int value = (EPT + R - 1) / R; };
 
 // One Stockham pass over every line of the workgroup.  `active` is false for
-// threads with no line; they still take part in both barriers.
-template <int R, int EPT, typename T, bool INV>
-__device__ __forceinline__ void fft_pass(C2<T> *line, const FftPlanDev &p, int t, bool active, int Ns)
+// threads with no line; they still take part in both barriers.  Ns is the
+// product of the radices of the previous passes; P2: every length involved
+// is a power of two and lNs = log2(Ns), so index arithmetic is shifts/masks.
+template <int R, int EPT, typename T, bool INV, bool P2>
+__device__ __forceinline__ void fft_pass(C2<T> *line, const C2<T> *__restrict__ tw, int L, int tline, int t,
+                                         bool active, int Ns, int lNs)
 {
     constexpr int MB = MaxB<R, EPT>::value;
-    const int L = p.L;
     const int nb = L / R;
     C2<T> v[MB][R];
 #pragma unroll
     for (int q = 0; q < MB; ++q) {
-        const int b = t + q * p.tline;
+        const int b = t + q * tline;
         if (active && b < nb) {
 #pragma unroll
             for (int r = 0; r < R; ++r) v[q][r] = line[sk(b + r * nb)];
         }
     }
     __syncthreads();
-    const C2<T> *tw = (const C2<T> *)p.tw;
-    const int tstep = L / (Ns * R);
+    const int tstep = P2 ? nb >> lNs : L / (Ns * R);
 #pragma unroll
     for (int q = 0; q < MB; ++q) {
-        const int b = t + q * p.tline;
+        const int b = t + q * tline;
         if (active && b < nb) {
-            const int k = b % Ns;
+            const int k = P2 ? (b & (Ns - 1)) : b % Ns;
             if (Ns > 1) {
 #pragma unroll
                 for (int r = 1; r < R; ++r) {
@@ -186,7 +199,7 @@ __device__ __forceinline__ void fft_pass(C2<T> *line, const FftPlanDev &p, int t
                 }
             }
             dftR<R, T, INV>(v[q]);
-            const int base = (b / Ns) * Ns * R + k;
+            const int base = P2 ? (((b >> lNs) << (lNs + Log2R<R>::value)) + k) : (b / Ns) * Ns * R + k;
 #pragma unroll
             for (int r = 0; r < R; ++r) line[sk(base + r * Ns)] = v[q][r];
         }
@@ -196,20 +209,29 @@ __device__ __forceinline__ void fft_pass(C2<T> *line, const FftPlanDev &p, int t
 
 // Full transform of one line per thread group.  All threads of the workgroup
 // must call this with the same plan (barriers inside).  EPT is the number of
-// elements per thread the plan was built for.
-template <typename T, bool INV, int EPT>
-__device__ __forceinline__ void lds_fft_ept(C2<T> *line, const FftPlanDev &p, int t, bool active)
+// elements per thread the plan was built for; tw: the plan's twiddle table in
+// LDS (load_twiddles).
+template <typename T, bool INV, int EPT, bool P2>
+__device__ __forceinline__ void lds_fft_ept(C2<T> *line, const FftPlanDev &p, const C2<T> *tw, int t, bool active)
 {
-    int Ns = 1;
+    int Ns = 1, lNs = 0;
     unsigned rs = p.radices;
     for (int ps = 0; ps < p.npass; ++ps, rs >>= 4) {
         const int R = (int)(rs & 15u);
-        switch (R) {
-        case 8: fft_pass<8, EPT, T, INV>(line, p, t, active, Ns); break;
-        case 5: fft_pass<5, EPT, T, INV>(line, p, t, active, Ns); break;
-        case 4: fft_pass<4, EPT, T, INV>(line, p, t, active, Ns); break;
-        case 3: fft_pass<3, EPT, T, INV>(line, p, t, active, Ns); break;
-        default: fft_pass<2, EPT, T, INV>(line, p, t, active, Ns); break;
+        if (P2) {
+            switch (R) {
+            case 8: fft_pass<8, EPT, T, INV, true>(line, tw, p.L, p.tline, t, active, Ns, lNs); lNs += 3; break;
+            case 4: fft_pass<4, EPT, T, INV, true>(line, tw, p.L, p.tline, t, active, Ns, lNs); lNs += 2; break;
+            default: fft_pass<2, EPT, T, INV, true>(line, tw, p.L, p.tline, t, active, Ns, lNs); lNs += 1; break;
+            }
+        } else {
+            switch (R) {
+            case 8: fft_pass<8, EPT, T, INV, false>(line, tw, p.L, p.tline, t, active, Ns, 0); break;
+            case 5: fft_pass<5, EPT, T, INV, false>(line, tw, p.L, p.tline, t, active, Ns, 0); break;
+            case 4: fft_pass<4, EPT, T, INV, false>(line, tw, p.L, p.tline, t, active, Ns, 0); break;
+            case 3: fft_pass<3, EPT, T, INV, false>(line, tw, p.L, p.tline, t, active, Ns, 0); break;
+            default: fft_pass<2, EPT, T, INV, false>(line, tw, p.L, p.tline, t, active, Ns, 0); break;
+            }
         }
         Ns *= R;
     }
@@ -218,9 +240,10 @@ __device__ __forceinline__ void lds_fft_ept(C2<T> *line, const FftPlanDev &p, in
 #define MM_FFT_EPT 8
 
 template <typename T, bool INV>
-__device__ __forceinline__ void lds_fft(C2<T> *line, const FftPlanDev &p, int t, bool active)
+__device__ __forceinline__ void lds_fft(C2<T> *line, const FftPlanDev &p, int t, bool active, const C2<T> *tw)
 {
-    lds_fft_ept<T, INV, MM_FFT_EPT>(line, p, t, active);
+    if (p.pow2) lds_fft_ept<T, INV, MM_FFT_EPT, true>(line, p, tw, t, active);
+    else lds_fft_ept<T, INV, MM_FFT_EPT, false>(line, p, tw, t, active);
 }
 
 #endif

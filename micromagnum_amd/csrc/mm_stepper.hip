@@ -312,7 +312,9 @@ __global__ void k_stage(SysDev<T> S, StageDev<T> G, StepCtl *ctl, const C2<T> *_
     const int row0 = blockIdx.x * rpw;
     const T *Y = G.Ys ? G.Ys : (ctl->cur ? G.yb1 : G.yb0);
     const int L = p.L, LP = lds_len(L);
+    C2<T> *twl = sm + (size_t)3 * rpw * LP;
     if (has_demag == 1) {
+        load_twiddles<T>(twl, p);
         const int l = threadIdx.x / p.tline, t = threadIdx.x % p.tline;
         const int c = l / rpw, r = l % rpw;
         const int gr = row0 + r;
@@ -323,7 +325,7 @@ __global__ void k_stage(SysDev<T> S, StageDev<T> G, StepCtl *ctl, const C2<T> *_
             for (int k = t; k < L; k += p.tline) line[sk(k)] = k < Hx ? src[k] : cconj<T>(src[L - k]);
         }
         __syncthreads();
-        lds_fft<T, true>(line, p, t, active);
+        lds_fft<T, true>(line, p, t, active, twl);
     }
     StageRed red = {0.0, 0.0, 0.0, 0.0};
     EnergyAcc ea = {0.0, 0.0, 0.0, 0.0};
@@ -357,7 +359,7 @@ __global__ void k_stage(SysDev<T> S, StageDev<T> G, StepCtl *ctl, const C2<T> *_
         const int gr = row0 + r;
         const bool active = (l < 3 * rpw) && gr < nrows;
         C2<T> *line = sm + (size_t)l * LP;
-        lds_fft<T, false>(line, p, t, active);
+        lds_fft<T, false>(line, p, t, active, twl);
         if (active) {
             C2<T> *dst = G.fwd_S1 + ((size_t)c * nrows + gr) * Hp;
             for (int k = t; k < Hx; k += p.tline) dst[k] = line[sk(k)];
@@ -557,6 +559,8 @@ __global__ void k_xfwd_prep(const T *__restrict__ y0, const T *__restrict__ y1, 
     const int g = blockIdx.x * lpw + l;
     const bool active = g < nrows;
     C2<T> *line = sm + (size_t)l * lds_len(L);
+    C2<T> *twl = sm + (size_t)lpw * lds_len(L);
+    load_twiddles<T>(twl, p);
     if (active) {
         const size_t base = (size_t)g * nx;
         for (int x = t; x < L; x += p.tline) {
@@ -569,7 +573,7 @@ __global__ void k_xfwd_prep(const T *__restrict__ y0, const T *__restrict__ y1, 
         }
     }
     __syncthreads();
-    lds_fft<T, false>(line, p, t, active);
+    lds_fft<T, false>(line, p, t, active, twl);
     if (active) {
         C2<T> *dst = S1 + (size_t)g * Hp;
         for (int kx = t; kx < Hx; kx += p.tline) dst[kx] = line[sk(kx)];
@@ -813,8 +817,8 @@ template <typename T> struct Stepper : public StepperBase {
             conv = new ConvImpl<T>();
             conv->init(d.N_host, d.nx, d.ny, d.nz, d.ex, d.ey, d.ez);
             const int tl = conv->px.tline;
-            split_x = 3 * tl > 1024 ||
-                      (size_t)3 * lds_len(conv->px.L) * sizeof(C2<T>) > (size_t)mm_max_lds_bytes();
+            const size_t cs = sizeof(C2<T>), L = conv->px.L;
+            split_x = 3 * tl > 1024 || (3 * lds_len((int)L) + L) * cs > (size_t)mm_max_lds_bytes();
             if (const char *e = std::getenv("MM_SPLIT_X")) split_x = split_x || std::atoi(e) != 0;
             if (split_x) {
                 // the inverse x transform runs as its own pass into Hdem
@@ -825,10 +829,9 @@ template <typename T> struct Stepper : public StepperBase {
             } else {
                 rpw = std::max(1, 192 / (3 * tl));
                 if (const char *e = std::getenv("MM_RPW")) rpw = std::max(1, std::atoi(e));
-                while (rpw > 1 && (size_t)3 * rpw * lds_len(conv->px.L) * sizeof(C2<T>) > (size_t)mm_max_lds_bytes())
-                    --rpw;
+                while (rpw > 1 && ((size_t)3 * rpw * lds_len((int)L) + L) * cs > (size_t)mm_max_lds_bytes()) --rpw;
                 threads = 3 * rpw * tl;
-                stage_lds = (size_t)3 * rpw * lds_len(conv->px.L) * sizeof(C2<T>);
+                stage_lds = ((size_t)3 * rpw * lds_len((int)L) + L) * cs;
             }
         } else {
             rpw = std::max(1, 256 / d.nx);
@@ -952,7 +955,7 @@ template <typename T> struct Stepper : public StepperBase {
                 const FftPlanHost &px = conv->px;
                 const int lpw = conv->ln.lpw_x;
                 const int nrows = 3 * d.ny * d.nz;
-                const size_t sh = (size_t)lpw * lds_len(px.L) * sizeof(C2<T>);
+                const size_t sh = ((size_t)lpw * lds_len(px.L) + px.L) * sizeof(C2<T>);
                 hipLaunchKernelGGL(k_xfwd_prep<T>, dim3((nrows + lpw - 1) / lpw), dim3(lpw * px.tline), sh, s, yb[0],
                                    yb[1], k[0], tmp[0], conv->S1, ctl, tab.b[1][0], d.nx, d.ny, d.nz, conv->g.Hx,
                                    conv->g.Hp, px.template dev<T>(), lpw);
