@@ -147,19 +147,15 @@ __global__ void k_xfwd(const TIN *__restrict__ in, size_t in_cstride, C2<T> *__r
     C2<T> *line = sm + (size_t)l * lds_len(L);
     C2<T> *twl = sm + (size_t)lpw * lds_len(L);
     load_twiddles<T>(twl, p);
-    int c = 0, r = 0;
-    if (active) {
-        c = g / (ny * nz);
-        r = g % (ny * nz);
-        const TIN *src = in + c * in_cstride + (size_t)r * nx_in;
-        for (int x = t; x < L; x += p.tline) line[sk(x)] = cmk<T>(x < nx_in ? (T)src[x] : (T)0, (T)0);
-    }
-    __syncthreads();
-    lds_fft<T, false>(line, p, t, active, twl);
-    if (active) {
-        C2<T> *dst = out + c * out_cstride + (size_t)r * Hp;
-        for (int kx = t; kx < Hx; kx += p.tline) dst[kx] = line[sk(kx)];
-    }
+    const int c = active ? g / (ny * nz) : 0, r = active ? g % (ny * nz) : 0;
+    const TIN *src = in + c * in_cstride + (size_t)r * nx_in;
+    C2<T> *dst = out + c * out_cstride + (size_t)r * Hp;
+    const LineMap m = {t, active};
+    fft_line<T, false, true, true>(
+        p, twl, line, m, line, m, [&](int x) { return cmk<T>(x < nx_in ? (T)src[x] : (T)0, (T)0); },
+        [&](int kx, C2<T> v) {
+            if (kx < Hx) dst[kx] = v;
+        });
 }
 
 // y forward: columns kx of planes (c, z); rows y < ny_in read, Py written
@@ -176,17 +172,17 @@ __global__ void k_yfwd(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, in
     C2<T> *dst = out + ((size_t)c * nzl + z) * (size_t)L * Hp;
     C2<T> *twl = sm + (size_t)tk * LP;
     load_twiddles<T>(twl, p);
-    for (int e = threadIdx.x; e < tk * L; e += blockDim.x) {
-        const int col = e % tk, y = e / tk, kx = kx0 + col;
-        sm[col * LP + sk(y)] = (y < ny_in && kx < Hx) ? src[(size_t)y * Hp + kx] : cmk<T>((T)0, (T)0);
-    }
-    __syncthreads();
-    const int l = threadIdx.x / p.tline, t = threadIdx.x % p.tline;
-    lds_fft<T, false>(sm + (size_t)l * LP, p, t, l < tk, twl);
-    for (int e = threadIdx.x; e < tk * L; e += blockDim.x) {
-        const int col = e % tk, y = e / tk, kx = kx0 + col;
-        if (kx < Hx) dst[(size_t)y * Hp + kx] = sm[col * LP + sk(y)];
-    }
+    // global passes: columns fastest across the threads (row segments of tk
+    // values per load); LDS passes: one line per group of tline threads
+    const int lg = threadIdx.x % tk, ls = threadIdx.x / p.tline;
+    const LineMap mg = {(int)(threadIdx.x / tk), true}, ms = {(int)(threadIdx.x % p.tline), ls < tk};
+    const int kx = kx0 + lg;
+    fft_line<T, false, true, true>(
+        p, twl, sm + (size_t)lg * LP, mg, sm + (size_t)ls * LP, ms,
+        [&](int y) { return (y < ny_in && kx < Hx) ? src[(size_t)y * Hp + kx] : cmk<T>((T)0, (T)0); },
+        [&](int y, C2<T> v) {
+            if (kx < Hx) dst[(size_t)y * Hp + kx] = v;
+        });
 }
 
 // y inverse: columns of S2 planes (c, z), all Py rows read, y < ny_out written
@@ -203,17 +199,15 @@ __global__ void k_yinv(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, in
     C2<T> *dst = out + ((size_t)c * nzl + z) * (size_t)ny_out * Hp;
     C2<T> *twl = sm + (size_t)tk * LP;
     load_twiddles<T>(twl, p);
-    for (int e = threadIdx.x; e < tk * L; e += blockDim.x) {
-        const int col = e % tk, y = e / tk, kx = kx0 + col;
-        sm[col * LP + sk(y)] = (kx < Hx) ? src[(size_t)y * Hp + kx] : cmk<T>((T)0, (T)0);
-    }
-    __syncthreads();
-    const int l = threadIdx.x / p.tline, t = threadIdx.x % p.tline;
-    lds_fft<T, true>(sm + (size_t)l * LP, p, t, l < tk, twl);
-    for (int e = threadIdx.x; e < tk * ny_out; e += blockDim.x) {
-        const int col = e % tk, y = e / tk, kx = kx0 + col;
-        if (kx < Hx) dst[(size_t)y * Hp + kx] = sm[col * LP + sk(y)];
-    }
+    const int lg = threadIdx.x % tk, ls = threadIdx.x / p.tline;
+    const LineMap mg = {(int)(threadIdx.x / tk), true}, ms = {(int)(threadIdx.x % p.tline), ls < tk};
+    const int kx = kx0 + lg;
+    fft_line<T, true, true, true>(
+        p, twl, sm + (size_t)lg * LP, mg, sm + (size_t)ls * LP, ms,
+        [&](int y) { return kx < Hx ? src[(size_t)y * Hp + kx] : cmk<T>((T)0, (T)0); },
+        [&](int y, C2<T> v) {
+            if (y < ny_out && kx < Hx) dst[(size_t)y * Hp + kx] = v;
+        });
 }
 
 // Pruned y transforms for zero-padded lines (P = 2h even, data only in
@@ -435,19 +429,15 @@ __global__ void k_xinv(const C2<T> *__restrict__ in, TOUT *__restrict__ out, siz
     C2<T> *line = sm + (size_t)l * lds_len(L);
     C2<T> *twl = sm + (size_t)lpw * lds_len(L);
     load_twiddles<T>(twl, p);
-    int c = 0, r = 0;
-    if (active) {
-        c = g / (ny * nz);
-        r = g % (ny * nz);
-        const C2<T> *src = in + (size_t)g * Hp;
-        for (int k = t; k < L; k += p.tline) line[sk(k)] = k < Hx ? src[k] : cconj<T>(src[L - k]);
-    }
-    __syncthreads();
-    lds_fft<T, true>(line, p, t, active, twl);
-    if (active) {
-        TOUT *dst = out + c * out_cstride + (size_t)r * nx;
-        for (int x = t; x < nx; x += p.tline) dst[x] = (TOUT)line[sk(x)].x;
-    }
+    const int c = active ? g / (ny * nz) : 0, r = active ? g % (ny * nz) : 0;
+    const C2<T> *src = in + (size_t)(active ? g : 0) * Hp;
+    TOUT *dst = out + c * out_cstride + (size_t)r * nx;
+    const LineMap m = {t, active};
+    fft_line<T, true, true, true>(
+        p, twl, line, m, line, m, [&](int k) { return k < Hx ? src[k] : cconj<T>(src[L - k]); },
+        [&](int x, C2<T> v) {
+            if (x < nx) dst[x] = (TOUT)v.x;
+        });
 }
 
 // setup: forward z over full lines (no multiply), all Pz outputs stored

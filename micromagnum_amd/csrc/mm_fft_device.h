@@ -246,4 +246,134 @@ __device__ __forceinline__ void lds_fft(C2<T> *line, const FftPlanDev &p, int t,
     else lds_fft_ept<T, INV, MM_FFT_EPT, false>(line, p, tw, t, active);
 }
 
+// ---- transforms with the first pass reading and/or the last pass writing
+// outside LDS ---------------------------------------------------------------
+// A thread takes part in a pass as butterfly owner `t` of a line.  Passes
+// that touch global memory use the global mapping (G: line pointer, t,
+// active) chosen by the kernel for coalescing; passes inside LDS use the
+// LDS mapping (S).  Reading the inputs straight into registers (ld(j) =
+// element j) saves the global -> LDS copy and its barrier; writing the last
+// pass's outputs straight out (st(j, v)) saves the final LDS round trip.
+struct LineMap {
+    int t;
+    bool active;
+};
+
+template <int R, int EPT, typename T, bool INV, bool P2, bool LDIN, bool STOUT, typename LD, typename ST>
+__device__ __forceinline__ void fft_pass_io(C2<T> *lineG, LineMap g, C2<T> *lineS, LineMap sm_, const C2<T> *tw,
+                                            int L, int tline, int Ns, int lNs, LD &ld, ST &st)
+{
+    constexpr int MB = MaxB<R, EPT>::value;
+    const int nb = L / R;
+    C2<T> v[MB][R];
+    // a pass that reads through ld or writes through st runs in the global
+    // mapping (ld and st address the thread's line of that mapping), others
+    // in the LDS mapping
+    constexpr bool useG = LDIN || STOUT;
+    const LineMap in = useG ? g : sm_;
+    C2<T> *lin = useG ? lineG : lineS;
+#pragma unroll
+    for (int q = 0; q < MB; ++q) {
+        const int b = in.t + q * tline;
+        if (in.active && b < nb) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[q][r] = LDIN ? ld(b + r * nb) : lin[sk(b + r * nb)];
+        }
+    }
+    if (!LDIN && !STOUT) __syncthreads();  // in place in LDS: every read before any write
+    const int tstep = P2 ? nb >> lNs : L / (Ns * R);
+    // outputs: a butterfly stays with the thread that read it
+#pragma unroll
+    for (int q = 0; q < MB; ++q) {
+        const int b = in.t + q * tline;
+        if (in.active && b < nb) {
+            const int k = P2 ? (b & (Ns - 1)) : b % Ns;
+            if (Ns > 1) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    C2<T> w = tw[r * k * tstep];
+                    if (INV) w.y = -w.y;
+                    v[q][r] = cmul<T>(v[q][r], w);
+                }
+            }
+            dftR<R, T, INV>(v[q]);
+            const int base = P2 ? (((b >> lNs) << (lNs + Log2R<R>::value)) + k) : (b / Ns) * Ns * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if (STOUT) st(base + r * Ns, v[q][r]);
+                else lin[sk(base + r * Ns)] = v[q][r];
+            }
+        }
+    }
+    if (!STOUT) __syncthreads();
+}
+
+template <typename T, bool INV, int EPT, bool P2, bool LDIN, bool STOUT, bool FIRST, bool LAST, typename LD,
+          typename ST>
+__device__ __forceinline__ void fft_pass_sel(int R, C2<T> *lineG, LineMap g, C2<T> *lineS, LineMap s_,
+                                             const C2<T> *tw, int L, int tline, int Ns, int lNs, LD &ld, ST &st)
+{
+    constexpr bool I = LDIN && FIRST, O = STOUT && LAST;
+    if (P2) {
+        switch (R) {
+        case 8: fft_pass_io<8, EPT, T, INV, true, I, O>(lineG, g, lineS, s_, tw, L, tline, Ns, lNs, ld, st); break;
+        case 4: fft_pass_io<4, EPT, T, INV, true, I, O>(lineG, g, lineS, s_, tw, L, tline, Ns, lNs, ld, st); break;
+        default: fft_pass_io<2, EPT, T, INV, true, I, O>(lineG, g, lineS, s_, tw, L, tline, Ns, lNs, ld, st); break;
+        }
+    } else {
+        switch (R) {
+        case 8: fft_pass_io<8, EPT, T, INV, false, I, O>(lineG, g, lineS, s_, tw, L, tline, Ns, 0, ld, st); break;
+        case 5: fft_pass_io<5, EPT, T, INV, false, I, O>(lineG, g, lineS, s_, tw, L, tline, Ns, 0, ld, st); break;
+        case 4: fft_pass_io<4, EPT, T, INV, false, I, O>(lineG, g, lineS, s_, tw, L, tline, Ns, 0, ld, st); break;
+        case 3: fft_pass_io<3, EPT, T, INV, false, I, O>(lineG, g, lineS, s_, tw, L, tline, Ns, 0, ld, st); break;
+        default: fft_pass_io<2, EPT, T, INV, false, I, O>(lineG, g, lineS, s_, tw, L, tline, Ns, 0, ld, st); break;
+        }
+    }
+}
+
+// Transform of one line per thread group, all threads of the workgroup
+// calling (barriers inside).  LDIN: inputs via ld(j) (else the LDS line holds
+// them); STOUT: outputs via st(j, v) (else they end in the LDS line, followed
+// by a barrier).  lineG/g: line and mapping of the passes that use ld/st,
+// lineS/s: of the LDS passes (the same line in both mappings).
+template <typename T, bool INV, bool P2, bool LDIN, bool STOUT, typename LD, typename ST>
+__device__ __forceinline__ void fft_line_p2(const FftPlanDev &p, const C2<T> *tw, C2<T> *lineG, LineMap g,
+                                            C2<T> *lineS, LineMap s_, LD &ld, ST &st)
+{
+    constexpr int EPT = MM_FFT_EPT;
+    const int np = p.npass;
+    if (np == 0) {
+        if (g.active && g.t == 0) {
+            const C2<T> v = LDIN ? ld(0) : lineG[sk(0)];
+            if (STOUT) st(0, v);
+            else lineG[sk(0)] = v;
+        }
+        if (!STOUT) __syncthreads();
+        return;
+    }
+    int Ns = 1, lNs = 0;
+    unsigned rs = p.radices;
+    for (int ps = 0; ps < np; ++ps, rs >>= 4) {
+        const int R = (int)(rs & 15u);
+        if (ps == 0 && ps == np - 1)
+            fft_pass_sel<T, INV, EPT, P2, LDIN, STOUT, true, true>(R, lineG, g, lineS, s_, tw, p.L, p.tline, Ns, lNs, ld, st);
+        else if (ps == 0)
+            fft_pass_sel<T, INV, EPT, P2, LDIN, STOUT, true, false>(R, lineG, g, lineS, s_, tw, p.L, p.tline, Ns, lNs, ld, st);
+        else if (ps == np - 1)
+            fft_pass_sel<T, INV, EPT, P2, LDIN, STOUT, false, true>(R, lineG, g, lineS, s_, tw, p.L, p.tline, Ns, lNs, ld, st);
+        else
+            fft_pass_sel<T, INV, EPT, P2, LDIN, STOUT, false, false>(R, lineG, g, lineS, s_, tw, p.L, p.tline, Ns, lNs, ld, st);
+        Ns *= R;
+        lNs += R == 8 ? 3 : (R == 4 ? 2 : 1);
+    }
+}
+
+template <typename T, bool INV, bool LDIN, bool STOUT, typename LD, typename ST>
+__device__ __forceinline__ void fft_line(const FftPlanDev &p, const C2<T> *tw, C2<T> *lineG, LineMap g, C2<T> *lineS,
+                                         LineMap s_, LD ld, ST st)
+{
+    if (p.pow2) fft_line_p2<T, INV, true, LDIN, STOUT>(p, tw, lineG, g, lineS, s_, ld, st);
+    else fft_line_p2<T, INV, false, LDIN, STOUT>(p, tw, lineG, g, lineS, s_, ld, st);
+}
+
 #endif

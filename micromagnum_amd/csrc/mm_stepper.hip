@@ -320,12 +320,11 @@ __global__ void k_stage(SysDev<T> S, StageDev<T> G, StepCtl *ctl, const C2<T> *_
         const int gr = row0 + r;
         const bool active = (l < 3 * rpw) && gr < nrows;
         C2<T> *line = sm + (size_t)l * LP;
-        if (active) {
-            const C2<T> *src = S1 + ((size_t)c * nrows + gr) * Hp;
-            for (int k = t; k < L; k += p.tline) line[sk(k)] = k < Hx ? src[k] : cconj<T>(src[L - k]);
-        }
-        __syncthreads();
-        lds_fft<T, true>(line, p, t, active, twl);
+        const C2<T> *src = S1 + ((size_t)c * nrows + (active ? gr : 0)) * Hp;
+        const LineMap m = {t, active};
+        fft_line<T, true, true, false>(
+            p, twl, line, m, line, m, [&](int k) { return k < Hx ? src[k] : cconj<T>(src[L - k]); },
+            [](int, C2<T>) {});
     }
     StageRed red = {0.0, 0.0, 0.0, 0.0};
     EnergyAcc ea = {0.0, 0.0, 0.0, 0.0};
@@ -359,11 +358,13 @@ __global__ void k_stage(SysDev<T> S, StageDev<T> G, StepCtl *ctl, const C2<T> *_
         const int gr = row0 + r;
         const bool active = (l < 3 * rpw) && gr < nrows;
         C2<T> *line = sm + (size_t)l * LP;
-        lds_fft<T, false>(line, p, t, active, twl);
-        if (active) {
-            C2<T> *dst = G.fwd_S1 + ((size_t)c * nrows + gr) * Hp;
-            for (int k = t; k < Hx; k += p.tline) dst[k] = line[sk(k)];
-        }
+        C2<T> *dst = G.fwd_S1 + ((size_t)c * nrows + (active ? gr : 0)) * Hp;
+        const LineMap m = {t, active};
+        fft_line<T, false, false, true>(
+            p, twl, line, m, line, m, [](int) { return cmk<T>((T)0, (T)0); },
+            [&](int k, C2<T> v) {
+                if (k < Hx) dst[k] = v;
+            });
     }
     __shared__ double wred[4][16];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (int)((blockDim.x + 63) >> 6);
@@ -561,23 +562,22 @@ __global__ void k_xfwd_prep(const T *__restrict__ y0, const T *__restrict__ y1, 
     C2<T> *line = sm + (size_t)l * lds_len(L);
     C2<T> *twl = sm + (size_t)lpw * lds_len(L);
     load_twiddles<T>(twl, p);
-    if (active) {
-        const size_t base = (size_t)g * nx;
-        for (int x = t; x < L; x += p.tline) {
+    const size_t base = (size_t)(active ? g : 0) * nx;
+    C2<T> *dst = S1 + (size_t)(active ? g : 0) * Hp;
+    const LineMap m = {t, active};
+    fft_line<T, false, true, true>(
+        p, twl, line, m, line, m,
+        [&](int x) {
             T v = (T)0;
             if (x < nx) {
                 v = y[base + x] + k0[base + x] * f;
                 out[base + x] = v;
             }
-            line[sk(x)] = cmk<T>(v, (T)0);
-        }
-    }
-    __syncthreads();
-    lds_fft<T, false>(line, p, t, active, twl);
-    if (active) {
-        C2<T> *dst = S1 + (size_t)g * Hp;
-        for (int kx = t; kx < Hx; kx += p.tline) dst[kx] = line[sk(kx)];
-    }
+            return cmk<T>(v, (T)0);
+        },
+        [&](int kx, C2<T> v) {
+            if (kx < Hx) dst[kx] = v;
+        });
 }
 
 template <typename T> __global__ void k_cast_in(size_t n, const double *__restrict__ a, T *__restrict__ b)
