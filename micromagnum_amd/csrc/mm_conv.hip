@@ -352,15 +352,20 @@ __global__ void k_zmi(C2<T> *__restrict__ S2, const T *__restrict__ K, size_t pl
     C2<T> *base = S2 + (size_t)ky * Hp;
     C2<T> *twl = sm + (size_t)3 * tk * LP;
     load_twiddles<T>(twl, p);
-    for (int e = threadIdx.x; e < 3 * tk * L; e += blockDim.x) {
-        const int col = e % tk, rest = e / tk, z = rest % L, c = rest / L, kx = kx0 + col;
-        sm[(c * tk + col) * LP + sk(z)] = (z < nz && kx < Hx) ? base[c * cstride + z * zstride + kx] : cmk<T>((T)0, (T)0);
-    }
-    __syncthreads();
+    // global passes: columns fastest, then components; LDS passes: one line
+    // (component c, column) per group of tline threads
+    const int colg = threadIdx.x % tk, cg = (threadIdx.x / tk) % 3;
+    const LineMap mg = {(int)(threadIdx.x / (3 * tk)), true};
+    C2<T> *lineG = sm + (size_t)(cg * tk + colg) * LP;
     const int l = threadIdx.x / p.tline, t = threadIdx.x % p.tline;
-    const bool active = l < 3 * tk;
+    const LineMap ms = {t, l < 3 * tk};
     C2<T> *line = sm + (size_t)l * LP;
-    lds_fft<T, false>(line, p, t, active, twl);
+    const int kxg = kx0 + colg;
+    C2<T> *colp = base + cg * cstride + kxg;
+    fft_line<T, false, true, false>(
+        p, twl, lineG, mg, line, ms,
+        [&](int z) { return (z < nz && kxg < Hx) ? colp[z * zstride] : cmk<T>((T)0, (T)0); },
+        [](int, C2<T>) {});
     for (int e = threadIdx.x; e < tk * L; e += blockDim.x) {
         const int col = e % tk, kz = e / tk, kx = kx0 + col;
         if (kx < Hx) {
@@ -370,11 +375,11 @@ __global__ void k_zmi(C2<T> *__restrict__ S2, const T *__restrict__ K, size_t pl
         }
     }
     __syncthreads();
-    lds_fft<T, true>(line, p, t, active, twl);
-    for (int e = threadIdx.x; e < 3 * tk * nz; e += blockDim.x) {
-        const int col = e % tk, rest = e / tk, z = rest % nz, c = rest / nz, kx = kx0 + col;
-        if (kx < Hx) base[c * cstride + z * zstride + kx] = sm[(c * tk + col) * LP + sk(z)];
-    }
+    fft_line<T, true, false, true>(
+        p, twl, lineG, mg, line, ms, [](int) { return cmk<T>((T)0, (T)0); },
+        [&](int z, C2<T> v) {
+            if (z < nz && kxg < Hx) colp[z * zstride] = v;
+        });
 }
 
 // 2-D: y forward (ny rows, zero-extended) + multiply (kz = 0) + y inverse,
@@ -390,15 +395,18 @@ __global__ void k_ymi2d(C2<T> *__restrict__ S1, const T *__restrict__ K, size_t 
     const size_t cstride = (size_t)ny * Hp;
     C2<T> *twl = sm + (size_t)3 * tk * LP;
     load_twiddles<T>(twl, p);
-    for (int e = threadIdx.x; e < 3 * tk * L; e += blockDim.x) {
-        const int col = e % tk, rest = e / tk, y = rest % L, c = rest / L, kx = kx0 + col;
-        sm[(c * tk + col) * LP + sk(y)] = (y < ny && kx < Hx) ? S1[c * cstride + (size_t)y * Hp + kx] : cmk<T>((T)0, (T)0);
-    }
-    __syncthreads();
+    const int colg = threadIdx.x % tk, cg = (threadIdx.x / tk) % 3;
+    const LineMap mg = {(int)(threadIdx.x / (3 * tk)), true};
+    C2<T> *lineG = sm + (size_t)(cg * tk + colg) * LP;
     const int l = threadIdx.x / p.tline, t = threadIdx.x % p.tline;
-    const bool active = l < 3 * tk;
+    const LineMap ms = {t, l < 3 * tk};
     C2<T> *line = sm + (size_t)l * LP;
-    lds_fft<T, false>(line, p, t, active, twl);
+    const int kxg = kx0 + colg;
+    C2<T> *colp = S1 + cg * cstride + kxg;
+    fft_line<T, false, true, false>(
+        p, twl, lineG, mg, line, ms,
+        [&](int y) { return (y < ny && kxg < Hx) ? colp[(size_t)y * Hp] : cmk<T>((T)0, (T)0); },
+        [](int, C2<T>) {});
     for (int e = threadIdx.x; e < tk * L; e += blockDim.x) {
         const int col = e % tk, ky = e / tk, kx = kx0 + col;
         if (kx < Hx) {
@@ -408,11 +416,11 @@ __global__ void k_ymi2d(C2<T> *__restrict__ S1, const T *__restrict__ K, size_t 
         }
     }
     __syncthreads();
-    lds_fft<T, true>(line, p, t, active, twl);
-    for (int e = threadIdx.x; e < 3 * tk * ny; e += blockDim.x) {
-        const int col = e % tk, rest = e / tk, y = rest % ny, c = rest / ny, kx = kx0 + col;
-        if (kx < Hx) S1[c * cstride + (size_t)y * Hp + kx] = sm[(c * tk + col) * LP + sk(y)];
-    }
+    fft_line<T, true, false, true>(
+        p, twl, lineG, mg, line, ms, [](int) { return cmk<T>((T)0, (T)0); },
+        [&](int y, C2<T> v) {
+            if (y < ny && kxg < Hx) colp[(size_t)y * Hp] = v;
+        });
 }
 
 // x inverse (C2R): rows (c, z, y) of S1, Hermitian extension, keep x < nx
