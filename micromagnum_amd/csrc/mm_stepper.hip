@@ -81,7 +81,41 @@ template <typename T> struct SysDev {
     T hext[3];
     int Ms_uniform;
     double Ms_u;
+    // exchange with uniform, non-zero Ms and uniform A: no masking, no
+    // per-neighbour Ms loads; H = exch_f * sum_j w_j (M_j - M_i) with
+    // exch_f = 2 A / (mu0 Ms^2), the reference's (2A/(mu0 Ms)) sum (m_j - m_i) w
+    int exch_uniform;
+    T exch_f;
 };
+
+// exchange field of cell i for uniform Ms and A (see exch_uniform)
+template <typename T>
+__device__ __forceinline__ V3<T> exchange_cell_uniform(const MeshDev &m, int x, int y, int z, size_t i, size_t n,
+                                                       T wx, T wy, T wz, T f, const T *__restrict__ M)
+{
+    const V3<T> mi = v3<T>(M[i], M[n + i], M[2 * n + i]);
+    V3<T> s = v3<T>(0, 0, 0);
+    const long sx = 1, sy = m.nx, sz = (long)m.nx * m.ny;
+    auto add = [&](long j, T w) {
+        s.x += (M[j] - mi.x) * w;
+        s.y += (M[n + j] - mi.y) * w;
+        s.z += (M[2 * n + j] - mi.z) * w;
+    };
+    const long li = (long)i;
+    if (x > 0) add(li - sx, wx);
+    else if (m.px) add(li - sx + sx * m.nx, wx);
+    if (x < m.nx - 1) add(li + sx, wx);
+    else if (m.px) add(li + sx - sx * m.nx, wx);
+    if (y > 0) add(li - sy, wy);
+    else if (m.py) add(li - sy + sy * m.ny, wy);
+    if (y < m.ny - 1) add(li + sy, wy);
+    else if (m.py) add(li + sy - sy * m.ny, wy);
+    if (z > 0) add(li - sz, wz);
+    else if (m.pz) add(li - sz + sz * m.nz, wz);
+    if (z < m.nz - 1) add(li + sz, wz);
+    else if (m.pz) add(li + sz - sz * m.nz, wz);
+    return v3<T>(f * s.x, f * s.y, f * s.z);
+}
 
 template <typename T> struct StageDev {
     const T *Ys;          // stage input (null: y[cur])
@@ -145,8 +179,10 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
     V3<T> H = Hd;
     if (EN) ea.demag += dotd<T>(Mv, Hd);
     if (S.has_exch && ms != (T)0) {
-        const V3<T> he = exchange_cell<T>(S.m, x, y, z, i, n, S.wx, S.wy, S.wz, S.coef_ex, S.Ms.p ? S.Ms.p : nullptr,
-                                          S.A.p ? S.A.p : nullptr, Y, ms);
+        const V3<T> he = S.exch_uniform
+                             ? exchange_cell_uniform<T>(S.m, x, y, z, i, n, S.wx, S.wy, S.wz, S.exch_f, Y)
+                             : exchange_cell<T>(S.m, x, y, z, i, n, S.wx, S.wy, S.wz, S.coef_ex,
+                                                S.Ms.p ? S.Ms.p : nullptr, S.A.p ? S.A.p : nullptr, Y, ms);
         H.x += he.x;
         H.y += he.y;
         H.z += he.z;
@@ -789,6 +825,8 @@ template <typename T> struct Stepper : public StepperBase {
             T *dp = dalloc(n);
             upload(dp, d.A_host, n);
             sys.A.p = dp;
+            sys.exch_uniform = sys.Ms_uniform && sys.Ms_u != 0.0 && all_equal(d.A_host, n);
+            if (sys.exch_uniform) sys.exch_f = (T)(2.0 * d.A_host[0] / (mu0 * sys.Ms_u * sys.Ms_u));
         }
         sys.has_uni = d.k_uni_host != nullptr;
         sys.has_cub = d.k_cub_host != nullptr;
