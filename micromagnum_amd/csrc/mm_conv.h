@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
 #include <vector>
 
 #include "mm_fft_device.h"
@@ -38,6 +39,31 @@ struct FftPlanHost {
         return d;
     }
 };
+
+// Calls f(std::integral_constant<int, LC>()) with LC = L when kernels carry a
+// compile-time plan for that length (powers of two 2..2048) and `generic` is
+// false, else LC = 0 (runtime plan).
+template <typename F> void mm_with_len(int L, bool generic, F &&f)
+{
+    if (generic) {
+        f(std::integral_constant<int, 0>());
+        return;
+    }
+    switch (L) {
+    case 2: f(std::integral_constant<int, 2>()); return;
+    case 4: f(std::integral_constant<int, 4>()); return;
+    case 8: f(std::integral_constant<int, 8>()); return;
+    case 16: f(std::integral_constant<int, 16>()); return;
+    case 32: f(std::integral_constant<int, 32>()); return;
+    case 64: f(std::integral_constant<int, 64>()); return;
+    case 128: f(std::integral_constant<int, 128>()); return;
+    case 256: f(std::integral_constant<int, 256>()); return;
+    case 512: f(std::integral_constant<int, 512>()); return;
+    case 1024: f(std::integral_constant<int, 1024>()); return;
+    case 2048: f(std::integral_constant<int, 2048>()); return;
+    default: f(std::integral_constant<int, 0>()); return;
+    }
+}
 
 bool mm_is_smooth(int n);                  // n has no prime factor > 5
 int mm_fft_size_for(int n, int given);     // radix-friendly padded size for n cells
@@ -97,6 +123,7 @@ struct ConvImpl {
     ConvLaunch ln;
     FftPlanHost px, py, pz;
     FftPlanHost pyh;        // length Py/2, for the pruned y transforms
+    bool fft_generic = false;  // runtime FFT plans only (no compile-time lengths)
     bool prune_y = false;
     int tk_yp = 1;          // kx columns per WG of the pruned y transforms
     T *kern = nullptr;      // 6 real planes [e][kz<Hz2][ky<Hy2][kx<Hx], pitch Hp, scaled 1/(PxPyPz)

@@ -131,7 +131,7 @@ template <typename K> static void set_lds(K, size_t bytes)
 
 // x forward: rows (c, z, y); input rows of nx_in reals with row stride nx_in
 // sel (may be null): device index choosing in (0) or in_alt (1)
-template <typename T, typename TIN>
+template <typename T, typename TIN, int LC = 0>
 __global__ void k_xfwd(const TIN *__restrict__ in, size_t in_cstride, C2<T> *__restrict__ out,
                        size_t out_cstride, int Hx, int Hp, int ny, int nz, int nrows, int nx_in,
                        FftPlanDev p, int lpw, const TIN *__restrict__ in_alt = nullptr,
@@ -140,18 +140,18 @@ __global__ void k_xfwd(const TIN *__restrict__ in, size_t in_cstride, C2<T> *__r
     if (sel && *sel) in = in_alt;
     extern __shared__ char smem_raw[];
     C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
-    const int L = p.L;
-    const int l = threadIdx.x / p.tline, t = threadIdx.x % p.tline;
+    const int L = PlanC<LC>::L(p);
+    const int l = threadIdx.x / PlanC<LC>::tline(p), t = threadIdx.x % PlanC<LC>::tline(p);
     const int g = blockIdx.x * lpw + l;
     const bool active = g < nrows;
     C2<T> *line = sm + (size_t)l * lds_len(L);
     C2<T> *twl = sm + (size_t)lpw * lds_len(L);
-    load_twiddles<T>(twl, p);
+    load_twiddles<T, LC>(twl, p);
     const int c = active ? g / (ny * nz) : 0, r = active ? g % (ny * nz) : 0;
     const TIN *src = in + c * in_cstride + (size_t)r * nx_in;
     C2<T> *dst = out + c * out_cstride + (size_t)r * Hp;
     const LineMap m = {t, active};
-    fft_line<T, false, true, true>(
+    fft_line<T, false, true, true, LC>(
         p, twl, line, m, line, m, [&](int x) { return cmk<T>(x < nx_in ? (T)src[x] : (T)0, (T)0); },
         [&](int kx, C2<T> v) {
             if (kx < Hx) dst[kx] = v;
@@ -159,25 +159,25 @@ __global__ void k_xfwd(const TIN *__restrict__ in, size_t in_cstride, C2<T> *__r
 }
 
 // y forward: columns kx of planes (c, z); rows y < ny_in read, Py written
-template <typename T>
+template <typename T, int LC = 0>
 __global__ void k_yfwd(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, int Hx, int Hp, int ny_in,
                        int nzl, FftPlanDev p, int tk)
 {
     extern __shared__ char smem_raw[];
     C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
-    const int L = p.L, LP = lds_len(L);
+    const int L = PlanC<LC>::L(p), LP = lds_len(L);
     const int kx0 = blockIdx.x * tk;
     const int z = blockIdx.y, c = blockIdx.z;
     const C2<T> *src = in + ((size_t)c * nzl + z) * (size_t)ny_in * Hp;
     C2<T> *dst = out + ((size_t)c * nzl + z) * (size_t)L * Hp;
     C2<T> *twl = sm + (size_t)tk * LP;
-    load_twiddles<T>(twl, p);
+    load_twiddles<T, LC>(twl, p);
     // global passes: columns fastest across the threads (row segments of tk
     // values per load); LDS passes: one line per group of tline threads
-    const int lg = threadIdx.x % tk, ls = threadIdx.x / p.tline;
-    const LineMap mg = {(int)(threadIdx.x / tk), true}, ms = {(int)(threadIdx.x % p.tline), ls < tk};
+    const int lg = threadIdx.x % tk, ls = threadIdx.x / PlanC<LC>::tline(p);
+    const LineMap mg = {(int)(threadIdx.x / tk), true}, ms = {(int)(threadIdx.x % PlanC<LC>::tline(p)), ls < tk};
     const int kx = kx0 + lg;
-    fft_line<T, false, true, true>(
+    fft_line<T, false, true, true, LC>(
         p, twl, sm + (size_t)lg * LP, mg, sm + (size_t)ls * LP, ms,
         [&](int y) { return (y < ny_in && kx < Hx) ? src[(size_t)y * Hp + kx] : cmk<T>((T)0, (T)0); },
         [&](int y, C2<T> v) {
@@ -186,23 +186,23 @@ __global__ void k_yfwd(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, in
 }
 
 // y inverse: columns of S2 planes (c, z), all Py rows read, y < ny_out written
-template <typename T>
+template <typename T, int LC = 0>
 __global__ void k_yinv(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, int Hx, int Hp, int ny_out,
                        int nzl, FftPlanDev p, int tk)
 {
     extern __shared__ char smem_raw[];
     C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
-    const int L = p.L, LP = lds_len(L);
+    const int L = PlanC<LC>::L(p), LP = lds_len(L);
     const int kx0 = blockIdx.x * tk;
     const int z = blockIdx.y, c = blockIdx.z;
     const C2<T> *src = in + ((size_t)c * nzl + z) * (size_t)L * Hp;
     C2<T> *dst = out + ((size_t)c * nzl + z) * (size_t)ny_out * Hp;
     C2<T> *twl = sm + (size_t)tk * LP;
-    load_twiddles<T>(twl, p);
-    const int lg = threadIdx.x % tk, ls = threadIdx.x / p.tline;
-    const LineMap mg = {(int)(threadIdx.x / tk), true}, ms = {(int)(threadIdx.x % p.tline), ls < tk};
+    load_twiddles<T, LC>(twl, p);
+    const int lg = threadIdx.x % tk, ls = threadIdx.x / PlanC<LC>::tline(p);
+    const LineMap mg = {(int)(threadIdx.x / tk), true}, ms = {(int)(threadIdx.x % PlanC<LC>::tline(p)), ls < tk};
     const int kx = kx0 + lg;
-    fft_line<T, true, true, true>(
+    fft_line<T, true, true, true, LC>(
         p, twl, sm + (size_t)lg * LP, mg, sm + (size_t)ls * LP, ms,
         [&](int y) { return kx < Hx ? src[(size_t)y * Hp + kx] : cmk<T>((T)0, (T)0); },
         [&](int y, C2<T> v) {
@@ -338,31 +338,31 @@ __device__ __forceinline__ void mult3(C2<T> &mx, C2<T> &my, C2<T> &mz, const T k
 
 // z pass (3-D): forward over z (nz rows, zero-extended), multiply, inverse,
 // keep z < nz.  Lines: 3 components x tk columns for one ky.
-template <typename T>
+template <typename T, int LC = 0>
 __global__ void k_zmi(C2<T> *__restrict__ S2, const T *__restrict__ K, size_t plane, int Hx, int Hp, int Hy2,
                       int Py, int nz, FftPlanDev p, int tk)
 {
     extern __shared__ char smem_raw[];
     C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
-    const int L = p.L, LP = lds_len(L);
+    const int L = PlanC<LC>::L(p), LP = lds_len(L);
     const int kx0 = blockIdx.x * tk;
     const int ky = blockIdx.y;
     const size_t cstride = (size_t)nz * Py * Hp;
     const size_t zstride = (size_t)Py * Hp;
     C2<T> *base = S2 + (size_t)ky * Hp;
     C2<T> *twl = sm + (size_t)3 * tk * LP;
-    load_twiddles<T>(twl, p);
+    load_twiddles<T, LC>(twl, p);
     // global passes: columns fastest, then components; LDS passes: one line
     // (component c, column) per group of tline threads
     const int colg = threadIdx.x % tk, cg = (threadIdx.x / tk) % 3;
     const LineMap mg = {(int)(threadIdx.x / (3 * tk)), true};
     C2<T> *lineG = sm + (size_t)(cg * tk + colg) * LP;
-    const int l = threadIdx.x / p.tline, t = threadIdx.x % p.tline;
+    const int l = threadIdx.x / PlanC<LC>::tline(p), t = threadIdx.x % PlanC<LC>::tline(p);
     const LineMap ms = {t, l < 3 * tk};
     C2<T> *line = sm + (size_t)l * LP;
     const int kxg = kx0 + colg;
     C2<T> *colp = base + cg * cstride + kxg;
-    fft_line<T, false, true, false>(
+    fft_line<T, false, true, false, LC>(
         p, twl, lineG, mg, line, ms,
         [&](int z) { return (z < nz && kxg < Hx) ? colp[z * zstride] : cmk<T>((T)0, (T)0); },
         [](int, C2<T>) {});
@@ -375,7 +375,7 @@ __global__ void k_zmi(C2<T> *__restrict__ S2, const T *__restrict__ K, size_t pl
         }
     }
     __syncthreads();
-    fft_line<T, true, false, true>(
+    fft_line<T, true, false, true, LC>(
         p, twl, lineG, mg, line, ms, [](int) { return cmk<T>((T)0, (T)0); },
         [&](int z, C2<T> v) {
             if (z < nz && kxg < Hx) colp[z * zstride] = v;
@@ -384,26 +384,26 @@ __global__ void k_zmi(C2<T> *__restrict__ S2, const T *__restrict__ K, size_t pl
 
 // 2-D: y forward (ny rows, zero-extended) + multiply (kz = 0) + y inverse,
 // keep y < ny, in place on S1 planes [c][y][kx].
-template <typename T>
+template <typename T, int LC = 0>
 __global__ void k_ymi2d(C2<T> *__restrict__ S1, const T *__restrict__ K, size_t plane, int Hx, int Hp, int Hy2,
                         int ny, FftPlanDev p, int tk)
 {
     extern __shared__ char smem_raw[];
     C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
-    const int L = p.L, LP = lds_len(L);
+    const int L = PlanC<LC>::L(p), LP = lds_len(L);
     const int kx0 = blockIdx.x * tk;
     const size_t cstride = (size_t)ny * Hp;
     C2<T> *twl = sm + (size_t)3 * tk * LP;
-    load_twiddles<T>(twl, p);
+    load_twiddles<T, LC>(twl, p);
     const int colg = threadIdx.x % tk, cg = (threadIdx.x / tk) % 3;
     const LineMap mg = {(int)(threadIdx.x / (3 * tk)), true};
     C2<T> *lineG = sm + (size_t)(cg * tk + colg) * LP;
-    const int l = threadIdx.x / p.tline, t = threadIdx.x % p.tline;
+    const int l = threadIdx.x / PlanC<LC>::tline(p), t = threadIdx.x % PlanC<LC>::tline(p);
     const LineMap ms = {t, l < 3 * tk};
     C2<T> *line = sm + (size_t)l * LP;
     const int kxg = kx0 + colg;
     C2<T> *colp = S1 + cg * cstride + kxg;
-    fft_line<T, false, true, false>(
+    fft_line<T, false, true, false, LC>(
         p, twl, lineG, mg, line, ms,
         [&](int y) { return (y < ny && kxg < Hx) ? colp[(size_t)y * Hp] : cmk<T>((T)0, (T)0); },
         [](int, C2<T>) {});
@@ -416,7 +416,7 @@ __global__ void k_ymi2d(C2<T> *__restrict__ S1, const T *__restrict__ K, size_t 
         }
     }
     __syncthreads();
-    fft_line<T, true, false, true>(
+    fft_line<T, true, false, true, LC>(
         p, twl, lineG, mg, line, ms, [](int) { return cmk<T>((T)0, (T)0); },
         [&](int y, C2<T> v) {
             if (y < ny && kxg < Hx) colp[(size_t)y * Hp] = v;
@@ -424,24 +424,24 @@ __global__ void k_ymi2d(C2<T> *__restrict__ S1, const T *__restrict__ K, size_t 
 }
 
 // x inverse (C2R): rows (c, z, y) of S1, Hermitian extension, keep x < nx
-template <typename T, typename TOUT>
+template <typename T, typename TOUT, int LC = 0>
 __global__ void k_xinv(const C2<T> *__restrict__ in, TOUT *__restrict__ out, size_t out_cstride, int Hx,
                        int Hp, int ny, int nz, int nrows, int nx, FftPlanDev p, int lpw)
 {
     extern __shared__ char smem_raw[];
     C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
-    const int L = p.L;
-    const int l = threadIdx.x / p.tline, t = threadIdx.x % p.tline;
+    const int L = PlanC<LC>::L(p);
+    const int l = threadIdx.x / PlanC<LC>::tline(p), t = threadIdx.x % PlanC<LC>::tline(p);
     const int g = blockIdx.x * lpw + l;
     const bool active = g < nrows;
     C2<T> *line = sm + (size_t)l * lds_len(L);
     C2<T> *twl = sm + (size_t)lpw * lds_len(L);
-    load_twiddles<T>(twl, p);
+    load_twiddles<T, LC>(twl, p);
     const int c = active ? g / (ny * nz) : 0, r = active ? g % (ny * nz) : 0;
     const C2<T> *src = in + (size_t)(active ? g : 0) * Hp;
     TOUT *dst = out + c * out_cstride + (size_t)r * nx;
     const LineMap m = {t, active};
-    fft_line<T, true, true, true>(
+    fft_line<T, true, true, true, LC>(
         p, twl, line, m, line, m, [&](int k) { return k < Hx ? src[k] : cconj<T>(src[L - k]); },
         [&](int x, C2<T> v) {
             if (x < nx) dst[x] = (TOUT)v.x;
@@ -550,6 +550,12 @@ void ConvImpl<T>::init(const double *N_host, int nx, int ny, int nz, int ex, int
     py.build(g.Py);
     pz.build(g.Pz);
     ln = choose_launch(g, px, py, pz, sizeof(C2<T>));
+    // MM_FFT_GENERIC=1: runtime FFT plans for every length (tests compare
+    // them with the compile-time plans)
+    {
+        const char *e = std::getenv("MM_FFT_GENERIC");
+        fft_generic = e && std::atoi(e) != 0;
+    }
     // pruned y transforms when the padding is at least half of the line
     {
         // measured on MI355X (256x256x16 cells): slower than the full-length
@@ -693,9 +699,11 @@ template <typename T> void ConvImpl<T>::forward_x(const T *M, hipStream_t s, con
     const size_t n = (size_t)g.nx * g.ny * g.nz;
     const int nrows = 3 * g.ny * g.nz;
     const size_t sh = lines_lds(ln.lpw_x, px.L, sizeof(C2<T>));
-    hipLaunchKernelGGL((k_xfwd<T, T>), dim3((nrows + ln.lpw_x - 1) / ln.lpw_x), dim3(ln.lpw_x * px.tline), sh, s,
-                       M, n, S1, (size_t)g.ny * g.nz * g.Hp, g.Hx, g.Hp, g.ny, g.nz, nrows, g.nx, px.dev<T>(), ln.lpw_x,
-                       M_alt, sel);
+    mm_with_len(px.L, fft_generic, [&](auto lc) {
+        hipLaunchKernelGGL((k_xfwd<T, T, decltype(lc)::value>), dim3((nrows + ln.lpw_x - 1) / ln.lpw_x),
+                           dim3(ln.lpw_x * px.tline), sh, s, M, n, S1, (size_t)g.ny * g.nz * g.Hp, g.Hx, g.Hp, g.ny,
+                           g.nz, nrows, g.nx, px.dev<T>(), ln.lpw_x, M_alt, sel);
+    });
     MM_HIP(hipGetLastError());
     if (klog) klog->mark(s, KC_XFWD, conv_bytes<T>(g, KC_XFWD));
 }
@@ -707,8 +715,10 @@ template <typename T> void ConvImpl<T>::spectral(hipStream_t s)
     if (g.is2d) {
         const int tk = ln.tk_y;
         const size_t sh = lines_lds(3 * tk, py.L, cs);
-        hipLaunchKernelGGL((k_ymi2d<T>), dim3((g.Hx + tk - 1) / tk), dim3(3 * tk * py.tline), sh, s, S1, kern, plane,
-                           g.Hx, g.Hp, g.Hy2, g.ny, py.dev<T>(), tk);
+        mm_with_len(py.L, fft_generic, [&](auto lc) {
+            hipLaunchKernelGGL((k_ymi2d<T, decltype(lc)::value>), dim3((g.Hx + tk - 1) / tk), dim3(3 * tk * py.tline),
+                               sh, s, S1, kern, plane, g.Hx, g.Hp, g.Hy2, g.ny, py.dev<T>(), tk);
+        });
         MM_HIP(hipGetLastError());
         if (klog) klog->mark(s, KC_YMI2D, conv_bytes<T>(g, KC_YMI2D));
         return;
@@ -721,23 +731,29 @@ template <typename T> void ConvImpl<T>::spectral(hipStream_t s)
         hipLaunchKernelGGL((k_yfwd_pr<T>), dim3((g.Hx + tkp - 1) / tkp, g.nz, 3), dim3(tkp * pyh.tline), shyp, s, S1, S2,
                            g.Hx, g.Hp, g.ny, g.nz, pyh.dev<T>(), (const C2<T> *)py.dev<T>().tw, tkp);
     } else {
-        hipLaunchKernelGGL((k_yfwd<T>), dim3((g.Hx + tk - 1) / tk, g.nz, 3), dim3(tk * py.tline), shy, s, S1, S2, g.Hx,
-                           g.Hp, g.ny, g.nz, py.dev<T>(), tk);
+        mm_with_len(py.L, fft_generic, [&](auto lc) {
+            hipLaunchKernelGGL((k_yfwd<T, decltype(lc)::value>), dim3((g.Hx + tk - 1) / tk, g.nz, 3),
+                               dim3(tk * py.tline), shy, s, S1, S2, g.Hx, g.Hp, g.ny, g.nz, py.dev<T>(), tk);
+        });
     }
     MM_HIP(hipGetLastError());
     if (klog) klog->mark(s, KC_YFWD, conv_bytes<T>(g, KC_YFWD));
     const int tz = ln.tk_z;
     const size_t shz = lines_lds(3 * tz, pz.L, cs);
-    hipLaunchKernelGGL((k_zmi<T>), dim3((g.Hx + tz - 1) / tz, g.Py), dim3(3 * tz * pz.tline), shz, s, S2, kern, plane,
-                       g.Hx, g.Hp, g.Hy2, g.Py, g.nz, pz.dev<T>(), tz);
+    mm_with_len(pz.L, fft_generic, [&](auto lc) {
+        hipLaunchKernelGGL((k_zmi<T, decltype(lc)::value>), dim3((g.Hx + tz - 1) / tz, g.Py), dim3(3 * tz * pz.tline),
+                           shz, s, S2, kern, plane, g.Hx, g.Hp, g.Hy2, g.Py, g.nz, pz.dev<T>(), tz);
+    });
     MM_HIP(hipGetLastError());
     if (klog) klog->mark(s, KC_ZMI, conv_bytes<T>(g, KC_ZMI));
     if (prune_y) {
         hipLaunchKernelGGL((k_yinv_pr<T>), dim3((g.Hx + tkp - 1) / tkp, g.nz, 3), dim3(tkp * pyh.tline), shyp, s, S2, S1,
                            g.Hx, g.Hp, g.ny, g.nz, pyh.dev<T>(), (const C2<T> *)py.dev<T>().tw, tkp);
     } else {
-        hipLaunchKernelGGL((k_yinv<T>), dim3((g.Hx + tk - 1) / tk, g.nz, 3), dim3(tk * py.tline), shy, s, S2, S1, g.Hx,
-                           g.Hp, g.ny, g.nz, py.dev<T>(), tk);
+        mm_with_len(py.L, fft_generic, [&](auto lc) {
+            hipLaunchKernelGGL((k_yinv<T, decltype(lc)::value>), dim3((g.Hx + tk - 1) / tk, g.nz, 3),
+                               dim3(tk * py.tline), shy, s, S2, S1, g.Hx, g.Hp, g.ny, g.nz, py.dev<T>(), tk);
+        });
     }
     MM_HIP(hipGetLastError());
     if (klog) klog->mark(s, KC_YINV, conv_bytes<T>(g, KC_YINV));
@@ -748,8 +764,11 @@ template <typename T> void ConvImpl<T>::inverse_x(T *H, hipStream_t s)
     const size_t n = (size_t)g.nx * g.ny * g.nz;
     const int nrows = 3 * g.ny * g.nz;
     const size_t sh = lines_lds(ln.lpw_x, px.L, sizeof(C2<T>));
-    hipLaunchKernelGGL((k_xinv<T, T>), dim3((nrows + ln.lpw_x - 1) / ln.lpw_x), dim3(ln.lpw_x * px.tline), sh, s, S1,
-                       H, n, g.Hx, g.Hp, g.ny, g.nz, nrows, g.nx, px.dev<T>(), ln.lpw_x);
+    mm_with_len(px.L, fft_generic, [&](auto lc) {
+        hipLaunchKernelGGL((k_xinv<T, T, decltype(lc)::value>), dim3((nrows + ln.lpw_x - 1) / ln.lpw_x),
+                           dim3(ln.lpw_x * px.tline), sh, s, S1, H, n, g.Hx, g.Hp, g.ny, g.nz, nrows, g.nx,
+                           px.dev<T>(), ln.lpw_x);
+    });
     MM_HIP(hipGetLastError());
 }
 

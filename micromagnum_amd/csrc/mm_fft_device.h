@@ -67,10 +67,11 @@ struct FftPlanDev {
 // The kernels copy the plan's twiddle table into LDS next to their lines
 // (one read of L values per workgroup) before the barrier that precedes the
 // first transform, so no pass waits on a global load.
-template <typename T> __device__ __forceinline__ void load_twiddles(C2<T> *twl, const FftPlanDev &p)
+template <typename T, int LC = 0> __device__ __forceinline__ void load_twiddles(C2<T> *twl, const FftPlanDev &p)
 {
     const C2<T> *g = (const C2<T> *)p.tw;
-    for (int i = threadIdx.x; i < p.L; i += blockDim.x) twl[i] = g[i];
+    const int L = LC > 0 ? LC : p.L;
+    for (int i = threadIdx.x; i < L; i += blockDim.x) twl[i] = g[i];
 }
 
 template <int R> struct Log2R { static const int value = R == 8 ? 3 : (R == 4 ? 2 : 1); };
@@ -355,25 +356,79 @@ __device__ __forceinline__ void fft_line_p2(const FftPlanDev &p, const C2<T> *tw
     unsigned rs = p.radices;
     for (int ps = 0; ps < np; ++ps, rs >>= 4) {
         const int R = (int)(rs & 15u);
+        // L and tline pass through an empty asm statement each pass, so the
+        // compiler cannot hoist the read addresses of every radix and
+        // position variant out of the pass loop (keeping them all live costs
+        // 150-250 VGPRs and spills; each variant computes its own in place)
+        int Lv = p.L, tlv = p.tline;
+        asm volatile("" : "+s"(Lv), "+s"(tlv));
         if (ps == 0 && ps == np - 1)
-            fft_pass_sel<T, INV, EPT, P2, LDIN, STOUT, true, true>(R, lineG, g, lineS, s_, tw, p.L, p.tline, Ns, lNs, ld, st);
+            fft_pass_sel<T, INV, EPT, P2, LDIN, STOUT, true, true>(R, lineG, g, lineS, s_, tw, Lv, tlv, Ns, lNs, ld, st);
         else if (ps == 0)
-            fft_pass_sel<T, INV, EPT, P2, LDIN, STOUT, true, false>(R, lineG, g, lineS, s_, tw, p.L, p.tline, Ns, lNs, ld, st);
+            fft_pass_sel<T, INV, EPT, P2, LDIN, STOUT, true, false>(R, lineG, g, lineS, s_, tw, Lv, tlv, Ns, lNs, ld, st);
         else if (ps == np - 1)
-            fft_pass_sel<T, INV, EPT, P2, LDIN, STOUT, false, true>(R, lineG, g, lineS, s_, tw, p.L, p.tline, Ns, lNs, ld, st);
+            fft_pass_sel<T, INV, EPT, P2, LDIN, STOUT, false, true>(R, lineG, g, lineS, s_, tw, Lv, tlv, Ns, lNs, ld, st);
         else
-            fft_pass_sel<T, INV, EPT, P2, LDIN, STOUT, false, false>(R, lineG, g, lineS, s_, tw, p.L, p.tline, Ns, lNs, ld, st);
+            fft_pass_sel<T, INV, EPT, P2, LDIN, STOUT, false, false>(R, lineG, g, lineS, s_, tw, Lv, tlv, Ns, lNs, ld, st);
         Ns *= R;
         lNs += R == 8 ? 3 : (R == 4 ? 2 : 1);
     }
 }
 
-template <typename T, bool INV, bool LDIN, bool STOUT, typename LD, typename ST>
+// ---- compile-time plans for power-of-two lengths ---------------------------
+// A kernel instantiated for a fixed power-of-two length LC runs the passes of
+// the runtime plan (radix-8 passes, then one radix-4 or radix-2 pass, as
+// FftPlanHost::build orders them) with every length, stride and pass count a
+// constant: the pass loop unrolls, the radix switch disappears and the index
+// arithmetic folds (shorter code, no divisions, fewer live registers).
+constexpr int mm_ilog2(int v) { return v <= 1 ? 0 : 1 + mm_ilog2(v >> 1); }
+
+template <int LC> struct FftLen {
+    static_assert(LC >= 2 && (LC & (LC - 1)) == 0, "compile-time plans are for powers of two");
+    static constexpr int lg = mm_ilog2(LC);
+    static constexpr int n8 = lg / 3, rem = lg % 3;
+    static constexpr int npass = n8 + (rem ? 1 : 0);
+    static constexpr int tline = LC > MM_FFT_EPT ? LC / MM_FFT_EPT : 1;
+    static constexpr int radix(int ps) { return ps < n8 ? 8 : (rem == 2 ? 4 : 2); }
+};
+
+// line length and threads per line of a kernel instantiated for LC (0: the
+// runtime plan p)
+template <int LC> struct PlanC {
+    __device__ static constexpr int L(const FftPlanDev &) { return LC; }
+    __device__ static constexpr int tline(const FftPlanDev &) { return FftLen<LC>::tline; }
+};
+template <> struct PlanC<0> {
+    __device__ static int L(const FftPlanDev &p) { return p.L; }
+    __device__ static int tline(const FftPlanDev &p) { return p.tline; }
+};
+
+template <typename T, bool INV, int LC, int PS, int NS, bool LDIN, bool STOUT, typename LD, typename ST>
+__device__ __forceinline__ void fft_line_c(const C2<T> *tw, C2<T> *lineG, LineMap g, C2<T> *lineS, LineMap s_,
+                                           LD &ld, ST &st)
+{
+    using F = FftLen<LC>;
+    if constexpr (PS < F::npass) {
+        constexpr int R = F::radix(PS);
+        constexpr bool I = LDIN && PS == 0, O = STOUT && PS == F::npass - 1;
+        fft_pass_io<R, MM_FFT_EPT, T, INV, true, I, O>(lineG, g, lineS, s_, tw, LC, F::tline, NS, mm_ilog2(NS), ld,
+                                                       st);
+        fft_line_c<T, INV, LC, PS + 1, NS * R, LDIN, STOUT>(tw, lineG, g, lineS, s_, ld, st);
+    }
+}
+
+// LC > 0: the compile-time plan of that length (p.L must equal LC); LC == 0:
+// the runtime plan p.
+template <typename T, bool INV, bool LDIN, bool STOUT, int LC = 0, typename LD, typename ST>
 __device__ __forceinline__ void fft_line(const FftPlanDev &p, const C2<T> *tw, C2<T> *lineG, LineMap g, C2<T> *lineS,
                                          LineMap s_, LD ld, ST st)
 {
-    if (p.pow2) fft_line_p2<T, INV, true, LDIN, STOUT>(p, tw, lineG, g, lineS, s_, ld, st);
-    else fft_line_p2<T, INV, false, LDIN, STOUT>(p, tw, lineG, g, lineS, s_, ld, st);
+    if constexpr (LC > 0) {
+        fft_line_c<T, INV, LC, 0, 1, LDIN, STOUT>(tw, lineG, g, lineS, s_, ld, st);
+    } else {
+        if (p.pow2) fft_line_p2<T, INV, true, LDIN, STOUT>(p, tw, lineG, g, lineS, s_, ld, st);
+        else fft_line_p2<T, INV, false, LDIN, STOUT>(p, tw, lineG, g, lineS, s_, ld, st);
+    }
 }
 
 #endif

@@ -335,7 +335,7 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
 // 1 inverse x transform of S1 rows in LDS, 2 H_demag read from Hdem (lines
 // too long for three in one workgroup's LDS).  EN: instead of the stage
 // epilogue, per-block sums of M.H per term into epart[q * gridDim.x + block].
-template <typename T, bool EN>
+template <typename T, bool EN, int LC = 0>
 __global__ void k_stage(SysDev<T> S, StageDev<T> G, StepCtl *ctl, const C2<T> *__restrict__ S1, FftPlanDev p,
                         int Hx, int Hp, int rpw, int has_demag, double *__restrict__ epart,
                         const T *__restrict__ Hdem)
@@ -347,18 +347,18 @@ __global__ void k_stage(SysDev<T> S, StageDev<T> G, StepCtl *ctl, const C2<T> *_
     const int nrows = ny * nz;
     const int row0 = blockIdx.x * rpw;
     const T *Y = G.Ys ? G.Ys : (ctl->cur ? G.yb1 : G.yb0);
-    const int L = p.L, LP = lds_len(L);
+    const int L = PlanC<LC>::L(p), LP = lds_len(L);
     C2<T> *twl = sm + (size_t)3 * rpw * LP;
     if (has_demag == 1) {
-        load_twiddles<T>(twl, p);
-        const int l = threadIdx.x / p.tline, t = threadIdx.x % p.tline;
+        load_twiddles<T, LC>(twl, p);
+        const int l = threadIdx.x / PlanC<LC>::tline(p), t = threadIdx.x % PlanC<LC>::tline(p);
         const int c = l / rpw, r = l % rpw;
         const int gr = row0 + r;
         const bool active = (l < 3 * rpw) && gr < nrows;
         C2<T> *line = sm + (size_t)l * LP;
         const C2<T> *src = S1 + ((size_t)c * nrows + (active ? gr : 0)) * Hp;
         const LineMap m = {t, active};
-        fft_line<T, true, true, false>(
+        fft_line<T, true, true, false, LC>(
             p, twl, line, m, line, m, [&](int k) { return k < Hx ? src[k] : cconj<T>(src[L - k]); },
             [](int, C2<T>) {});
     }
@@ -389,14 +389,14 @@ __global__ void k_stage(SysDev<T> S, StageDev<T> G, StepCtl *ctl, const C2<T> *_
             sm[(size_t)l * LP + sk(x)] = cmk<T>((T)0, (T)0);
         }
         __syncthreads();
-        const int l = threadIdx.x / p.tline, t = threadIdx.x % p.tline;
+        const int l = threadIdx.x / PlanC<LC>::tline(p), t = threadIdx.x % PlanC<LC>::tline(p);
         const int c = l / rpw, r = l % rpw;
         const int gr = row0 + r;
         const bool active = (l < 3 * rpw) && gr < nrows;
         C2<T> *line = sm + (size_t)l * LP;
         C2<T> *dst = G.fwd_S1 + ((size_t)c * nrows + (active ? gr : 0)) * Hp;
         const LineMap m = {t, active};
-        fft_line<T, false, false, true>(
+        fft_line<T, false, false, true, LC>(
             p, twl, line, m, line, m, [](int) { return cmk<T>((T)0, (T)0); },
             [&](int k, C2<T> v) {
                 if (k < Hx) dst[k] = v;
@@ -580,7 +580,7 @@ __global__ void k_prep_first(size_t m3, const T *__restrict__ y0, const T *__res
 // k_prep_first fused with the x forward transform of its result: rows
 // (c, z, y) of ytmp_1 = y + k_0 (h b10), written to `out` and, transformed
 // along x (zero-extended to L), to S1 rows (pitch Hp)
-template <typename T>
+template <typename T, int LC = 0>
 __global__ void k_xfwd_prep(const T *__restrict__ y0, const T *__restrict__ y1, const T *__restrict__ k0,
                             T *__restrict__ out, C2<T> *__restrict__ S1, const StepCtl *ctl, double b10, int nx,
                             int ny, int nz, int Hx, int Hp, FftPlanDev p, int lpw)
@@ -590,18 +590,18 @@ __global__ void k_xfwd_prep(const T *__restrict__ y0, const T *__restrict__ y1, 
     C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
     const T *y = ctl->cur ? y1 : y0;
     const T f = (T)(ctl->h * b10);
-    const int L = p.L;
+    const int L = PlanC<LC>::L(p);
     const int nrows = 3 * ny * nz;
-    const int l = threadIdx.x / p.tline, t = threadIdx.x % p.tline;
+    const int l = threadIdx.x / PlanC<LC>::tline(p), t = threadIdx.x % PlanC<LC>::tline(p);
     const int g = blockIdx.x * lpw + l;
     const bool active = g < nrows;
     C2<T> *line = sm + (size_t)l * lds_len(L);
     C2<T> *twl = sm + (size_t)lpw * lds_len(L);
-    load_twiddles<T>(twl, p);
+    load_twiddles<T, LC>(twl, p);
     const size_t base = (size_t)(active ? g : 0) * nx;
     C2<T> *dst = S1 + (size_t)(active ? g : 0) * Hp;
     const LineMap m = {t, active};
-    fft_line<T, false, true, true>(
+    fft_line<T, false, true, true, LC>(
         p, twl, line, m, line, m,
         [&](int x) {
             T v = (T)0;
@@ -945,6 +945,8 @@ template <typename T> struct Stepper : public StepperBase {
 
     // one stage: dM/dt at stage input Ys (null = y[cur])
     int demag_mode() const { return conv ? (split_x ? 2 : 1) : 0; }
+    // x transform length inside the stage kernel (0: it runs none)
+    int stage_len() const { return demag_mode() == 1 ? conv->px.L : 0; }
 
     // need_fwd: S1 does not yet hold the x transform of the stage input
     void launch_stage(const T *Ys, const StageDev<T> &G, bool need_fwd = true)
@@ -958,9 +960,12 @@ template <typename T> struct Stepper : public StepperBase {
             conv->spectral(s);
             if (split_x) conv->inverse_x(Hdem, s);
         }
-        hipLaunchKernelGGL((k_stage<T, false>), dim3(grid_rows), dim3(threads), stage_lds, s, sys, G, ctl,
-                           conv ? conv->S1 : nullptr, conv ? conv->px.template dev<T>() : FftPlanDev(),
-                           conv ? conv->g.Hx : 0, conv ? conv->g.Hp : 0, rpw, demag_mode(), nullptr, Hdem);
+        mm_with_len(stage_len(), conv && conv->fft_generic, [&](auto lc) {
+            hipLaunchKernelGGL((k_stage<T, false, decltype(lc)::value>), dim3(grid_rows), dim3(threads), stage_lds, s,
+                               sys, G, ctl, conv ? conv->S1 : nullptr,
+                               conv ? conv->px.template dev<T>() : FftPlanDev(), conv ? conv->g.Hx : 0,
+                               conv ? conv->g.Hp : 0, rpw, demag_mode(), nullptr, Hdem);
+        });
         MM_HIP(hipGetLastError());
         if (klog) klog->mark(s, KC_STAGE, stage_bytes(G));
     }
@@ -994,9 +999,12 @@ template <typename T> struct Stepper : public StepperBase {
                 const int lpw = conv->ln.lpw_x;
                 const int nrows = 3 * d.ny * d.nz;
                 const size_t sh = ((size_t)lpw * lds_len(px.L) + px.L) * sizeof(C2<T>);
-                hipLaunchKernelGGL(k_xfwd_prep<T>, dim3((nrows + lpw - 1) / lpw), dim3(lpw * px.tline), sh, s, yb[0],
-                                   yb[1], k[0], tmp[0], conv->S1, ctl, tab.b[1][0], d.nx, d.ny, d.nz, conv->g.Hx,
-                                   conv->g.Hp, px.template dev<T>(), lpw);
+                mm_with_len(px.L, conv->fft_generic, [&](auto lc) {
+                    hipLaunchKernelGGL((k_xfwd_prep<T, decltype(lc)::value>), dim3((nrows + lpw - 1) / lpw),
+                                       dim3(lpw * px.tline), sh, s, yb[0], yb[1], k[0], tmp[0], conv->S1, ctl,
+                                       tab.b[1][0], d.nx, d.ny, d.nz, conv->g.Hx, conv->g.Hp, px.template dev<T>(),
+                                       lpw);
+                });
                 MM_HIP(hipGetLastError());
                 if (klog)
                     klog->mark(s, KC_XFWD, 3.0 * 3.0 * sizeof(T) * n + 3.0 * 2.0 * sizeof(T) * conv->g.Hx * d.ny * d.nz);
@@ -1149,9 +1157,12 @@ template <typename T> struct Stepper : public StepperBase {
             conv->spectral(s);
             if (split_x) conv->inverse_x(Hdem, s);
         }
-        hipLaunchKernelGGL((k_stage<T, true>), dim3(grid_rows), dim3(threads), stage_lds, s, sys, G, ctl,
-                           conv ? conv->S1 : nullptr, conv ? conv->px.template dev<T>() : FftPlanDev(),
-                           conv ? conv->g.Hx : 0, conv ? conv->g.Hp : 0, rpw, demag_mode(), epart, Hdem);
+        mm_with_len(stage_len(), conv && conv->fft_generic, [&](auto lc) {
+            hipLaunchKernelGGL((k_stage<T, true, decltype(lc)::value>), dim3(grid_rows), dim3(threads), stage_lds, s,
+                               sys, G, ctl, conv ? conv->S1 : nullptr,
+                               conv ? conv->px.template dev<T>() : FftPlanDev(), conv ? conv->g.Hx : 0,
+                               conv ? conv->g.Hp : 0, rpw, demag_mode(), epart, Hdem);
+        });
         MM_HIP(hipGetLastError());
         std::vector<double> h((size_t)4 * grid_rows);
         MM_HIP(hipMemcpyAsync(h.data(), epart, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));

@@ -282,6 +282,26 @@ def test_conv_fp32_vs_oracle(mg, shape):
     assert rel_err(H, Ho) < 1e-5, rel_err(H, Ho)  # FP32 FFT, FP32 accumulation
 
 
+# padded lengths 4..2048 along every axis: the kernels' compile-time FFT plans
+POW2_SHAPES = [(8, 8, 8), (16, 32, 4), (64, 128, 2), (256, 4, 3), (512, 2, 1), (1024, 1, 1), (3, 2, 1)]
+
+
+@pytest.mark.parametrize("shape", POW2_SHAPES)
+def test_conv_compiled_plans_vs_oracle_and_runtime_plans(mg, shape, monkeypatch):
+    # power-of-two lengths run kernels instantiated for that length; the
+    # runtime plans (forced) run the same passes in generic kernels
+    H, Ho, _ = _conv_case(mg, shape, (2e-9, 3e-9, 2.5e-9), "fp64", seed=5)
+    assert rel_err(H, Ho) < TOL64, rel_err(H, Ho)
+    assert max_rel_err(H, Ho) < 1e-11
+    monkeypatch.setenv("MM_FFT_GENERIC", "1")
+    Hg, _, _ = _conv_case(mg, shape, (2e-9, 3e-9, 2.5e-9), "fp64", seed=5)
+    assert rel_err(H, Hg) < 1e-14, rel_err(H, Hg)
+    H32, _, _ = _conv_case(mg, shape, (2e-9, 3e-9, 2.5e-9), "fp32", seed=5)
+    monkeypatch.delenv("MM_FFT_GENERIC")
+    H32c, _, _ = _conv_case(mg, shape, (2e-9, 3e-9, 2.5e-9), "fp32", seed=5)
+    assert rel_err(H32c, Ho) < 1e-5 and rel_err(H32, H32c) < 1e-6
+
+
 def test_conv_non_smooth_padding_reembedded(mg):
     # round4(2*13-1) = 28 = 4*7 is not radix-friendly: the library re-embeds the
     # tensor into 30; the linear convolution is unchanged.
@@ -646,6 +666,35 @@ def test_ensemble_equals_single_steppers_and_oracle():
     assert s["attempts"] == so["attempts"] and s["rejections"] == so["rejections"]
     assert np.abs(ens.get_m(0) - Mo).max() / sp["Ms"] < 1e-10
     ens.close()
+
+
+@pytest.mark.parametrize("shape", [(32, 16, 2), (64, 8, 4)])
+def test_stepper_compiled_plans_match_runtime_plans_and_oracle(shape, monkeypatch):
+    # padded lengths (64, 32, 4) and (128, 16, 8): the stage kernel's x
+    # transforms and every pass of the field run compile-time plans
+    from micromagnum_amd.solver import Stepper
+    delta = (3e-9, 3e-9, 2e-9)
+    kw = dict(Ms=8.6e5, alpha=0.1, A=1.3e-11, P=0.7, xi=0.05, j=(1e12, 0.0, 0.0))
+    M = vortex_field(shape, delta, (shape[0] * 1.5e-9, shape[1] * 1.5e-9), Ms=8.6e5)
+    out = {}
+    for generic in ("0", "1"):
+        monkeypatch.setenv("MM_FFT_GENERIC", generic)
+        st = Stepper(shape, delta, precision="fp64", method="rkf45", eps_abs=1e-5, eps_rel=1e-5, **kw)
+        st.set_m(M)
+        st.run(10)
+        out[generic] = (st.get_m(), st.stats(), st.energy(), st.N)
+        st.close()
+    s0, s1 = out["0"][1], out["1"][1]
+    assert s0["attempts"] == s1["attempts"] and s0["rejections"] == s1["rejections"]
+    assert abs(s0["t"] - s1["t"]) <= 1e-10 * s0["t"]
+    assert np.abs(out["0"][0] - out["1"][0]).max() / 8.6e5 < 1e-10
+    for key, v in out["1"][2].items():
+        assert v == pytest.approx(out["0"][2][key], rel=1e-10, abs=1e-300)
+    sysO = O.System(shape, delta, N=out["0"][3], **kw)
+    Mo, so = sysO.run(M, method="rkf45", steps=10, eps_abs=1e-5, eps_rel=1e-5)
+    assert s0["attempts"] == so["attempts"] and s0["rejections"] == so["rejections"]
+    assert abs(s0["t"] - so["t"]) <= 1e-9 * so["t"]
+    assert np.abs(out["0"][0] - Mo).max() / 8.6e5 < 1e-9
 
 
 @pytest.mark.parametrize("method", ["rkf45", "dp54"])
