@@ -124,7 +124,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--cpu-steps", type=int, default=1, help="RKF45 steps in the CPU-baseline sample")
+    ap.add_argument("--cpu-steps", type=int, default=16, help="RKF45 steps in the CPU-baseline sample")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: all host cores (capped at 16)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")

@@ -124,6 +124,7 @@ struct ConvImpl {
     FftPlanHost px, py, pz;
     FftPlanHost pyh;        // length Py/2, for the pruned y transforms
     bool fft_generic = false;  // runtime FFT plans only (no compile-time lengths)
+    int xcd_remap = 1;         // XCD-contiguous tile order (xcd_tile)
     bool prune_y = false;
     int tk_yp = 1;          // kx columns per WG of the pruned y transforms
     T *kern = nullptr;      // 6 real planes [e][kz<Hz2][ky<Hy2][kx<Hx], pitch Hp, scaled 1/(PxPyPz)

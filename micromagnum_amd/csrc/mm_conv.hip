@@ -161,13 +161,15 @@ __global__ void k_xfwd(const TIN *__restrict__ in, size_t in_cstride, C2<T> *__r
 // y forward: columns kx of planes (c, z); rows y < ny_in read, Py written
 template <typename T, int LC = 0>
 __global__ void k_yfwd(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, int Hx, int Hp, int ny_in,
-                       int nzl, FftPlanDev p, int tk)
+                       int nzl, FftPlanDev p, int tk, int remap)
 {
     extern __shared__ char smem_raw[];
     C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
     const int L = PlanC<LC>::L(p), LP = lds_len(L);
-    const int kx0 = blockIdx.x * tk;
-    const int z = blockIdx.y, c = blockIdx.z;
+    // tiles (kx tile, z, c), kx tile fastest
+    const int tile = block_tile(remap), ntx = (int)gridDim.x;
+    const int kx0 = (tile % ntx) * tk;
+    const int z = (tile / ntx) % nzl, c = tile / (ntx * nzl);
     const C2<T> *src = in + ((size_t)c * nzl + z) * (size_t)ny_in * Hp;
     C2<T> *dst = out + ((size_t)c * nzl + z) * (size_t)L * Hp;
     C2<T> *twl = sm + (size_t)tk * LP;
@@ -188,13 +190,14 @@ __global__ void k_yfwd(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, in
 // y inverse: columns of S2 planes (c, z), all Py rows read, y < ny_out written
 template <typename T, int LC = 0>
 __global__ void k_yinv(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, int Hx, int Hp, int ny_out,
-                       int nzl, FftPlanDev p, int tk)
+                       int nzl, FftPlanDev p, int tk, int remap)
 {
     extern __shared__ char smem_raw[];
     C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
     const int L = PlanC<LC>::L(p), LP = lds_len(L);
-    const int kx0 = blockIdx.x * tk;
-    const int z = blockIdx.y, c = blockIdx.z;
+    const int tile = block_tile(remap), ntx = (int)gridDim.x;
+    const int kx0 = (tile % ntx) * tk;
+    const int z = (tile / ntx) % nzl, c = tile / (ntx * nzl);
     const C2<T> *src = in + ((size_t)c * nzl + z) * (size_t)L * Hp;
     C2<T> *dst = out + ((size_t)c * nzl + z) * (size_t)ny_out * Hp;
     C2<T> *twl = sm + (size_t)tk * LP;
@@ -340,13 +343,17 @@ __device__ __forceinline__ void mult3(C2<T> &mx, C2<T> &my, C2<T> &mz, const T k
 // keep z < nz.  Lines: 3 components x tk columns for one ky.
 template <typename T, int LC = 0>
 __global__ void k_zmi(C2<T> *__restrict__ S2, const T *__restrict__ K, size_t plane, int Hx, int Hp, int Hy2,
-                      int Py, int nz, FftPlanDev p, int tk)
+                      int Py, int nz, FftPlanDev p, int tk, int remap)
 {
     extern __shared__ char smem_raw[];
     C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
     const int L = PlanC<LC>::L(p), LP = lds_len(L);
-    const int kx0 = blockIdx.x * tk;
-    const int ky = blockIdx.y;
+    // tiles (kx tile, m), kx tile fastest.  With remap, rows come in the order
+    // 0, 1, Py-1, 2, Py-2, ...: ky and Py-ky read the same kernel row
+    const int tile = block_tile(remap), ntx = (int)gridDim.x;
+    const int kx0 = (tile % ntx) * tk;
+    const int m = tile / ntx, hm = (m + 1) >> 1;
+    const int ky = !remap || m == 0 ? m : ((m & 1) ? hm : Py - hm);
     const size_t cstride = (size_t)nz * Py * Hp;
     const size_t zstride = (size_t)Py * Hp;
     C2<T> *base = S2 + (size_t)ky * Hp;
@@ -556,6 +563,8 @@ void ConvImpl<T>::init(const double *N_host, int nx, int ny, int nz, int ex, int
         const char *e = std::getenv("MM_FFT_GENERIC");
         fft_generic = e && std::atoi(e) != 0;
     }
+    // MM_XCD_REMAP=0: hardware workgroup order for the tiles (measurements)
+    xcd_remap = env_int("MM_XCD_REMAP", 1) != 0 ? 1 : 0;
     // pruned y transforms when the padding is at least half of the line
     {
         // measured on MI355X (256x256x16 cells): slower than the full-length
@@ -632,7 +641,7 @@ void ConvImpl<T>::init(const double *N_host, int nx, int ny, int nz, int ex, int
                            (const int *)nullptr);
         MM_HIP(hipGetLastError());
         hipLaunchKernelGGL((k_yfwd<double>), dim3((g.Hx + tky - 1) / tky, Pz, 1), dim3(tky * py.tline), shy, 0,
-                           T1, T2, g.Hx, g.Hp, Py, Pz, dy, tky);
+                           T1, T2, g.Hx, g.Hp, Py, Pz, dy, tky, 0);
         MM_HIP(hipGetLastError());
         if (Pz > 1) {
             hipLaunchKernelGGL(k_zfwd_setup, dim3((g.Hx + tkz - 1) / tkz, Py, 1), dim3(tkz * pz.tline), shz, 0,
@@ -733,7 +742,8 @@ template <typename T> void ConvImpl<T>::spectral(hipStream_t s)
     } else {
         mm_with_len(py.L, fft_generic, [&](auto lc) {
             hipLaunchKernelGGL((k_yfwd<T, decltype(lc)::value>), dim3((g.Hx + tk - 1) / tk, g.nz, 3),
-                               dim3(tk * py.tline), shy, s, S1, S2, g.Hx, g.Hp, g.ny, g.nz, py.dev<T>(), tk);
+                               dim3(tk * py.tline), shy, s, S1, S2, g.Hx, g.Hp, g.ny, g.nz, py.dev<T>(), tk,
+                               xcd_remap);
         });
     }
     MM_HIP(hipGetLastError());
@@ -742,7 +752,7 @@ template <typename T> void ConvImpl<T>::spectral(hipStream_t s)
     const size_t shz = lines_lds(3 * tz, pz.L, cs);
     mm_with_len(pz.L, fft_generic, [&](auto lc) {
         hipLaunchKernelGGL((k_zmi<T, decltype(lc)::value>), dim3((g.Hx + tz - 1) / tz, g.Py), dim3(3 * tz * pz.tline),
-                           shz, s, S2, kern, plane, g.Hx, g.Hp, g.Hy2, g.Py, g.nz, pz.dev<T>(), tz);
+                           shz, s, S2, kern, plane, g.Hx, g.Hp, g.Hy2, g.Py, g.nz, pz.dev<T>(), tz, xcd_remap);
     });
     MM_HIP(hipGetLastError());
     if (klog) klog->mark(s, KC_ZMI, conv_bytes<T>(g, KC_ZMI));
@@ -752,7 +762,8 @@ template <typename T> void ConvImpl<T>::spectral(hipStream_t s)
     } else {
         mm_with_len(py.L, fft_generic, [&](auto lc) {
             hipLaunchKernelGGL((k_yinv<T, decltype(lc)::value>), dim3((g.Hx + tk - 1) / tk, g.nz, 3),
-                               dim3(tk * py.tline), shy, s, S2, S1, g.Hx, g.Hp, g.ny, g.nz, py.dev<T>(), tk);
+                               dim3(tk * py.tline), shy, s, S2, S1, g.Hx, g.Hp, g.ny, g.nz, py.dev<T>(), tk,
+                               xcd_remap);
         });
     }
     MM_HIP(hipGetLastError());

@@ -53,6 +53,28 @@ template <typename T, bool INV> __device__ __forceinline__ C2<T> cmul_mi(C2<T> a
 __host__ __device__ __forceinline__ int sk(int j) { return MM_LDS_SKEW ? j + (j >> MM_LDS_SKEW) : j; }
 __host__ __device__ __forceinline__ int lds_len(int L) { return (MM_LDS_SKEW ? L + (L >> MM_LDS_SKEW) : L) + 1; }
 
+// Workgroups of a launch are dispatched round-robin over gfx950's 8 XCDs,
+// each with its own L2: hardware workgroup b runs on XCD b % 8.  xcd_tile(b,
+// n) maps the n workgroups of a launch to logical tiles so that every XCD
+// takes one contiguous range of tiles, in order; tiles that share cache lines
+// (neighbouring rows, the halves of a 128-byte line, mirrored kernel rows)
+// then meet in one L2 instead of being fetched once per XCD.
+#define MM_NUM_XCD 8
+__device__ __forceinline__ int xcd_tile(int b, int n)
+{
+    const int x = b % MM_NUM_XCD, i = b / MM_NUM_XCD, q = n / MM_NUM_XCD, r = n % MM_NUM_XCD;
+    return x * q + (x < r ? x : r) + i;
+}
+
+// linear workgroup index of a 3-D grid (x fastest), remapped by xcd_tile when
+// remap is set
+__device__ __forceinline__ int block_tile(int remap)
+{
+    const int n = (int)(gridDim.x * gridDim.y * gridDim.z);
+    const int b = (int)(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
+    return remap ? xcd_tile(b, n) : b;
+}
+
 // device-side plan of one transform length (passed by value).  The radix of
 // pass p is (radices >> 4p) & 15, so no array is indexed at run time.
 struct FftPlanDev {

@@ -86,36 +86,50 @@ template <typename T> struct SysDev {
     // exch_f = 2 A / (mu0 Ms^2), the reference's (2A/(mu0 Ms)) sum (m_j - m_i) w
     int exch_uniform;
     T exch_f;
+    // rows of the stage launch in XCD-contiguous ranges (xcd_tile): the
+    // exchange stencil's neighbour rows are then read from the same L2
+    int xcd_remap;
 };
 
-// exchange field of cell i for uniform Ms and A (see exch_uniform)
+// exchange field of cell i for uniform Ms and A (see exch_uniform).  Each
+// direction's neighbour is the periodic image or, at an open boundary, the
+// cell itself (M_i - M_i adds an exact zero), so the six neighbours load
+// without branches, in one batch, and sum in the fixed order -x, +x, -y, +y,
+// -z, +z.
 template <typename T>
 __device__ __forceinline__ V3<T> exchange_cell_uniform(const MeshDev &m, int x, int y, int z, size_t i, size_t n,
                                                        T wx, T wy, T wz, T f, const T *__restrict__ M)
 {
+    const long li = (long)i, sy = m.nx, sz = (long)m.nx * m.ny;
+    long j[6];
+    j[0] = x > 0 ? li - 1 : (m.px ? li + (m.nx - 1) : li);
+    j[1] = x < m.nx - 1 ? li + 1 : (m.px ? li - (m.nx - 1) : li);
+    j[2] = y > 0 ? li - sy : (m.py ? li + sy * (m.ny - 1) : li);
+    j[3] = y < m.ny - 1 ? li + sy : (m.py ? li - sy * (m.ny - 1) : li);
+    j[4] = z > 0 ? li - sz : (m.pz ? li + sz * (m.nz - 1) : li);
+    j[5] = z < m.nz - 1 ? li + sz : (m.pz ? li - sz * (m.nz - 1) : li);
+    T a[6][3];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        a[q][0] = M[j[q]];
+        a[q][1] = M[n + j[q]];
+        a[q][2] = M[2 * n + j[q]];
+    }
     const V3<T> mi = v3<T>(M[i], M[n + i], M[2 * n + i]);
+    const T w[6] = {wx, wx, wy, wy, wz, wz};
     V3<T> s = v3<T>(0, 0, 0);
-    const long sx = 1, sy = m.nx, sz = (long)m.nx * m.ny;
-    auto add = [&](long j, T w) {
-        s.x += (M[j] - mi.x) * w;
-        s.y += (M[n + j] - mi.y) * w;
-        s.z += (M[2 * n + j] - mi.z) * w;
-    };
-    const long li = (long)i;
-    if (x > 0) add(li - sx, wx);
-    else if (m.px) add(li - sx + sx * m.nx, wx);
-    if (x < m.nx - 1) add(li + sx, wx);
-    else if (m.px) add(li + sx - sx * m.nx, wx);
-    if (y > 0) add(li - sy, wy);
-    else if (m.py) add(li - sy + sy * m.ny, wy);
-    if (y < m.ny - 1) add(li + sy, wy);
-    else if (m.py) add(li + sy - sy * m.ny, wy);
-    if (z > 0) add(li - sz, wz);
-    else if (m.pz) add(li - sz + sz * m.nz, wz);
-    if (z < m.nz - 1) add(li + sz, wz);
-    else if (m.pz) add(li + sz - sz * m.nz, wz);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        s.x += (a[q][0] - mi.x) * w[q];
+        s.y += (a[q][1] - mi.y) * w[q];
+        s.z += (a[q][2] - mi.z) * w[q];
+    }
     return v3<T>(f * s.x, f * s.y, f * s.z);
 }
+
+// stored stage vectors one stage combination reads at most (DP54's last
+// stage: k_0 .. k_5)
+#define MM_STAGE_MAXK 6
 
 template <typename T> struct StageDev {
     const T *Ys;          // stage input (null: y[cur])
@@ -127,10 +141,10 @@ template <typename T> struct StageDev {
     int form;             // FORM_SUM / FORM_SEQ
     T *prep_out;
     int nk;               // number of stored k_j used (j < s)
-    const T *kin[8];
-    double b[8];          // coefficients for kin
-    double bself;         // coefficient for k_s
-    double ec[8], ecself; // error coefficients (final stage)
+    const T *kin[MM_STAGE_MAXK];
+    double b[MM_STAGE_MAXK];  // coefficients for kin
+    double bself;             // coefficient for k_s
+    double ec[MM_STAGE_MAXK], ecself;  // error coefficients (final stage)
     int want_dydt;        // reduce max|k_s| (stage 0)
     int norm_mode;
     // FSAL (dp54, rk23): the last stage takes max|k_0| from kin[0] and then
@@ -237,43 +251,64 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
         red.dydt = fmax(red.dydt, q);
     }
     if (!G.prep) return;
-    const double h = ctl->h;
+    // the combination's inputs (stored stage vectors, the attempt's start
+    // state) load in one batch of independent loads before any arithmetic
+    // (loading them here rather than at the top keeps them out of the field
+    // computation's live registers)
     const T *yc = ctl->cur ? G.yb1 : G.yb0;
+    T kv[MM_STAGE_MAXK][3], ycv[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) ycv[c] = yc[(size_t)c * n + i];
+#pragma unroll
+    for (int q = 0; q < MM_STAGE_MAXK; ++q)
+        if (q < G.nk)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) kv[q][c] = G.kin[q][(size_t)c * n + i];
+    const double h = ctl->h;
     T kc[3] = {k.x, k.y, k.z};
     T out[3];
     double er[3] = {0.0, 0.0, 0.0};
+#pragma unroll
     for (int c = 0; c < 3; ++c) {
-        const size_t o = (size_t)c * n + i;
         if (G.form == FORM_SUM) {
             T s = (T)0;
             bool first = true;
-            for (int q = 0; q < G.nk; ++q) {
-                const T term = (T)G.b[q] * G.kin[q][o];
-                s = first ? term : s + term;
-                first = false;
+#pragma unroll
+            for (int q = 0; q < MM_STAGE_MAXK; ++q) {
+                if (q < G.nk) {
+                    const T term = (T)G.b[q] * kv[q][c];
+                    s = first ? term : s + term;
+                    first = false;
+                }
             }
             const T ts = (T)G.bself * kc[c];
             s = first ? ts : s + ts;
-            out[c] = yc[o] + (T)h * s;
+            out[c] = ycv[c] + (T)h * s;
             if (G.prep == 2) {
                 double e = 0.0;
                 bool f2 = true;
-                for (int q = 0; q < G.nk; ++q) {
-                    const double term = G.ec[q] * (double)G.kin[q][o];
-                    e = f2 ? term : e + term;
-                    f2 = false;
+#pragma unroll
+                for (int q = 0; q < MM_STAGE_MAXK; ++q) {
+                    if (q < G.nk) {
+                        const double term = G.ec[q] * (double)kv[q][c];
+                        e = f2 ? term : e + term;
+                        f2 = false;
+                    }
                 }
                 const double tse = G.ecself * (double)kc[c];
                 e = f2 ? tse : e + tse;
                 er[c] = h * e;
             }
         } else {
-            T v = yc[o];
+            T v = ycv[c];
             double e = 0.0;
-            for (int q = 0; q < G.nk; ++q) {
-                const T kq = G.kin[q][o];
-                v += kq * (T)(h * G.b[q]);
-                if (G.prep == 2) e += (double)kq * (h * G.ec[q]);
+#pragma unroll
+            for (int q = 0; q < MM_STAGE_MAXK; ++q) {
+                if (q < G.nk) {
+                    const T kq = kv[q][c];
+                    v += kq * (T)(h * G.b[q]);
+                    if (G.prep == 2) e += (double)kq * (h * G.ec[q]);
+                }
             }
             v += kc[c] * (T)(h * G.bself);
             if (G.prep == 2) e += (double)kc[c] * (h * G.ecself);
@@ -295,7 +330,7 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
         const double a = G.kin[0][i], b = G.kin[0][n + i], c = G.kin[0][2 * n + i];
         double q = a * a + b * b + c * c;
         red.draw = fmax(red.draw, q);
-        const double y0 = yc[i], y1 = yc[n + i], y2 = yc[2 * n + i];
+        const double y0 = ycv[0], y1 = ycv[1], y2 = ycv[2];
         red.mraw = fmax(red.mraw, y0 * y0 + y1 * y1 + y2 * y2);
         if (!S.Ms_uniform) {
             const double s = ms;
@@ -345,7 +380,7 @@ __global__ void k_stage(SysDev<T> S, StageDev<T> G, StepCtl *ctl, const C2<T> *_
     C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
     const int nx = S.m.nx, ny = S.m.ny, nz = S.m.nz;
     const int nrows = ny * nz;
-    const int row0 = blockIdx.x * rpw;
+    const int row0 = (S.xcd_remap ? xcd_tile((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x) * rpw;
     const T *Y = G.Ys ? G.Ys : (ctl->cur ? G.yb1 : G.yb0);
     const int L = PlanC<LC>::L(p), LP = lds_len(L);
     C2<T> *twl = sm + (size_t)3 * rpw * LP;
@@ -789,6 +824,10 @@ template <typename T> struct Stepper : public StepperBase {
             }
         }
         std::memset(&sys, 0, sizeof(sys));
+        {
+            const char *e = std::getenv("MM_XCD_REMAP");
+            sys.xcd_remap = e ? (std::atoi(e) != 0) : 1;
+        }
         sys.m.nx = d.nx;
         sys.m.ny = d.ny;
         sys.m.nz = d.nz;
@@ -877,6 +916,17 @@ template <typename T> struct Stepper : public StepperBase {
             stage_lds = 0;
         }
         threads = std::max(64, ((threads + 63) / 64) * 64);
+        if (conv && !split_x) {
+            // one epilogue cell per thread: the workgroup covers its rpw rows
+            // (threads beyond the 3 rpw x transform lines idle in the FFT
+            // passes).  Measured on MI355X, 256x256x16 cells FP32: 62 us per
+            // stage with 256 threads against 67 us with the 192 the
+            // transforms use.  MM_STAGE_THREADS overrides (measurements).
+            const int fft_threads = threads;
+            threads = std::max(fft_threads, std::min(1024, (rpw * d.nx + 63) / 64 * 64));
+            if (const char *e = std::getenv("MM_STAGE_THREADS"))
+                threads = std::min(1024, std::max(fft_threads, (std::atoi(e) + 63) / 64 * 64));
+        }
         grid_rows = (d.ny * d.nz + rpw - 1) / rpw;
         MM_HIP(hipMalloc(&epart, sizeof(double) * 4 * grid_rows));
         MM_HIP(hipMalloc(&mpart, sizeof(double) * 4 * grid_rows));
@@ -951,6 +1001,7 @@ template <typename T> struct Stepper : public StepperBase {
     // need_fwd: S1 does not yet hold the x transform of the stage input
     void launch_stage(const T *Ys, const StageDev<T> &G, bool need_fwd = true)
     {
+        MM_REQUIRE(G.nk >= 0 && G.nk <= MM_STAGE_MAXK, "stage combination reads too many stage vectors");
         if (conv) {
             conv->klog = klog;
             if (need_fwd) {
