@@ -339,6 +339,9 @@ __device__ __forceinline__ void mult3(C2<T> &mx, C2<T> &my, C2<T> &mz, const T k
     mz = cmk<T>(k[2] * x.x + k[4] * y.x + k[5] * z.x, k[2] * x.y + k[4] * y.y + k[5] * z.y);
 }
 
+// kernel items (column, kz pair) per thread of the z pass
+#define ZM_ITEMS 2
+
 // z pass (3-D): forward over z (nz rows, zero-extended), multiply, inverse,
 // keep z < nz.  Lines: 3 components x tk columns for one ky.
 template <typename T, int LC = 0>
@@ -369,16 +372,45 @@ __global__ void k_zmi(C2<T> *__restrict__ S2, const T *__restrict__ K, size_t pl
     C2<T> *line = sm + (size_t)l * LP;
     const int kxg = kx0 + colg;
     C2<T> *colp = base + cg * cstride + kxg;
+    // The kernel entries this thread multiplies load before the forward
+    // transform, so their latency hides behind it.  Items (col, kzf) with
+    // kzf <= L/2 each serve kz = kzf and the mirror kz = L - kzf, whose xz
+    // and yz entries change sign (real octant storage, as kernel_at); at most
+    // ZM_ITEMS per thread (the host checks).
+    const int nitems = tk * (L / 2 + 1);
+    const bool fy = ky > Py / 2;
+    const int kyf = fy ? Py - ky : ky;
+    const T sy = fy ? (T)-1 : (T)1;
+    T kk[ZM_ITEMS][6];
+#pragma unroll
+    for (int j = 0; j < ZM_ITEMS; ++j) {
+        const int e = threadIdx.x + j * blockDim.x;
+        const int col = e % tk, kzf = e / tk, kx = kx0 + col;
+        if (e < nitems && kx < Hx) {
+            const size_t o = ((size_t)kzf * Hy2 + kyf) * Hp + kx;
+#pragma unroll
+            for (int q = 0; q < 6; ++q) kk[j][q] = K[q * plane + o];
+        }
+    }
     fft_line<T, false, true, false, LC>(
         p, twl, lineG, mg, line, ms,
         [&](int z) { return (z < nz && kxg < Hx) ? colp[z * zstride] : cmk<T>((T)0, (T)0); },
         [](int, C2<T>) {});
-    for (int e = threadIdx.x; e < tk * L; e += blockDim.x) {
-        const int col = e % tk, kz = e / tk, kx = kx0 + col;
-        if (kx < Hx) {
-            T k[6];
-            kernel_at<T>(K, plane, Hp, Hy2, Py, L, kx, ky, kz, k);
-            mult3<T>(sm[(0 * tk + col) * LP + sk(kz)], sm[(1 * tk + col) * LP + sk(kz)], sm[(2 * tk + col) * LP + sk(kz)], k);
+#pragma unroll
+    for (int j = 0; j < ZM_ITEMS; ++j) {
+        const int e = threadIdx.x + j * blockDim.x;
+        const int col = e % tk, kzf = e / tk, kx = kx0 + col;
+        if (e < nitems && kx < Hx) {
+            T k[6] = {kk[j][0], sy * kk[j][1], kk[j][2], kk[j][3], sy * kk[j][4], kk[j][5]};
+            C2<T> *l0 = sm + (size_t)(0 * tk + col) * LP, *l1 = sm + (size_t)(1 * tk + col) * LP,
+                  *l2 = sm + (size_t)(2 * tk + col) * LP;
+            mult3<T>(l0[sk(kzf)], l1[sk(kzf)], l2[sk(kzf)], k);
+            const int kz2 = L - kzf;
+            if (kzf > 0 && kz2 != kzf) {
+                k[2] = -k[2];
+                k[4] = -k[4];
+                mult3<T>(l0[sk(kz2)], l1[sk(kz2)], l2[sk(kz2)], k);
+            }
         }
     }
     __syncthreads();
@@ -750,6 +782,7 @@ template <typename T> void ConvImpl<T>::spectral(hipStream_t s)
     if (klog) klog->mark(s, KC_YFWD, conv_bytes<T>(g, KC_YFWD));
     const int tz = ln.tk_z;
     const size_t shz = lines_lds(3 * tz, pz.L, cs);
+    MM_REQUIRE((size_t)tz * (pz.L / 2 + 1) <= (size_t)ZM_ITEMS * 3 * tz * pz.tline, "z pass items per thread");
     mm_with_len(pz.L, fft_generic, [&](auto lc) {
         hipLaunchKernelGGL((k_zmi<T, decltype(lc)::value>), dim3((g.Hx + tz - 1) / tz, g.Py), dim3(3 * tz * pz.tline),
                            shz, s, S2, kern, plane, g.Hx, g.Hp, g.Hy2, g.Py, g.nz, pz.dev<T>(), tz, xcd_remap);

@@ -180,8 +180,11 @@ struct EnergyAcc {
     double demag, exch, aniso, zee;
 };
 
-// Per-cell epilogue: everything after the demag field of one cell.
-template <typename T, bool EN>
+// Per-cell epilogue: everything after the demag field of one cell.  STT:
+// the Zhang-Li term is compiled in; GEN: cubic anisotropy and the exchange
+// with per-cell Ms/A are compiled in (without them the epilogue is shorter
+// and needs fewer registers).
+template <typename T, bool EN, bool STT, bool GEN>
 __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T> &G, const StepCtl *ctl,
                                            const T *__restrict__ Y, int x, int y, int z, const V3<T> &Hd,
                                            StageRed &red, EnergyAcc &ea, T *nxt)
@@ -193,7 +196,7 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
     V3<T> H = Hd;
     if (EN) ea.demag += dotd<T>(Mv, Hd);
     if (S.has_exch && ms != (T)0) {
-        const V3<T> he = S.exch_uniform
+        const V3<T> he = (!GEN || S.exch_uniform)
                              ? exchange_cell_uniform<T>(S.m, x, y, z, i, n, S.wx, S.wy, S.wz, S.exch_f, Y)
                              : exchange_cell<T>(S.m, x, y, z, i, n, S.wx, S.wy, S.wz, S.coef_ex,
                                                 S.Ms.p ? S.Ms.p : nullptr, S.A.p ? S.A.p : nullptr, Y, ms);
@@ -202,14 +205,14 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
         H.z += he.z;
         if (EN) ea.exch += dotd<T>(Mv, he);
     }
-    if (S.has_uni || S.has_cub) {
+    if (S.has_uni || (GEN && S.has_cub)) {
         T e0 = (T)0, e1 = (T)0;
         V3<T> ha = v3<T>(0, 0, 0);
-        if (S.has_cub) ha = cubic_cell<T>(Mv, S.axis1.at(n, i), S.axis2.at(n, i), S.kcub.at(i), ms, S.cm2mu0, &e1);
+        if (GEN && S.has_cub) ha = cubic_cell<T>(Mv, S.axis1.at(n, i), S.axis2.at(n, i), S.kcub.at(i), ms, S.cm2mu0, &e1);
         if (S.has_uni) {
             const V3<T> hu = uniaxial_cell<T>(Mv, S.axis1.at(n, i), S.kuni.at(i), ms, S.mu0, &e0);
             if (EN) ea.aniso += (double)e0;
-            if (S.has_cub) {
+            if (GEN && S.has_cub) {
                 ha.x += hu.x;
                 ha.y += hu.y;
                 ha.z += hu.z;
@@ -230,7 +233,7 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
     }
     if (EN) return;
     V3<T> k = llge_cell<T>(Mv, H, S.f1.at(i), S.f2.at(i));
-    if (S.has_stt) {
+    if (STT && S.has_stt) {
         const V3<T> st = zhangli_cell<T>(S.m, x, y, z, i, n, S.dx, S.dy, S.dz, S.do_precess != 0, S.P.at(i), S.xi.at(i),
                                          ms, S.alpha.at(i), S.j.at(n, i), Y, S.mu_b, S.e_charge);
         k.x += st.x;
@@ -370,7 +373,7 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
 // 1 inverse x transform of S1 rows in LDS, 2 H_demag read from Hdem (lines
 // too long for three in one workgroup's LDS).  EN: instead of the stage
 // epilogue, per-block sums of M.H per term into epart[q * gridDim.x + block].
-template <typename T, bool EN, int LC = 0>
+template <typename T, bool EN, int LC, bool STT, bool GEN>
 __global__ void k_stage(SysDev<T> S, StageDev<T> G, StepCtl *ctl, const C2<T> *__restrict__ S1, FftPlanDev p,
                         int Hx, int Hp, int rpw, int has_demag, double *__restrict__ epart,
                         const T *__restrict__ Hdem)
@@ -412,7 +415,7 @@ __global__ void k_stage(SysDev<T> S, StageDev<T> G, StepCtl *ctl, const C2<T> *_
             Hd = v3<T>(Hdem[i], Hdem[S.n + i], Hdem[2 * S.n + i]);
         }
         T nxt[3];
-        stage_cell<T, EN>(S, G, ctl, Y, x, gr % ny, gr / ny, Hd, red, ea, nxt);
+        stage_cell<T, EN, STT, GEN>(S, G, ctl, Y, x, gr % ny, gr / ny, Hd, red, ea, nxt);
         // the slot (r, x) of the three lines is this thread's alone: the next
         // stage input replaces H_demag in place
         if (fused)
@@ -998,6 +1001,30 @@ template <typename T> struct Stepper : public StepperBase {
     // x transform length inside the stage kernel (0: it runs none)
     int stage_len() const { return demag_mode() == 1 ? conv->px.L : 0; }
 
+    // the stage kernel instantiated for the x transform length and the
+    // physics present: STT (Zhang-Li) and GEN (cubic anisotropy or exchange
+    // with per-cell Ms/A); the energy launch (EN) has no spin torque term
+    template <bool EN, int LC, bool STT, bool GEN> void stage_launch(const StageDev<T> &G, double *ep)
+    {
+        hipLaunchKernelGGL((k_stage<T, EN, LC, STT, GEN>), dim3(grid_rows), dim3(threads), stage_lds, s, sys, G, ctl,
+                           conv ? conv->S1 : nullptr, conv ? conv->px.template dev<T>() : FftPlanDev(),
+                           conv ? conv->g.Hx : 0, conv ? conv->g.Hp : 0, rpw, demag_mode(), ep, Hdem);
+    }
+    template <bool EN> void stage_kernel(const StageDev<T> &G, double *ep)
+    {
+        const bool stt = !EN && sys.has_stt, gen = sys.has_cub || (sys.has_exch && !sys.exch_uniform);
+        mm_with_len(stage_len(), conv && conv->fft_generic, [&](auto lc) {
+            constexpr int LC = decltype(lc)::value;
+            if (stt) {
+                if (gen) stage_launch<EN, LC, !EN, true>(G, ep);
+                else stage_launch<EN, LC, !EN, false>(G, ep);
+            } else {
+                if (gen) stage_launch<EN, LC, false, true>(G, ep);
+                else stage_launch<EN, LC, false, false>(G, ep);
+            }
+        });
+    }
+
     // need_fwd: S1 does not yet hold the x transform of the stage input
     void launch_stage(const T *Ys, const StageDev<T> &G, bool need_fwd = true)
     {
@@ -1011,12 +1038,7 @@ template <typename T> struct Stepper : public StepperBase {
             conv->spectral(s);
             if (split_x) conv->inverse_x(Hdem, s);
         }
-        mm_with_len(stage_len(), conv && conv->fft_generic, [&](auto lc) {
-            hipLaunchKernelGGL((k_stage<T, false, decltype(lc)::value>), dim3(grid_rows), dim3(threads), stage_lds, s,
-                               sys, G, ctl, conv ? conv->S1 : nullptr,
-                               conv ? conv->px.template dev<T>() : FftPlanDev(), conv ? conv->g.Hx : 0,
-                               conv ? conv->g.Hp : 0, rpw, demag_mode(), nullptr, Hdem);
-        });
+        stage_kernel<false>(G, nullptr);
         MM_HIP(hipGetLastError());
         if (klog) klog->mark(s, KC_STAGE, stage_bytes(G));
     }
@@ -1208,12 +1230,7 @@ template <typename T> struct Stepper : public StepperBase {
             conv->spectral(s);
             if (split_x) conv->inverse_x(Hdem, s);
         }
-        mm_with_len(stage_len(), conv && conv->fft_generic, [&](auto lc) {
-            hipLaunchKernelGGL((k_stage<T, true, decltype(lc)::value>), dim3(grid_rows), dim3(threads), stage_lds, s,
-                               sys, G, ctl, conv ? conv->S1 : nullptr,
-                               conv ? conv->px.template dev<T>() : FftPlanDev(), conv ? conv->g.Hx : 0,
-                               conv ? conv->g.Hp : 0, rpw, demag_mode(), epart, Hdem);
-        });
+        stage_kernel<true>(G, epart);
         MM_HIP(hipGetLastError());
         std::vector<double> h((size_t)4 * grid_rows);
         MM_HIP(hipMemcpyAsync(h.data(), epart, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
