@@ -137,6 +137,7 @@ __global__ void k_xfwd(const TIN *__restrict__ in, size_t in_cstride, C2<T> *__r
                        FftPlanDev p, int lpw, const TIN *__restrict__ in_alt = nullptr,
                        const int *__restrict__ sel = nullptr)
 {
+    constexpr int SK = MM_SKEW_X;
     if (sel && *sel) in = in_alt;
     extern __shared__ char smem_raw[];
     C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
@@ -144,14 +145,14 @@ __global__ void k_xfwd(const TIN *__restrict__ in, size_t in_cstride, C2<T> *__r
     const int l = threadIdx.x / PlanC<LC>::tline(p), t = threadIdx.x % PlanC<LC>::tline(p);
     const int g = blockIdx.x * lpw + l;
     const bool active = g < nrows;
-    C2<T> *line = sm + (size_t)l * lds_len(L);
-    C2<T> *twl = sm + (size_t)lpw * lds_len(L);
+    C2<T> *line = sm + (size_t)l * lds_len<SK>(L);
+    C2<T> *twl = sm + (size_t)lpw * lds_len<SK>(L);
     load_twiddles<T, LC>(twl, p);
     const int c = active ? g / (ny * nz) : 0, r = active ? g % (ny * nz) : 0;
     const TIN *src = in + c * in_cstride + (size_t)r * nx_in;
     C2<T> *dst = out + c * out_cstride + (size_t)r * Hp;
     const LineMap m = {t, active};
-    fft_line<T, false, true, true, LC>(
+    fft_line<T, false, true, true, LC, SK>(
         p, twl, line, m, line, m, [&](int x) { return cmk<T>(x < nx_in ? (T)src[x] : (T)0, (T)0); },
         [&](int kx, C2<T> v) {
             if (kx < Hx) dst[kx] = v;
@@ -163,9 +164,10 @@ template <typename T, int LC = 0>
 __global__ void k_yfwd(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, int Hx, int Hp, int ny_in,
                        int nzl, FftPlanDev p, int tk, int remap)
 {
+    constexpr int SK = MM_SKEW_Y;
     extern __shared__ char smem_raw[];
     C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
-    const int L = PlanC<LC>::L(p), LP = lds_len(L);
+    const int L = PlanC<LC>::L(p), LP = lds_len<SK>(L);
     // tiles (kx tile, z, c), kx tile fastest
     const int tile = block_tile(remap), ntx = (int)gridDim.x;
     const int kx0 = (tile % ntx) * tk;
@@ -179,7 +181,7 @@ __global__ void k_yfwd(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, in
     const int lg = threadIdx.x % tk, ls = threadIdx.x / PlanC<LC>::tline(p);
     const LineMap mg = {(int)(threadIdx.x / tk), true}, ms = {(int)(threadIdx.x % PlanC<LC>::tline(p)), ls < tk};
     const int kx = kx0 + lg;
-    fft_line<T, false, true, true, LC>(
+    fft_line<T, false, true, true, LC, SK>(
         p, twl, sm + (size_t)lg * LP, mg, sm + (size_t)ls * LP, ms,
         [&](int y) { return (y < ny_in && kx < Hx) ? src[(size_t)y * Hp + kx] : cmk<T>((T)0, (T)0); },
         [&](int y, C2<T> v) {
@@ -192,9 +194,10 @@ template <typename T, int LC = 0>
 __global__ void k_yinv(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, int Hx, int Hp, int ny_out,
                        int nzl, FftPlanDev p, int tk, int remap)
 {
+    constexpr int SK = MM_SKEW_Y;
     extern __shared__ char smem_raw[];
     C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
-    const int L = PlanC<LC>::L(p), LP = lds_len(L);
+    const int L = PlanC<LC>::L(p), LP = lds_len<SK>(L);
     const int tile = block_tile(remap), ntx = (int)gridDim.x;
     const int kx0 = (tile % ntx) * tk;
     const int z = (tile / ntx) % nzl, c = tile / (ntx * nzl);
@@ -205,7 +208,7 @@ __global__ void k_yinv(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, in
     const int lg = threadIdx.x % tk, ls = threadIdx.x / PlanC<LC>::tline(p);
     const LineMap mg = {(int)(threadIdx.x / tk), true}, ms = {(int)(threadIdx.x % PlanC<LC>::tline(p)), ls < tk};
     const int kx = kx0 + lg;
-    fft_line<T, true, true, true, LC>(
+    fft_line<T, true, true, true, LC, SK>(
         p, twl, sm + (size_t)lg * LP, mg, sm + (size_t)ls * LP, ms,
         [&](int y) { return kx < Hx ? src[(size_t)y * Hp + kx] : cmk<T>((T)0, (T)0); },
         [&](int y, C2<T> v) {
@@ -225,9 +228,10 @@ template <typename T>
 __global__ void k_yfwd_pr(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, int Hx, int Hp, int ny_in,
                           int nzl, FftPlanDev ph, const C2<T> *__restrict__ twP, int tk)
 {
+    constexpr int SK = MM_SKEW_Y;
     extern __shared__ char smem_raw[];
     C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
-    const int h = ph.L, LP = lds_len(h);
+    const int h = ph.L, LP = lds_len<SK>(h);
     const int kx0 = blockIdx.x * tk;
     const int z = blockIdx.y, c = blockIdx.z;
     const C2<T> *src = in + ((size_t)c * nzl + z) * (size_t)ny_in * Hp;
@@ -243,28 +247,28 @@ __global__ void k_yfwd_pr(const C2<T> *__restrict__ in, C2<T> *__restrict__ out,
         xr[q] = cmk<T>((T)0, (T)0);
         if (e < tk * h) {
             if (n < ny_in && kx < Hx) xr[q] = src[(size_t)n * Hp + kx];
-            sm[col * LP + sk(n)] = xr[q];
+            sm[col * LP + sk<SK>(n)] = xr[q];
         }
     }
     __syncthreads();
     const int l = threadIdx.x / ph.tline, t = threadIdx.x % ph.tline;
-    lds_fft<T, false>(sm + (size_t)l * LP, ph, t, l < tk, twl);
+    lds_fft<T, false, SK>(sm + (size_t)l * LP, ph, t, l < tk, twl);
     for (int e = threadIdx.x; e < tk * h; e += blockDim.x) {
         const int col = e % tk, m = e / tk, kx = kx0 + col;
-        if (kx < Hx) dst[(size_t)(2 * m) * Hp + kx] = sm[col * LP + sk(m)];
+        if (kx < Hx) dst[(size_t)(2 * m) * Hp + kx] = sm[col * LP + sk<SK>(m)];
     }
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < MAXE; ++q) {
         const int e = threadIdx.x + q * blockDim.x;
         const int col = e % tk, n = e / tk;
-        if (e < tk * h) sm[col * LP + sk(n)] = cmul<T>(xr[q], twP[n]);
+        if (e < tk * h) sm[col * LP + sk<SK>(n)] = cmul<T>(xr[q], twP[n]);
     }
     __syncthreads();
-    lds_fft<T, false>(sm + (size_t)l * LP, ph, t, l < tk, twl);
+    lds_fft<T, false, SK>(sm + (size_t)l * LP, ph, t, l < tk, twl);
     for (int e = threadIdx.x; e < tk * h; e += blockDim.x) {
         const int col = e % tk, m = e / tk, kx = kx0 + col;
-        if (kx < Hx) dst[(size_t)(2 * m + 1) * Hp + kx] = sm[col * LP + sk(m)];
+        if (kx < Hx) dst[(size_t)(2 * m + 1) * Hp + kx] = sm[col * LP + sk<SK>(m)];
     }
 }
 
@@ -272,9 +276,10 @@ template <typename T>
 __global__ void k_yinv_pr(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, int Hx, int Hp, int ny_out,
                           int nzl, FftPlanDev ph, const C2<T> *__restrict__ twP, int tk)
 {
+    constexpr int SK = MM_SKEW_Y;
     extern __shared__ char smem_raw[];
     C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
-    const int h = ph.L, LP = lds_len(h);
+    const int h = ph.L, LP = lds_len<SK>(h);
     const int kx0 = blockIdx.x * tk;
     const int z = blockIdx.y, c = blockIdx.z;
     const C2<T> *src = in + ((size_t)c * nzl + z) * (size_t)(2 * h) * Hp;
@@ -286,29 +291,29 @@ __global__ void k_yinv_pr(const C2<T> *__restrict__ in, C2<T> *__restrict__ out,
     C2<T> ar[MAXE];
     for (int e = threadIdx.x; e < tk * h; e += blockDim.x) {
         const int col = e % tk, m = e / tk, kx = kx0 + col;
-        sm[col * LP + sk(m)] = (kx < Hx) ? src[(size_t)(2 * m) * Hp + kx] : cmk<T>((T)0, (T)0);
+        sm[col * LP + sk<SK>(m)] = (kx < Hx) ? src[(size_t)(2 * m) * Hp + kx] : cmk<T>((T)0, (T)0);
     }
     __syncthreads();
-    lds_fft<T, true>(sm + (size_t)l * LP, ph, t, l < tk, twl);
+    lds_fft<T, true, SK>(sm + (size_t)l * LP, ph, t, l < tk, twl);
 #pragma unroll
     for (int q = 0; q < MAXE; ++q) {
         const int e = threadIdx.x + q * blockDim.x;
         const int col = e % tk, n = e / tk;
-        ar[q] = (e < tk * h) ? sm[col * LP + sk(n)] : cmk<T>((T)0, (T)0);
+        ar[q] = (e < tk * h) ? sm[col * LP + sk<SK>(n)] : cmk<T>((T)0, (T)0);
     }
     __syncthreads();
     for (int e = threadIdx.x; e < tk * h; e += blockDim.x) {
         const int col = e % tk, m = e / tk, kx = kx0 + col;
-        sm[col * LP + sk(m)] = (kx < Hx) ? src[(size_t)(2 * m + 1) * Hp + kx] : cmk<T>((T)0, (T)0);
+        sm[col * LP + sk<SK>(m)] = (kx < Hx) ? src[(size_t)(2 * m + 1) * Hp + kx] : cmk<T>((T)0, (T)0);
     }
     __syncthreads();
-    lds_fft<T, true>(sm + (size_t)l * LP, ph, t, l < tk, twl);
+    lds_fft<T, true, SK>(sm + (size_t)l * LP, ph, t, l < tk, twl);
 #pragma unroll
     for (int q = 0; q < MAXE; ++q) {
         const int e = threadIdx.x + q * blockDim.x;
         const int col = e % tk, n = e / tk, kx = kx0 + col;
         if (e < tk * h && n < ny_out && kx < Hx)
-            dst[(size_t)n * Hp + kx] = cadd<T>(ar[q], cmul<T>(sm[col * LP + sk(n)], cconj<T>(twP[n])));
+            dst[(size_t)n * Hp + kx] = cadd<T>(ar[q], cmul<T>(sm[col * LP + sk<SK>(n)], cconj<T>(twP[n])));
     }
 }
 
@@ -348,9 +353,10 @@ template <typename T, int LC = 0>
 __global__ void k_zmi(C2<T> *__restrict__ S2, const T *__restrict__ K, size_t plane, int Hx, int Hp, int Hy2,
                       int Py, int nz, FftPlanDev p, int tk, int remap)
 {
+    constexpr int SK = MM_SKEW_Z;
     extern __shared__ char smem_raw[];
     C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
-    const int L = PlanC<LC>::L(p), LP = lds_len(L);
+    const int L = PlanC<LC>::L(p), LP = lds_len<SK>(L);
     // tiles (kx tile, m), kx tile fastest.  With remap, rows come in the order
     // 0, 1, Py-1, 2, Py-2, ...: ky and Py-ky read the same kernel row
     const int tile = block_tile(remap), ntx = (int)gridDim.x;
@@ -392,7 +398,7 @@ __global__ void k_zmi(C2<T> *__restrict__ S2, const T *__restrict__ K, size_t pl
             for (int q = 0; q < 6; ++q) kk[j][q] = K[q * plane + o];
         }
     }
-    fft_line<T, false, true, false, LC>(
+    fft_line<T, false, true, false, LC, SK>(
         p, twl, lineG, mg, line, ms,
         [&](int z) { return (z < nz && kxg < Hx) ? colp[z * zstride] : cmk<T>((T)0, (T)0); },
         [](int, C2<T>) {});
@@ -404,17 +410,17 @@ __global__ void k_zmi(C2<T> *__restrict__ S2, const T *__restrict__ K, size_t pl
             T k[6] = {kk[j][0], sy * kk[j][1], kk[j][2], kk[j][3], sy * kk[j][4], kk[j][5]};
             C2<T> *l0 = sm + (size_t)(0 * tk + col) * LP, *l1 = sm + (size_t)(1 * tk + col) * LP,
                   *l2 = sm + (size_t)(2 * tk + col) * LP;
-            mult3<T>(l0[sk(kzf)], l1[sk(kzf)], l2[sk(kzf)], k);
+            mult3<T>(l0[sk<SK>(kzf)], l1[sk<SK>(kzf)], l2[sk<SK>(kzf)], k);
             const int kz2 = L - kzf;
             if (kzf > 0 && kz2 != kzf) {
                 k[2] = -k[2];
                 k[4] = -k[4];
-                mult3<T>(l0[sk(kz2)], l1[sk(kz2)], l2[sk(kz2)], k);
+                mult3<T>(l0[sk<SK>(kz2)], l1[sk<SK>(kz2)], l2[sk<SK>(kz2)], k);
             }
         }
     }
     __syncthreads();
-    fft_line<T, true, false, true, LC>(
+    fft_line<T, true, false, true, LC, SK>(
         p, twl, lineG, mg, line, ms, [](int) { return cmk<T>((T)0, (T)0); },
         [&](int z, C2<T> v) {
             if (z < nz && kxg < Hx) colp[z * zstride] = v;
@@ -427,9 +433,10 @@ template <typename T, int LC = 0>
 __global__ void k_ymi2d(C2<T> *__restrict__ S1, const T *__restrict__ K, size_t plane, int Hx, int Hp, int Hy2,
                         int ny, FftPlanDev p, int tk)
 {
+    constexpr int SK = MM_SKEW_Y;
     extern __shared__ char smem_raw[];
     C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
-    const int L = PlanC<LC>::L(p), LP = lds_len(L);
+    const int L = PlanC<LC>::L(p), LP = lds_len<SK>(L);
     const int kx0 = blockIdx.x * tk;
     const size_t cstride = (size_t)ny * Hp;
     C2<T> *twl = sm + (size_t)3 * tk * LP;
@@ -442,7 +449,7 @@ __global__ void k_ymi2d(C2<T> *__restrict__ S1, const T *__restrict__ K, size_t 
     C2<T> *line = sm + (size_t)l * LP;
     const int kxg = kx0 + colg;
     C2<T> *colp = S1 + cg * cstride + kxg;
-    fft_line<T, false, true, false, LC>(
+    fft_line<T, false, true, false, LC, SK>(
         p, twl, lineG, mg, line, ms,
         [&](int y) { return (y < ny && kxg < Hx) ? colp[(size_t)y * Hp] : cmk<T>((T)0, (T)0); },
         [](int, C2<T>) {});
@@ -451,11 +458,11 @@ __global__ void k_ymi2d(C2<T> *__restrict__ S1, const T *__restrict__ K, size_t 
         if (kx < Hx) {
             T k[6];
             kernel_at<T>(K, plane, Hp, Hy2, L, 1, kx, ky, 0, k);
-            mult3<T>(sm[(0 * tk + col) * LP + sk(ky)], sm[(1 * tk + col) * LP + sk(ky)], sm[(2 * tk + col) * LP + sk(ky)], k);
+            mult3<T>(sm[(0 * tk + col) * LP + sk<SK>(ky)], sm[(1 * tk + col) * LP + sk<SK>(ky)], sm[(2 * tk + col) * LP + sk<SK>(ky)], k);
         }
     }
     __syncthreads();
-    fft_line<T, true, false, true, LC>(
+    fft_line<T, true, false, true, LC, SK>(
         p, twl, lineG, mg, line, ms, [](int) { return cmk<T>((T)0, (T)0); },
         [&](int y, C2<T> v) {
             if (y < ny && kxg < Hx) colp[(size_t)y * Hp] = v;
@@ -467,20 +474,21 @@ template <typename T, typename TOUT, int LC = 0>
 __global__ void k_xinv(const C2<T> *__restrict__ in, TOUT *__restrict__ out, size_t out_cstride, int Hx,
                        int Hp, int ny, int nz, int nrows, int nx, FftPlanDev p, int lpw)
 {
+    constexpr int SK = MM_SKEW_X;
     extern __shared__ char smem_raw[];
     C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
     const int L = PlanC<LC>::L(p);
     const int l = threadIdx.x / PlanC<LC>::tline(p), t = threadIdx.x % PlanC<LC>::tline(p);
     const int g = blockIdx.x * lpw + l;
     const bool active = g < nrows;
-    C2<T> *line = sm + (size_t)l * lds_len(L);
-    C2<T> *twl = sm + (size_t)lpw * lds_len(L);
+    C2<T> *line = sm + (size_t)l * lds_len<SK>(L);
+    C2<T> *twl = sm + (size_t)lpw * lds_len<SK>(L);
     load_twiddles<T, LC>(twl, p);
     const int c = active ? g / (ny * nz) : 0, r = active ? g % (ny * nz) : 0;
     const C2<T> *src = in + (size_t)(active ? g : 0) * Hp;
     TOUT *dst = out + c * out_cstride + (size_t)r * nx;
     const LineMap m = {t, active};
-    fft_line<T, true, true, true, LC>(
+    fft_line<T, true, true, true, LC, SK>(
         p, twl, line, m, line, m, [&](int k) { return k < Hx ? src[k] : cconj<T>(src[L - k]); },
         [&](int x, C2<T> v) {
             if (x < nx) dst[x] = (TOUT)v.x;
@@ -490,9 +498,10 @@ __global__ void k_xinv(const C2<T> *__restrict__ in, TOUT *__restrict__ out, siz
 // setup: forward z over full lines (no multiply), all Pz outputs stored
 __global__ void k_zfwd_setup(double2 *__restrict__ S, int Hx, int Hp, int Py, FftPlanDev p, int tk)
 {
+    constexpr int SK = MM_SKEW_Z;
     extern __shared__ char smem_raw[];
     double2 *sm = reinterpret_cast<double2 *>(smem_raw);
-    const int L = p.L, LP = lds_len(L);
+    const int L = p.L, LP = lds_len<SK>(L);
     const int kx0 = blockIdx.x * tk, ky = blockIdx.y;
     const size_t zstride = (size_t)Py * Hp;
     double2 *base = S + (size_t)ky * Hp;
@@ -500,14 +509,14 @@ __global__ void k_zfwd_setup(double2 *__restrict__ S, int Hx, int Hp, int Py, Ff
     load_twiddles<double>(twl, p);
     for (int e = threadIdx.x; e < tk * L; e += blockDim.x) {
         const int col = e % tk, z = e / tk, kx = kx0 + col;
-        sm[col * LP + sk(z)] = kx < Hx ? base[z * zstride + kx] : cmk<double>(0.0, 0.0);
+        sm[col * LP + sk<SK>(z)] = kx < Hx ? base[z * zstride + kx] : cmk<double>(0.0, 0.0);
     }
     __syncthreads();
     const int l = threadIdx.x / p.tline, t = threadIdx.x % p.tline;
-    lds_fft<double, false>(sm + (size_t)l * LP, p, t, l < tk, twl);
+    lds_fft<double, false, SK>(sm + (size_t)l * LP, p, t, l < tk, twl);
     for (int e = threadIdx.x; e < tk * L; e += blockDim.x) {
         const int col = e % tk, z = e / tk, kx = kx0 + col;
-        if (kx < Hx) base[z * zstride + kx] = sm[col * LP + sk(z)];
+        if (kx < Hx) base[z * zstride + kx] = sm[col * LP + sk<SK>(z)];
     }
 }
 
@@ -546,9 +555,9 @@ static int env_int(const char *name, int dflt)
 // that global loads of one overlap the FFT of another.  MM_LPW_X, MM_TK_Y and
 // MM_TK_Z override the choice (launch-shape sweeps).
 // LDS bytes of `nlines` transform lines of length L plus the plan's twiddle table
-static size_t lines_lds(int nlines, int L, size_t csize)
+template <int SK> static size_t lines_lds(int nlines, int L, size_t csize)
 {
-    return ((size_t)nlines * lds_len(L) + (size_t)L) * csize;
+    return ((size_t)nlines * lds_len<SK>(L) + (size_t)L) * csize;
 }
 
 static ConvLaunch choose_launch(const ConvGeom &g, const FftPlanHost &px, const FftPlanHost &py,
@@ -557,15 +566,15 @@ static ConvLaunch choose_launch(const ConvGeom &g, const FftPlanHost &px, const 
     const int lds = mm_max_lds_bytes();
     ConvLaunch ln;
     ln.lpw_x = env_int("MM_LPW_X", std::max(1, 256 / px.tline));
-    while (ln.lpw_x > 1 && (lines_lds(ln.lpw_x, px.L, csize) > (size_t)lds || ln.lpw_x * px.tline > 1024)) --ln.lpw_x;
+    while (ln.lpw_x > 1 && (lines_lds<MM_SKEW_X>(ln.lpw_x, px.L, csize) > (size_t)lds || ln.lpw_x * px.tline > 1024)) --ln.lpw_x;
     const int nl_y = g.is2d ? 3 : 1;
     int tk = env_int("MM_TK_Y", 8);
-    while (tk > 1 && ((size_t)nl_y * tk * py.tline > 1024 || lines_lds(nl_y * tk, py.L, csize) > (size_t)lds)) tk /= 2;
+    while (tk > 1 && ((size_t)nl_y * tk * py.tline > 1024 || lines_lds<MM_SKEW_Y>(nl_y * tk, py.L, csize) > (size_t)lds)) tk /= 2;
     ln.tk_y = tk;
-    MM_REQUIRE((size_t)nl_y * tk * py.tline <= 1024 && lines_lds(nl_y * tk, py.L, csize) <= (size_t)lds,
+    MM_REQUIRE((size_t)nl_y * tk * py.tline <= 1024 && lines_lds<MM_SKEW_Y>(nl_y * tk, py.L, csize) <= (size_t)lds,
                "y transform length too large for one workgroup");
     int tz = env_int("MM_TK_Z", 16);
-    while (tz > 1 && ((size_t)3 * tz * pz.tline > 1024 || lines_lds(3 * tz, pz.L, csize) > (size_t)lds)) tz /= 2;
+    while (tz > 1 && ((size_t)3 * tz * pz.tline > 1024 || lines_lds<MM_SKEW_Z>(3 * tz, pz.L, csize) > (size_t)lds)) tz /= 2;
     ln.tk_z = tz;
     return ln;
 }
@@ -609,7 +618,7 @@ void ConvImpl<T>::init(const double *N_host, int nx, int ny, int nz, int ex, int
             const int lds = mm_max_lds_bytes();
             const size_t cs = sizeof(C2<T>);
             int tk = env_int("MM_TK_YP", (int)(128 / cs));
-            while (tk > 1 && ((size_t)tk * pyh.tline > 1024 || lines_lds(tk, pyh.L, cs) > (size_t)lds)) tk /= 2;
+            while (tk > 1 && ((size_t)tk * pyh.tline > 1024 || lines_lds<MM_SKEW_Y>(tk, pyh.L, cs) > (size_t)lds)) tk /= 2;
             tk_yp = tk;
         }
     }
@@ -654,13 +663,13 @@ void ConvImpl<T>::init(const double *N_host, int nx, int ny, int nz, int ex, int
     const int lds = mm_max_lds_bytes();
     const FftPlanDev dx = px.dev<double>(), dy = py.dev<double>(), dz = pz.dev<double>();
     int lpw = std::max(1, 256 / px.tline);
-    while (lpw > 1 && lines_lds(lpw, px.L, 16) > (size_t)lds) --lpw;
+    while (lpw > 1 && lines_lds<MM_SKEW_X>(lpw, px.L, 16) > (size_t)lds) --lpw;
     int tky = 8;
-    while (tky > 1 && ((size_t)tky * py.tline > 1024 || lines_lds(tky, py.L, 16) > (size_t)lds)) tky /= 2;
+    while (tky > 1 && ((size_t)tky * py.tline > 1024 || lines_lds<MM_SKEW_Y>(tky, py.L, 16) > (size_t)lds)) tky /= 2;
     int tkz = 8;
-    while (tkz > 1 && ((size_t)tkz * pz.tline > 1024 || lines_lds(tkz, pz.L, 16) > (size_t)lds)) tkz /= 2;
-    const size_t shx = lines_lds(lpw, px.L, 16), shy = lines_lds(tky, py.L, 16),
-                 shz = lines_lds(tkz, pz.L, 16);
+    while (tkz > 1 && ((size_t)tkz * pz.tline > 1024 || lines_lds<MM_SKEW_Z>(tkz, pz.L, 16) > (size_t)lds)) tkz /= 2;
+    const size_t shx = lines_lds<MM_SKEW_X>(lpw, px.L, 16), shy = lines_lds<MM_SKEW_Y>(tky, py.L, 16),
+                 shz = lines_lds<MM_SKEW_Z>(tkz, pz.L, 16);
     set_lds(k_xfwd<double, double>, shx);
     set_lds(k_yfwd<double>, shy);
     set_lds(k_zfwd_setup, shz);
@@ -693,14 +702,14 @@ void ConvImpl<T>::init(const double *N_host, int nx, int ny, int nz, int ex, int
     if (!g.is2d) MM_HIP(hipMalloc(&S2, sizeof(C2<T>) * 3 * (size_t)nz * Py * g.Hp));
 
     const size_t cs = sizeof(C2<T>);
-    set_lds(k_xfwd<T, T>, lines_lds(ln.lpw_x, px.L, cs));
-    set_lds(k_xinv<T, T>, lines_lds(ln.lpw_x, px.L, cs));
+    set_lds(k_xfwd<T, T>, lines_lds<MM_SKEW_X>(ln.lpw_x, px.L, cs));
+    set_lds(k_xinv<T, T>, lines_lds<MM_SKEW_X>(ln.lpw_x, px.L, cs));
     if (g.is2d) {
-        set_lds(k_ymi2d<T>, lines_lds(3 * ln.tk_y, py.L, cs));
+        set_lds(k_ymi2d<T>, lines_lds<MM_SKEW_Y>(3 * ln.tk_y, py.L, cs));
     } else {
-        set_lds(k_yfwd<T>, lines_lds(ln.tk_y, py.L, cs));
-        set_lds(k_yinv<T>, lines_lds(ln.tk_y, py.L, cs));
-        set_lds(k_zmi<T>, lines_lds(3 * ln.tk_z, pz.L, cs));
+        set_lds(k_yfwd<T>, lines_lds<MM_SKEW_Y>(ln.tk_y, py.L, cs));
+        set_lds(k_yinv<T>, lines_lds<MM_SKEW_Y>(ln.tk_y, py.L, cs));
+        set_lds(k_zmi<T>, lines_lds<MM_SKEW_Z>(3 * ln.tk_z, pz.L, cs));
     }
 }
 
@@ -739,7 +748,7 @@ template <typename T> void ConvImpl<T>::forward_x(const T *M, hipStream_t s, con
 {
     const size_t n = (size_t)g.nx * g.ny * g.nz;
     const int nrows = 3 * g.ny * g.nz;
-    const size_t sh = lines_lds(ln.lpw_x, px.L, sizeof(C2<T>));
+    const size_t sh = lines_lds<MM_SKEW_X>(ln.lpw_x, px.L, sizeof(C2<T>));
     mm_with_len(px.L, fft_generic, [&](auto lc) {
         hipLaunchKernelGGL((k_xfwd<T, T, decltype(lc)::value>), dim3((nrows + ln.lpw_x - 1) / ln.lpw_x),
                            dim3(ln.lpw_x * px.tline), sh, s, M, n, S1, (size_t)g.ny * g.nz * g.Hp, g.Hx, g.Hp, g.ny,
@@ -755,7 +764,7 @@ template <typename T> void ConvImpl<T>::spectral(hipStream_t s)
     const size_t plane = (size_t)g.Hp * g.Hy2 * g.Hz2;
     if (g.is2d) {
         const int tk = ln.tk_y;
-        const size_t sh = lines_lds(3 * tk, py.L, cs);
+        const size_t sh = lines_lds<MM_SKEW_Y>(3 * tk, py.L, cs);
         mm_with_len(py.L, fft_generic, [&](auto lc) {
             hipLaunchKernelGGL((k_ymi2d<T, decltype(lc)::value>), dim3((g.Hx + tk - 1) / tk), dim3(3 * tk * py.tline),
                                sh, s, S1, kern, plane, g.Hx, g.Hp, g.Hy2, g.ny, py.dev<T>(), tk);
@@ -765,9 +774,9 @@ template <typename T> void ConvImpl<T>::spectral(hipStream_t s)
         return;
     }
     const int tk = ln.tk_y;
-    const size_t shy = lines_lds(tk, py.L, cs);
+    const size_t shy = lines_lds<MM_SKEW_Y>(tk, py.L, cs);
     const int tkp = tk_yp;
-    const size_t shyp = lines_lds(tkp, pyh.L, cs);
+    const size_t shyp = lines_lds<MM_SKEW_Y>(tkp, pyh.L, cs);
     if (prune_y) {
         hipLaunchKernelGGL((k_yfwd_pr<T>), dim3((g.Hx + tkp - 1) / tkp, g.nz, 3), dim3(tkp * pyh.tline), shyp, s, S1, S2,
                            g.Hx, g.Hp, g.ny, g.nz, pyh.dev<T>(), (const C2<T> *)py.dev<T>().tw, tkp);
@@ -781,7 +790,7 @@ template <typename T> void ConvImpl<T>::spectral(hipStream_t s)
     MM_HIP(hipGetLastError());
     if (klog) klog->mark(s, KC_YFWD, conv_bytes<T>(g, KC_YFWD));
     const int tz = ln.tk_z;
-    const size_t shz = lines_lds(3 * tz, pz.L, cs);
+    const size_t shz = lines_lds<MM_SKEW_Z>(3 * tz, pz.L, cs);
     MM_REQUIRE((size_t)tz * (pz.L / 2 + 1) <= (size_t)ZM_ITEMS * 3 * tz * pz.tline, "z pass items per thread");
     mm_with_len(pz.L, fft_generic, [&](auto lc) {
         hipLaunchKernelGGL((k_zmi<T, decltype(lc)::value>), dim3((g.Hx + tz - 1) / tz, g.Py), dim3(3 * tz * pz.tline),
@@ -807,7 +816,7 @@ template <typename T> void ConvImpl<T>::inverse_x(T *H, hipStream_t s)
 {
     const size_t n = (size_t)g.nx * g.ny * g.nz;
     const int nrows = 3 * g.ny * g.nz;
-    const size_t sh = lines_lds(ln.lpw_x, px.L, sizeof(C2<T>));
+    const size_t sh = lines_lds<MM_SKEW_X>(ln.lpw_x, px.L, sizeof(C2<T>));
     mm_with_len(px.L, fft_generic, [&](auto lc) {
         hipLaunchKernelGGL((k_xinv<T, T, decltype(lc)::value>), dim3((nrows + ln.lpw_x - 1) / ln.lpw_x),
                            dim3(ln.lpw_x * px.tline), sh, s, S1, H, n, g.Hx, g.Hp, g.ny, g.nz, nrows, g.nx,

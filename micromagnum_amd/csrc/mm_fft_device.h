@@ -41,17 +41,19 @@ template <typename T, bool INV> __device__ __forceinline__ C2<T> cmul_mi(C2<T> a
     return INV ? cmk<T>(-a.y, a.x) : cmk<T>(a.y, -a.x);
 }
 
-// LDS layout of a line: with MM_LDS_SKEW = s > 0 element j lives at
-// j + (j >> s) (one pad slot per 2^s elements, spreading strided Stockham
-// writes over the LDS banks); a line of length L occupies lds_len(L) slots.
-// Measured on MI355X (256x256x16 cells, RKF45) the skew costs more in index
-// arithmetic than it saves in bank conflicts (544 vs 599 steps/s FP32), so
-// it is off by default.
-#ifndef MM_LDS_SKEW
-#define MM_LDS_SKEW 0
-#endif
-__host__ __device__ __forceinline__ int sk(int j) { return MM_LDS_SKEW ? j + (j >> MM_LDS_SKEW) : j; }
-__host__ __device__ __forceinline__ int lds_len(int L) { return (MM_LDS_SKEW ? L + (L >> MM_LDS_SKEW) : L) + 1; }
+// LDS layout of a line: with skew SK > 0 element j lives at j + (j >> SK)
+// (one pad slot per 2^SK elements), a line of length L occupies
+// lds_len<SK>(L) slots.  The first radix-8 passes write at strides 8 and 64
+// elements, which without the pad puts many lanes of a wave on one LDS bank.
+// Measured on MI355X (256x256x16 cells): SK = 3 takes the FP32 z pass from
+// 41.7 to 35.0 us and the x transforms from 20.6 to 16.2 us; the y passes,
+// whose column tiles already spread over the banks and whose LDS footprint
+// sets their occupancy, are faster without it.
+#define MM_SKEW_X 3  // x lines: x transforms and the stage kernel
+#define MM_SKEW_Y 0  // y column tiles
+#define MM_SKEW_Z 3  // z column tiles
+template <int SK> __host__ __device__ __forceinline__ int sk(int j) { return SK ? j + (j >> SK) : j; }
+template <int SK> __host__ __device__ __forceinline__ int lds_len(int L) { return (SK ? L + (L >> SK) : L) + 1; }
 
 // Workgroups of a launch are dispatched round-robin over gfx950's 8 XCDs,
 // each with its own L2: hardware workgroup b runs on XCD b % 8.  xcd_tile(b,
@@ -191,7 +193,7 @@ template <int R, int EPT> struct MaxB { static const int value = (EPT + R - 1) /
 // threads with no line; they still take part in both barriers.  Ns is the
 // product of the radices of the previous passes; P2: every length involved
 // is a power of two and lNs = log2(Ns), so index arithmetic is shifts/masks.
-template <int R, int EPT, typename T, bool INV, bool P2>
+template <int R, int EPT, typename T, bool INV, bool P2, int SK>
 __device__ __forceinline__ void fft_pass(C2<T> *line, const C2<T> *__restrict__ tw, int L, int tline, int t,
                                          bool active, int Ns, int lNs)
 {
@@ -203,7 +205,7 @@ __device__ __forceinline__ void fft_pass(C2<T> *line, const C2<T> *__restrict__ 
         const int b = t + q * tline;
         if (active && b < nb) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[q][r] = line[sk(b + r * nb)];
+            for (int r = 0; r < R; ++r) v[q][r] = line[sk<SK>(b + r * nb)];
         }
     }
     __syncthreads();
@@ -224,7 +226,7 @@ __device__ __forceinline__ void fft_pass(C2<T> *line, const C2<T> *__restrict__ 
             dftR<R, T, INV>(v[q]);
             const int base = P2 ? (((b >> lNs) << (lNs + Log2R<R>::value)) + k) : (b / Ns) * Ns * R + k;
 #pragma unroll
-            for (int r = 0; r < R; ++r) line[sk(base + r * Ns)] = v[q][r];
+            for (int r = 0; r < R; ++r) line[sk<SK>(base + r * Ns)] = v[q][r];
         }
     }
     __syncthreads();
@@ -234,7 +236,7 @@ __device__ __forceinline__ void fft_pass(C2<T> *line, const C2<T> *__restrict__ 
 // must call this with the same plan (barriers inside).  EPT is the number of
 // elements per thread the plan was built for; tw: the plan's twiddle table in
 // LDS (load_twiddles).
-template <typename T, bool INV, int EPT, bool P2>
+template <typename T, bool INV, int EPT, bool P2, int SK>
 __device__ __forceinline__ void lds_fft_ept(C2<T> *line, const FftPlanDev &p, const C2<T> *tw, int t, bool active)
 {
     int Ns = 1, lNs = 0;
@@ -243,17 +245,17 @@ __device__ __forceinline__ void lds_fft_ept(C2<T> *line, const FftPlanDev &p, co
         const int R = (int)(rs & 15u);
         if (P2) {
             switch (R) {
-            case 8: fft_pass<8, EPT, T, INV, true>(line, tw, p.L, p.tline, t, active, Ns, lNs); lNs += 3; break;
-            case 4: fft_pass<4, EPT, T, INV, true>(line, tw, p.L, p.tline, t, active, Ns, lNs); lNs += 2; break;
-            default: fft_pass<2, EPT, T, INV, true>(line, tw, p.L, p.tline, t, active, Ns, lNs); lNs += 1; break;
+            case 8: fft_pass<8, EPT, T, INV, true, SK>(line, tw, p.L, p.tline, t, active, Ns, lNs); lNs += 3; break;
+            case 4: fft_pass<4, EPT, T, INV, true, SK>(line, tw, p.L, p.tline, t, active, Ns, lNs); lNs += 2; break;
+            default: fft_pass<2, EPT, T, INV, true, SK>(line, tw, p.L, p.tline, t, active, Ns, lNs); lNs += 1; break;
             }
         } else {
             switch (R) {
-            case 8: fft_pass<8, EPT, T, INV, false>(line, tw, p.L, p.tline, t, active, Ns, 0); break;
-            case 5: fft_pass<5, EPT, T, INV, false>(line, tw, p.L, p.tline, t, active, Ns, 0); break;
-            case 4: fft_pass<4, EPT, T, INV, false>(line, tw, p.L, p.tline, t, active, Ns, 0); break;
-            case 3: fft_pass<3, EPT, T, INV, false>(line, tw, p.L, p.tline, t, active, Ns, 0); break;
-            default: fft_pass<2, EPT, T, INV, false>(line, tw, p.L, p.tline, t, active, Ns, 0); break;
+            case 8: fft_pass<8, EPT, T, INV, false, SK>(line, tw, p.L, p.tline, t, active, Ns, 0); break;
+            case 5: fft_pass<5, EPT, T, INV, false, SK>(line, tw, p.L, p.tline, t, active, Ns, 0); break;
+            case 4: fft_pass<4, EPT, T, INV, false, SK>(line, tw, p.L, p.tline, t, active, Ns, 0); break;
+            case 3: fft_pass<3, EPT, T, INV, false, SK>(line, tw, p.L, p.tline, t, active, Ns, 0); break;
+            default: fft_pass<2, EPT, T, INV, false, SK>(line, tw, p.L, p.tline, t, active, Ns, 0); break;
             }
         }
         Ns *= R;
@@ -262,11 +264,11 @@ __device__ __forceinline__ void lds_fft_ept(C2<T> *line, const FftPlanDev &p, co
 
 #define MM_FFT_EPT 8
 
-template <typename T, bool INV>
+template <typename T, bool INV, int SK>
 __device__ __forceinline__ void lds_fft(C2<T> *line, const FftPlanDev &p, int t, bool active, const C2<T> *tw)
 {
-    if (p.pow2) lds_fft_ept<T, INV, MM_FFT_EPT, true>(line, p, tw, t, active);
-    else lds_fft_ept<T, INV, MM_FFT_EPT, false>(line, p, tw, t, active);
+    if (p.pow2) lds_fft_ept<T, INV, MM_FFT_EPT, true, SK>(line, p, tw, t, active);
+    else lds_fft_ept<T, INV, MM_FFT_EPT, false, SK>(line, p, tw, t, active);
 }
 
 // ---- transforms with the first pass reading and/or the last pass writing
@@ -282,7 +284,7 @@ struct LineMap {
     bool active;
 };
 
-template <int R, int EPT, typename T, bool INV, bool P2, bool LDIN, bool STOUT, typename LD, typename ST>
+template <int R, int EPT, typename T, bool INV, bool P2, bool LDIN, bool STOUT, int SK, typename LD, typename ST>
 __device__ __forceinline__ void fft_pass_io(C2<T> *lineG, LineMap g, C2<T> *lineS, LineMap sm_, const C2<T> *tw,
                                             int L, int tline, int Ns, int lNs, LD &ld, ST &st)
 {
@@ -300,7 +302,7 @@ __device__ __forceinline__ void fft_pass_io(C2<T> *lineG, LineMap g, C2<T> *line
         const int b = in.t + q * tline;
         if (in.active && b < nb) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[q][r] = LDIN ? ld(b + r * nb) : lin[sk(b + r * nb)];
+            for (int r = 0; r < R; ++r) v[q][r] = LDIN ? ld(b + r * nb) : lin[sk<SK>(b + r * nb)];
         }
     }
     if (!LDIN && !STOUT) __syncthreads();  // in place in LDS: every read before any write
@@ -324,32 +326,32 @@ __device__ __forceinline__ void fft_pass_io(C2<T> *lineG, LineMap g, C2<T> *line
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 if (STOUT) st(base + r * Ns, v[q][r]);
-                else lin[sk(base + r * Ns)] = v[q][r];
+                else lin[sk<SK>(base + r * Ns)] = v[q][r];
             }
         }
     }
     if (!STOUT) __syncthreads();
 }
 
-template <typename T, bool INV, int EPT, bool P2, bool LDIN, bool STOUT, bool FIRST, bool LAST, typename LD,
-          typename ST>
+template <typename T, bool INV, int EPT, bool P2, bool LDIN, bool STOUT, bool FIRST, bool LAST, int SK,
+          typename LD, typename ST>
 __device__ __forceinline__ void fft_pass_sel(int R, C2<T> *lineG, LineMap g, C2<T> *lineS, LineMap s_,
                                              const C2<T> *tw, int L, int tline, int Ns, int lNs, LD &ld, ST &st)
 {
     constexpr bool I = LDIN && FIRST, O = STOUT && LAST;
     if (P2) {
         switch (R) {
-        case 8: fft_pass_io<8, EPT, T, INV, true, I, O>(lineG, g, lineS, s_, tw, L, tline, Ns, lNs, ld, st); break;
-        case 4: fft_pass_io<4, EPT, T, INV, true, I, O>(lineG, g, lineS, s_, tw, L, tline, Ns, lNs, ld, st); break;
-        default: fft_pass_io<2, EPT, T, INV, true, I, O>(lineG, g, lineS, s_, tw, L, tline, Ns, lNs, ld, st); break;
+        case 8: fft_pass_io<8, EPT, T, INV, true, I, O, SK>(lineG, g, lineS, s_, tw, L, tline, Ns, lNs, ld, st); break;
+        case 4: fft_pass_io<4, EPT, T, INV, true, I, O, SK>(lineG, g, lineS, s_, tw, L, tline, Ns, lNs, ld, st); break;
+        default: fft_pass_io<2, EPT, T, INV, true, I, O, SK>(lineG, g, lineS, s_, tw, L, tline, Ns, lNs, ld, st); break;
         }
     } else {
         switch (R) {
-        case 8: fft_pass_io<8, EPT, T, INV, false, I, O>(lineG, g, lineS, s_, tw, L, tline, Ns, 0, ld, st); break;
-        case 5: fft_pass_io<5, EPT, T, INV, false, I, O>(lineG, g, lineS, s_, tw, L, tline, Ns, 0, ld, st); break;
-        case 4: fft_pass_io<4, EPT, T, INV, false, I, O>(lineG, g, lineS, s_, tw, L, tline, Ns, 0, ld, st); break;
-        case 3: fft_pass_io<3, EPT, T, INV, false, I, O>(lineG, g, lineS, s_, tw, L, tline, Ns, 0, ld, st); break;
-        default: fft_pass_io<2, EPT, T, INV, false, I, O>(lineG, g, lineS, s_, tw, L, tline, Ns, 0, ld, st); break;
+        case 8: fft_pass_io<8, EPT, T, INV, false, I, O, SK>(lineG, g, lineS, s_, tw, L, tline, Ns, 0, ld, st); break;
+        case 5: fft_pass_io<5, EPT, T, INV, false, I, O, SK>(lineG, g, lineS, s_, tw, L, tline, Ns, 0, ld, st); break;
+        case 4: fft_pass_io<4, EPT, T, INV, false, I, O, SK>(lineG, g, lineS, s_, tw, L, tline, Ns, 0, ld, st); break;
+        case 3: fft_pass_io<3, EPT, T, INV, false, I, O, SK>(lineG, g, lineS, s_, tw, L, tline, Ns, 0, ld, st); break;
+        default: fft_pass_io<2, EPT, T, INV, false, I, O, SK>(lineG, g, lineS, s_, tw, L, tline, Ns, 0, ld, st); break;
         }
     }
 }
@@ -359,7 +361,7 @@ __device__ __forceinline__ void fft_pass_sel(int R, C2<T> *lineG, LineMap g, C2<
 // them); STOUT: outputs via st(j, v) (else they end in the LDS line, followed
 // by a barrier).  lineG/g: line and mapping of the passes that use ld/st,
 // lineS/s: of the LDS passes (the same line in both mappings).
-template <typename T, bool INV, bool P2, bool LDIN, bool STOUT, typename LD, typename ST>
+template <typename T, bool INV, bool P2, bool LDIN, bool STOUT, int SK, typename LD, typename ST>
 __device__ __forceinline__ void fft_line_p2(const FftPlanDev &p, const C2<T> *tw, C2<T> *lineG, LineMap g,
                                             C2<T> *lineS, LineMap s_, LD &ld, ST &st)
 {
@@ -367,9 +369,9 @@ __device__ __forceinline__ void fft_line_p2(const FftPlanDev &p, const C2<T> *tw
     const int np = p.npass;
     if (np == 0) {
         if (g.active && g.t == 0) {
-            const C2<T> v = LDIN ? ld(0) : lineG[sk(0)];
+            const C2<T> v = LDIN ? ld(0) : lineG[sk<SK>(0)];
             if (STOUT) st(0, v);
-            else lineG[sk(0)] = v;
+            else lineG[sk<SK>(0)] = v;
         }
         if (!STOUT) __syncthreads();
         return;
@@ -385,13 +387,13 @@ __device__ __forceinline__ void fft_line_p2(const FftPlanDev &p, const C2<T> *tw
         int Lv = p.L, tlv = p.tline;
         asm volatile("" : "+s"(Lv), "+s"(tlv));
         if (ps == 0 && ps == np - 1)
-            fft_pass_sel<T, INV, EPT, P2, LDIN, STOUT, true, true>(R, lineG, g, lineS, s_, tw, Lv, tlv, Ns, lNs, ld, st);
+            fft_pass_sel<T, INV, EPT, P2, LDIN, STOUT, true, true, SK>(R, lineG, g, lineS, s_, tw, Lv, tlv, Ns, lNs, ld, st);
         else if (ps == 0)
-            fft_pass_sel<T, INV, EPT, P2, LDIN, STOUT, true, false>(R, lineG, g, lineS, s_, tw, Lv, tlv, Ns, lNs, ld, st);
+            fft_pass_sel<T, INV, EPT, P2, LDIN, STOUT, true, false, SK>(R, lineG, g, lineS, s_, tw, Lv, tlv, Ns, lNs, ld, st);
         else if (ps == np - 1)
-            fft_pass_sel<T, INV, EPT, P2, LDIN, STOUT, false, true>(R, lineG, g, lineS, s_, tw, Lv, tlv, Ns, lNs, ld, st);
+            fft_pass_sel<T, INV, EPT, P2, LDIN, STOUT, false, true, SK>(R, lineG, g, lineS, s_, tw, Lv, tlv, Ns, lNs, ld, st);
         else
-            fft_pass_sel<T, INV, EPT, P2, LDIN, STOUT, false, false>(R, lineG, g, lineS, s_, tw, Lv, tlv, Ns, lNs, ld, st);
+            fft_pass_sel<T, INV, EPT, P2, LDIN, STOUT, false, false, SK>(R, lineG, g, lineS, s_, tw, Lv, tlv, Ns, lNs, ld, st);
         Ns *= R;
         lNs += R == 8 ? 3 : (R == 4 ? 2 : 1);
     }
@@ -425,7 +427,7 @@ template <> struct PlanC<0> {
     __device__ static int tline(const FftPlanDev &p) { return p.tline; }
 };
 
-template <typename T, bool INV, int LC, int PS, int NS, bool LDIN, bool STOUT, typename LD, typename ST>
+template <typename T, bool INV, int LC, int PS, int NS, bool LDIN, bool STOUT, int SK, typename LD, typename ST>
 __device__ __forceinline__ void fft_line_c(const C2<T> *tw, C2<T> *lineG, LineMap g, C2<T> *lineS, LineMap s_,
                                            LD &ld, ST &st)
 {
@@ -433,23 +435,23 @@ __device__ __forceinline__ void fft_line_c(const C2<T> *tw, C2<T> *lineG, LineMa
     if constexpr (PS < F::npass) {
         constexpr int R = F::radix(PS);
         constexpr bool I = LDIN && PS == 0, O = STOUT && PS == F::npass - 1;
-        fft_pass_io<R, MM_FFT_EPT, T, INV, true, I, O>(lineG, g, lineS, s_, tw, LC, F::tline, NS, mm_ilog2(NS), ld,
-                                                       st);
-        fft_line_c<T, INV, LC, PS + 1, NS * R, LDIN, STOUT>(tw, lineG, g, lineS, s_, ld, st);
+        fft_pass_io<R, MM_FFT_EPT, T, INV, true, I, O, SK>(lineG, g, lineS, s_, tw, LC, F::tline, NS, mm_ilog2(NS),
+                                                           ld, st);
+        fft_line_c<T, INV, LC, PS + 1, NS * R, LDIN, STOUT, SK>(tw, lineG, g, lineS, s_, ld, st);
     }
 }
 
 // LC > 0: the compile-time plan of that length (p.L must equal LC); LC == 0:
-// the runtime plan p.
-template <typename T, bool INV, bool LDIN, bool STOUT, int LC = 0, typename LD, typename ST>
+// the runtime plan p.  SK: the LDS skew of the lines (sk).
+template <typename T, bool INV, bool LDIN, bool STOUT, int LC, int SK, typename LD, typename ST>
 __device__ __forceinline__ void fft_line(const FftPlanDev &p, const C2<T> *tw, C2<T> *lineG, LineMap g, C2<T> *lineS,
                                          LineMap s_, LD ld, ST st)
 {
     if constexpr (LC > 0) {
-        fft_line_c<T, INV, LC, 0, 1, LDIN, STOUT>(tw, lineG, g, lineS, s_, ld, st);
+        fft_line_c<T, INV, LC, 0, 1, LDIN, STOUT, SK>(tw, lineG, g, lineS, s_, ld, st);
     } else {
-        if (p.pow2) fft_line_p2<T, INV, true, LDIN, STOUT>(p, tw, lineG, g, lineS, s_, ld, st);
-        else fft_line_p2<T, INV, false, LDIN, STOUT>(p, tw, lineG, g, lineS, s_, ld, st);
+        if (p.pow2) fft_line_p2<T, INV, true, LDIN, STOUT, SK>(p, tw, lineG, g, lineS, s_, ld, st);
+        else fft_line_p2<T, INV, false, LDIN, STOUT, SK>(p, tw, lineG, g, lineS, s_, ld, st);
     }
 }
 

@@ -378,6 +378,7 @@ __global__ void k_stage(SysDev<T> S, StageDev<T> G, StepCtl *ctl, const C2<T> *_
                         int Hx, int Hp, int rpw, int has_demag, double *__restrict__ epart,
                         const T *__restrict__ Hdem)
 {
+    constexpr int SK = MM_SKEW_X;
     if (!EN && ctl->stop) return;
     extern __shared__ char smem_raw[];
     C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
@@ -385,7 +386,7 @@ __global__ void k_stage(SysDev<T> S, StageDev<T> G, StepCtl *ctl, const C2<T> *_
     const int nrows = ny * nz;
     const int row0 = (S.xcd_remap ? xcd_tile((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x) * rpw;
     const T *Y = G.Ys ? G.Ys : (ctl->cur ? G.yb1 : G.yb0);
-    const int L = PlanC<LC>::L(p), LP = lds_len(L);
+    const int L = PlanC<LC>::L(p), LP = lds_len<SK>(L);
     C2<T> *twl = sm + (size_t)3 * rpw * LP;
     if (has_demag == 1) {
         load_twiddles<T, LC>(twl, p);
@@ -396,7 +397,7 @@ __global__ void k_stage(SysDev<T> S, StageDev<T> G, StepCtl *ctl, const C2<T> *_
         C2<T> *line = sm + (size_t)l * LP;
         const C2<T> *src = S1 + ((size_t)c * nrows + (active ? gr : 0)) * Hp;
         const LineMap m = {t, active};
-        fft_line<T, true, true, false, LC>(
+        fft_line<T, true, true, false, LC, SK>(
             p, twl, line, m, line, m, [&](int k) { return k < Hx ? src[k] : cconj<T>(src[L - k]); },
             [](int, C2<T>) {});
     }
@@ -409,7 +410,7 @@ __global__ void k_stage(SysDev<T> S, StageDev<T> G, StepCtl *ctl, const C2<T> *_
         if (gr >= nrows) continue;
         V3<T> Hd = v3<T>(0, 0, 0);
         if (has_demag == 1) {
-            Hd = v3<T>(sm[(0 * rpw + r) * LP + sk(x)].x, sm[(1 * rpw + r) * LP + sk(x)].x, sm[(2 * rpw + r) * LP + sk(x)].x);
+            Hd = v3<T>(sm[(0 * rpw + r) * LP + sk<SK>(x)].x, sm[(1 * rpw + r) * LP + sk<SK>(x)].x, sm[(2 * rpw + r) * LP + sk<SK>(x)].x);
         } else if (has_demag == 2) {
             const size_t i = (size_t)gr * nx + x;
             Hd = v3<T>(Hdem[i], Hdem[S.n + i], Hdem[2 * S.n + i]);
@@ -419,12 +420,12 @@ __global__ void k_stage(SysDev<T> S, StageDev<T> G, StepCtl *ctl, const C2<T> *_
         // the slot (r, x) of the three lines is this thread's alone: the next
         // stage input replaces H_demag in place
         if (fused)
-            for (int c = 0; c < 3; ++c) sm[(c * rpw + r) * LP + sk(x)] = cmk<T>(nxt[c], (T)0);
+            for (int c = 0; c < 3; ++c) sm[(c * rpw + r) * LP + sk<SK>(x)] = cmk<T>(nxt[c], (T)0);
     }
     if (fused) {
         for (int e = threadIdx.x; e < 3 * rpw * (L - nx); e += blockDim.x) {
             const int l = e / (L - nx), x = nx + e % (L - nx);
-            sm[(size_t)l * LP + sk(x)] = cmk<T>((T)0, (T)0);
+            sm[(size_t)l * LP + sk<SK>(x)] = cmk<T>((T)0, (T)0);
         }
         __syncthreads();
         const int l = threadIdx.x / PlanC<LC>::tline(p), t = threadIdx.x % PlanC<LC>::tline(p);
@@ -434,7 +435,7 @@ __global__ void k_stage(SysDev<T> S, StageDev<T> G, StepCtl *ctl, const C2<T> *_
         C2<T> *line = sm + (size_t)l * LP;
         C2<T> *dst = G.fwd_S1 + ((size_t)c * nrows + (active ? gr : 0)) * Hp;
         const LineMap m = {t, active};
-        fft_line<T, false, false, true, LC>(
+        fft_line<T, false, false, true, LC, SK>(
             p, twl, line, m, line, m, [](int) { return cmk<T>((T)0, (T)0); },
             [&](int k, C2<T> v) {
                 if (k < Hx) dst[k] = v;
@@ -623,6 +624,7 @@ __global__ void k_xfwd_prep(const T *__restrict__ y0, const T *__restrict__ y1, 
                             T *__restrict__ out, C2<T> *__restrict__ S1, const StepCtl *ctl, double b10, int nx,
                             int ny, int nz, int Hx, int Hp, FftPlanDev p, int lpw)
 {
+    constexpr int SK = MM_SKEW_X;
     if (ctl->stop) return;
     extern __shared__ char smem_raw[];
     C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
@@ -633,13 +635,13 @@ __global__ void k_xfwd_prep(const T *__restrict__ y0, const T *__restrict__ y1, 
     const int l = threadIdx.x / PlanC<LC>::tline(p), t = threadIdx.x % PlanC<LC>::tline(p);
     const int g = blockIdx.x * lpw + l;
     const bool active = g < nrows;
-    C2<T> *line = sm + (size_t)l * lds_len(L);
-    C2<T> *twl = sm + (size_t)lpw * lds_len(L);
+    C2<T> *line = sm + (size_t)l * lds_len<SK>(L);
+    C2<T> *twl = sm + (size_t)lpw * lds_len<SK>(L);
     load_twiddles<T, LC>(twl, p);
     const size_t base = (size_t)(active ? g : 0) * nx;
     C2<T> *dst = S1 + (size_t)(active ? g : 0) * Hp;
     const LineMap m = {t, active};
-    fft_line<T, false, true, true, LC>(
+    fft_line<T, false, true, true, LC, SK>(
         p, twl, line, m, line, m,
         [&](int x) {
             T v = (T)0;
@@ -898,7 +900,7 @@ template <typename T> struct Stepper : public StepperBase {
             conv->init(d.N_host, d.nx, d.ny, d.nz, d.ex, d.ey, d.ez);
             const int tl = conv->px.tline;
             const size_t cs = sizeof(C2<T>), L = conv->px.L;
-            split_x = 3 * tl > 1024 || (3 * lds_len((int)L) + L) * cs > (size_t)mm_max_lds_bytes();
+            split_x = 3 * tl > 1024 || (3 * lds_len<MM_SKEW_X>((int)L) + L) * cs > (size_t)mm_max_lds_bytes();
             if (const char *e = std::getenv("MM_SPLIT_X")) split_x = split_x || std::atoi(e) != 0;
             if (split_x) {
                 // the inverse x transform runs as its own pass into Hdem
@@ -909,9 +911,9 @@ template <typename T> struct Stepper : public StepperBase {
             } else {
                 rpw = std::max(1, 192 / (3 * tl));
                 if (const char *e = std::getenv("MM_RPW")) rpw = std::max(1, std::atoi(e));
-                while (rpw > 1 && ((size_t)3 * rpw * lds_len((int)L) + L) * cs > (size_t)mm_max_lds_bytes()) --rpw;
+                while (rpw > 1 && ((size_t)3 * rpw * lds_len<MM_SKEW_X>((int)L) + L) * cs > (size_t)mm_max_lds_bytes()) --rpw;
                 threads = 3 * rpw * tl;
-                stage_lds = ((size_t)3 * rpw * lds_len((int)L) + L) * cs;
+                stage_lds = ((size_t)3 * rpw * lds_len<MM_SKEW_X>((int)L) + L) * cs;
             }
         } else {
             rpw = std::max(1, 256 / d.nx);
@@ -1071,7 +1073,7 @@ template <typename T> struct Stepper : public StepperBase {
                 const FftPlanHost &px = conv->px;
                 const int lpw = conv->ln.lpw_x;
                 const int nrows = 3 * d.ny * d.nz;
-                const size_t sh = ((size_t)lpw * lds_len(px.L) + px.L) * sizeof(C2<T>);
+                const size_t sh = ((size_t)lpw * lds_len<MM_SKEW_X>(px.L) + px.L) * sizeof(C2<T>);
                 mm_with_len(px.L, conv->fft_generic, [&](auto lc) {
                     hipLaunchKernelGGL((k_xfwd_prep<T, decltype(lc)::value>), dim3((nrows + lpw - 1) / lpw),
                                        dim3(lpw * px.tline), sh, s, yb[0], yb[1], k[0], tmp[0], conv->S1, ctl,
