@@ -1,0 +1,215 @@
+"""GPU runs at the full sizes of the benchmark workloads, checked through
+size-independent properties (the oracle does not run at these sizes):
+
+  * the demag convolution is linear, and reciprocal: sum M1.H(M2) equals
+    sum M2.H(M1) because the cell-cell tensor is symmetric and even;
+  * trace rule: with M uniform along each axis in turn, the volume means of
+    the field component along that axis add up to -Ms (the trace of the
+    cell-cell tensor summed over all partners is 1 for every cell; the far
+    entries' series and asymptotics leave ~1e-6 at these sizes);
+  * RK steps keep |M| = Ms in every cell (normalisation after each step);
+  * the mixed FP32 path tracks the FP64 path (same attempts and a nearby
+    state, or, where FP32 rounding changes the step sequence, a nearby state
+    at a common end time).
+
+Workloads: block 256x256x16 cells (RKF45), film 1024x1024x2 with spin torque
+(DP54), grid 512x512x64 (its Newell tensor takes about a minute and a half on
+the host, so that grid runs a point-dipole tensor of the same symmetries).
+"""
+import numpy as np
+import pytest
+
+from conftest import gpu_available, vortex_field
+
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not gpu_available(), reason="needs a HIP GPU")]
+
+
+def _dev(a, dtype):
+    from micromagnum_amd.device import to_device
+    return to_device(np.ascontiguousarray(a, dtype=np.float64), dtype)
+
+
+def _conv_apply(conv, M, dtype=np.float64):
+    from micromagnum_amd.device import empty_like
+    Md = _dev(M, dtype)
+    H = empty_like(Md)
+    conv.execute(Md, H)
+    return H.numpy().astype(np.float64)
+
+
+def _random_m(shape, seed, Ms):
+    nx, ny, nz = shape
+    m = np.random.default_rng(seed).standard_normal((3, nz, ny, nx))
+    m /= np.sqrt((m ** 2).sum(axis=0))
+    return m * Ms
+
+
+def _check_conv_properties(conv, shape, trace_tol=None):
+    Ms = 8e5
+    M1, M2 = _random_m(shape, 1, Ms), _random_m(shape, 2, Ms)
+    H1, H2 = _conv_apply(conv, M1), _conv_apply(conv, M2)
+    # reciprocity
+    a, b = float((M1 * H2).sum()), float((M2 * H1).sum())
+    scale = np.sqrt((M1 ** 2).sum() * (H2 ** 2).sum())
+    assert abs(a - b) <= 1e-13 * scale, (a, b)
+    # linearity
+    H12 = _conv_apply(conv, 0.3 * M1 - 1.7 * M2)
+    ref = 0.3 * H1 - 1.7 * H2
+    assert np.sqrt(((H12 - ref) ** 2).sum() / (ref ** 2).sum()) < 1e-13
+    if trace_tol is not None:
+        nx, ny, nz = shape
+        tot = 0.0
+        for c in range(3):
+            M = np.zeros((3, nz, ny, nx))
+            M[c] = Ms
+            tot += _conv_apply(conv, M)[c].mean()
+        assert abs(tot / Ms + 1.0) < trace_tol, tot / Ms + 1.0
+
+
+def _norm_err(M, Ms):
+    return np.abs(np.sqrt((M ** 2).sum(axis=0)) / Ms - 1.0).max()
+
+
+def _fp32_tracks_fp64(shape, delta, N, M0, Ms, steps, method, tol_m, **kw):
+    from micromagnum_amd.solver import Stepper
+    out = {}
+    for prec in ("fp64", "fp32"):
+        st = Stepper(shape, delta, N=N, precision=prec, method=method, Ms=Ms, **kw)
+        st.set_m(M0)
+        st.run(steps)
+        out[prec] = (st.get_m(), st.stats())
+        st.close()
+    (M64, s64), (M32, s32) = out["fp64"], out["fp32"]
+    assert s64["attempts"] - s64["rejections"] == steps and s32["attempts"] - s32["rejections"] == steps
+    assert s32["attempts"] == s64["attempts"] and s32["rejections"] == s64["rejections"]
+    assert abs(s32["t"] - s64["t"]) <= 1e-3 * s64["t"]
+    # the reference's normalisation takes the norm in single precision
+    # (float_cast_norm), in both paths
+    assert _norm_err(M64, Ms) < 1e-6
+    assert _norm_err(M32, Ms) < 1e-6
+    assert np.abs(M32 - M64).max() / Ms < tol_m
+    return M64
+
+
+def test_block_256x256x16_full_size():
+    from micromagnum_amd import magneto as mg
+    import bench
+    shape, delta = bench.SHAPE, bench.DELTA
+    N = mg.GenerateDemagTensor(*shape, *delta)
+    conv = mg.SymmetricMatrixVectorConvolution_FFT(N, *shape)
+    _check_conv_properties(conv, shape, trace_tol=2e-5)
+    del conv
+    p = dict(bench.PARAMS)
+    Ms = p.pop("Ms")
+    _fp32_tracks_fp64(shape, delta, N, bench.initial_m(shape, Ms), Ms, 20, "rkf45", 1e-4,
+                      eps_abs=bench.TOL, eps_rel=bench.TOL, **p)
+
+
+def test_film_1024x1024x2_spin_torque_full_size():
+    from micromagnum_amd import magneto as mg
+    shape, delta = (1024, 1024, 2), (3e-9, 3e-9, 2e-9)
+    N = mg.GenerateDemagTensor(*shape, *delta)
+    conv = mg.SymmetricMatrixVectorConvolution_FFT(N, *shape)
+    _check_conv_properties(conv, shape, trace_tol=2e-5)
+    del conv
+    Ms = 8.6e5
+    # vortex (core +z, 10 nm) plus 1 % seeded noise, renormalised
+    M0 = vortex_field(shape, delta, (shape[0] * 1.5e-9, shape[1] * 1.5e-9), Ms=Ms)
+    M0 = M0 + 0.01 * Ms * np.random.default_rng(7).standard_normal(M0.shape)
+    M0 *= Ms / np.sqrt((M0 ** 2).sum(axis=0))
+    # The noise makes exchange fields large and the first attempts are
+    # rejected down to h ~ 1e-15 s.  From there the relative error criterion
+    # lets h grow by at most 5x per step; in FP32 the stage inputs' rounding,
+    # amplified by the exchange stencil, leaves a floor of ~3e-7 in that
+    # relative error, which caps the growth at ~1.7x per step (RKF45 from a
+    # step size the true error limits is unaffected).  The two precisions
+    # therefore take different step sequences here and are compared at a
+    # common end time instead.
+    from micromagnum_amd.solver import Stepper
+    kw = dict(alpha=0.1, A=1.3e-11, P=0.7, xi=0.05, j=(1e12, 0.0, 0.0), eps_abs=1e-5, eps_rel=1e-5)
+    t_end = 2.5e-13
+    out = {}
+    for prec in ("fp64", "fp32"):
+        st = Stepper(shape, delta, N=N, precision=prec, method="dp54", Ms=Ms, **kw)
+        st.set_m(M0)
+        st.run_until(t_end, max_steps=200)
+        out[prec] = (st.get_m(), st.stats())
+        st.close()
+    (M64, s64), (M32, s32) = out["fp64"], out["fp32"]
+    assert s64["t"] == pytest.approx(t_end, rel=1e-15) and s32["t"] == pytest.approx(t_end, rel=1e-15)
+    assert _norm_err(M64, Ms) < 1e-6 and _norm_err(M32, Ms) < 1e-6
+    assert np.abs(M32 - M64).max() / Ms < 1e-3
+    assert np.abs(M64 - M0).max() / Ms > 1e-3
+
+
+def _dipole_tensor(shape, delta):
+    """Point-dipole cell-cell tensor in the padded layout and sign convention
+    of GenerateDemagTensor (H = N M; round4(2n-1) per open axis, offsets
+    0..n-1 first, -(n-1)..-1 last, zero between; self term -1/3): symmetric
+    and even like the Newell tensor.  Evaluated on the octant of non-negative
+    offsets and mirrored (xy, xz, yz odd in the axes they name)."""
+    from micromagnum_amd import magneto as mg
+    nx, ny, nz = shape
+    ex, ey, ez = (mg.round_tensor_dimension(n, False, mg.PADDING_ROUND_4) for n in shape)
+    x = np.arange(nx) * delta[0]
+    y = np.arange(ny) * delta[1]
+    f0 = delta[0] * delta[1] * delta[2] / (4 * np.pi)
+    O = np.empty((nz, ny, nx, 6))
+    X, Y = x[None, :], y[:, None]
+    for k in range(nz):
+        z = k * delta[2]
+        r2 = X * X + Y * Y + z * z
+        r2[0, 0] = np.inf if k == 0 else r2[0, 0]
+        i3 = f0 / (r2 * np.sqrt(r2))
+        i5 = 3.0 * i3 / r2
+        O[k, :, :, 0] = i5 * X * X - i3
+        O[k, :, :, 1] = i5 * X * Y
+        O[k, :, :, 2] = i5 * X * z
+        O[k, :, :, 3] = i5 * Y * Y - i3
+        O[k, :, :, 4] = i5 * Y * z
+        O[k, :, :, 5] = i5 * z * z - i3
+    O[0, 0, 0] = (-1.0 / 3.0, 0.0, 0.0, -1.0 / 3.0, 0.0, -1.0 / 3.0)
+    N = np.zeros((ez, ey, ex, 6))
+    odd = np.array([[0, 1, 1, 0, 0, 0], [0, 1, 0, 0, 1, 0], [0, 0, 1, 0, 1, 0]])  # odd in x, y, z
+    for sz in (1, -1):
+        for sy in (1, -1):
+            for sx in (1, -1):
+                sign = np.ones(6)
+                src, dst = [], []
+                for a, (s_, n, e) in enumerate(((sz, nz, ez), (sy, ny, ey), (sx, nx, ex))):
+                    if s_ > 0:
+                        src.append(slice(0, n))
+                        dst.append(slice(0, n))
+                    else:
+                        src.append(slice(1, n))
+                        dst.append(slice(e - 1, e - n, -1))
+                        sign = sign * np.where(odd[2 - a] == 1, -1.0, 1.0)
+                if any(d.start == d.stop for d in dst):
+                    continue
+                N[tuple(dst)] = O[tuple(src)] * sign
+    return N
+
+
+def test_grid_512x512x64_full_size():
+    from micromagnum_amd import magneto as mg
+    from micromagnum_amd.solver import Stepper
+    shape, delta = (512, 512, 64), (2e-9, 2e-9, 2e-9)
+    N = _dipole_tensor(shape, delta)
+    conv = mg.SymmetricMatrixVectorConvolution_FFT(N, *shape)
+    assert conv.fft_size == (1024, 1024, 128)
+    _check_conv_properties(conv, shape)
+    del conv
+    # the relaxation workload's physics (cubic anisotropy, alpha 1), a few
+    # steps in FP32: the norm stays, the state moves, nothing overflows
+    Ms = 8e5
+    M0 = _random_m(shape, 3, Ms)
+    st = Stepper(shape, delta, N=N, precision="fp32", method="rkf45", Ms=Ms, alpha=1.0, A=1.3e-11,
+                 k_cub=-5e3, axis1=(1.0, 0.0, 0.0), axis2=(0.0, 1.0, 0.0), eps_abs=1e-4, eps_rel=1e-4)
+    st.set_m(M0)
+    st.run(5)
+    M = st.get_m()
+    s = st.stats()
+    st.close()
+    assert s["attempts"] - s["rejections"] == 5 and np.isfinite(M).all()
+    assert _norm_err(M, Ms) < 1e-6
+    assert np.abs(M - M0).max() > 1e-3 * Ms

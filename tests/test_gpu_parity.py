@@ -668,10 +668,11 @@ def test_ensemble_equals_single_steppers_and_oracle():
     ens.close()
 
 
-@pytest.mark.parametrize("shape", [(32, 16, 2), (64, 8, 4)])
-def test_stepper_compiled_plans_match_runtime_plans_and_oracle(shape, monkeypatch):
-    # padded lengths (64, 32, 4) and (128, 16, 8): the stage kernel's x
-    # transforms and every pass of the field run compile-time plans
+@pytest.mark.parametrize("shape,method", [((32, 16, 2), "rkf45"), ((64, 8, 4), "rkf45"), ((1024, 4, 2), "dp54"),
+                                          ((1024, 4, 2), "rkf45")])
+def test_stepper_compiled_plans_match_runtime_plans_and_oracle(shape, method, monkeypatch):
+    # padded lengths (64, 32, 4), (128, 16, 8) and (2048, 8, 4): the stage
+    # kernel's x transforms and every pass of the field run compile-time plans
     from micromagnum_amd.solver import Stepper
     delta = (3e-9, 3e-9, 2e-9)
     kw = dict(Ms=8.6e5, alpha=0.1, A=1.3e-11, P=0.7, xi=0.05, j=(1e12, 0.0, 0.0))
@@ -679,7 +680,7 @@ def test_stepper_compiled_plans_match_runtime_plans_and_oracle(shape, monkeypatc
     out = {}
     for generic in ("0", "1"):
         monkeypatch.setenv("MM_FFT_GENERIC", generic)
-        st = Stepper(shape, delta, precision="fp64", method="rkf45", eps_abs=1e-5, eps_rel=1e-5, **kw)
+        st = Stepper(shape, delta, precision="fp64", method=method, eps_abs=1e-5, eps_rel=1e-5, **kw)
         st.set_m(M)
         st.run(10)
         out[generic] = (st.get_m(), st.stats(), st.energy(), st.N)
@@ -691,7 +692,7 @@ def test_stepper_compiled_plans_match_runtime_plans_and_oracle(shape, monkeypatc
     for key, v in out["1"][2].items():
         assert v == pytest.approx(out["0"][2][key], rel=1e-10, abs=1e-300)
     sysO = O.System(shape, delta, N=out["0"][3], **kw)
-    Mo, so = sysO.run(M, method="rkf45", steps=10, eps_abs=1e-5, eps_rel=1e-5)
+    Mo, so = sysO.run(M, method=method, steps=10, eps_abs=1e-5, eps_rel=1e-5)
     assert s0["attempts"] == so["attempts"] and s0["rejections"] == so["rejections"]
     assert abs(s0["t"] - so["t"]) <= 1e-9 * so["t"]
     assert np.abs(out["0"][0] - Mo).max() / 8.6e5 < 1e-9
