@@ -374,10 +374,15 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
 // too long for three in one workgroup's LDS).  EN: instead of the stage
 // epilogue, per-block sums of M.H per term into epart[q * gridDim.x + block].
 template <typename T, bool EN, int LC, bool STT, bool GEN>
-__global__ void k_stage(SysDev<T> S, StageDev<T> G, StepCtl *ctl, const C2<T> *__restrict__ S1, FftPlanDev p,
-                        int Hx, int Hp, int rpw, int has_demag, double *__restrict__ epart,
-                        const T *__restrict__ Hdem)
+__global__ void k_stage(const SysDev<T> *__restrict__ Sp, const StageDev<T> *__restrict__ Gp, StepCtl *ctl,
+                        const C2<T> *__restrict__ S1, FftPlanDev p, int Hx, int Hp, int rpw, int has_demag,
+                        double *__restrict__ epart, const T *__restrict__ Hdem)
 {
+    // the system and stage records live in device memory (read where used)
+    // rather than in the kernel arguments: by value they need ~160 SGPRs,
+    // and the compiler spilled most of them to VGPR lanes
+    const SysDev<T> &S = *Sp;
+    const StageDev<T> &G = *Gp;
     constexpr int SK = MM_SKEW_X;
     if (!EN && ctl->stop) return;
     extern __shared__ char smem_raw[];
@@ -948,6 +953,7 @@ template <typename T> struct Stepper : public StepperBase {
         c0.h_last = d.h0;
         MM_HIP(hipMemcpy(ctl, &c0, sizeof(c0), hipMemcpyHostToDevice));
         MM_HIP(hipMemset(yb[0], 0, sizeof(T) * 3 * n));
+        build_tables();
     }
 
     void release()
@@ -968,6 +974,10 @@ template <typename T> struct Stepper : public StepperBase {
         if (part) (void)hipFree(part);
         if (epart) (void)hipFree(epart);
         if (mpart) (void)hipFree(mpart);
+        if (gtab) (void)hipFree(gtab);
+        if (sysd) (void)hipFree(sysd);
+        gtab = nullptr;
+        sysd = nullptr;
         if (ctl) (void)hipFree(ctl);
         if (s && own_stream) (void)hipStreamDestroy(s);
         dbuf = part = epart = mpart = nullptr;
@@ -1006,31 +1016,31 @@ template <typename T> struct Stepper : public StepperBase {
     // the stage kernel instantiated for the x transform length and the
     // physics present: STT (Zhang-Li) and GEN (cubic anisotropy or exchange
     // with per-cell Ms/A); the energy launch (EN) has no spin torque term
-    template <bool EN, int LC, bool STT, bool GEN> void stage_launch(const StageDev<T> &G, double *ep)
+    template <bool EN, int LC, bool STT, bool GEN> void stage_launch(const StageDev<T> *Gd, double *ep)
     {
-        hipLaunchKernelGGL((k_stage<T, EN, LC, STT, GEN>), dim3(grid_rows), dim3(threads), stage_lds, s, sys, G, ctl,
-                           conv ? conv->S1 : nullptr, conv ? conv->px.template dev<T>() : FftPlanDev(),
+        hipLaunchKernelGGL((k_stage<T, EN, LC, STT, GEN>), dim3(grid_rows), dim3(threads), stage_lds, s, sysd, Gd,
+                           ctl, conv ? conv->S1 : nullptr, conv ? conv->px.template dev<T>() : FftPlanDev(),
                            conv ? conv->g.Hx : 0, conv ? conv->g.Hp : 0, rpw, demag_mode(), ep, Hdem);
     }
-    template <bool EN> void stage_kernel(const StageDev<T> &G, double *ep)
+    // Gd: the stage record in device memory (gtab)
+    template <bool EN> void stage_kernel(const StageDev<T> *Gd, double *ep)
     {
         const bool stt = !EN && sys.has_stt, gen = sys.has_cub || (sys.has_exch && !sys.exch_uniform);
         mm_with_len(stage_len(), conv && conv->fft_generic, [&](auto lc) {
             constexpr int LC = decltype(lc)::value;
             if (stt) {
-                if (gen) stage_launch<EN, LC, !EN, true>(G, ep);
-                else stage_launch<EN, LC, !EN, false>(G, ep);
+                if (gen) stage_launch<EN, LC, !EN, true>(Gd, ep);
+                else stage_launch<EN, LC, !EN, false>(Gd, ep);
             } else {
-                if (gen) stage_launch<EN, LC, false, true>(G, ep);
-                else stage_launch<EN, LC, false, false>(G, ep);
+                if (gen) stage_launch<EN, LC, false, true>(Gd, ep);
+                else stage_launch<EN, LC, false, false>(Gd, ep);
             }
         });
     }
 
     // need_fwd: S1 does not yet hold the x transform of the stage input
-    void launch_stage(const T *Ys, const StageDev<T> &G, bool need_fwd = true)
+    void launch_stage(const T *Ys, const StageDev<T> &G, const StageDev<T> *Gd, bool need_fwd = true)
     {
-        MM_REQUIRE(G.nk >= 0 && G.nk <= MM_STAGE_MAXK, "stage combination reads too many stage vectors");
         if (conv) {
             conv->klog = klog;
             if (need_fwd) {
@@ -1040,12 +1050,12 @@ template <typename T> struct Stepper : public StepperBase {
             conv->spectral(s);
             if (split_x) conv->inverse_x(Hdem, s);
         }
-        stage_kernel<false>(G, nullptr);
+        stage_kernel<false>(Gd, nullptr);
         MM_HIP(hipGetLastError());
         if (klog) klog->mark(s, KC_STAGE, stage_bytes(G));
     }
 
-    StageDev<T> blank_stage(const T *Ys)
+    StageDev<T> blank_stage(const T *Ys) const
     {
         StageDev<T> G;
         std::memset(&G, 0, sizeof(G));
@@ -1057,13 +1067,92 @@ template <typename T> struct Stepper : public StepperBase {
         return G;
     }
 
+    // the record of stage st of an attempt (the same for attempts that start
+    // from a carried-over k_0: those skip stage 0)
+    StageDev<T> stage_record(int st) const
+    {
+        const int ns = tab.num_steps;
+        const bool six = ns == 6;
+        const bool fsal = tab.fsal != 0;
+        const T *Ys = st == 0 ? nullptr : tmp[(st - 1) & 1];
+        StageDev<T> G = blank_stage(Ys);
+        G.kout = (st < ns - 1) ? k[st] : nullptr;
+        G.want_dydt = (st == 0) && !fsal;
+        if (conv && !split_x && st < ns - 1) {
+            G.fwd = 1;
+            G.fwd_S1 = conv->S1;
+        }
+        G.form = six ? FORM_SUM : FORM_SEQ;
+        if (st < ns - 1) {
+            G.prep = 1;
+            G.prep_out = tmp[st & 1];
+            G.nk = 0;
+            for (int q = 0; q < st; ++q) {
+                G.kin[G.nk] = k[q];
+                G.b[G.nk++] = tab.b[st + 1][q];
+            }
+            G.bself = tab.b[st + 1][st];
+            if (!six) {
+                // sequential form of runge_kutta.py:107-111 / runge_kutta_4.py keeps
+                // zero coefficients out (RK4 only adds k_s with h*b)
+                int m2 = 0;
+                for (int q = 0; q < G.nk; ++q)
+                    if (G.b[q] != 0.0) {
+                        G.kin[m2] = G.kin[q];
+                        G.b[m2++] = G.b[q];
+                    }
+                G.nk = m2;
+            }
+        } else {
+            G.prep = 2;
+            G.nk = 0;
+            const bool skip1 = six && tab.c[1] == 0.0 && tab.ec[1] == 0.0;
+            for (int q = 0; q < st; ++q) {
+                if (q == 1 && skip1) continue;
+                G.kin[G.nk] = k[q];
+                G.b[G.nk] = tab.c[q];
+                G.ec[G.nk++] = tab.ec[q];
+            }
+            G.bself = tab.c[st];
+            G.ecself = tab.ec[st];
+            if (fsal) {
+                G.dydt_from_k0 = 1;
+                G.fsal_out = k[0];
+            }
+        }
+        return G;
+    }
+
+    // device copies of the stage records: stages 0..ns-1, then dM/dt, then
+    // the energy launch; and of the system record.  Uploaded once at set-up,
+    // so captured graphs and eager launches read the same fixed addresses.
+    std::vector<StageDev<T>> htab;
+    StageDev<T> *gtab = nullptr;
+    SysDev<T> *sysd = nullptr;
+    void build_tables()
+    {
+        const int ns = tab.num_steps;
+        htab.clear();
+        for (int st = 0; st < ns; ++st) htab.push_back(stage_record(st));
+        StageDev<T> G = blank_stage(nullptr);
+        G.kout = tmp[0];
+        G.Hout = Hbuf;
+        htab.push_back(G);                      // dM/dt
+        htab.push_back(blank_stage(nullptr));   // energy
+        for (const StageDev<T> &r : htab)
+            MM_REQUIRE(r.nk >= 0 && r.nk <= MM_STAGE_MAXK, "stage combination reads too many stage vectors");
+        MM_HIP(hipMalloc(&gtab, sizeof(StageDev<T>) * htab.size()));
+        MM_HIP(hipMemcpy(gtab, htab.data(), sizeof(StageDev<T>) * htab.size(), hipMemcpyHostToDevice));
+        MM_HIP(hipMalloc(&sysd, sizeof(SysDev<T>)));
+        MM_HIP(hipMemcpy(sysd, &sys, sizeof(SysDev<T>), hipMemcpyHostToDevice));
+    }
+
     bool have_fsal = false;
 
     // one attempt of the tableau from y[cur] with h from the control block
     void enqueue_attempt()
     {
         const int ns = tab.num_steps;
-        const bool six = ns == 6;
         const bool fsal = tab.fsal != 0;
         int st0 = 0;
         bool s1_ready = false;  // S1 holds the x transform of the next stage input
@@ -1094,53 +1183,8 @@ template <typename T> struct Stepper : public StepperBase {
             st0 = 1;
         }
         for (int st = st0; st < ns; ++st) {
-            const T *Ys = st == 0 ? nullptr : tmp[(st - 1) & 1];
-            StageDev<T> G = blank_stage(Ys);
-            G.kout = (st < ns - 1) ? k[st] : nullptr;
-            G.want_dydt = (st == 0) && !fsal;
-            if (conv && !split_x && st < ns - 1) {
-                G.fwd = 1;
-                G.fwd_S1 = conv->S1;
-            }
-            G.form = six ? FORM_SUM : FORM_SEQ;
-            if (st < ns - 1) {
-                G.prep = 1;
-                G.prep_out = tmp[st & 1];
-                G.nk = 0;
-                for (int q = 0; q < st; ++q) {
-                    G.kin[G.nk] = k[q];
-                    G.b[G.nk++] = tab.b[st + 1][q];
-                }
-                G.bself = tab.b[st + 1][st];
-                if (!six) {
-                    // sequential form of runge_kutta.py:107-111 / runge_kutta_4.py keeps
-                    // zero coefficients out (RK4 only adds k_s with h*b)
-                    int m2 = 0;
-                    for (int q = 0; q < G.nk; ++q)
-                        if (G.b[q] != 0.0) {
-                            G.kin[m2] = G.kin[q];
-                            G.b[m2++] = G.b[q];
-                        }
-                    G.nk = m2;
-                }
-            } else {
-                G.prep = 2;
-                G.nk = 0;
-                const bool skip1 = six && tab.c[1] == 0.0 && tab.ec[1] == 0.0;
-                for (int q = 0; q < st; ++q) {
-                    if (q == 1 && skip1) continue;
-                    G.kin[G.nk] = k[q];
-                    G.b[G.nk] = tab.c[q];
-                    G.ec[G.nk++] = tab.ec[q];
-                }
-                G.bself = tab.c[st];
-                G.ecself = tab.ec[st];
-                if (fsal) {
-                    G.dydt_from_k0 = 1;
-                    G.fsal_out = k[0];
-                }
-            }
-            launch_stage(Ys, G, !s1_ready);
+            const StageDev<T> &G = htab[st];
+            launch_stage(G.Ys, G, gtab + st, !s1_ready);
             s1_ready = G.fwd != 0;
         }
         if (fsal) have_fsal = true;
@@ -1214,10 +1258,8 @@ template <typename T> struct Stepper : public StepperBase {
 
     void dmdt(double *dM_host, double *H_host) override
     {
-        StageDev<T> G = blank_stage(nullptr);
-        G.kout = tmp[0];
-        G.Hout = Hbuf;
-        launch_stage(nullptr, G);
+        const int slot = tab.num_steps;
+        launch_stage(nullptr, htab[slot], gtab + slot);
         if (dM_host) download(dM_host, tmp[0], 3 * n);
         if (H_host) download(H_host, Hbuf, 3 * n);
     }
@@ -1225,14 +1267,13 @@ template <typename T> struct Stepper : public StepperBase {
     // E_stray, E_exch, E_aniso, E_ext, E_tot at the current state
     void energy(double *E5) override
     {
-        StageDev<T> G = blank_stage(nullptr);
         if (conv) {
             conv->klog = nullptr;
             conv->forward_x(yb[0], s, yb[1], &ctl->cur);
             conv->spectral(s);
             if (split_x) conv->inverse_x(Hdem, s);
         }
-        stage_kernel<true>(G, epart);
+        stage_kernel<true>(gtab + tab.num_steps + 1, epart);
         MM_HIP(hipGetLastError());
         std::vector<double> h((size_t)4 * grid_rows);
         MM_HIP(hipMemcpyAsync(h.data(), epart, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
