@@ -22,8 +22,11 @@ struct FftPlanHost {
     int L = 1;
     int tline = 1;
     std::vector<int> rad;
-    void *tw32 = nullptr;  // float2[L]
+    void *tw32 = nullptr;  // float2[L], exp(-2 pi i m / L)
     void *tw64 = nullptr;  // double2[L]
+    void *twp32 = nullptr; // float2[ntw], per-pass layout (FftPlanDev)
+    void *twp64 = nullptr; // double2[ntw]
+    int ntw = 0;
     void build(int len);
     void release();
     template <typename T> FftPlanDev dev() const
@@ -34,10 +37,13 @@ struct FftPlanHost {
         d.npass = (int)rad.size();
         d.radices = 0u;
         for (int i = 0; i < d.npass; ++i) d.radices |= (unsigned)rad[i] << (4 * i);
-        d.tw = sizeof(T) == 4 ? tw32 : tw64;
+        d.tw = sizeof(T) == 4 ? twp32 : twp64;
+        d.ntw = ntw;
         d.pow2 = (L & (L - 1)) == 0 ? 1 : 0;
         return d;
     }
+    // the plain table exp(-2 pi i m / L), m < L
+    template <typename T> const void *plain() const { return sizeof(T) == 4 ? tw32 : tw64; }
 };
 
 // Calls f(std::integral_constant<int, LC>()) with LC = L when kernels carry a

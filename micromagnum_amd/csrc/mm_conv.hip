@@ -65,13 +65,39 @@ void FftPlanHost::build(int len)
     MM_HIP(hipMalloc(&tw64, sizeof(double2) * L));
     MM_HIP(hipMemcpy(tw32, t32.data(), sizeof(float2) * L, hipMemcpyHostToDevice));
     MM_HIP(hipMemcpy(tw64, t64.data(), sizeof(double2) * L, hipMemcpyHostToDevice));
+    // per-pass layout: pass p (radix R, Ns > 1) holds W_{Ns R}^{r k} at
+    // (r-1) Ns + k, copied from the plain table at index r k L / (Ns R)
+    std::vector<float2> p32;
+    std::vector<double2> p64;
+    int Ns = 1;
+    for (int R : rad) {
+        if (Ns > 1)
+            for (int r = 1; r < R; ++r)
+                for (int k = 0; k < Ns; ++k) {
+                    const int m = r * k * (L / (Ns * R));
+                    p32.push_back(t32[m]);
+                    p64.push_back(t64[m]);
+                }
+        Ns *= R;
+    }
+    ntw = (int)p64.size();
+    MM_REQUIRE(ntw < L || L == 1, "per-pass twiddle table larger than the line");
+    const size_t nalloc = std::max<size_t>(1, p64.size());
+    MM_HIP(hipMalloc(&twp32, sizeof(float2) * nalloc));
+    MM_HIP(hipMalloc(&twp64, sizeof(double2) * nalloc));
+    if (ntw > 0) {
+        MM_HIP(hipMemcpy(twp32, p32.data(), sizeof(float2) * ntw, hipMemcpyHostToDevice));
+        MM_HIP(hipMemcpy(twp64, p64.data(), sizeof(double2) * ntw, hipMemcpyHostToDevice));
+    }
 }
 
 void FftPlanHost::release()
 {
     if (tw32) (void)hipFree(tw32);
     if (tw64) (void)hipFree(tw64);
-    tw32 = tw64 = nullptr;
+    if (twp32) (void)hipFree(twp32);
+    if (twp64) (void)hipFree(twp64);
+    tw32 = tw64 = twp32 = twp64 = nullptr;
 }
 
 bool mm_is_smooth(int n)
@@ -779,7 +805,7 @@ template <typename T> void ConvImpl<T>::spectral(hipStream_t s)
     const size_t shyp = lines_lds<MM_SKEW_Y>(tkp, pyh.L, cs);
     if (prune_y) {
         hipLaunchKernelGGL((k_yfwd_pr<T>), dim3((g.Hx + tkp - 1) / tkp, g.nz, 3), dim3(tkp * pyh.tline), shyp, s, S1, S2,
-                           g.Hx, g.Hp, g.ny, g.nz, pyh.dev<T>(), (const C2<T> *)py.dev<T>().tw, tkp);
+                           g.Hx, g.Hp, g.ny, g.nz, pyh.dev<T>(), (const C2<T> *)py.plain<T>(), tkp);
     } else {
         mm_with_len(py.L, fft_generic, [&](auto lc) {
             hipLaunchKernelGGL((k_yfwd<T, decltype(lc)::value>), dim3((g.Hx + tk - 1) / tk, g.nz, 3),
@@ -800,7 +826,7 @@ template <typename T> void ConvImpl<T>::spectral(hipStream_t s)
     if (klog) klog->mark(s, KC_ZMI, conv_bytes<T>(g, KC_ZMI));
     if (prune_y) {
         hipLaunchKernelGGL((k_yinv_pr<T>), dim3((g.Hx + tkp - 1) / tkp, g.nz, 3), dim3(tkp * pyh.tline), shyp, s, S2, S1,
-                           g.Hx, g.Hp, g.ny, g.nz, pyh.dev<T>(), (const C2<T> *)py.dev<T>().tw, tkp);
+                           g.Hx, g.Hp, g.ny, g.nz, pyh.dev<T>(), (const C2<T> *)py.plain<T>(), tkp);
     } else {
         mm_with_len(py.L, fft_generic, [&](auto lc) {
             hipLaunchKernelGGL((k_yinv<T, decltype(lc)::value>), dim3((g.Hx + tk - 1) / tk, g.nz, 3),

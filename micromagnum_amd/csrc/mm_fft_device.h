@@ -79,23 +79,31 @@ __device__ __forceinline__ int block_tile(int remap)
 
 // device-side plan of one transform length (passed by value).  The radix of
 // pass p is (radices >> 4p) & 15, so no array is indexed at run time.
+// Twiddles are stored per pass: pass p (radix R, Ns = product of the
+// earlier radices, Ns > 1) owns (R-1) Ns entries at offset toff_p,
+// tw[toff_p + (r-1) Ns + k] = exp(-2 pi i r k / (Ns R)) for r = 1..R-1 and
+// k = 0..Ns-1 (the values of the plain table at index r k L/(Ns R)).  The
+// lanes of a wave take consecutive k, so each twiddle read of a pass is one
+// contiguous run in LDS (the plain table read at stride r hit 2-8 lanes per
+// bank).  ntw = sum over passes with Ns > 1 of (R-1) Ns < L entries.
 struct FftPlanDev {
     int L;
     int tline;             // threads per line
     int npass;
     unsigned radices;
-    const void *tw;        // L entries, tw[m] = exp(-2 pi i m / L), precision T
+    const void *tw;        // per-pass twiddles (ntw entries), precision T
+    int ntw;
     int pow2;              // L is a power of two (all radices 8, 4, 2)
 };
 
 // The kernels copy the plan's twiddle table into LDS next to their lines
-// (one read of L values per workgroup) before the barrier that precedes the
-// first transform, so no pass waits on a global load.
+// (one read of ntw values per workgroup) before the barrier that precedes
+// the first transform, so no pass waits on a global load.
 template <typename T, int LC = 0> __device__ __forceinline__ void load_twiddles(C2<T> *twl, const FftPlanDev &p)
 {
     const C2<T> *g = (const C2<T> *)p.tw;
-    const int L = LC > 0 ? LC : p.L;
-    for (int i = threadIdx.x; i < L; i += blockDim.x) twl[i] = g[i];
+    const int n = p.ntw;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) twl[i] = g[i];
 }
 
 template <int R> struct Log2R { static const int value = R == 8 ? 3 : (R == 4 ? 2 : 1); };
@@ -195,7 +203,7 @@ template <int R, int EPT> struct MaxB { static const int value = (EPT + R - 1) /
 // is a power of two and lNs = log2(Ns), so index arithmetic is shifts/masks.
 template <int R, int EPT, typename T, bool INV, bool P2, int SK>
 __device__ __forceinline__ void fft_pass(C2<T> *line, const C2<T> *__restrict__ tw, int L, int tline, int t,
-                                         bool active, int Ns, int lNs)
+                                         bool active, int Ns, int lNs, int toff)
 {
     constexpr int MB = MaxB<R, EPT>::value;
     const int nb = L / R;
@@ -209,7 +217,6 @@ __device__ __forceinline__ void fft_pass(C2<T> *line, const C2<T> *__restrict__ 
         }
     }
     __syncthreads();
-    const int tstep = P2 ? nb >> lNs : L / (Ns * R);
 #pragma unroll
     for (int q = 0; q < MB; ++q) {
         const int b = t + q * tline;
@@ -218,7 +225,7 @@ __device__ __forceinline__ void fft_pass(C2<T> *line, const C2<T> *__restrict__ 
             if (Ns > 1) {
 #pragma unroll
                 for (int r = 1; r < R; ++r) {
-                    C2<T> w = tw[r * k * tstep];
+                    C2<T> w = tw[toff + (r - 1) * Ns + k];
                     if (INV) w.y = -w.y;
                     v[q][r] = cmul<T>(v[q][r], w);
                 }
@@ -239,25 +246,26 @@ __device__ __forceinline__ void fft_pass(C2<T> *line, const C2<T> *__restrict__ 
 template <typename T, bool INV, int EPT, bool P2, int SK>
 __device__ __forceinline__ void lds_fft_ept(C2<T> *line, const FftPlanDev &p, const C2<T> *tw, int t, bool active)
 {
-    int Ns = 1, lNs = 0;
+    int Ns = 1, lNs = 0, toff = 0;
     unsigned rs = p.radices;
     for (int ps = 0; ps < p.npass; ++ps, rs >>= 4) {
         const int R = (int)(rs & 15u);
         if (P2) {
             switch (R) {
-            case 8: fft_pass<8, EPT, T, INV, true, SK>(line, tw, p.L, p.tline, t, active, Ns, lNs); lNs += 3; break;
-            case 4: fft_pass<4, EPT, T, INV, true, SK>(line, tw, p.L, p.tline, t, active, Ns, lNs); lNs += 2; break;
-            default: fft_pass<2, EPT, T, INV, true, SK>(line, tw, p.L, p.tline, t, active, Ns, lNs); lNs += 1; break;
+            case 8: fft_pass<8, EPT, T, INV, true, SK>(line, tw, p.L, p.tline, t, active, Ns, lNs, toff); lNs += 3; break;
+            case 4: fft_pass<4, EPT, T, INV, true, SK>(line, tw, p.L, p.tline, t, active, Ns, lNs, toff); lNs += 2; break;
+            default: fft_pass<2, EPT, T, INV, true, SK>(line, tw, p.L, p.tline, t, active, Ns, lNs, toff); lNs += 1; break;
             }
         } else {
             switch (R) {
-            case 8: fft_pass<8, EPT, T, INV, false, SK>(line, tw, p.L, p.tline, t, active, Ns, 0); break;
-            case 5: fft_pass<5, EPT, T, INV, false, SK>(line, tw, p.L, p.tline, t, active, Ns, 0); break;
-            case 4: fft_pass<4, EPT, T, INV, false, SK>(line, tw, p.L, p.tline, t, active, Ns, 0); break;
-            case 3: fft_pass<3, EPT, T, INV, false, SK>(line, tw, p.L, p.tline, t, active, Ns, 0); break;
-            default: fft_pass<2, EPT, T, INV, false, SK>(line, tw, p.L, p.tline, t, active, Ns, 0); break;
+            case 8: fft_pass<8, EPT, T, INV, false, SK>(line, tw, p.L, p.tline, t, active, Ns, 0, toff); break;
+            case 5: fft_pass<5, EPT, T, INV, false, SK>(line, tw, p.L, p.tline, t, active, Ns, 0, toff); break;
+            case 4: fft_pass<4, EPT, T, INV, false, SK>(line, tw, p.L, p.tline, t, active, Ns, 0, toff); break;
+            case 3: fft_pass<3, EPT, T, INV, false, SK>(line, tw, p.L, p.tline, t, active, Ns, 0, toff); break;
+            default: fft_pass<2, EPT, T, INV, false, SK>(line, tw, p.L, p.tline, t, active, Ns, 0, toff); break;
             }
         }
+        if (Ns > 1) toff += (R - 1) * Ns;
         Ns *= R;
     }
 }
@@ -286,7 +294,7 @@ struct LineMap {
 
 template <int R, int EPT, typename T, bool INV, bool P2, bool LDIN, bool STOUT, int SK, typename LD, typename ST>
 __device__ __forceinline__ void fft_pass_io(C2<T> *lineG, LineMap g, C2<T> *lineS, LineMap sm_, const C2<T> *tw,
-                                            int L, int tline, int Ns, int lNs, LD &ld, ST &st)
+                                            int L, int tline, int Ns, int lNs, int toff, LD &ld, ST &st)
 {
     constexpr int MB = MaxB<R, EPT>::value;
     const int nb = L / R;
@@ -306,7 +314,6 @@ __device__ __forceinline__ void fft_pass_io(C2<T> *lineG, LineMap g, C2<T> *line
         }
     }
     if (!LDIN && !STOUT) __syncthreads();  // in place in LDS: every read before any write
-    const int tstep = P2 ? nb >> lNs : L / (Ns * R);
     // outputs: a butterfly stays with the thread that read it
 #pragma unroll
     for (int q = 0; q < MB; ++q) {
@@ -316,7 +323,7 @@ __device__ __forceinline__ void fft_pass_io(C2<T> *lineG, LineMap g, C2<T> *line
             if (Ns > 1) {
 #pragma unroll
                 for (int r = 1; r < R; ++r) {
-                    C2<T> w = tw[r * k * tstep];
+                    C2<T> w = tw[toff + (r - 1) * Ns + k];
                     if (INV) w.y = -w.y;
                     v[q][r] = cmul<T>(v[q][r], w);
                 }
@@ -336,22 +343,23 @@ __device__ __forceinline__ void fft_pass_io(C2<T> *lineG, LineMap g, C2<T> *line
 template <typename T, bool INV, int EPT, bool P2, bool LDIN, bool STOUT, bool FIRST, bool LAST, int SK,
           typename LD, typename ST>
 __device__ __forceinline__ void fft_pass_sel(int R, C2<T> *lineG, LineMap g, C2<T> *lineS, LineMap s_,
-                                             const C2<T> *tw, int L, int tline, int Ns, int lNs, LD &ld, ST &st)
+                                             const C2<T> *tw, int L, int tline, int Ns, int lNs, int toff, LD &ld,
+                                             ST &st)
 {
     constexpr bool I = LDIN && FIRST, O = STOUT && LAST;
     if (P2) {
         switch (R) {
-        case 8: fft_pass_io<8, EPT, T, INV, true, I, O, SK>(lineG, g, lineS, s_, tw, L, tline, Ns, lNs, ld, st); break;
-        case 4: fft_pass_io<4, EPT, T, INV, true, I, O, SK>(lineG, g, lineS, s_, tw, L, tline, Ns, lNs, ld, st); break;
-        default: fft_pass_io<2, EPT, T, INV, true, I, O, SK>(lineG, g, lineS, s_, tw, L, tline, Ns, lNs, ld, st); break;
+        case 8: fft_pass_io<8, EPT, T, INV, true, I, O, SK>(lineG, g, lineS, s_, tw, L, tline, Ns, lNs, toff, ld, st); break;
+        case 4: fft_pass_io<4, EPT, T, INV, true, I, O, SK>(lineG, g, lineS, s_, tw, L, tline, Ns, lNs, toff, ld, st); break;
+        default: fft_pass_io<2, EPT, T, INV, true, I, O, SK>(lineG, g, lineS, s_, tw, L, tline, Ns, lNs, toff, ld, st); break;
         }
     } else {
         switch (R) {
-        case 8: fft_pass_io<8, EPT, T, INV, false, I, O, SK>(lineG, g, lineS, s_, tw, L, tline, Ns, 0, ld, st); break;
-        case 5: fft_pass_io<5, EPT, T, INV, false, I, O, SK>(lineG, g, lineS, s_, tw, L, tline, Ns, 0, ld, st); break;
-        case 4: fft_pass_io<4, EPT, T, INV, false, I, O, SK>(lineG, g, lineS, s_, tw, L, tline, Ns, 0, ld, st); break;
-        case 3: fft_pass_io<3, EPT, T, INV, false, I, O, SK>(lineG, g, lineS, s_, tw, L, tline, Ns, 0, ld, st); break;
-        default: fft_pass_io<2, EPT, T, INV, false, I, O, SK>(lineG, g, lineS, s_, tw, L, tline, Ns, 0, ld, st); break;
+        case 8: fft_pass_io<8, EPT, T, INV, false, I, O, SK>(lineG, g, lineS, s_, tw, L, tline, Ns, 0, toff, ld, st); break;
+        case 5: fft_pass_io<5, EPT, T, INV, false, I, O, SK>(lineG, g, lineS, s_, tw, L, tline, Ns, 0, toff, ld, st); break;
+        case 4: fft_pass_io<4, EPT, T, INV, false, I, O, SK>(lineG, g, lineS, s_, tw, L, tline, Ns, 0, toff, ld, st); break;
+        case 3: fft_pass_io<3, EPT, T, INV, false, I, O, SK>(lineG, g, lineS, s_, tw, L, tline, Ns, 0, toff, ld, st); break;
+        default: fft_pass_io<2, EPT, T, INV, false, I, O, SK>(lineG, g, lineS, s_, tw, L, tline, Ns, 0, toff, ld, st); break;
         }
     }
 }
@@ -376,7 +384,7 @@ __device__ __forceinline__ void fft_line_p2(const FftPlanDev &p, const C2<T> *tw
         if (!STOUT) __syncthreads();
         return;
     }
-    int Ns = 1, lNs = 0;
+    int Ns = 1, lNs = 0, toff = 0;
     unsigned rs = p.radices;
     for (int ps = 0; ps < np; ++ps, rs >>= 4) {
         const int R = (int)(rs & 15u);
@@ -387,13 +395,14 @@ __device__ __forceinline__ void fft_line_p2(const FftPlanDev &p, const C2<T> *tw
         int Lv = p.L, tlv = p.tline;
         asm volatile("" : "+s"(Lv), "+s"(tlv));
         if (ps == 0 && ps == np - 1)
-            fft_pass_sel<T, INV, EPT, P2, LDIN, STOUT, true, true, SK>(R, lineG, g, lineS, s_, tw, Lv, tlv, Ns, lNs, ld, st);
+            fft_pass_sel<T, INV, EPT, P2, LDIN, STOUT, true, true, SK>(R, lineG, g, lineS, s_, tw, Lv, tlv, Ns, lNs, toff, ld, st);
         else if (ps == 0)
-            fft_pass_sel<T, INV, EPT, P2, LDIN, STOUT, true, false, SK>(R, lineG, g, lineS, s_, tw, Lv, tlv, Ns, lNs, ld, st);
+            fft_pass_sel<T, INV, EPT, P2, LDIN, STOUT, true, false, SK>(R, lineG, g, lineS, s_, tw, Lv, tlv, Ns, lNs, toff, ld, st);
         else if (ps == np - 1)
-            fft_pass_sel<T, INV, EPT, P2, LDIN, STOUT, false, true, SK>(R, lineG, g, lineS, s_, tw, Lv, tlv, Ns, lNs, ld, st);
+            fft_pass_sel<T, INV, EPT, P2, LDIN, STOUT, false, true, SK>(R, lineG, g, lineS, s_, tw, Lv, tlv, Ns, lNs, toff, ld, st);
         else
-            fft_pass_sel<T, INV, EPT, P2, LDIN, STOUT, false, false, SK>(R, lineG, g, lineS, s_, tw, Lv, tlv, Ns, lNs, ld, st);
+            fft_pass_sel<T, INV, EPT, P2, LDIN, STOUT, false, false, SK>(R, lineG, g, lineS, s_, tw, Lv, tlv, Ns, lNs, toff, ld, st);
+        if (Ns > 1) toff += (R - 1) * Ns;
         Ns *= R;
         lNs += R == 8 ? 3 : (R == 4 ? 2 : 1);
     }
@@ -414,6 +423,16 @@ template <int LC> struct FftLen {
     static constexpr int npass = n8 + (rem ? 1 : 0);
     static constexpr int tline = LC > MM_FFT_EPT ? LC / MM_FFT_EPT : 1;
     static constexpr int radix(int ps) { return ps < n8 ? 8 : (rem == 2 ? 4 : 2); }
+    // offset of pass ps in the per-pass twiddle table
+    static constexpr int toff(int ps)
+    {
+        int o = 0, ns = 1;
+        for (int q = 0; q < ps; ++q) {
+            if (ns > 1) o += (radix(q) - 1) * ns;
+            ns *= radix(q);
+        }
+        return o;
+    }
 };
 
 // line length and threads per line of a kernel instantiated for LC (0: the
@@ -436,7 +455,7 @@ __device__ __forceinline__ void fft_line_c(const C2<T> *tw, C2<T> *lineG, LineMa
         constexpr int R = F::radix(PS);
         constexpr bool I = LDIN && PS == 0, O = STOUT && PS == F::npass - 1;
         fft_pass_io<R, MM_FFT_EPT, T, INV, true, I, O, SK>(lineG, g, lineS, s_, tw, LC, F::tline, NS, mm_ilog2(NS),
-                                                           ld, st);
+                                                           F::toff(PS), ld, st);
         fft_line_c<T, INV, LC, PS + 1, NS * R, LDIN, STOUT, SK>(tw, lineG, g, lineS, s_, ld, st);
     }
 }
