@@ -125,6 +125,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=16, help="RKF45 steps in the CPU-baseline sample")
+    ap.add_argument("--cpu-steps-1t", type=int, default=2, help="RKF45 steps in the single-thread CPU sample")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: all host cores (capped at 16)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
@@ -214,6 +215,13 @@ def main():
                                "unit": "RKF45 steps/s (accepted)", "cores": threads, "kind": "port",
                                "sample": "%d RKF45 steps (%d attempts) of the same 256x256x16 workload, "
                                          "FP64 oracle, %.1f s" % (cb["steps"], cb["attempts"], cb["seconds"])}
+        if args.cpu_steps_1t > 0:
+            # the reference's default single-thread configuration
+            c1 = cpu_baseline(N, M0, args.cpu_steps_1t, 1)
+            out["cpu_baseline"]["single_thread"] = {
+                "value": round(c1["steps"] / c1["seconds"], 5), "cores": 1,
+                "sample": "%d RKF45 steps (%d attempts), FP64 oracle, %.1f s" % (c1["steps"], c1["attempts"],
+                                                                                c1["seconds"])}
         out["gpu_over_cpu"] = round(steps_s / out["cpu_baseline"]["value"], 1)
     print(json.dumps(out))
 

@@ -167,22 +167,41 @@ __global__ void k_xfwd(const TIN *__restrict__ in, size_t in_cstride, C2<T> *__r
     if (sel && *sel) in = in_alt;
     extern __shared__ char smem_raw[];
     C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
-    const int L = PlanC<LC>::L(p);
-    const int l = threadIdx.x / PlanC<LC>::tline(p), t = threadIdx.x % PlanC<LC>::tline(p);
-    const int g = blockIdx.x * lpw + l;
-    const bool active = g < nrows;
-    C2<T> *line = sm + (size_t)l * lds_len<SK>(L);
-    C2<T> *twl = sm + (size_t)lpw * lds_len<SK>(L);
+    const int L = PlanC<LC>::L(p), LP = lds_len<SK>(L), TL = PlanC<LC>::tline(p);
+    // two real rows per complex line: line l of the block holds rows
+    // 2 g and 2 g + 1 (g = blockIdx.x lpw + l) as real and imaginary part
+    const int l = threadIdx.x / TL, t = threadIdx.x % TL;
+    const int g0 = 2 * (blockIdx.x * lpw + l);
+    const bool active = l < lpw && g0 < nrows;
+    const bool has1 = g0 + 1 < nrows;
+    C2<T> *line = sm + (size_t)l * LP;
+    C2<T> *twl = sm + (size_t)lpw * LP;
     load_twiddles<T, LC>(twl, p);
-    const int c = active ? g / (ny * nz) : 0, r = active ? g % (ny * nz) : 0;
-    const TIN *src = in + c * in_cstride + (size_t)r * nx_in;
-    C2<T> *dst = out + c * out_cstride + (size_t)r * Hp;
+    const int rows_c = ny * nz;
+    auto row_src = [&](int g) { return in + (g / rows_c) * in_cstride + (size_t)(g % rows_c) * nx_in; };
+    const TIN *src0 = row_src(active ? g0 : 0);
+    const TIN *src1 = row_src(active && has1 ? g0 + 1 : 0);
     const LineMap m = {t, active};
-    fft_line<T, false, true, true, LC, SK>(
-        p, twl, line, m, line, m, [&](int x) { return cmk<T>(x < nx_in ? (T)src[x] : (T)0, (T)0); },
-        [&](int kx, C2<T> v) {
-            if (kx < Hx) dst[kx] = v;
-        });
+    fft_line<T, false, true, false, LC, SK>(
+        p, twl, line, m, line, m,
+        [&](int x) {
+            const bool in_row = x < nx_in;
+            return cmk<T>(in_row ? (T)src0[x] : (T)0, in_row && has1 ? (T)src1[x] : (T)0);
+        },
+        [](int, C2<T>) {});
+    // X0[k] = (Z[k] + conj Z[L-k]) / 2, X1[k] = (Z[k] - conj Z[L-k]) / 2i
+    for (int e = threadIdx.x; e < lpw * Hx; e += blockDim.x) {
+        const int l2 = e / Hx, k = e % Hx;
+        const int h0 = 2 * (blockIdx.x * lpw + l2);
+        if (h0 >= nrows) continue;
+        const C2<T> *ln = sm + (size_t)l2 * LP;
+        const C2<T> zk = ln[sk<SK>(k)], zm = ln[sk<SK>(k == 0 ? 0 : L - k)];
+        out[(h0 / rows_c) * out_cstride + (size_t)(h0 % rows_c) * Hp + k] =
+            cmk<T>((zk.x + zm.x) * (T)0.5, (zk.y - zm.y) * (T)0.5);
+        if (h0 + 1 < nrows)
+            out[((h0 + 1) / rows_c) * out_cstride + (size_t)((h0 + 1) % rows_c) * Hp + k] =
+                cmk<T>((zk.y + zm.y) * (T)0.5, (zm.x - zk.x) * (T)0.5);
+    }
 }
 
 // y forward: columns kx of planes (c, z); rows y < ny_in read, Py written
@@ -703,7 +722,7 @@ void ConvImpl<T>::init(const double *N_host, int nx, int ny, int nz, int ex, int
     for (int e = 0; e < 6; ++e) {
         MM_HIP(hipMemcpy(dN, Nr.data() + e * plane_full, sizeof(double) * plane_full, hipMemcpyHostToDevice));
         const int nrows = Py * Pz;
-        hipLaunchKernelGGL((k_xfwd<double, double>), dim3((nrows + lpw - 1) / lpw), dim3(lpw * px.tline), shx, 0,
+        hipLaunchKernelGGL((k_xfwd<double, double>), dim3(((nrows + 1) / 2 + lpw - 1) / lpw), dim3(lpw * px.tline), shx, 0,
                            dN, 0, T1, 0, g.Hx, g.Hp, Py, Pz, nrows, Px, dx, lpw, (const double *)nullptr,
                            (const int *)nullptr);
         MM_HIP(hipGetLastError());
@@ -776,7 +795,7 @@ template <typename T> void ConvImpl<T>::forward_x(const T *M, hipStream_t s, con
     const int nrows = 3 * g.ny * g.nz;
     const size_t sh = lines_lds<MM_SKEW_X>(ln.lpw_x, px.L, sizeof(C2<T>));
     mm_with_len(px.L, fft_generic, [&](auto lc) {
-        hipLaunchKernelGGL((k_xfwd<T, T, decltype(lc)::value>), dim3((nrows + ln.lpw_x - 1) / ln.lpw_x),
+        hipLaunchKernelGGL((k_xfwd<T, T, decltype(lc)::value>), dim3(((nrows + 1) / 2 + ln.lpw_x - 1) / ln.lpw_x),
                            dim3(ln.lpw_x * px.tline), sh, s, M, n, S1, (size_t)g.ny * g.nz * g.Hp, g.Hx, g.Hp, g.ny,
                            g.nz, nrows, g.nx, px.dev<T>(), ln.lpw_x, M_alt, sel);
     });

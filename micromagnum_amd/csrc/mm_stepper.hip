@@ -391,19 +391,36 @@ __global__ void k_stage(const SysDev<T> *__restrict__ Sp, const StageDev<T> *__r
     const int nrows = ny * nz;
     const int row0 = (S.xcd_remap ? xcd_tile((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x) * rpw;
     const T *Y = G.Ys ? G.Ys : (ctl->cur ? G.yb1 : G.yb0);
-    const int L = PlanC<LC>::L(p), LP = lds_len<SK>(L);
-    C2<T> *twl = sm + (size_t)3 * rpw * LP;
+    const int L = PlanC<LC>::L(p), LP = lds_len<SK>(L), TL = PlanC<LC>::tline(p);
+    // Rows are transformed in pairs, two real rows per complex line: line
+    // l = (component c, pair q) holds rows gr0 = row0 + 2q (real part) and
+    // gr0 + 1 (imaginary part).  With has_demag == 1, rpw is even.
+    const int np = rpw >> 1;
+    C2<T> *twl = sm + (size_t)3 * np * LP;
     if (has_demag == 1) {
         load_twiddles<T, LC>(twl, p);
-        const int l = threadIdx.x / PlanC<LC>::tline(p), t = threadIdx.x % PlanC<LC>::tline(p);
-        const int c = l / rpw, r = l % rpw;
-        const int gr = row0 + r;
-        const bool active = (l < 3 * rpw) && gr < nrows;
+        // inverse: Z = X0 + i X1 with both half spectra extended Hermitian
+        // (X[L-k] = conj X[k]; X[0], X[L/2] taken real), so that Re and Im of
+        // the inverse transform are the two rows
+        const int l = threadIdx.x / TL, t = threadIdx.x % TL;
+        const int c = l / (np > 0 ? np : 1), q = l % (np > 0 ? np : 1);
+        const int gr0 = row0 + 2 * q;
+        const bool active = (l < 3 * np) && gr0 < nrows;
+        const bool has1 = gr0 + 1 < nrows;
         C2<T> *line = sm + (size_t)l * LP;
-        const C2<T> *src = S1 + ((size_t)c * nrows + (active ? gr : 0)) * Hp;
+        const C2<T> *src0 = S1 + ((size_t)c * nrows + (active ? gr0 : 0)) * Hp;
+        const C2<T> *src1 = src0 + (has1 ? Hp : 0);
         const LineMap m = {t, active};
         fft_line<T, true, true, false, LC, SK>(
-            p, twl, line, m, line, m, [&](int k) { return k < Hx ? src[k] : cconj<T>(src[L - k]); },
+            p, twl, line, m, line, m,
+            [&](int k) {
+                const bool lo = k < Hx;
+                const int kk = lo ? k : L - k;
+                const C2<T> a = src0[kk];
+                const C2<T> b = has1 ? src1[kk] : cmk<T>((T)0, (T)0);
+                if (k == 0 || 2 * k == L) return cmk<T>(a.x, b.x);
+                return lo ? cmk<T>(a.x - b.y, a.y + b.x) : cmk<T>(a.x + b.y, b.x - a.y);
+            },
             [](int, C2<T>) {});
     }
     StageRed red = {0.0, 0.0, 0.0, 0.0};
@@ -412,39 +429,56 @@ __global__ void k_stage(const SysDev<T> *__restrict__ Sp, const StageDev<T> *__r
     for (int e = threadIdx.x; e < rpw * nx; e += blockDim.x) {
         const int r = e / nx, x = e % nx;
         const int gr = row0 + r;
-        if (gr >= nrows) continue;
+        const int q = r >> 1, hi = r & 1;
+        if (gr >= nrows) {
+            // the absent second row of a pair transforms as zeros
+            if (fused)
+                for (int c = 0; c < 3; ++c) reinterpret_cast<T *>(&sm[(c * np + q) * LP + sk<SK>(x)])[hi] = (T)0;
+            continue;
+        }
         V3<T> Hd = v3<T>(0, 0, 0);
         if (has_demag == 1) {
-            Hd = v3<T>(sm[(0 * rpw + r) * LP + sk<SK>(x)].x, sm[(1 * rpw + r) * LP + sk<SK>(x)].x, sm[(2 * rpw + r) * LP + sk<SK>(x)].x);
+            T h3[3];
+            for (int c = 0; c < 3; ++c) h3[c] = reinterpret_cast<const T *>(&sm[(c * np + q) * LP + sk<SK>(x)])[hi];
+            Hd = v3<T>(h3[0], h3[1], h3[2]);
         } else if (has_demag == 2) {
             const size_t i = (size_t)gr * nx + x;
             Hd = v3<T>(Hdem[i], Hdem[S.n + i], Hdem[2 * S.n + i]);
         }
         T nxt[3];
         stage_cell<T, EN, STT, GEN>(S, G, ctl, Y, x, gr % ny, gr / ny, Hd, red, ea, nxt);
-        // the slot (r, x) of the three lines is this thread's alone: the next
-        // stage input replaces H_demag in place
+        // the half slot (row, x) of the three lines is this thread's alone:
+        // the next stage input replaces H_demag in place
         if (fused)
-            for (int c = 0; c < 3; ++c) sm[(c * rpw + r) * LP + sk<SK>(x)] = cmk<T>(nxt[c], (T)0);
+            for (int c = 0; c < 3; ++c) reinterpret_cast<T *>(&sm[(c * np + q) * LP + sk<SK>(x)])[hi] = nxt[c];
     }
     if (fused) {
-        for (int e = threadIdx.x; e < 3 * rpw * (L - nx); e += blockDim.x) {
+        for (int e = threadIdx.x; e < 3 * np * (L - nx); e += blockDim.x) {
             const int l = e / (L - nx), x = nx + e % (L - nx);
             sm[(size_t)l * LP + sk<SK>(x)] = cmk<T>((T)0, (T)0);
         }
         __syncthreads();
-        const int l = threadIdx.x / PlanC<LC>::tline(p), t = threadIdx.x % PlanC<LC>::tline(p);
-        const int c = l / rpw, r = l % rpw;
-        const int gr = row0 + r;
-        const bool active = (l < 3 * rpw) && gr < nrows;
-        C2<T> *line = sm + (size_t)l * LP;
-        C2<T> *dst = G.fwd_S1 + ((size_t)c * nrows + (active ? gr : 0)) * Hp;
-        const LineMap m = {t, active};
-        fft_line<T, false, false, true, LC, SK>(
-            p, twl, line, m, line, m, [](int) { return cmk<T>((T)0, (T)0); },
-            [&](int k, C2<T> v) {
-                if (k < Hx) dst[k] = v;
-            });
+        {
+            const int l = threadIdx.x / TL, t = threadIdx.x % TL;
+            const bool active = (l < 3 * np) && row0 + 2 * (l % (np > 0 ? np : 1)) < nrows;
+            C2<T> *line = sm + (size_t)l * LP;
+            const LineMap m = {t, active};
+            fft_line<T, false, false, false, LC, SK>(
+                p, twl, line, m, line, m, [](int) { return cmk<T>((T)0, (T)0); }, [](int, C2<T>) {});
+        }
+        // forward: Z = FFT(x0 + i x1); X0[k] = (Z[k] + conj Z[L-k]) / 2,
+        // X1[k] = (Z[k] - conj Z[L-k]) / 2i for k < Hx
+        for (int e = threadIdx.x; e < 3 * np * Hx; e += blockDim.x) {
+            const int l = e / Hx, k = e % Hx;
+            const int c = l / np, q = l % np;
+            const int gr0 = row0 + 2 * q;
+            if (gr0 >= nrows) continue;
+            const C2<T> *ln = sm + (size_t)l * LP;
+            const C2<T> zk = ln[sk<SK>(k)], zm = ln[sk<SK>(k == 0 ? 0 : L - k)];
+            C2<T> *d0 = G.fwd_S1 + ((size_t)c * nrows + gr0) * Hp;
+            d0[k] = cmk<T>((zk.x + zm.x) * (T)0.5, (zk.y - zm.y) * (T)0.5);
+            if (gr0 + 1 < nrows) d0[(size_t)Hp + k] = cmk<T>((zk.y + zm.y) * (T)0.5, (zm.x - zk.x) * (T)0.5);
+        }
     }
     __shared__ double wred[4][16];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (int)((blockDim.x + 63) >> 6);
@@ -635,30 +669,45 @@ __global__ void k_xfwd_prep(const T *__restrict__ y0, const T *__restrict__ y1, 
     C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
     const T *y = ctl->cur ? y1 : y0;
     const T f = (T)(ctl->h * b10);
-    const int L = PlanC<LC>::L(p);
+    const int L = PlanC<LC>::L(p), LP = lds_len<SK>(L), TL = PlanC<LC>::tline(p);
     const int nrows = 3 * ny * nz;
-    const int l = threadIdx.x / PlanC<LC>::tline(p), t = threadIdx.x % PlanC<LC>::tline(p);
-    const int g = blockIdx.x * lpw + l;
-    const bool active = g < nrows;
-    C2<T> *line = sm + (size_t)l * lds_len<SK>(L);
-    C2<T> *twl = sm + (size_t)lpw * lds_len<SK>(L);
+    // two real rows per complex line (rows 2 g and 2 g + 1, real and
+    // imaginary part); S1 rows and `out` rows follow the row index
+    const int l = threadIdx.x / TL, t = threadIdx.x % TL;
+    const int g0 = 2 * (blockIdx.x * lpw + l);
+    const bool active = l < lpw && g0 < nrows;
+    const bool has1 = g0 + 1 < nrows;
+    C2<T> *line = sm + (size_t)l * LP;
+    C2<T> *twl = sm + (size_t)lpw * LP;
     load_twiddles<T, LC>(twl, p);
-    const size_t base = (size_t)(active ? g : 0) * nx;
-    C2<T> *dst = S1 + (size_t)(active ? g : 0) * Hp;
+    const size_t base0 = (size_t)(active ? g0 : 0) * nx;
+    const size_t base1 = base0 + nx;
     const LineMap m = {t, active};
-    fft_line<T, false, true, true, LC, SK>(
+    fft_line<T, false, true, false, LC, SK>(
         p, twl, line, m, line, m,
         [&](int x) {
-            T v = (T)0;
+            T v0 = (T)0, v1 = (T)0;
             if (x < nx) {
-                v = y[base + x] + k0[base + x] * f;
-                out[base + x] = v;
+                v0 = y[base0 + x] + k0[base0 + x] * f;
+                out[base0 + x] = v0;
+                if (has1) {
+                    v1 = y[base1 + x] + k0[base1 + x] * f;
+                    out[base1 + x] = v1;
+                }
             }
-            return cmk<T>(v, (T)0);
+            return cmk<T>(v0, v1);
         },
-        [&](int kx, C2<T> v) {
-            if (kx < Hx) dst[kx] = v;
-        });
+        [](int, C2<T>) {});
+    // X0[k] = (Z[k] + conj Z[L-k]) / 2, X1[k] = (Z[k] - conj Z[L-k]) / 2i
+    for (int e = threadIdx.x; e < lpw * Hx; e += blockDim.x) {
+        const int l2 = e / Hx, k = e % Hx;
+        const int h0 = 2 * (blockIdx.x * lpw + l2);
+        if (h0 >= nrows) continue;
+        const C2<T> *ln = sm + (size_t)l2 * LP;
+        const C2<T> zk = ln[sk<SK>(k)], zm = ln[sk<SK>(k == 0 ? 0 : L - k)];
+        S1[(size_t)h0 * Hp + k] = cmk<T>((zk.x + zm.x) * (T)0.5, (zk.y - zm.y) * (T)0.5);
+        if (h0 + 1 < nrows) S1[(size_t)(h0 + 1) * Hp + k] = cmk<T>((zk.y + zm.y) * (T)0.5, (zm.x - zk.x) * (T)0.5);
+    }
 }
 
 template <typename T> __global__ void k_cast_in(size_t n, const double *__restrict__ a, T *__restrict__ b)
@@ -914,11 +963,14 @@ template <typename T> struct Stepper : public StepperBase {
                 stage_lds = 0;
                 Hdem = dalloc(3 * n);
             } else {
-                rpw = std::max(1, 192 / (3 * tl));
-                if (const char *e = std::getenv("MM_RPW")) rpw = std::max(1, std::atoi(e));
-                while (rpw > 1 && ((size_t)3 * rpw * lds_len<MM_SKEW_X>((int)L) + L) * cs > (size_t)mm_max_lds_bytes()) --rpw;
-                threads = 3 * rpw * tl;
-                stage_lds = ((size_t)3 * rpw * lds_len<MM_SKEW_X>((int)L) + L) * cs;
+                // rows go through the x transforms in pairs (two real rows per
+                // complex line): np pairs, 3 np lines, rpw = 2 np rows per block
+                int np = std::max(1, 192 / (3 * tl));
+                if (const char *e = std::getenv("MM_RPW")) np = std::max(1, std::atoi(e) / 2);
+                while (np > 1 && ((size_t)3 * np * lds_len<MM_SKEW_X>((int)L) + L) * cs > (size_t)mm_max_lds_bytes()) --np;
+                rpw = 2 * np;
+                threads = 3 * np * tl;
+                stage_lds = ((size_t)3 * np * lds_len<MM_SKEW_X>((int)L) + L) * cs;
             }
         } else {
             rpw = std::max(1, 256 / d.nx);
@@ -927,13 +979,13 @@ template <typename T> struct Stepper : public StepperBase {
         }
         threads = std::max(64, ((threads + 63) / 64) * 64);
         if (conv && !split_x) {
-            // one epilogue cell per thread: the workgroup covers its rpw rows
-            // (threads beyond the 3 rpw x transform lines idle in the FFT
-            // passes).  Measured on MI355X, 256x256x16 cells FP32: 62 us per
-            // stage with 256 threads against 67 us with the 192 the
-            // transforms use.  MM_STAGE_THREADS overrides (measurements).
+            // two epilogue cells per thread, the two rows of a pair at one x
+            // (threads beyond the 3 np x transform lines idle in the FFT
+            // passes).  Measured on MI355X, 256x256x16 cells: 256 threads per
+            // 2-row block against 512 (one cell each), stage 44.8 vs 49.4 us
+            // FP32 and 74.9 vs 83.8 us FP64.  MM_STAGE_THREADS overrides.
             const int fft_threads = threads;
-            threads = std::max(fft_threads, std::min(1024, (rpw * d.nx + 63) / 64 * 64));
+            threads = std::max(fft_threads, std::min(1024, (rpw * d.nx / 2 + 63) / 64 * 64));
             if (const char *e = std::getenv("MM_STAGE_THREADS"))
                 threads = std::min(1024, std::max(fft_threads, (std::atoi(e) + 63) / 64 * 64));
         }
@@ -1164,7 +1216,7 @@ template <typename T> struct Stepper : public StepperBase {
                 const int nrows = 3 * d.ny * d.nz;
                 const size_t sh = ((size_t)lpw * lds_len<MM_SKEW_X>(px.L) + px.L) * sizeof(C2<T>);
                 mm_with_len(px.L, conv->fft_generic, [&](auto lc) {
-                    hipLaunchKernelGGL((k_xfwd_prep<T, decltype(lc)::value>), dim3((nrows + lpw - 1) / lpw),
+                    hipLaunchKernelGGL((k_xfwd_prep<T, decltype(lc)::value>), dim3(((nrows + 1) / 2 + lpw - 1) / lpw),
                                        dim3(lpw * px.tline), sh, s, yb[0], yb[1], k[0], tmp[0], conv->S1, ctl,
                                        tab.b[1][0], d.nx, d.ny, d.nz, conv->g.Hx, conv->g.Hp, px.template dev<T>(),
                                        lpw);
