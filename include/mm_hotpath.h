@@ -185,6 +185,12 @@ typedef struct {
     int ex, ey, ez;
     int px, py, pz;         /* periodic axes (exchange neighbours wrap; the tensor
                                must be generated with the same flags)           */
+    /* AlternatingField (alternating_field.py:57-84): with use_ac the Zeeman
+     * field is h_ext + h_amp * sin(t * h_freq + h_phase) per component, at the
+     * time of each evaluation (RK stage s of an attempt from t with step h:
+     * t + a_s h, runge_kutta.py:115).  use_zeeman must be set too. */
+    int use_ac;
+    double h_amp[3], h_freq[3], h_phase[3];
 } mm_system_desc_t;
 
 typedef struct mm_stepper mm_stepper_t;

@@ -47,7 +47,8 @@ class SystemDesc(ctypes.Structure):
                 ("P_host", _dp), ("xi_host", _dp), ("j_host", _dp),
                 ("use_zeeman", _i), ("h_ext", _d * 3),
                 ("N_host", _dp), ("ex", _i), ("ey", _i), ("ez", _i),
-                ("px", _i), ("py", _i), ("pz", _i)]
+                ("px", _i), ("py", _i), ("pz", _i),
+                ("use_ac", _i), ("h_amp", _d * 3), ("h_freq", _d * 3), ("h_phase", _d * 3)]
 
 
 # name -> (restype, argtypes); every symbol of include/mm_hotpath.h

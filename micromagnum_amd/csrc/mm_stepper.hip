@@ -79,6 +79,9 @@ template <typename T> struct SysDev {
     VParam<T> axis1, axis2, j;
     int has_exch, has_uni, has_cub, has_zee, has_stt, do_precess;
     T hext[3];
+    // alternating Zeeman field: hoff + ac_amp sin(t ac_freq + ac_phase)
+    int has_ac;
+    double hoff[3], ac_amp[3], ac_freq[3], ac_phase[3];
     int Ms_uniform;
     double Ms_u;
     // exchange with uniform, non-zero Ms and uniform A: no masking, no
@@ -159,6 +162,9 @@ template <typename T> struct StageDev {
     // [2] |k0|^2, [3] |y|^2, one entry per block of the launch; the
     // controller reduces them (no same-address atomics across the grid)
     double *mpart;
+    // node a_s of this stage: the stage is evaluated at t + a_s h (the
+    // alternating Zeeman field depends on it)
+    double tnode;
 };
 
 
@@ -187,7 +193,7 @@ struct EnergyAcc {
 template <typename T, bool EN, bool STT, bool GEN>
 __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T> &G, const StepCtl *ctl,
                                            const T *__restrict__ Y, int x, int y, int z, const V3<T> &Hd,
-                                           StageRed &red, EnergyAcc &ea, T *nxt)
+                                           const V3<T> &hz, StageRed &red, EnergyAcc &ea, T *nxt)
 {
     const size_t n = S.n;
     const size_t i = (size_t)x + (size_t)S.m.nx * ((size_t)y + (size_t)S.m.ny * z);
@@ -226,10 +232,10 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
         H.z += ha.z;
     }
     if (S.has_zee) {
-        H.x += S.hext[0];
-        H.y += S.hext[1];
-        H.z += S.hext[2];
-        if (EN) ea.zee += (double)Mv.x * S.hext[0] + (double)Mv.y * S.hext[1] + (double)Mv.z * S.hext[2];
+        H.x += hz.x;
+        H.y += hz.y;
+        H.z += hz.z;
+        if (EN) ea.zee += (double)Mv.x * hz.x + (double)Mv.y * hz.y + (double)Mv.z * hz.z;
     }
     if (EN) return;
     V3<T> k = llge_cell<T>(Mv, H, S.f1.at(i), S.f2.at(i));
@@ -384,7 +390,9 @@ __global__ void k_stage(const SysDev<T> *__restrict__ Sp, const StageDev<T> *__r
     const SysDev<T> &S = *Sp;
     const StageDev<T> &G = *Gp;
     constexpr int SK = MM_SKEW_X;
-    if (!EN && ctl->stop) return;
+    // stages of an attempt (prep != 0) do nothing once the run has stopped;
+    // evaluations outside attempts (dM/dt, energies) always run
+    if (!EN && G.prep && ctl->stop) return;
     extern __shared__ char smem_raw[];
     C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
     const int nx = S.m.nx, ny = S.m.ny, nz = S.m.nz;
@@ -426,6 +434,15 @@ __global__ void k_stage(const SysDev<T> *__restrict__ Sp, const StageDev<T> *__r
     StageRed red = {0.0, 0.0, 0.0, 0.0};
     EnergyAcc ea = {0.0, 0.0, 0.0, 0.0};
     const bool fused = !EN && has_demag == 1 && G.fwd;
+    // Zeeman field of this evaluation (alternating_field.py:57-84), at the
+    // stage time t + a_s h
+    V3<T> hz = v3<T>(S.hext[0], S.hext[1], S.hext[2]);
+    if (S.has_zee && S.has_ac) {
+        const double ts = ctl->t + G.tnode * ctl->h;
+        hz = v3<T>((T)(S.hoff[0] + S.ac_amp[0] * sin(ts * S.ac_freq[0] + S.ac_phase[0])),
+                   (T)(S.hoff[1] + S.ac_amp[1] * sin(ts * S.ac_freq[1] + S.ac_phase[1])),
+                   (T)(S.hoff[2] + S.ac_amp[2] * sin(ts * S.ac_freq[2] + S.ac_phase[2])));
+    }
     for (int e = threadIdx.x; e < rpw * nx; e += blockDim.x) {
         const int r = e / nx, x = e % nx;
         const int gr = row0 + r;
@@ -446,7 +463,7 @@ __global__ void k_stage(const SysDev<T> *__restrict__ Sp, const StageDev<T> *__r
             Hd = v3<T>(Hdem[i], Hdem[S.n + i], Hdem[2 * S.n + i]);
         }
         T nxt[3];
-        stage_cell<T, EN, STT, GEN>(S, G, ctl, Y, x, gr % ny, gr / ny, Hd, red, ea, nxt);
+        stage_cell<T, EN, STT, GEN>(S, G, ctl, Y, x, gr % ny, gr / ny, Hd, hz, red, ea, nxt);
         // the half slot (row, x) of the three lines is this thread's alone:
         // the next stage input replaces H_demag in place
         if (fused)
@@ -939,6 +956,13 @@ template <typename T> struct Stepper : public StepperBase {
             sys.axis2 = vparam(d.axis2_host);
         }
         sys.has_zee = d.use_zeeman;
+        sys.has_ac = d.use_zeeman && d.use_ac;
+        for (int c = 0; c < 3; ++c) {
+            sys.hoff[c] = d.h_ext[c];
+            sys.ac_amp[c] = d.h_amp[c];
+            sys.ac_freq[c] = d.h_freq[c];
+            sys.ac_phase[c] = d.h_phase[c];
+        }
         for (int c = 0; c < 3; ++c) sys.hext[c] = (T)d.h_ext[c];
         sys.has_stt = d.P_host != nullptr;
         sys.do_precess = d.do_precess;
@@ -1128,6 +1152,7 @@ template <typename T> struct Stepper : public StepperBase {
         const bool fsal = tab.fsal != 0;
         const T *Ys = st == 0 ? nullptr : tmp[(st - 1) & 1];
         StageDev<T> G = blank_stage(Ys);
+        G.tnode = tab.a[st];
         G.kout = (st < ns - 1) ? k[st] : nullptr;
         G.want_dydt = (st == 0) && !fsal;
         if (conv && !split_x && st < ns - 1) {

@@ -33,7 +33,10 @@ class SystemSpec:
                  axis1=None, axis2=None, h_ext=None, P=None, xi=None, j=None, do_precess=True,
                  method="rkf45", eps_abs=1e-4, eps_rel=1e-4, h0=1e-13, precision="fp64",
                  normalize_mode=0, padding=PADDING_ROUND_4, pbc=(False, False, False), periodic_repeat=None,
-                 cache_dir=None):
+                 cache_dir=None, h_ext_amp=None, h_ext_freq=None, h_ext_phase=None):
+        """h_ext: the Zeeman field (the offset of the reference's
+        ExternalField); h_ext_amp, h_ext_freq, h_ext_phase (3-tuples) add
+        amp sin(t freq + phase) per component (alternating_field.py:57-84)."""
         nx, ny, nz = shape
         self.shape = tuple(shape)
         self.delta = tuple(float(v) for v in delta)
@@ -65,10 +68,16 @@ class SystemSpec:
         d.P_host = f(P)
         d.xi_host = f(xi)
         d.j_host = f(j, 3)
-        if h_ext is not None:
+        if h_ext is not None or h_ext_amp is not None:
             d.use_zeeman = 1
             for c in range(3):
-                d.h_ext[c] = float(h_ext[c])
+                d.h_ext[c] = float(h_ext[c]) if h_ext is not None else 0.0
+        if h_ext_amp is not None:
+            d.use_ac = 1
+            for c in range(3):
+                d.h_amp[c] = float(h_ext_amp[c])
+                d.h_freq[c] = float(h_ext_freq[c]) if h_ext_freq is not None else 0.0
+                d.h_phase[c] = float(h_ext_phase[c]) if h_ext_phase is not None else 0.0
         d.px, d.py, d.pz = (int(bool(b)) for b in pbc)
         if periodic_repeat is None:
             # RectangularMesh defaults (src/magnum/mesh/rectangular_mesh.py:27-31)

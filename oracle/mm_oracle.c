@@ -777,7 +777,17 @@ void orc_llge_factors(const orc_system *s, double *f1, double *f2)
 
 /* H_tot = 0 + H_stray + H_exch + H_aniso + H_ext (landau_lifshitz_gilbert.py:92-101,
  * module order of the list given to create_solver; anisotropy_field.py:41-66). */
-void orc_h_tot(const orc_system *s, const double *M, double *H_tot)
+/* Zeeman field component c at time t: ExternalField = AlternatingField
+ * (external_field.py:21-37, alternating_field.py:57-84) */
+static double zeeman_at(const orc_system *s, int c, double t)
+{
+    if (!s->use_ac) return s->h_ext[c];
+    return s->h_ext[c] + s->h_amp[c] * sin(t * s->h_freq[c] + s->h_phase[c]);
+}
+
+void orc_h_tot(const orc_system *s, const double *M, double *H_tot) { orc_h_tot_at(s, M, 0.0, H_tot); }
+
+void orc_h_tot_at(const orc_system *s, const double *M, double t, double *H_tot)
 {
     const int n = s->nx * s->ny * s->nz;
     const long m = 3L * n;
@@ -805,8 +815,10 @@ void orc_h_tot(const orc_system *s, const double *M, double *H_tot)
         for (long i = 0; i < m; ++i) H_tot[i] += tmp[i];
     }
     if (s->use_zeeman) {
-        for (int c = 0; c < 3; ++c)
-            for (int i = 0; i < n; ++i) H_tot[c * (long)n + i] += s->h_ext[c];
+        for (int c = 0; c < 3; ++c) {
+            const double hz = zeeman_at(s, c, t);
+            for (int i = 0; i < n; ++i) H_tot[c * (long)n + i] += hz;
+        }
     }
     free(tmp); free(tmp2);
 }
@@ -815,10 +827,16 @@ void orc_h_tot(const orc_system *s, const double *M, double *H_tot)
 void orc_dmdt(const orc_system *s, const double *f1, const double *f2,
               const double *M, double *dMdt, double *H_tot_out)
 {
+    orc_dmdt_at(s, f1, f2, M, 0.0, dMdt, H_tot_out);
+}
+
+void orc_dmdt_at(const orc_system *s, const double *f1, const double *f2, const double *M, double t,
+                 double *dMdt, double *H_tot_out)
+{
     const int n = s->nx * s->ny * s->nz;
     const long m = 3L * n;
     double *H = H_tot_out ? H_tot_out : (double *)malloc(sizeof(double) * m);
-    orc_h_tot(s, M, H);
+    orc_h_tot_at(s, M, t, H);
     orc_llge(n, f1, f2, M, H, dMdt);
     if (s->P) {
         double *st = (double *)malloc(sizeof(double) * m);
@@ -893,17 +911,17 @@ static int relax_test(relax_state *r, const orc_system *s, int n, const double *
  * accepted or not (runge_kutta.py:97-101,131-134). */
 static void apply_attempt(const orc_system *s, int method, const orc_tableau *t, const double *f1,
                           const double *f2, double **k, double *M, double *ytmp, double *yerr,
-                          double *fsal, int *have_fsal, double h, int k0_given, int n)
+                          double *fsal, int *have_fsal, double t0, double h, int k0_given, int n)
 {
     const long m = 3L * n;
     if (!k0_given) {
         if (method != ORC_RK4 && t->fsal && *have_fsal) memcpy(k[0], fsal, sizeof(double) * m);
-        else orc_dmdt(s, f1, f2, M, k[0], NULL);
+        else orc_dmdt_at(s, f1, f2, M, t0, k[0], NULL);
     }
     if (method == ORC_RK4) {
         for (int sg = 1; sg < 4; ++sg) {
             orc_rk_prepare_step(sg, h, t, k, M, ytmp, n);
-            orc_dmdt(s, f1, f2, ytmp, k[sg], NULL);
+            orc_dmdt_at(s, f1, f2, ytmp, t0 + h * t->a[sg], k[sg], NULL);
         }
         for (int sg = 0; sg < 4; ++sg) {
             const double f = h * (sg == 0 || sg == 3 ? 1.0 : 2.0) / 6.0;
@@ -913,7 +931,7 @@ static void apply_attempt(const orc_system *s, int method, const orc_tableau *t,
     }
     for (int sg = 1; sg < t->num_steps; ++sg) {
         orc_rk_prepare_step(sg, h, t, k, M, ytmp, n);
-        orc_dmdt(s, f1, f2, ytmp, k[sg], NULL);
+        orc_dmdt_at(s, f1, f2, ytmp, t0 + h * t->a[sg], k[sg], NULL);
     }
     orc_rk_combine_result(h, t, k, M, yerr, n);
     if (t->fsal) {
@@ -954,7 +972,7 @@ static long run_impl(const orc_system *s, int method, double eps_abs, double eps
         int k0_ready = 0;
         if (rx && st % rx->every == 0) {
             if (method != ORC_RK4 && t.fsal && have_fsal) memcpy(k[0], fsal, sizeof(double) * m);
-            else orc_dmdt(s, f1, f2, M, k[0], NULL);
+            else orc_dmdt_at(s, f1, f2, M, tnow, k[0], NULL);
             k0_ready = 1;
             if (relax_test(rx, s, n, k[0], M, h_last)) break;
         }
@@ -962,7 +980,7 @@ static long run_impl(const orc_system *s, int method, double eps_abs, double eps
         for (;;) {
             double h = method == ORC_RK4 ? h0 : h_try;
             attempts++;
-            apply_attempt(s, method, &t, f1, f2, k, M, ytmp, yerr, fsal, &have_fsal, h, k0_ready, n);
+            apply_attempt(s, method, &t, f1, f2, k, M, ytmp, yerr, fsal, &have_fsal, tnow, h, k0_ready, n);
             k0_ready = 0;
             double hn = h;
             int acc = 1;
@@ -981,7 +999,7 @@ static long run_impl(const orc_system *s, int method, double eps_abs, double eps
                 h = t_max - tnow;
                 memcpy(M, y0, sizeof(double) * m);
                 attempts++;
-                apply_attempt(s, method, &t, f1, f2, k, M, ytmp, yerr, fsal, &have_fsal, h, 0, n);
+                apply_attempt(s, method, &t, f1, f2, k, M, ytmp, yerr, fsal, &have_fsal, tnow, h, 0, n);
             }
             tnow += h; h_last = h; h_next = hn;
             h_try = hn;
@@ -1033,7 +1051,9 @@ long orc_relax(const orc_system *s, int method, double eps_abs, double eps_rel, 
  * anisotropy_field.py:84-87 (uniaxial + cubic summed), external_field.py:33-35,
  * E_tot landau_lifshitz_gilbert.py:103-106; MU0 as the Python front-end's
  * 4.0*math.pi*1e-7 (constants.py:21). */
-void orc_energy(const orc_system *s, const double *M, double *E5)
+void orc_energy(const orc_system *s, const double *M, double *E5) { orc_energy_at(s, M, 0.0, E5); }
+
+void orc_energy_at(const orc_system *s, const double *M, double t, double *E5)
 {
     const int n = s->nx * s->ny * s->nz;
     const long m = 3L * n;
@@ -1056,8 +1076,10 @@ void orc_energy(const orc_system *s, const double *M, double *E5)
         E5[2] = e * V;
     }
     if (s->use_zeeman) {
-        for (int c = 0; c < 3; ++c)
-            for (int i = 0; i < n; ++i) tmp[c * (long)n + i] = s->h_ext[c];
+        for (int c = 0; c < 3; ++c) {
+            const double hz = zeeman_at(s, c, t);
+            for (int i = 0; i < n; ++i) tmp[c * (long)n + i] = hz;
+        }
         E5[3] = -mu0 * V * orc_dot_sum(n, M, tmp);
     }
     E5[4] = E5[0] + E5[1] + E5[2] + E5[3];

@@ -90,12 +90,22 @@ typedef struct {
     const double *P, *xi, *j;     /* NULL P: no spin torque                         */
     int do_precess;
     int normalize_mode;           /* 0 reference float-norm, 1 fp64                 */
+    /* AlternatingField (alternating_field.py:57-84): with use_ac the Zeeman
+     * field is h_ext + h_amp * sin(t * h_freq + h_phase) per component, at
+     * the time t of the evaluation (stage s of an RK attempt from t0 with
+     * step h: t0 + a_s h, runge_kutta.py:115) */
+    int use_ac;
+    double h_amp[3], h_freq[3], h_phase[3];
 } orc_system;
 
 void orc_llge_factors(const orc_system *s, double *f1, double *f2);
 void orc_h_tot(const orc_system *s, const double *M, double *H_tot);
 void orc_dmdt(const orc_system *s, const double *f1, const double *f2,
               const double *M, double *dMdt, double *H_tot_out /* may be NULL */);
+/* the same at time t (orc_h_tot / orc_dmdt evaluate at t = 0) */
+void orc_h_tot_at(const orc_system *s, const double *M, double t, double *H_tot);
+void orc_dmdt_at(const orc_system *s, const double *f1, const double *f2, const double *M, double t,
+                 double *dMdt, double *H_tot_out);
 
 /* Adaptive RK (rkf45/cc45/dp54/rk23) driven by MicroMagneticsStepSizeController,
  * or fixed-step classical RK4.  Performs `steps` accepted steps in place on M.
@@ -119,6 +129,7 @@ long orc_run_until(const orc_system *s, int method, double eps_abs, double eps_r
 
 /* E_stray, E_exch, E_aniso, E_ext, E_tot at M [J]. */
 void orc_energy(const orc_system *s, const double *M, double *E5);
+void orc_energy_at(const orc_system *s, const double *M, double t, double *E5);
 
 #ifdef __cplusplus
 }

@@ -200,7 +200,9 @@ class _System(ctypes.Structure):
                 ("k_uni", _dp), ("k_cub", _dp), ("axis1", _dp), ("axis2", _dp),
                 ("use_zeeman", ctypes.c_int), ("h_ext", ctypes.c_double * 3),
                 ("P", _dp), ("xi", _dp), ("j", _dp), ("do_precess", ctypes.c_int),
-                ("normalize_mode", ctypes.c_int)]
+                ("normalize_mode", ctypes.c_int), ("use_ac", ctypes.c_int),
+                ("h_amp", ctypes.c_double * 3), ("h_freq", ctypes.c_double * 3),
+                ("h_phase", ctypes.c_double * 3)]
 
 
 class System:
@@ -209,7 +211,10 @@ class System:
 
     def __init__(self, shape, delta, Ms, alpha, A=None, N=None, k_uni=None, k_cub=None,
                  axis1=None, axis2=None, h_ext=None, P=None, xi=None, j=None, do_precess=True,
-                 normalize_mode=0, pbc=(False, False, False)):
+                 normalize_mode=0, pbc=(False, False, False), h_ext_amp=None, h_ext_freq=None,
+                 h_ext_phase=None):
+        """h_ext is the Zeeman offset; h_ext_amp/freq/phase (3-tuples) make it
+        the reference's AlternatingField, h_ext + amp sin(t freq + phase)."""
         nx, ny, nz = shape
         self._keep = []
 
@@ -241,10 +246,16 @@ class System:
         s.k_cub = _p(field(k_cub))
         s.axis1 = _p(field(axis1, 3))
         s.axis2 = _p(field(axis2, 3))
-        if h_ext is not None:
+        if h_ext is not None or h_ext_amp is not None:
             s.use_zeeman = 1
             for c in range(3):
-                s.h_ext[c] = float(h_ext[c])
+                s.h_ext[c] = float(h_ext[c]) if h_ext is not None else 0.0
+        if h_ext_amp is not None:
+            s.use_ac = 1
+            for c in range(3):
+                s.h_amp[c] = float(h_ext_amp[c])
+                s.h_freq[c] = float(h_ext_freq[c]) if h_ext_freq is not None else 0.0
+                s.h_phase[c] = float(h_ext_phase[c]) if h_ext_phase is not None else 0.0
         s.P = _p(field(P))
         s.xi = _p(field(xi))
         s.j = _p(field(j, 3))
@@ -260,18 +271,18 @@ class System:
         lib().orc_llge_factors(ctypes.byref(self.s), _p(f1), _p(f2))
         return f1, f2
 
-    def h_tot(self, M):
+    def h_tot(self, M, t=0.0):
         M = _c(M)
         H = np.empty_like(M)
-        lib().orc_h_tot(ctypes.byref(self.s), _p(M), _p(H))
+        lib().orc_h_tot_at(ctypes.byref(self.s), _p(M), ctypes.c_double(t), _p(H))
         return H
 
-    def dmdt(self, M):
+    def dmdt(self, M, t=0.0):
         M = _c(M)
         f1, f2 = self.llge_factors()
         dM = np.empty_like(M)
         H = np.empty_like(M)
-        lib().orc_dmdt(ctypes.byref(self.s), _p(f1), _p(f2), _p(M), _p(dM), _p(H))
+        lib().orc_dmdt_at(ctypes.byref(self.s), _p(f1), _p(f2), _p(M), ctypes.c_double(t), _p(dM), _p(H))
         return dM, H
 
     def run(self, M, method="rkf45", steps=1, eps_abs=1e-4, eps_rel=1e-4, h0=1e-13,
@@ -290,11 +301,11 @@ class System:
 
     ENERGY_TERMS = ("E_stray", "E_exch", "E_aniso", "E_ext", "E_tot")
 
-    def energy(self, M):
-        """E_stray, E_exch, E_aniso, E_ext, E_tot at M (joule)."""
+    def energy(self, M, t=0.0):
+        """E_stray, E_exch, E_aniso, E_ext, E_tot at M and time t (joule)."""
         M = _c(M)
         e = np.zeros(5)
-        lib().orc_energy(ctypes.byref(self.s), _p(M), _p(e))
+        lib().orc_energy_at(ctypes.byref(self.s), _p(M), ctypes.c_double(t), _p(e))
         return dict(zip(self.ENERGY_TERMS, e.tolist()))
 
     def relax(self, M, method="rkf45", threshold=1e7, criterion="dmdt", check_every=1,

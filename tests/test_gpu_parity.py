@@ -428,6 +428,38 @@ def test_stepper_fsal_trajectory_fp64_vs_oracle(method):
 
 
 @pytest.mark.parametrize("method", ["rkf45", "dp54", "rk4"])
+def test_stepper_alternating_field_vs_oracle(method):
+    # Zeeman field offs + amp sin(t freq + phase) (alternating_field.py:57-84)
+    # evaluated at every stage's time t + a_s h; then dM/dt and the energies
+    # at the reached time
+    from micromagnum_amd.solver import Stepper
+    shape, delta = (20, 16, 2), (3e-9, 3e-9, 2e-9)
+    ac = dict(h_ext=(1e4, 0.0, -5e3), h_ext_amp=(2e4, 3e4, 5e4), h_ext_freq=(2e10, 3e10, 5e10),
+              h_ext_phase=(0.1, -0.7, 1.2))
+    kw = dict(Ms=8.6e5, alpha=0.1, A=1.3e-11, **ac)
+    # (h0 1e-13: no rejections, so DP54's carried-over k_0 keeps the
+    # trajectory well conditioned; dM/dt after the run also checks that
+    # evaluations outside an attempt ignore the finished run's stop flag)
+    st = Stepper(shape, delta, precision="fp64", method=method, eps_abs=1e-5, eps_rel=1e-5, h0=1e-13, **kw)
+    M = vortex_field(shape, delta, (30e-9, 24e-9), Ms=8.6e5)
+    st.set_m(M)
+    st.run(15)
+    s = st.stats()
+    sysO = O.System(shape, delta, N=st.N, **kw)
+    Mo, so = sysO.run(M, method=method, steps=15, eps_abs=1e-5, eps_rel=1e-5, h0=1e-13)
+    assert s["attempts"] == so["attempts"] and s["rejections"] == so["rejections"]
+    assert abs(s["t"] - so["t"]) <= 1e-9 * so["t"]
+    Mg = st.get_m()
+    assert np.abs(Mg - Mo).max() / 8.6e5 < 1e-9
+    dM, H = st.dmdt()
+    dMo, Ho = sysO.dmdt(Mg, s["t"])
+    assert rel_err(dM, dMo) < TOL64 and rel_err(H, Ho) < TOL64
+    E, Eo = st.energy(), sysO.energy(Mg, s["t"])
+    assert E["E_ext"] == pytest.approx(Eo["E_ext"], rel=1e-12)
+    st.close()
+
+
+@pytest.mark.parametrize("method", ["rkf45", "dp54", "rk4"])
 def test_stepper_run_until_vs_oracle(method):
     # stepping to a fixed end time: the last accepted step is redone with
     # h = t_end - t (runge_kutta.py:74-78)
