@@ -190,17 +190,17 @@ __global__ void k_xfwd(const TIN *__restrict__ in, size_t in_cstride, C2<T> *__r
         },
         [](int, C2<T>) {});
     // X0[k] = (Z[k] + conj Z[L-k]) / 2, X1[k] = (Z[k] - conj Z[L-k]) / 2i
-    for (int e = threadIdx.x; e < lpw * Hx; e += blockDim.x) {
-        const int l2 = e / Hx, k = e % Hx;
+    for (int l2 = 0; l2 < lpw; ++l2) {
         const int h0 = 2 * (blockIdx.x * lpw + l2);
-        if (h0 >= nrows) continue;
+        if (h0 >= nrows) break;
         const C2<T> *ln = sm + (size_t)l2 * LP;
-        const C2<T> zk = ln[sk<SK>(k)], zm = ln[sk<SK>(k == 0 ? 0 : L - k)];
-        out[(h0 / rows_c) * out_cstride + (size_t)(h0 % rows_c) * Hp + k] =
-            cmk<T>((zk.x + zm.x) * (T)0.5, (zk.y - zm.y) * (T)0.5);
-        if (h0 + 1 < nrows)
-            out[((h0 + 1) / rows_c) * out_cstride + (size_t)((h0 + 1) % rows_c) * Hp + k] =
-                cmk<T>((zk.y + zm.y) * (T)0.5, (zm.x - zk.x) * (T)0.5);
+        C2<T> *d0 = out + (h0 / rows_c) * out_cstride + (size_t)(h0 % rows_c) * Hp;
+        C2<T> *d1 = h0 + 1 < nrows ? out + ((h0 + 1) / rows_c) * out_cstride + (size_t)((h0 + 1) % rows_c) * Hp : nullptr;
+        for (int k = threadIdx.x; k < Hx; k += blockDim.x) {
+            const C2<T> zk = ln[sk<SK>(k)], zm = ln[sk<SK>(k == 0 ? 0 : L - k)];
+            d0[k] = cmk<T>((zk.x + zm.x) * (T)0.5, (zk.y - zm.y) * (T)0.5);
+            if (d1) d1[k] = cmk<T>((zk.y + zm.y) * (T)0.5, (zm.x - zk.x) * (T)0.5);
+        }
     }
 }
 

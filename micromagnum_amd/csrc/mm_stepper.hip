@@ -92,7 +92,51 @@ template <typename T> struct SysDev {
     // rows of the stage launch in XCD-contiguous ranges (xcd_tile): the
     // exchange stencil's neighbour rows are then read from the same L2
     int xcd_remap;
+    // uniaxial anisotropy with uniform k, Ms (non-zero) and axis:
+    // H = uni_c (M.a) a with uni_c = 2 k / (mu0 Ms^2), the reference's
+    // 2 k (m.a) / (mu0 Ms) a with m = M / Ms; energy density uni_e |a x M|^2,
+    // uni_e = k / Ms^2 (no per-cell divisions)
+    int uni_fast;
+    T uni_c, uni_e;
+    // Zhang-Li torque with uniform P, xi, Ms (non-zero), alpha and j: the
+    // factors f1, f2 of fdm_zhangli_cpu.cpp:95-114 and the inverse cell sizes
+    // computed once (no per-cell divisions)
+    int stt_fast;
+    T stt_f1, stt_f2, inv_d[3];
 };
+
+// Zhang-Li torque of cell (x,y,z) with the precomputed uniform factors
+// (zhangli_cell with its divisions hoisted): gradients central inside,
+// one-sided at the faces, zero along axes of one cell
+template <typename T>
+__device__ __forceinline__ V3<T> zhangli_cell_uniform(const MeshDev &m, int x, int y, int z, size_t i, size_t n,
+                                                      const T *inv_d, T f1, T f2, const V3<T> &j,
+                                                      const T *__restrict__ M)
+{
+    const long li = (long)i;
+    auto grad = [&](int pos, int dim, long st, T id) -> V3<T> {
+        if (dim <= 1) return v3<T>(0, 0, 0);
+        long a = li - st, b = li + st;
+        T f = id * (T)0.5;
+        if (pos == 0) {
+            a = li;
+            f = id;
+        } else if (pos == dim - 1) {
+            b = li;
+            f = id;
+        }
+        return v3<T>((M[b] - M[a]) * f, (M[n + b] - M[n + a]) * f, (M[2 * n + b] - M[2 * n + a]) * f);
+    };
+    const V3<T> gx = grad(x, m.nx, 1, inv_d[0]);
+    const V3<T> gy = grad(y, m.ny, m.nx, inv_d[1]);
+    const V3<T> gz = grad(z, m.nz, (long)m.nx * m.ny, inv_d[2]);
+    const V3<T> jg = v3<T>(j.x * gx.x + j.y * gy.x + j.z * gz.x, j.x * gx.y + j.y * gy.y + j.z * gz.y,
+                           j.x * gx.z + j.y * gy.z + j.z * gz.z);
+    const V3<T> Mv = v3<T>(M[li], M[n + li], M[2 * n + li]);
+    const V3<T> B = cross3(Mv, jg);
+    const V3<T> Av = cross3(Mv, B);
+    return v3<T>(f1 * Av.x + f2 * B.x, f1 * Av.y + f2 * B.y, f1 * Av.z + f2 * B.z);
+}
 
 // exchange field of cell i for uniform Ms and A (see exch_uniform).  Each
 // direction's neighbour is the periodic image or, at an open boundary, the
@@ -216,7 +260,18 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
         V3<T> ha = v3<T>(0, 0, 0);
         if (GEN && S.has_cub) ha = cubic_cell<T>(Mv, S.axis1.at(n, i), S.axis2.at(n, i), S.kcub.at(i), ms, S.cm2mu0, &e1);
         if (S.has_uni) {
-            const V3<T> hu = uniaxial_cell<T>(Mv, S.axis1.at(n, i), S.kuni.at(i), ms, S.mu0, &e0);
+            V3<T> hu;
+            if (S.uni_fast) {
+                const V3<T> ax = v3<T>(S.axis1.u[0], S.axis1.u[1], S.axis1.u[2]);
+                const T f = S.uni_c * dot3(Mv, ax);
+                hu = v3<T>(f * ax.x, f * ax.y, f * ax.z);
+                if (EN) {
+                    const V3<T> c = cross3(ax, Mv);
+                    e0 = S.uni_e * dot3(c, c);
+                }
+            } else {
+                hu = uniaxial_cell<T>(Mv, S.axis1.at(n, i), S.kuni.at(i), ms, S.mu0, &e0);
+            }
             if (EN) ea.aniso += (double)e0;
             if (GEN && S.has_cub) {
                 ha.x += hu.x;
@@ -240,8 +295,11 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
     if (EN) return;
     V3<T> k = llge_cell<T>(Mv, H, S.f1.at(i), S.f2.at(i));
     if (STT && S.has_stt) {
-        const V3<T> st = zhangli_cell<T>(S.m, x, y, z, i, n, S.dx, S.dy, S.dz, S.do_precess != 0, S.P.at(i), S.xi.at(i),
-                                         ms, S.alpha.at(i), S.j.at(n, i), Y, S.mu_b, S.e_charge);
+        const V3<T> st =
+            S.stt_fast ? zhangli_cell_uniform<T>(S.m, x, y, z, i, n, S.inv_d, S.stt_f1, S.stt_f2,
+                                                 v3<T>(S.j.u[0], S.j.u[1], S.j.u[2]), Y)
+                       : zhangli_cell<T>(S.m, x, y, z, i, n, S.dx, S.dy, S.dz, S.do_precess != 0, S.P.at(i),
+                                         S.xi.at(i), ms, S.alpha.at(i), S.j.at(n, i), Y, S.mu_b, S.e_charge);
         k.x += st.x;
         k.y += st.y;
         k.z += st.z;
@@ -411,7 +469,7 @@ __global__ void k_stage(const SysDev<T> *__restrict__ Sp, const StageDev<T> *__r
         // (X[L-k] = conj X[k]; X[0], X[L/2] taken real), so that Re and Im of
         // the inverse transform are the two rows
         const int l = threadIdx.x / TL, t = threadIdx.x % TL;
-        const int c = l / (np > 0 ? np : 1), q = l % (np > 0 ? np : 1);
+        const int c = np == 1 ? l : l / (np > 0 ? np : 1), q = np == 1 ? 0 : l % (np > 0 ? np : 1);
         const int gr0 = row0 + 2 * q;
         const bool active = (l < 3 * np) && gr0 < nrows;
         const bool has1 = gr0 + 1 < nrows;
@@ -443,41 +501,47 @@ __global__ void k_stage(const SysDev<T> *__restrict__ Sp, const StageDev<T> *__r
                    (T)(S.hoff[1] + S.ac_amp[1] * sin(ts * S.ac_freq[1] + S.ac_phase[1])),
                    (T)(S.hoff[2] + S.ac_amp[2] * sin(ts * S.ac_freq[2] + S.ac_phase[2])));
     }
-    for (int e = threadIdx.x; e < rpw * nx; e += blockDim.x) {
-        const int r = e / nx, x = e % nx;
+    // rows of the workgroup in (y, z): from row0 once, then stepped (no
+    // per-cell integer division)
+    int yr = row0 % ny, zr = row0 / ny;
+    for (int r = 0; r < rpw; ++r, ++yr) {
+        if (yr == ny) {
+            yr = 0;
+            ++zr;
+        }
         const int gr = row0 + r;
         const int q = r >> 1, hi = r & 1;
-        if (gr >= nrows) {
-            // the absent second row of a pair transforms as zeros
+        for (int x = threadIdx.x; x < nx; x += blockDim.x) {
+            if (gr >= nrows) {
+                // the absent second row of a pair transforms as zeros
+                if (fused)
+                    for (int c = 0; c < 3; ++c) reinterpret_cast<T *>(&sm[(c * np + q) * LP + sk<SK>(x)])[hi] = (T)0;
+                continue;
+            }
+            V3<T> Hd = v3<T>(0, 0, 0);
+            if (has_demag == 1) {
+                T h3[3];
+                for (int c = 0; c < 3; ++c) h3[c] = reinterpret_cast<const T *>(&sm[(c * np + q) * LP + sk<SK>(x)])[hi];
+                Hd = v3<T>(h3[0], h3[1], h3[2]);
+            } else if (has_demag == 2) {
+                const size_t i = (size_t)gr * nx + x;
+                Hd = v3<T>(Hdem[i], Hdem[S.n + i], Hdem[2 * S.n + i]);
+            }
+            T nxt[3];
+            stage_cell<T, EN, STT, GEN>(S, G, ctl, Y, x, yr, zr, Hd, hz, red, ea, nxt);
+            // the half slot (row, x) of the three lines is this thread's
+            // alone: the next stage input replaces H_demag in place
             if (fused)
-                for (int c = 0; c < 3; ++c) reinterpret_cast<T *>(&sm[(c * np + q) * LP + sk<SK>(x)])[hi] = (T)0;
-            continue;
+                for (int c = 0; c < 3; ++c) reinterpret_cast<T *>(&sm[(c * np + q) * LP + sk<SK>(x)])[hi] = nxt[c];
         }
-        V3<T> Hd = v3<T>(0, 0, 0);
-        if (has_demag == 1) {
-            T h3[3];
-            for (int c = 0; c < 3; ++c) h3[c] = reinterpret_cast<const T *>(&sm[(c * np + q) * LP + sk<SK>(x)])[hi];
-            Hd = v3<T>(h3[0], h3[1], h3[2]);
-        } else if (has_demag == 2) {
-            const size_t i = (size_t)gr * nx + x;
-            Hd = v3<T>(Hdem[i], Hdem[S.n + i], Hdem[2 * S.n + i]);
-        }
-        T nxt[3];
-        stage_cell<T, EN, STT, GEN>(S, G, ctl, Y, x, gr % ny, gr / ny, Hd, hz, red, ea, nxt);
-        // the half slot (row, x) of the three lines is this thread's alone:
-        // the next stage input replaces H_demag in place
-        if (fused)
-            for (int c = 0; c < 3; ++c) reinterpret_cast<T *>(&sm[(c * np + q) * LP + sk<SK>(x)])[hi] = nxt[c];
     }
     if (fused) {
-        for (int e = threadIdx.x; e < 3 * np * (L - nx); e += blockDim.x) {
-            const int l = e / (L - nx), x = nx + e % (L - nx);
-            sm[(size_t)l * LP + sk<SK>(x)] = cmk<T>((T)0, (T)0);
-        }
+        for (int l = 0; l < 3 * np; ++l)
+            for (int x = nx + threadIdx.x; x < L; x += blockDim.x) sm[(size_t)l * LP + sk<SK>(x)] = cmk<T>((T)0, (T)0);
         __syncthreads();
         {
             const int l = threadIdx.x / TL, t = threadIdx.x % TL;
-            const bool active = (l < 3 * np) && row0 + 2 * (l % (np > 0 ? np : 1)) < nrows;
+            const bool active = (l < 3 * np) && row0 + 2 * (np == 1 ? 0 : l % np) < nrows;
             C2<T> *line = sm + (size_t)l * LP;
             const LineMap m = {t, active};
             fft_line<T, false, false, false, LC, SK>(
@@ -485,16 +549,18 @@ __global__ void k_stage(const SysDev<T> *__restrict__ Sp, const StageDev<T> *__r
         }
         // forward: Z = FFT(x0 + i x1); X0[k] = (Z[k] + conj Z[L-k]) / 2,
         // X1[k] = (Z[k] - conj Z[L-k]) / 2i for k < Hx
-        for (int e = threadIdx.x; e < 3 * np * Hx; e += blockDim.x) {
-            const int l = e / Hx, k = e % Hx;
+        for (int l = 0; l < 3 * np; ++l) {
             const int c = l / np, q = l % np;
             const int gr0 = row0 + 2 * q;
             if (gr0 >= nrows) continue;
+            const bool has1 = gr0 + 1 < nrows;
             const C2<T> *ln = sm + (size_t)l * LP;
-            const C2<T> zk = ln[sk<SK>(k)], zm = ln[sk<SK>(k == 0 ? 0 : L - k)];
             C2<T> *d0 = G.fwd_S1 + ((size_t)c * nrows + gr0) * Hp;
-            d0[k] = cmk<T>((zk.x + zm.x) * (T)0.5, (zk.y - zm.y) * (T)0.5);
-            if (gr0 + 1 < nrows) d0[(size_t)Hp + k] = cmk<T>((zk.y + zm.y) * (T)0.5, (zm.x - zk.x) * (T)0.5);
+            for (int k = threadIdx.x; k < Hx; k += blockDim.x) {
+                const C2<T> zk = ln[sk<SK>(k)], zm = ln[sk<SK>(k == 0 ? 0 : L - k)];
+                d0[k] = cmk<T>((zk.x + zm.x) * (T)0.5, (zk.y - zm.y) * (T)0.5);
+                if (has1) d0[(size_t)Hp + k] = cmk<T>((zk.y + zm.y) * (T)0.5, (zm.x - zk.x) * (T)0.5);
+            }
         }
     }
     __shared__ double wred[4][16];
@@ -716,14 +782,17 @@ __global__ void k_xfwd_prep(const T *__restrict__ y0, const T *__restrict__ y1, 
         },
         [](int, C2<T>) {});
     // X0[k] = (Z[k] + conj Z[L-k]) / 2, X1[k] = (Z[k] - conj Z[L-k]) / 2i
-    for (int e = threadIdx.x; e < lpw * Hx; e += blockDim.x) {
-        const int l2 = e / Hx, k = e % Hx;
+    for (int l2 = 0; l2 < lpw; ++l2) {
         const int h0 = 2 * (blockIdx.x * lpw + l2);
-        if (h0 >= nrows) continue;
+        if (h0 >= nrows) break;
         const C2<T> *ln = sm + (size_t)l2 * LP;
-        const C2<T> zk = ln[sk<SK>(k)], zm = ln[sk<SK>(k == 0 ? 0 : L - k)];
-        S1[(size_t)h0 * Hp + k] = cmk<T>((zk.x + zm.x) * (T)0.5, (zk.y - zm.y) * (T)0.5);
-        if (h0 + 1 < nrows) S1[(size_t)(h0 + 1) * Hp + k] = cmk<T>((zk.y + zm.y) * (T)0.5, (zm.x - zk.x) * (T)0.5);
+        C2<T> *d0 = S1 + (size_t)h0 * Hp;
+        const bool has1 = h0 + 1 < nrows;
+        for (int k = threadIdx.x; k < Hx; k += blockDim.x) {
+            const C2<T> zk = ln[sk<SK>(k)], zm = ln[sk<SK>(k == 0 ? 0 : L - k)];
+            d0[k] = cmk<T>((zk.x + zm.x) * (T)0.5, (zk.y - zm.y) * (T)0.5);
+            if (has1) d0[(size_t)Hp + k] = cmk<T>((zk.y + zm.y) * (T)0.5, (zm.x - zk.x) * (T)0.5);
+        }
     }
 }
 
@@ -955,6 +1024,12 @@ template <typename T> struct Stepper : public StepperBase {
             MM_REQUIRE(d.axis2_host, "cubic anisotropy needs axis2");
             sys.axis2 = vparam(d.axis2_host);
         }
+        sys.uni_fast = sys.has_uni && !sys.kuni.p && !sys.axis1.p && sys.Ms_uniform && sys.Ms_u != 0.0;
+        if (sys.uni_fast) {
+            const double k = d.k_uni_host[0], ms = sys.Ms_u;
+            sys.uni_c = (T)(2.0 * k / (mu0 * ms * ms));
+            sys.uni_e = (T)(k / (ms * ms));
+        }
         sys.has_zee = d.use_zeeman;
         sys.has_ac = d.use_zeeman && d.use_ac;
         for (int c = 0; c < 3; ++c) {
@@ -971,6 +1046,18 @@ template <typename T> struct Stepper : public StepperBase {
             sys.P = param(d.P_host, 0.0);
             sys.xi = param(d.xi_host, 0.0);
             sys.j = vparam(d.j_host);
+            sys.stt_fast = !sys.P.p && !sys.xi.p && !sys.j.p && !sys.alpha.p && sys.Ms_uniform && sys.Ms_u != 0.0;
+            if (sys.stt_fast) {
+                // fdm_zhangli_cpu.cpp:95-114 in double
+                const double P = d.P_host[0], xi = d.xi_host[0], al = d.alpha_host[0], ms = sys.Ms_u;
+                const double bj = P * MM_MU_BOHR / (MM_E_CHARGE * ms * (1.0 + xi * xi));
+                const double bjp = bj / (1.0 + al * al);
+                sys.stt_f1 = (T)(d.do_precess ? -bjp / (ms * ms) * (1.0 + al * xi) : -bjp / (ms * ms) * (al * xi));
+                sys.stt_f2 = (T)(d.do_precess ? -bjp / ms * (xi - al) : -bjp / ms * (-al));
+                sys.inv_d[0] = (T)(1.0 / d.dx);
+                sys.inv_d[1] = (T)(1.0 / d.dy);
+                sys.inv_d[2] = (T)(1.0 / d.dz);
+            }
         }
 
         if (d.N_host) {
