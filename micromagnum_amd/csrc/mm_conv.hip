@@ -176,7 +176,7 @@ __global__ void k_xfwd(const TIN *__restrict__ in, size_t in_cstride, C2<T> *__r
     const bool has1 = g0 + 1 < nrows;
     C2<T> *line = sm + (size_t)l * LP;
     C2<T> *twl = sm + (size_t)lpw * LP;
-    load_twiddles<T, LC>(twl, p);
+    const TwFetch<T> twf = TwFetch<T>::fetch(p);
     const int rows_c = ny * nz;
     auto row_src = [&](int g) { return in + (g / rows_c) * in_cstride + (size_t)(g % rows_c) * nx_in; };
     const TIN *src0 = row_src(active ? g0 : 0);
@@ -188,7 +188,7 @@ __global__ void k_xfwd(const TIN *__restrict__ in, size_t in_cstride, C2<T> *__r
             const bool in_row = x < nx_in;
             return cmk<T>(in_row ? (T)src0[x] : (T)0, in_row && has1 ? (T)src1[x] : (T)0);
         },
-        [](int, C2<T>) {});
+        [](int, C2<T>) {}, [&] { twf.store(twl); });
     // X0[k] = (Z[k] + conj Z[L-k]) / 2, X1[k] = (Z[k] - conj Z[L-k]) / 2i
     for (int l2 = 0; l2 < lpw; ++l2) {
         const int h0 = 2 * (blockIdx.x * lpw + l2);
@@ -220,7 +220,7 @@ __global__ void k_yfwd(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, in
     const C2<T> *src = in + ((size_t)c * nzl + z) * (size_t)ny_in * Hp;
     C2<T> *dst = out + ((size_t)c * nzl + z) * (size_t)L * Hp;
     C2<T> *twl = sm + (size_t)tk * LP;
-    load_twiddles<T, LC>(twl, p);
+    const TwFetch<T> twf = TwFetch<T>::fetch(p);
     // global passes: columns fastest across the threads (row segments of tk
     // values per load); LDS passes: one line per group of tline threads
     const int lg = threadIdx.x % tk, ls = threadIdx.x / PlanC<LC>::tline(p);
@@ -231,7 +231,7 @@ __global__ void k_yfwd(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, in
         [&](int y) { return (y < ny_in && kx < Hx) ? src[(size_t)y * Hp + kx] : cmk<T>((T)0, (T)0); },
         [&](int y, C2<T> v) {
             if (kx < Hx) dst[(size_t)y * Hp + kx] = v;
-        });
+        }, [&] { twf.store(twl); });
 }
 
 // y inverse: columns of S2 planes (c, z), all Py rows read, y < ny_out written
@@ -249,7 +249,7 @@ __global__ void k_yinv(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, in
     const C2<T> *src = in + ((size_t)c * nzl + z) * (size_t)L * Hp;
     C2<T> *dst = out + ((size_t)c * nzl + z) * (size_t)ny_out * Hp;
     C2<T> *twl = sm + (size_t)tk * LP;
-    load_twiddles<T, LC>(twl, p);
+    const TwFetch<T> twf = TwFetch<T>::fetch(p);
     const int lg = threadIdx.x % tk, ls = threadIdx.x / PlanC<LC>::tline(p);
     const LineMap mg = {(int)(threadIdx.x / tk), true}, ms = {(int)(threadIdx.x % PlanC<LC>::tline(p)), ls < tk};
     const int kx = kx0 + lg;
@@ -258,7 +258,7 @@ __global__ void k_yinv(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, in
         [&](int y) { return kx < Hx ? src[(size_t)y * Hp + kx] : cmk<T>((T)0, (T)0); },
         [&](int y, C2<T> v) {
             if (y < ny_out && kx < Hx) dst[(size_t)y * Hp + kx] = v;
-        });
+        }, [&] { twf.store(twl); });
 }
 
 // Pruned y transforms for zero-padded lines (P = 2h even, data only in
@@ -412,7 +412,7 @@ __global__ void k_zmi(C2<T> *__restrict__ S2, const T *__restrict__ K, size_t pl
     const size_t zstride = (size_t)Py * Hp;
     C2<T> *base = S2 + (size_t)ky * Hp;
     C2<T> *twl = sm + (size_t)3 * tk * LP;
-    load_twiddles<T, LC>(twl, p);
+    const TwFetch<T> twf = TwFetch<T>::fetch(p);
     // global passes: columns fastest, then components; LDS passes: one line
     // (component c, column) per group of tline threads
     const int colg = threadIdx.x % tk, cg = (threadIdx.x / tk) % 3;
@@ -446,7 +446,7 @@ __global__ void k_zmi(C2<T> *__restrict__ S2, const T *__restrict__ K, size_t pl
     fft_line<T, false, true, false, LC, SK>(
         p, twl, lineG, mg, line, ms,
         [&](int z) { return (z < nz && kxg < Hx) ? colp[z * zstride] : cmk<T>((T)0, (T)0); },
-        [](int, C2<T>) {});
+        [](int, C2<T>) {}, [&] { twf.store(twl); });
 #pragma unroll
     for (int j = 0; j < ZM_ITEMS; ++j) {
         const int e = threadIdx.x + j * blockDim.x;
@@ -485,7 +485,7 @@ __global__ void k_ymi2d(C2<T> *__restrict__ S1, const T *__restrict__ K, size_t 
     const int kx0 = blockIdx.x * tk;
     const size_t cstride = (size_t)ny * Hp;
     C2<T> *twl = sm + (size_t)3 * tk * LP;
-    load_twiddles<T, LC>(twl, p);
+    const TwFetch<T> twf = TwFetch<T>::fetch(p);
     const int colg = threadIdx.x % tk, cg = (threadIdx.x / tk) % 3;
     const LineMap mg = {(int)(threadIdx.x / (3 * tk)), true};
     C2<T> *lineG = sm + (size_t)(cg * tk + colg) * LP;
@@ -497,7 +497,7 @@ __global__ void k_ymi2d(C2<T> *__restrict__ S1, const T *__restrict__ K, size_t 
     fft_line<T, false, true, false, LC, SK>(
         p, twl, lineG, mg, line, ms,
         [&](int y) { return (y < ny && kxg < Hx) ? colp[(size_t)y * Hp] : cmk<T>((T)0, (T)0); },
-        [](int, C2<T>) {});
+        [](int, C2<T>) {}, [&] { twf.store(twl); });
     for (int e = threadIdx.x; e < tk * L; e += blockDim.x) {
         const int col = e % tk, ky = e / tk, kx = kx0 + col;
         if (kx < Hx) {
@@ -528,7 +528,7 @@ __global__ void k_xinv(const C2<T> *__restrict__ in, TOUT *__restrict__ out, siz
     const bool active = g < nrows;
     C2<T> *line = sm + (size_t)l * lds_len<SK>(L);
     C2<T> *twl = sm + (size_t)lpw * lds_len<SK>(L);
-    load_twiddles<T, LC>(twl, p);
+    const TwFetch<T> twf = TwFetch<T>::fetch(p);
     const int c = active ? g / (ny * nz) : 0, r = active ? g % (ny * nz) : 0;
     const C2<T> *src = in + (size_t)(active ? g : 0) * Hp;
     TOUT *dst = out + c * out_cstride + (size_t)r * nx;
@@ -537,7 +537,7 @@ __global__ void k_xinv(const C2<T> *__restrict__ in, TOUT *__restrict__ out, siz
         p, twl, line, m, line, m, [&](int k) { return k < Hx ? src[k] : cconj<T>(src[L - k]); },
         [&](int x, C2<T> v) {
             if (x < nx) dst[x] = (TOUT)v.x;
-        });
+        }, [&] { twf.store(twl); });
 }
 
 // setup: forward z over full lines (no multiply), all Pz outputs stored

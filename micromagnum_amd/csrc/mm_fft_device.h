@@ -292,9 +292,10 @@ struct LineMap {
     bool active;
 };
 
-template <int R, int EPT, typename T, bool INV, bool P2, bool LDIN, bool STOUT, int SK, typename LD, typename ST>
+template <int R, int EPT, typename T, bool INV, bool P2, bool LDIN, bool STOUT, int SK, bool PRE, typename LD,
+          typename ST, typename PF>
 __device__ __forceinline__ void fft_pass_io(C2<T> *lineG, LineMap g, C2<T> *lineS, LineMap sm_, const C2<T> *tw,
-                                            int L, int tline, int Ns, int lNs, int toff, LD &ld, ST &st)
+                                            int L, int tline, int Ns, int lNs, int toff, LD &ld, ST &st, PF &pre)
 {
     constexpr int MB = MaxB<R, EPT>::value;
     const int nb = L / R;
@@ -313,6 +314,9 @@ __device__ __forceinline__ void fft_pass_io(C2<T> *lineG, LineMap g, C2<T> *line
             for (int r = 0; r < R; ++r) v[q][r] = LDIN ? ld(b + r * nb) : lin[sk<SK>(b + r * nb)];
         }
     }
+    // first pass: the caller's hook runs while this pass's reads are in
+    // flight (kernels store their prefetched twiddles to LDS here)
+    if constexpr (PRE) pre();
     if (!LDIN && !STOUT) __syncthreads();  // in place in LDS: every read before any write
     // outputs: a butterfly stays with the thread that read it
 #pragma unroll
@@ -341,25 +345,25 @@ __device__ __forceinline__ void fft_pass_io(C2<T> *lineG, LineMap g, C2<T> *line
 }
 
 template <typename T, bool INV, int EPT, bool P2, bool LDIN, bool STOUT, bool FIRST, bool LAST, int SK,
-          typename LD, typename ST>
+          typename LD, typename ST, typename PF>
 __device__ __forceinline__ void fft_pass_sel(int R, C2<T> *lineG, LineMap g, C2<T> *lineS, LineMap s_,
                                              const C2<T> *tw, int L, int tline, int Ns, int lNs, int toff, LD &ld,
-                                             ST &st)
+                                             ST &st, PF &pre)
 {
     constexpr bool I = LDIN && FIRST, O = STOUT && LAST;
     if (P2) {
         switch (R) {
-        case 8: fft_pass_io<8, EPT, T, INV, true, I, O, SK>(lineG, g, lineS, s_, tw, L, tline, Ns, lNs, toff, ld, st); break;
-        case 4: fft_pass_io<4, EPT, T, INV, true, I, O, SK>(lineG, g, lineS, s_, tw, L, tline, Ns, lNs, toff, ld, st); break;
-        default: fft_pass_io<2, EPT, T, INV, true, I, O, SK>(lineG, g, lineS, s_, tw, L, tline, Ns, lNs, toff, ld, st); break;
+        case 8: fft_pass_io<8, EPT, T, INV, true, I, O, SK, FIRST>(lineG, g, lineS, s_, tw, L, tline, Ns, lNs, toff, ld, st, pre); break;
+        case 4: fft_pass_io<4, EPT, T, INV, true, I, O, SK, FIRST>(lineG, g, lineS, s_, tw, L, tline, Ns, lNs, toff, ld, st, pre); break;
+        default: fft_pass_io<2, EPT, T, INV, true, I, O, SK, FIRST>(lineG, g, lineS, s_, tw, L, tline, Ns, lNs, toff, ld, st, pre); break;
         }
     } else {
         switch (R) {
-        case 8: fft_pass_io<8, EPT, T, INV, false, I, O, SK>(lineG, g, lineS, s_, tw, L, tline, Ns, 0, toff, ld, st); break;
-        case 5: fft_pass_io<5, EPT, T, INV, false, I, O, SK>(lineG, g, lineS, s_, tw, L, tline, Ns, 0, toff, ld, st); break;
-        case 4: fft_pass_io<4, EPT, T, INV, false, I, O, SK>(lineG, g, lineS, s_, tw, L, tline, Ns, 0, toff, ld, st); break;
-        case 3: fft_pass_io<3, EPT, T, INV, false, I, O, SK>(lineG, g, lineS, s_, tw, L, tline, Ns, 0, toff, ld, st); break;
-        default: fft_pass_io<2, EPT, T, INV, false, I, O, SK>(lineG, g, lineS, s_, tw, L, tline, Ns, 0, toff, ld, st); break;
+        case 8: fft_pass_io<8, EPT, T, INV, false, I, O, SK, FIRST>(lineG, g, lineS, s_, tw, L, tline, Ns, 0, toff, ld, st, pre); break;
+        case 5: fft_pass_io<5, EPT, T, INV, false, I, O, SK, FIRST>(lineG, g, lineS, s_, tw, L, tline, Ns, 0, toff, ld, st, pre); break;
+        case 4: fft_pass_io<4, EPT, T, INV, false, I, O, SK, FIRST>(lineG, g, lineS, s_, tw, L, tline, Ns, 0, toff, ld, st, pre); break;
+        case 3: fft_pass_io<3, EPT, T, INV, false, I, O, SK, FIRST>(lineG, g, lineS, s_, tw, L, tline, Ns, 0, toff, ld, st, pre); break;
+        default: fft_pass_io<2, EPT, T, INV, false, I, O, SK, FIRST>(lineG, g, lineS, s_, tw, L, tline, Ns, 0, toff, ld, st, pre); break;
         }
     }
 }
@@ -369,13 +373,14 @@ __device__ __forceinline__ void fft_pass_sel(int R, C2<T> *lineG, LineMap g, C2<
 // them); STOUT: outputs via st(j, v) (else they end in the LDS line, followed
 // by a barrier).  lineG/g: line and mapping of the passes that use ld/st,
 // lineS/s: of the LDS passes (the same line in both mappings).
-template <typename T, bool INV, bool P2, bool LDIN, bool STOUT, int SK, typename LD, typename ST>
+template <typename T, bool INV, bool P2, bool LDIN, bool STOUT, int SK, typename LD, typename ST, typename PF>
 __device__ __forceinline__ void fft_line_p2(const FftPlanDev &p, const C2<T> *tw, C2<T> *lineG, LineMap g,
-                                            C2<T> *lineS, LineMap s_, LD &ld, ST &st)
+                                            C2<T> *lineS, LineMap s_, LD &ld, ST &st, PF &pre)
 {
     constexpr int EPT = MM_FFT_EPT;
     const int np = p.npass;
     if (np == 0) {
+        pre();
         if (g.active && g.t == 0) {
             const C2<T> v = LDIN ? ld(0) : lineG[sk<SK>(0)];
             if (STOUT) st(0, v);
@@ -395,13 +400,13 @@ __device__ __forceinline__ void fft_line_p2(const FftPlanDev &p, const C2<T> *tw
         int Lv = p.L, tlv = p.tline;
         asm volatile("" : "+s"(Lv), "+s"(tlv));
         if (ps == 0 && ps == np - 1)
-            fft_pass_sel<T, INV, EPT, P2, LDIN, STOUT, true, true, SK>(R, lineG, g, lineS, s_, tw, Lv, tlv, Ns, lNs, toff, ld, st);
+            fft_pass_sel<T, INV, EPT, P2, LDIN, STOUT, true, true, SK>(R, lineG, g, lineS, s_, tw, Lv, tlv, Ns, lNs, toff, ld, st, pre);
         else if (ps == 0)
-            fft_pass_sel<T, INV, EPT, P2, LDIN, STOUT, true, false, SK>(R, lineG, g, lineS, s_, tw, Lv, tlv, Ns, lNs, toff, ld, st);
+            fft_pass_sel<T, INV, EPT, P2, LDIN, STOUT, true, false, SK>(R, lineG, g, lineS, s_, tw, Lv, tlv, Ns, lNs, toff, ld, st, pre);
         else if (ps == np - 1)
-            fft_pass_sel<T, INV, EPT, P2, LDIN, STOUT, false, true, SK>(R, lineG, g, lineS, s_, tw, Lv, tlv, Ns, lNs, toff, ld, st);
+            fft_pass_sel<T, INV, EPT, P2, LDIN, STOUT, false, true, SK>(R, lineG, g, lineS, s_, tw, Lv, tlv, Ns, lNs, toff, ld, st, pre);
         else
-            fft_pass_sel<T, INV, EPT, P2, LDIN, STOUT, false, false, SK>(R, lineG, g, lineS, s_, tw, Lv, tlv, Ns, lNs, toff, ld, st);
+            fft_pass_sel<T, INV, EPT, P2, LDIN, STOUT, false, false, SK>(R, lineG, g, lineS, s_, tw, Lv, tlv, Ns, lNs, toff, ld, st, pre);
         if (Ns > 1) toff += (R - 1) * Ns;
         Ns *= R;
         lNs += R == 8 ? 3 : (R == 4 ? 2 : 1);
@@ -446,32 +451,71 @@ template <> struct PlanC<0> {
     __device__ static int tline(const FftPlanDev &p) { return p.tline; }
 };
 
-template <typename T, bool INV, int LC, int PS, int NS, bool LDIN, bool STOUT, int SK, typename LD, typename ST>
+template <typename T, bool INV, int LC, int PS, int NS, bool LDIN, bool STOUT, int SK, typename LD, typename ST,
+          typename PF>
 __device__ __forceinline__ void fft_line_c(const C2<T> *tw, C2<T> *lineG, LineMap g, C2<T> *lineS, LineMap s_,
-                                           LD &ld, ST &st)
+                                           LD &ld, ST &st, PF &pre)
 {
     using F = FftLen<LC>;
     if constexpr (PS < F::npass) {
         constexpr int R = F::radix(PS);
         constexpr bool I = LDIN && PS == 0, O = STOUT && PS == F::npass - 1;
-        fft_pass_io<R, MM_FFT_EPT, T, INV, true, I, O, SK>(lineG, g, lineS, s_, tw, LC, F::tline, NS, mm_ilog2(NS),
-                                                           F::toff(PS), ld, st);
-        fft_line_c<T, INV, LC, PS + 1, NS * R, LDIN, STOUT, SK>(tw, lineG, g, lineS, s_, ld, st);
+        fft_pass_io<R, MM_FFT_EPT, T, INV, true, I, O, SK, PS == 0>(lineG, g, lineS, s_, tw, LC, F::tline, NS,
+                                                                    mm_ilog2(NS), F::toff(PS), ld, st, pre);
+        fft_line_c<T, INV, LC, PS + 1, NS * R, LDIN, STOUT, SK>(tw, lineG, g, lineS, s_, ld, st, pre);
     }
 }
 
 // LC > 0: the compile-time plan of that length (p.L must equal LC); LC == 0:
 // the runtime plan p.  SK: the LDS skew of the lines (sk).
-template <typename T, bool INV, bool LDIN, bool STOUT, int LC, int SK, typename LD, typename ST>
+struct FftNoPre {
+    __device__ void operator()() const {}
+};
+
+// pre(): called once in the first pass after its reads are issued and
+// before its barrier (FftNoPre: nothing)
+template <typename T, bool INV, bool LDIN, bool STOUT, int LC, int SK, typename LD, typename ST,
+          typename PF = FftNoPre>
 __device__ __forceinline__ void fft_line(const FftPlanDev &p, const C2<T> *tw, C2<T> *lineG, LineMap g, C2<T> *lineS,
-                                         LineMap s_, LD ld, ST st)
+                                         LineMap s_, LD ld, ST st, PF pre = PF())
 {
     if constexpr (LC > 0) {
-        fft_line_c<T, INV, LC, 0, 1, LDIN, STOUT, SK>(tw, lineG, g, lineS, s_, ld, st);
+        fft_line_c<T, INV, LC, 0, 1, LDIN, STOUT, SK>(tw, lineG, g, lineS, s_, ld, st, pre);
     } else {
-        if (p.pow2) fft_line_p2<T, INV, true, LDIN, STOUT, SK>(p, tw, lineG, g, lineS, s_, ld, st);
-        else fft_line_p2<T, INV, false, LDIN, STOUT, SK>(p, tw, lineG, g, lineS, s_, ld, st);
+        if (p.pow2) fft_line_p2<T, INV, true, LDIN, STOUT, SK>(p, tw, lineG, g, lineS, s_, ld, st, pre);
+        else fft_line_p2<T, INV, false, LDIN, STOUT, SK>(p, tw, lineG, g, lineS, s_, ld, st, pre);
     }
 }
+
+// Twiddles fetched into registers at kernel start and stored to LDS by the
+// first FFT pass's pre() hook, so the first pass's data reads do not wait for
+// the twiddle read: TwFetch<T>::fetch(p) / .store(twl).
+template <typename T> struct TwFetch {
+    static constexpr int N = 2;  // per thread in registers; more go through a loop
+    C2<T> v[N];
+    const C2<T> *g;
+    int n;
+    __device__ static TwFetch fetch(const FftPlanDev &p)
+    {
+        TwFetch f;
+        f.g = (const C2<T> *)p.tw;
+        f.n = p.ntw;
+#pragma unroll
+        for (int q = 0; q < N; ++q) {
+            const int i = threadIdx.x + q * blockDim.x;
+            if (i < f.n) f.v[q] = f.g[i];
+        }
+        return f;
+    }
+    __device__ void store(C2<T> *twl) const
+    {
+#pragma unroll
+        for (int q = 0; q < N; ++q) {
+            const int i = threadIdx.x + q * blockDim.x;
+            if (i < n) twl[i] = v[q];
+        }
+        for (int i = threadIdx.x + N * blockDim.x; i < n; i += blockDim.x) twl[i] = g[i];
+    }
+};
 
 #endif

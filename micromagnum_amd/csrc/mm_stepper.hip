@@ -464,7 +464,7 @@ __global__ void k_stage(const SysDev<T> *__restrict__ Sp, const StageDev<T> *__r
     const int np = rpw >> 1;
     C2<T> *twl = sm + (size_t)3 * np * LP;
     if (has_demag == 1) {
-        load_twiddles<T, LC>(twl, p);
+        const TwFetch<T> twf = TwFetch<T>::fetch(p);
         // inverse: Z = X0 + i X1 with both half spectra extended Hermitian
         // (X[L-k] = conj X[k]; X[0], X[L/2] taken real), so that Re and Im of
         // the inverse transform are the two rows
@@ -487,7 +487,7 @@ __global__ void k_stage(const SysDev<T> *__restrict__ Sp, const StageDev<T> *__r
                 if (k == 0 || 2 * k == L) return cmk<T>(a.x, b.x);
                 return lo ? cmk<T>(a.x - b.y, a.y + b.x) : cmk<T>(a.x + b.y, b.x - a.y);
             },
-            [](int, C2<T>) {});
+            [](int, C2<T>) {}, [&] { twf.store(twl); });
     }
     StageRed red = {0.0, 0.0, 0.0, 0.0};
     EnergyAcc ea = {0.0, 0.0, 0.0, 0.0};
@@ -762,7 +762,7 @@ __global__ void k_xfwd_prep(const T *__restrict__ y0, const T *__restrict__ y1, 
     const bool has1 = g0 + 1 < nrows;
     C2<T> *line = sm + (size_t)l * LP;
     C2<T> *twl = sm + (size_t)lpw * LP;
-    load_twiddles<T, LC>(twl, p);
+    const TwFetch<T> twf = TwFetch<T>::fetch(p);
     const size_t base0 = (size_t)(active ? g0 : 0) * nx;
     const size_t base1 = base0 + nx;
     const LineMap m = {t, active};
@@ -780,7 +780,7 @@ __global__ void k_xfwd_prep(const T *__restrict__ y0, const T *__restrict__ y1, 
             }
             return cmk<T>(v0, v1);
         },
-        [](int, C2<T>) {});
+        [](int, C2<T>) {}, [&] { twf.store(twl); });
     // X0[k] = (Z[k] + conj Z[L-k]) / 2, X1[k] = (Z[k] - conj Z[L-k]) / 2i
     for (int l2 = 0; l2 < lpw; ++l2) {
         const int h0 = 2 * (blockIdx.x * lpw + l2);
