@@ -5,6 +5,21 @@
 
 #include <hip/hip_runtime.h>
 
+// A pointer read from device memory (a parameter record) is generic to the
+// compiler, which then emits flat loads and stores (they count against the
+// LDS counter too, so waits drain both).  gfield reads such a record field as
+// a global-address-space pointer, so address-space inference turns every
+// access through it into a global load or store.  (A cast of the loaded
+// generic pointer to global and back is folded away and does not do this.)
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Wincompatible-pointer-types-discards-qualifiers"
+template <typename T> __device__ __forceinline__ T *gfield(T *const &field)
+{
+    typedef __attribute__((address_space(1))) T *gptr;
+    return (T *)*reinterpret_cast<const gptr *>(&field);
+}
+#pragma clang diagnostic pop
+
 template <typename T> struct V3 {
     T x, y, z;
 };

@@ -57,14 +57,14 @@ struct StepCtl {
 template <typename T> struct Param {
     const T *p;
     T u;
-    __device__ __forceinline__ T at(size_t i) const { return p ? p[i] : u; }
+    __device__ __forceinline__ T at(size_t i) const { return p ? gfield(p)[i] : u; }
 };
 template <typename T> struct VParam {
     const T *p; // 3n SoA or null
     T u[3];
     __device__ __forceinline__ V3<T> at(size_t n, size_t i) const
     {
-        return p ? v3<T>(p[i], p[n + i], p[2 * n + i]) : v3<T>(u[0], u[1], u[2]);
+        return p ? v3<T>(gfield(p)[i], gfield(p)[n + i], gfield(p)[2 * n + i]) : v3<T>(u[0], u[1], u[2]);
     }
 };
 
@@ -249,7 +249,8 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
         const V3<T> he = (!GEN || S.exch_uniform)
                              ? exchange_cell_uniform<T>(S.m, x, y, z, i, n, S.wx, S.wy, S.wz, S.exch_f, Y)
                              : exchange_cell<T>(S.m, x, y, z, i, n, S.wx, S.wy, S.wz, S.coef_ex,
-                                                S.Ms.p ? S.Ms.p : nullptr, S.A.p ? S.A.p : nullptr, Y, ms);
+                                                S.Ms.p ? gfield(S.Ms.p) : nullptr, S.A.p ? gfield(S.A.p) : nullptr,
+                                                Y, ms);
         H.x += he.x;
         H.y += he.y;
         H.z += he.z;
@@ -304,8 +305,8 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
         k.y += st.y;
         k.z += st.z;
     }
-    if (G.kout) store3<T>(G.kout, n, i, k);
-    if (G.Hout) store3<T>(G.Hout, n, i, H);
+    if (G.kout) store3<T>(gfield(G.kout), n, i, k);
+    if (G.Hout) store3<T>(gfield(G.Hout), n, i, H);
     if (G.want_dydt) {
         const double a = k.x, b = k.y, c = k.z;
         double q = a * a + b * b + c * c;
@@ -322,15 +323,17 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
     // state) load in one batch of independent loads before any arithmetic
     // (loading them here rather than at the top keeps them out of the field
     // computation's live registers)
-    const T *yc = ctl->cur ? G.yb1 : G.yb0;
+    const T *yc = ctl->cur ? gfield(G.yb1) : gfield(G.yb0);
     T kv[MM_STAGE_MAXK][3], ycv[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) ycv[c] = yc[(size_t)c * n + i];
 #pragma unroll
     for (int q = 0; q < MM_STAGE_MAXK; ++q)
-        if (q < G.nk)
+        if (q < G.nk) {
+            const T *kq = gfield(G.kin[q]);
 #pragma unroll
-            for (int c = 0; c < 3; ++c) kv[q][c] = G.kin[q][(size_t)c * n + i];
+            for (int c = 0; c < 3; ++c) kv[q][c] = kq[(size_t)c * n + i];
+        }
     const double h = ctl->h;
     T kc[3] = {k.x, k.y, k.z};
     T out[3];
@@ -384,9 +387,10 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
         }
     }
     if (G.prep == 1) {
-        G.prep_out[i] = out[0];
-        G.prep_out[n + i] = out[1];
-        G.prep_out[2 * n + i] = out[2];
+        T *po = gfield(G.prep_out);
+        po[i] = out[0];
+        po[n + i] = out[1];
+        po[2 * n + i] = out[2];
         nxt[0] = out[0];
         nxt[1] = out[1];
         nxt[2] = out[2];
@@ -394,7 +398,8 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
     }
     // final stage: error norm and normalised new state into the other buffer
     if (G.dydt_from_k0) {
-        const double a = G.kin[0][i], b = G.kin[0][n + i], c = G.kin[0][2 * n + i];
+        const T *k0 = gfield(G.kin[0]);
+        const double a = k0[i], b = k0[n + i], c = k0[2 * n + i];
         double q = a * a + b * b + c * c;
         red.draw = fmax(red.draw, q);
         const double y0 = ycv[0], y1 = ycv[1], y2 = ycv[2];
@@ -405,7 +410,7 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
         }
         red.dydt = fmax(red.dydt, q);
     }
-    if (G.fsal_out) store3<T>(G.fsal_out, n, i, k);
+    if (G.fsal_out) store3<T>(gfield(G.fsal_out), n, i, k);
     {
         double q = er[0] * er[0] + er[1] * er[1] + er[2] * er[2];
         if (!S.Ms_uniform) {
@@ -414,7 +419,7 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
         }
         red.err = fmax(red.err, q);
     }
-    T *yn = ctl->cur ? G.yb0 : G.yb1;
+    T *yn = ctl->cur ? gfield(G.yb0) : gfield(G.yb1);
     double norm;
     if (G.norm_mode == 0) {
         norm = float_cast_norm((double)out[0], (double)out[1], (double)out[2]);
@@ -456,7 +461,7 @@ __global__ void k_stage(const SysDev<T> *__restrict__ Sp, const StageDev<T> *__r
     const int nx = S.m.nx, ny = S.m.ny, nz = S.m.nz;
     const int nrows = ny * nz;
     const int row0 = (S.xcd_remap ? xcd_tile((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x) * rpw;
-    const T *Y = G.Ys ? G.Ys : (ctl->cur ? G.yb1 : G.yb0);
+    const T *Y = G.Ys ? gfield(G.Ys) : (ctl->cur ? gfield(G.yb1) : gfield(G.yb0));
     const int L = PlanC<LC>::L(p), LP = lds_len<SK>(L), TL = PlanC<LC>::tline(p);
     // Rows are transformed in pairs, two real rows per complex line: line
     // l = (component c, pair q) holds rows gr0 = row0 + 2q (real part) and
@@ -555,7 +560,7 @@ __global__ void k_stage(const SysDev<T> *__restrict__ Sp, const StageDev<T> *__r
             if (gr0 >= nrows) continue;
             const bool has1 = gr0 + 1 < nrows;
             const C2<T> *ln = sm + (size_t)l * LP;
-            C2<T> *d0 = G.fwd_S1 + ((size_t)c * nrows + gr0) * Hp;
+            C2<T> *d0 = gfield(G.fwd_S1) + ((size_t)c * nrows + gr0) * Hp;
             for (int k = threadIdx.x; k < Hx; k += blockDim.x) {
                 const C2<T> zk = ln[sk<SK>(k)], zm = ln[sk<SK>(k == 0 ? 0 : L - k)];
                 d0[k] = cmk<T>((zk.x + zm.x) * (T)0.5, (zk.y - zm.y) * (T)0.5);
@@ -599,11 +604,12 @@ __global__ void k_stage(const SysDev<T> *__restrict__ Sp, const StageDev<T> *__r
                 ed = fmax(ed, wred[3][q]);
             }
             const size_t nb = gridDim.x, b = blockIdx.x;
-            if (G.prep == 2) G.mpart[b] = ea0;
+            double *mp = gfield(G.mpart);
+            if (G.prep == 2) mp[b] = ea0;
             if (G.want_dydt || G.dydt_from_k0) {
-                G.mpart[nb + b] = eb;
-                G.mpart[2 * nb + b] = ec;
-                G.mpart[3 * nb + b] = ed;
+                mp[nb + b] = eb;
+                mp[2 * nb + b] = ec;
+                mp[3 * nb + b] = ed;
             }
         }
     }
