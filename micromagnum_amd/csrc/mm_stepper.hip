@@ -92,6 +92,13 @@ template <typename T> struct SysDev {
     // rows of the stage launch in XCD-contiguous ranges (xcd_tile): the
     // exchange stencil's neighbour rows are then read from the same L2
     int xcd_remap;
+    // tile_zcol > 0: tiles of rpw rows align with the z planes (tile_zcol =
+    // ny / rpw tiles per plane) and consecutive tiles walk a column of
+    // planes (same y range, z = 0 .. nz-1) before moving on in y.  The
+    // z +- 1 neighbour rows of the exchange stencil are then read by tiles
+    // that run next to each other, while still in L2; in plane order they
+    // come back only after a whole plane of tiles, which evicts them.
+    int tile_zcol;
     // uniaxial anisotropy with uniform k, Ms (non-zero) and axis:
     // H = uni_c (M.a) a with uni_c = 2 k / (mu0 Ms^2), the reference's
     // 2 k (m.a) / (mu0 Ms) a with m = M / Ms; energy density uni_e |a x M|^2,
@@ -460,7 +467,12 @@ __global__ void k_stage(const SysDev<T> *__restrict__ Sp, const StageDev<T> *__r
     C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
     const int nx = S.m.nx, ny = S.m.ny, nz = S.m.nz;
     const int nrows = ny * nz;
-    const int row0 = (S.xcd_remap ? xcd_tile((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x) * rpw;
+    int tile = S.xcd_remap ? xcd_tile((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
+    if (S.tile_zcol > 0) {
+        const int ty = tile / nz;
+        tile = (tile - ty * nz) * S.tile_zcol + ty;
+    }
+    const int row0 = tile * rpw;
     const T *Y = G.Ys ? gfield(G.Ys) : (ctl->cur ? gfield(G.yb1) : gfield(G.yb0));
     const int L = PlanC<LC>::L(p), LP = lds_len<SK>(L), TL = PlanC<LC>::tline(p);
     // Rows are transformed in pairs, two real rows per complex line: line
@@ -1107,6 +1119,11 @@ template <typename T> struct Stepper : public StepperBase {
                 threads = std::min(1024, std::max(fft_threads, (std::atoi(e) + 63) / 64 * 64));
         }
         grid_rows = (d.ny * d.nz + rpw - 1) / rpw;
+        {
+            const char *e = std::getenv("MM_TILE_ZCOL");
+            const bool zcol = e ? std::atoi(e) != 0 : true;
+            sys.tile_zcol = (zcol && d.nz > 1 && d.ny % rpw == 0) ? d.ny / rpw : 0;
+        }
         MM_HIP(hipMalloc(&epart, sizeof(double) * 4 * grid_rows));
         MM_HIP(hipMalloc(&mpart, sizeof(double) * 4 * grid_rows));
         MM_HIP(hipMemset(mpart, 0, sizeof(double) * 4 * grid_rows));
