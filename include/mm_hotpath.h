@@ -251,6 +251,19 @@ int mm_stepper_energy(mm_stepper_t *st, double *E5);
 enum { MM_RELAX_DMDT = 1, MM_RELAX_DEG_PER_NS = 2 };
 int mm_stepper_relax(mm_stepper_t *st, int criterion, double threshold, int check_every,
                      long long max_steps, double *info3);
+/* ExternalField / AlternatingField with a user function `H_ext_fn`
+ * (src/magnum/micromagnetics/alternating_field.py:60-75, A = fn(t) returned
+ * as a uniform 3-vector): fn is called on the host with every evaluation
+ * time -- the stage times t + a_s h of each attempt, and t for
+ * mm_stepper_dmdt and mm_stepper_energy -- and writes the field [A/m] into
+ * h3; it takes the place of h_ext.  The description must then have no other
+ * Zeeman parameters (use_zeeman = 0): the reference raises ValueError when
+ * offs/amp/freq/phase are set beside fn (alternating_field.py:68-69).  While
+ * a function is set each attempt waits for the previous one, since its stage
+ * times depend on the step size the device chose.  fn = NULL removes it.
+ * Ensembles (mm_batch_*) do not take a function. */
+typedef void (*mm_field_fn_t)(double t, double *h3, void *user);
+int mm_stepper_set_field_fn(mm_stepper_t *st, mm_field_fn_t fn, void *user);
 int mm_stepper_destroy(mm_stepper_t *st);
 
 /* ------------------------------------------------------ ensembles */

@@ -51,6 +51,9 @@ class SystemDesc(ctypes.Structure):
                 ("use_ac", _i), ("h_amp", _d * 3), ("h_freq", _d * 3), ("h_phase", _d * 3)]
 
 
+# mm_field_fn_t: void (*)(double t, double *h3, void *user)
+FIELD_FN = ctypes.CFUNCTYPE(None, _d, _dp, _vp)
+
 # name -> (restype, argtypes); every symbol of include/mm_hotpath.h
 SIGNATURES = {
     "mm_last_error": (ctypes.c_char_p, []),
@@ -95,6 +98,7 @@ SIGNATURES = {
     "mm_stepper_kernel_profile": (_i, [_vp, _i, _dp]),
     "mm_stepper_energy": (_i, [_vp, _dp]),
     "mm_stepper_relax": (_i, [_vp, _i, _d, _i, ctypes.c_longlong, _dp]),
+    "mm_stepper_set_field_fn": (_i, [_vp, FIELD_FN, _vp]),
     "mm_stepper_destroy": (_i, [_vp]),
     "mm_batch_create": (_i, [ctypes.POINTER(SystemDesc), _i, _i, ctypes.POINTER(_vp)]),
     "mm_batch_set_m": (_i, [_vp, _i, _dp]),

@@ -99,6 +99,11 @@ template <typename T> struct SysDev {
     // that run next to each other, while still in L2; in plane order they
     // come back only after a whole plane of tiles, which evicts them.
     int tile_zcol;
+    // Zeeman field from a host function (has_fn): fn_tab[3 slot + c] is the
+    // field of stage record `slot`, evaluated on the host at that
+    // evaluation's time before the launch
+    int has_fn;
+    const double *fn_tab;
     // uniaxial anisotropy with uniform k, Ms (non-zero) and axis:
     // H = uni_c (M.a) a with uni_c = 2 k / (mu0 Ms^2), the reference's
     // 2 k (m.a) / (mu0 Ms) a with m = M / Ms; energy density uni_e |a x M|^2,
@@ -184,6 +189,9 @@ __device__ __forceinline__ V3<T> exchange_cell_uniform(const MeshDev &m, int x, 
 // stored stage vectors one stage combination reads at most (DP54's last
 // stage: k_0 .. k_5)
 #define MM_STAGE_MAXK 6
+// rows of the host-evaluated Zeeman table (stage records: at most 7 stages,
+// dM/dt and energy)
+#define MM_FN_SLOTS 16
 
 template <typename T> struct StageDev {
     const T *Ys;          // stage input (null: y[cur])
@@ -216,6 +224,9 @@ template <typename T> struct StageDev {
     // node a_s of this stage: the stage is evaluated at t + a_s h (the
     // alternating Zeeman field depends on it)
     double tnode;
+    // index of this record (stages 0..ns-1, then dM/dt, then energy): its
+    // row of the host-evaluated Zeeman table
+    int slot;
 };
 
 
@@ -517,6 +528,9 @@ __global__ void k_stage(const SysDev<T> *__restrict__ Sp, const StageDev<T> *__r
         hz = v3<T>((T)(S.hoff[0] + S.ac_amp[0] * sin(ts * S.ac_freq[0] + S.ac_phase[0])),
                    (T)(S.hoff[1] + S.ac_amp[1] * sin(ts * S.ac_freq[1] + S.ac_phase[1])),
                    (T)(S.hoff[2] + S.ac_amp[2] * sin(ts * S.ac_freq[2] + S.ac_phase[2])));
+    } else if (S.has_zee && S.has_fn) {
+        const double *f = gfield(S.fn_tab) + 3 * G.slot;
+        hz = v3<T>((T)f[0], (T)f[1], (T)f[2]);
     }
     // rows of the workgroup in (y, z): from row0 once, then stepped (no
     // per-cell integer division)
@@ -855,6 +869,7 @@ struct StepperBase {
     virtual void energy(double *E5) = 0;
     virtual void relax(int criterion, double thr, int every, long long max_steps, double *info3) = 0;
     virtual void run_until(double t_end, long long max_steps) = 0;
+    virtual void set_field_fn(mm_field_fn_t fn, void *user) = 0;
     // ensemble driving: arm(steps) sets the target, then rounds of
     // enqueue_some() on the stepper's stream and refresh() after the stream
     // is synchronised, while pending()
@@ -1127,6 +1142,14 @@ template <typename T> struct Stepper : public StepperBase {
         MM_HIP(hipMalloc(&epart, sizeof(double) * 4 * grid_rows));
         MM_HIP(hipMalloc(&mpart, sizeof(double) * 4 * grid_rows));
         MM_HIP(hipMemset(mpart, 0, sizeof(double) * 4 * grid_rows));
+        {
+            void *ft = nullptr;
+            MM_HIP(hipMalloc(&ft, sizeof(double) * 3 * MM_FN_SLOTS));
+            allocs.push_back(ft);
+            fn_tab = (double *)ft;
+            MM_HIP(hipMemset(fn_tab, 0, sizeof(double) * 3 * MM_FN_SLOTS));
+            MM_HIP(hipHostMalloc((void **)&fn_host, sizeof(double) * 3 * MM_FN_SLOTS, hipHostMallocDefault));
+        }
 
         for (int b = 0; b < 2; ++b) yb[b] = dalloc(3 * n);
         for (int q = 0; q < tab.num_steps; ++q) k[q] = dalloc(3 * n);
@@ -1165,6 +1188,8 @@ template <typename T> struct Stepper : public StepperBase {
         gtab = nullptr;
         sysd = nullptr;
         if (ctl) (void)hipFree(ctl);
+        if (fn_host) (void)hipHostFree(fn_host);
+        fn_host = nullptr;
         if (s && own_stream) (void)hipStreamDestroy(s);
         dbuf = part = epart = mpart = nullptr;
         ctl = nullptr;
@@ -1326,6 +1351,8 @@ template <typename T> struct Stepper : public StepperBase {
         G.Hout = Hbuf;
         htab.push_back(G);                      // dM/dt
         htab.push_back(blank_stage(nullptr));   // energy
+        for (size_t q = 0; q < htab.size(); ++q) htab[q].slot = (int)q;
+        MM_REQUIRE((int)htab.size() <= MM_FN_SLOTS, "too many stage records");
         for (const StageDev<T> &r : htab)
             MM_REQUIRE(r.nk >= 0 && r.nk <= MM_STAGE_MAXK, "stage combination reads too many stage vectors");
         MM_HIP(hipMalloc(&gtab, sizeof(StageDev<T>) * htab.size()));
@@ -1446,6 +1473,7 @@ template <typename T> struct Stepper : public StepperBase {
     void dmdt(double *dM_host, double *H_host) override
     {
         const int slot = tab.num_steps;
+        if (ffn) eval_field_at_t(slot);
         launch_stage(nullptr, htab[slot], gtab + slot);
         if (dM_host) download(dM_host, tmp[0], 3 * n);
         if (H_host) download(H_host, Hbuf, 3 * n);
@@ -1454,6 +1482,7 @@ template <typename T> struct Stepper : public StepperBase {
     // E_stray, E_exch, E_aniso, E_ext, E_tot at the current state
     void energy(double *E5) override
     {
+        if (ffn) eval_field_at_t(tab.num_steps + 1);
         if (conv) {
             conv->klog = nullptr;
             conv->forward_x(yb[0], s, yb[1], &ctl->cur);
@@ -1521,6 +1550,14 @@ template <typename T> struct Stepper : public StepperBase {
     // control block on entry (already on the device) and on return
     void drive(StepCtl &c, long long batch)
     {
+        while (ffn && !c.stop) {
+            // one attempt at a time: its stage times t + a_s h follow the
+            // step size the device chose
+            eval_stage_fields(c);
+            launch_attempt();
+            MM_HIP(hipStreamSynchronize(s));
+            MM_HIP(hipMemcpy(&c, ctl, sizeof(c), hipMemcpyDeviceToHost));
+        }
         while (!c.stop) {
             const long long q = std::max(1ll, std::min(batch, c.target - c.accepted));
             for (long long r = 0; r < q; ++r) launch_attempt();
@@ -1568,10 +1605,61 @@ template <typename T> struct Stepper : public StepperBase {
         MM_HIP(hipGraphLaunch(gexec[g], s));
     }
 
+    // Zeeman field from a host function (AlternatingField with H_ext_fn,
+    // alternating_field.py:60-75): evaluated at every evaluation time and
+    // uploaded to fn_tab before the launch that uses it
+    mm_field_fn_t ffn = nullptr;
+    void *fuser = nullptr;
+    double *fn_tab = nullptr;
+    double *fn_host = nullptr;  // pinned staging rows, rewritten only with the stream idle
+
+    void set_field_fn(mm_field_fn_t fn, void *user) override
+    {
+        // alternating_field.py:68-69: offs, amp, freq, phase must be zero
+        // when a function is given
+        MM_REQUIRE(!fn || !d.use_zeeman, "a Zeeman field function excludes h_ext and the alternating field parameters");
+        MM_HIP(hipStreamSynchronize(s));
+        ffn = fn;
+        fuser = user;
+        sys.has_fn = fn ? 1 : 0;
+        sys.fn_tab = fn_tab;
+        sys.has_zee = fn ? 1 : d.use_zeeman;
+        MM_HIP(hipMemcpy(sysd, &sys, sizeof(SysDev<T>), hipMemcpyHostToDevice));
+    }
+
+    // rows slot0 .. slot0+nslots-1 of fn_host to the device table, ordered
+    // on the stream before the launches that read them (the caller has
+    // synchronised the stream before filling fn_host)
+    void upload_fields(int slot0, int nslots)
+    {
+        MM_HIP(hipMemcpyAsync(fn_tab + 3 * slot0, fn_host + 3 * slot0, sizeof(double) * 3 * nslots,
+                              hipMemcpyHostToDevice, s));
+    }
+
+    // the stage times of the next attempt, as the stage kernel forms them
+    // (t + a_s h, runge_kutta.py:115); the stream is idle
+    void eval_stage_fields(const StepCtl &c)
+    {
+        const int ns = tab.num_steps;
+        for (int st = 0; st < ns; ++st) ffn(c.t + htab[st].tnode * c.h, fn_host + 3 * st, fuser);
+        upload_fields(0, ns);
+    }
+
+    // dM/dt and energies: the current time
+    void eval_field_at_t(int slot)
+    {
+        StepCtl c;
+        MM_HIP(hipStreamSynchronize(s));
+        MM_HIP(hipMemcpy(&c, ctl, sizeof(c), hipMemcpyDeviceToHost));
+        ffn(c.t, fn_host + 3 * slot, fuser);
+        upload_fields(slot, 1);
+    }
+
     StepCtl cached;
 
     void arm(long long steps) override
     {
+        MM_REQUIRE(!ffn, "a Zeeman field function is not supported in ensembles");
         MM_HIP(hipStreamSynchronize(s));
         MM_HIP(hipMemcpy(&cached, ctl, sizeof(cached), hipMemcpyDeviceToHost));
         cached.target = cached.accepted + std::max(0ll, steps);
@@ -1754,6 +1842,14 @@ extern "C" int mm_stepper_kernel_profile(mm_stepper_t *st, int attempts, double 
     MM_CAPI_BEGIN
     MM_REQUIRE(st && out && attempts > 0, "bad argument");
     st->impl->profile(attempts, out);
+    MM_CAPI_END
+}
+
+extern "C" int mm_stepper_set_field_fn(mm_stepper_t *st, mm_field_fn_t fn, void *user)
+{
+    MM_CAPI_BEGIN
+    MM_REQUIRE(st, "null argument");
+    st->impl->set_field_fn(fn, user);
     MM_CAPI_END
 }
 

@@ -97,15 +97,70 @@ class SystemSpec:
         self.method = method
 
 
+def _zeeman_fn_args(kw):
+    """Take h_ext_fn out of the keyword arguments.  As the reference's
+    AlternatingField (alternating_field.py:67-69), a function excludes the
+    offset/amplitude/frequency/phase parameters unless they are zero vectors."""
+    fn = kw.pop("h_ext_fn", None)
+    if fn is None:
+        return None
+    names = ("h_ext", "h_ext_amp", "h_ext_freq", "h_ext_phase")
+    for name in names:
+        v = kw.get(name)
+        if v is not None and any(float(x) != 0.0 for x in v):
+            raise ValueError("If h_ext_fn is defined, the parameters %s must be zero vectors, i.e. (0.0, 0.0, 0.0)"
+                             % ", ".join(names))
+        kw.pop(name, None)
+    return fn
+
+
 class Stepper:
     def __init__(self, shape, delta, Ms, alpha, **kw):
+        """Keyword arguments as SystemSpec, plus h_ext_fn: a function of the
+        time t returning the uniform Zeeman field (a 3-vector, A/m), the
+        reference's ExternalField with H_ext_fn; it is called on the host at
+        every stage time."""
         L = _lib.device_lib()
+        kw = dict(kw)
+        fn = _zeeman_fn_args(kw)
         spec = SystemSpec(shape, delta, Ms, alpha, **kw)
         self.shape, self.delta, self.n, self.pbc = spec.shape, spec.delta, spec.n, spec.pbc
         self.N, self.precision, self.method = spec.N, spec.precision, spec.method
         h = ctypes.c_void_p()
         _lib.check(L.mm_stepper_create(ctypes.byref(spec.desc), ctypes.byref(h)))
         self._h = h
+        self._fn_cb = None
+        self._fn_error = None
+        if fn is not None:
+            self.set_field_fn(fn)
+
+    def set_field_fn(self, fn):
+        """Zeeman field from fn(t) -> 3-vector (None removes it)."""
+        if fn is None:
+            _lib.check(_lib.load().mm_stepper_set_field_fn(self._h, _lib.FIELD_FN(), None))
+            self._fn_cb = None
+            return
+
+        def cb(t, h, user):
+            try:
+                v = fn(t)
+                v = [float(x) for x in v]
+                if len(v) != 3:
+                    raise TypeError("h_ext_fn must return a 3-vector (a uniform field)")
+                h[0], h[1], h[2] = v
+            except BaseException as e:  # reported after the call that evaluated fn
+                if self._fn_error is None:
+                    self._fn_error = e
+                h[0] = h[1] = h[2] = float("nan")
+
+        cb_c = _lib.FIELD_FN(cb)
+        _lib.check(_lib.load().mm_stepper_set_field_fn(self._h, cb_c, None))
+        self._fn_cb = cb_c
+
+    def _fn_check(self):
+        if self._fn_error is not None:
+            e, self._fn_error = self._fn_error, None
+            raise e
 
     def _dp(self, a):
         return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
@@ -125,15 +180,18 @@ class Stepper:
         dM = np.empty((3, nz, ny, nx))
         H = np.empty((3, nz, ny, nx))
         _lib.check(_lib.load().mm_stepper_dmdt(self._h, self._dp(dM), self._dp(H)))
+        self._fn_check()
         return dM, H
 
     def run(self, steps):
         _lib.check(_lib.load().mm_stepper_run(self._h, int(steps)))
+        self._fn_check()
 
     def run_until(self, t_end, max_steps=10000000):
         """Step until t == t_end (the last step clipped as in the reference's
         RungeKutta.evolve) or max_steps accepted steps."""
         _lib.check(_lib.load().mm_stepper_run_until(self._h, float(t_end), int(max_steps)))
+        self._fn_check()
 
     def sync(self):
         _lib.check(_lib.load().mm_stepper_sync(self._h))
@@ -155,6 +213,7 @@ class Stepper:
         E_stray, E_exch, E_aniso, E_ext and their sum E_tot)."""
         e = np.zeros(5)
         _lib.check(_lib.load().mm_stepper_energy(self._h, self._dp(e)))
+        self._fn_check()
         return dict(zip(self.ENERGY_TERMS, e.tolist()))
 
     def relax(self, threshold=1e7, criterion="dmdt", check_every=1, max_steps=1000000):
@@ -165,6 +224,7 @@ class Stepper:
         info = np.zeros(3)
         _lib.check(_lib.load().mm_stepper_relax(self._h, crit, float(threshold), int(check_every),
                                                 int(max_steps), self._dp(info)))
+        self._fn_check()
         return dict(steps=int(info[0]), relaxed=bool(info[1]), value=float(info[2]))
 
     KERNEL_CLASSES = ("x_fwd", "y_fwd", "z_mult", "y_inv", "y_mult_2d", "x_inv_stage", "controller")

@@ -781,6 +781,11 @@ void orc_llge_factors(const orc_system *s, double *f1, double *f2)
  * (external_field.py:21-37, alternating_field.py:57-84) */
 static double zeeman_at(const orc_system *s, int c, double t)
 {
+    if (s->h_fn) {
+        double h[3];
+        s->h_fn(t, h, s->h_user);
+        return h[c];
+    }
     if (!s->use_ac) return s->h_ext[c];
     return s->h_ext[c] + s->h_amp[c] * sin(t * s->h_freq[c] + s->h_phase[c]);
 }

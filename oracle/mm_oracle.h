@@ -96,6 +96,11 @@ typedef struct {
      * step h: t0 + a_s h, runge_kutta.py:115) */
     int use_ac;
     double h_amp[3], h_freq[3], h_phase[3];
+    /* AlternatingField with a user function (alternating_field.py:60-75):
+     * h_fn(t, h3, h_user) gives the uniform field at t; it replaces h_ext
+     * and the alternating parameters (use_zeeman must be set) */
+    void (*h_fn)(double t, double *h3, void *user);
+    void *h_user;
 } orc_system;
 
 void orc_llge_factors(const orc_system *s, double *f1, double *f2);

@@ -202,7 +202,9 @@ class _System(ctypes.Structure):
                 ("P", _dp), ("xi", _dp), ("j", _dp), ("do_precess", ctypes.c_int),
                 ("normalize_mode", ctypes.c_int), ("use_ac", ctypes.c_int),
                 ("h_amp", ctypes.c_double * 3), ("h_freq", ctypes.c_double * 3),
-                ("h_phase", ctypes.c_double * 3)]
+                ("h_phase", ctypes.c_double * 3),
+                ("h_fn", ctypes.CFUNCTYPE(None, ctypes.c_double, _dp, ctypes.c_void_p)),
+                ("h_user", ctypes.c_void_p)]
 
 
 class System:
@@ -212,9 +214,11 @@ class System:
     def __init__(self, shape, delta, Ms, alpha, A=None, N=None, k_uni=None, k_cub=None,
                  axis1=None, axis2=None, h_ext=None, P=None, xi=None, j=None, do_precess=True,
                  normalize_mode=0, pbc=(False, False, False), h_ext_amp=None, h_ext_freq=None,
-                 h_ext_phase=None):
+                 h_ext_phase=None, h_ext_fn=None):
         """h_ext is the Zeeman offset; h_ext_amp/freq/phase (3-tuples) make it
-        the reference's AlternatingField, h_ext + amp sin(t freq + phase)."""
+        the reference's AlternatingField, h_ext + amp sin(t freq + phase);
+        h_ext_fn(t) -> 3-vector replaces them (AlternatingField with a user
+        function)."""
         nx, ny, nz = shape
         self._keep = []
 
@@ -256,6 +260,15 @@ class System:
                 s.h_amp[c] = float(h_ext_amp[c])
                 s.h_freq[c] = float(h_ext_freq[c]) if h_ext_freq is not None else 0.0
                 s.h_phase[c] = float(h_ext_phase[c]) if h_ext_phase is not None else 0.0
+        if h_ext_fn is not None:
+            s.use_zeeman = 1
+
+            def cb(t, h, user):
+                v = h_ext_fn(t)
+                h[0], h[1], h[2] = float(v[0]), float(v[1]), float(v[2])
+
+            self._fn_cb = _System._fields_[-2][1](cb)
+            s.h_fn = self._fn_cb
         s.P = _p(field(P))
         s.xi = _p(field(xi))
         s.j = _p(field(j, 3))

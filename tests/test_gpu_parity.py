@@ -459,6 +459,123 @@ def test_stepper_alternating_field_vs_oracle(method):
     st.close()
 
 
+def _pulse_field(t):
+    # a static x field, a Gaussian pulse along y and a ramp along z
+    import math
+    g = math.exp(-((t - 4e-12) / 2e-12) ** 2)
+    return (2e4, 6e4 * g, -3e3 + 1e15 * t)
+
+
+@pytest.mark.parametrize("method", ["rkf45", "dp54", "rk4"])
+def test_stepper_field_function_vs_oracle(method):
+    # Zeeman field from a user function of t (alternating_field.py:60-75),
+    # evaluated on the host at every stage time t + a_s h; then dM/dt and
+    # the energies at the reached time
+    from micromagnum_amd.solver import Stepper
+    shape, delta = (20, 16, 2), (3e-9, 3e-9, 2e-9)
+    kw = dict(Ms=8.6e5, alpha=0.1, A=1.3e-11)
+    st = Stepper(shape, delta, precision="fp64", method=method, eps_abs=1e-5, eps_rel=1e-5, h0=1e-13,
+                 h_ext_fn=_pulse_field, **kw)
+    M = vortex_field(shape, delta, (30e-9, 24e-9), Ms=8.6e5)
+    st.set_m(M)
+    st.run(15)
+    s = st.stats()
+    sysO = O.System(shape, delta, N=st.N, h_ext_fn=_pulse_field, **kw)
+    Mo, so = sysO.run(M, method=method, steps=15, eps_abs=1e-5, eps_rel=1e-5, h0=1e-13)
+    assert s["attempts"] == so["attempts"] and s["rejections"] == so["rejections"]
+    assert abs(s["t"] - so["t"]) <= 1e-9 * so["t"]
+    Mg = st.get_m()
+    assert np.abs(Mg - Mo).max() / 8.6e5 < 1e-9
+    dM, H = st.dmdt()
+    dMo, Ho = sysO.dmdt(Mg, s["t"])
+    assert rel_err(dM, dMo) < TOL64 and rel_err(H, Ho) < TOL64
+    E, Eo = st.energy(), sysO.energy(Mg, s["t"])
+    assert E["E_ext"] == pytest.approx(Eo["E_ext"], rel=1e-12)
+    st.close()
+
+
+def test_stepper_field_function_run_until_and_precisions():
+    # run_until with the last step clipped, FP64 against the oracle and the
+    # mixed FP32 path against FP64
+    from micromagnum_amd import magneto as mg
+    from micromagnum_amd.solver import Stepper
+    shape, delta = (24, 20, 2), (3e-9, 3e-9, 2e-9)
+    N = mg.GenerateDemagTensor(*shape, *delta)
+    phys = dict(Ms=8.6e5, alpha=0.05, A=1.3e-11)
+    M = vortex_field(shape, delta, (36e-9, 30e-9), Ms=8.6e5)
+    t_end = 7.3e-12
+    out = {}
+    for prec in ("fp64", "fp32"):
+        st = Stepper(shape, delta, N=N, precision=prec, method="rkf45", eps_abs=1e-5, eps_rel=1e-5, h0=1e-13,
+                     h_ext_fn=_pulse_field, **phys)
+        st.set_m(M)
+        st.run_until(t_end)
+        out[prec] = (st.get_m(), st.stats())
+        st.close()
+    Mo, so = O.System(shape, delta, N=N, h_ext_fn=_pulse_field, **phys).run_until(
+        M, t_end, method="rkf45", eps_abs=1e-5, eps_rel=1e-5, h0=1e-13)
+    (M64, s64), (M32, s32) = out["fp64"], out["fp32"]
+    assert s64["t"] == pytest.approx(t_end, rel=1e-15) and s32["t"] == pytest.approx(t_end, rel=1e-15)
+    assert s64["attempts"] == so["attempts"]
+    assert np.abs(M64 - Mo).max() / 8.6e5 < 1e-9
+    assert np.abs(M32 - M64).max() / 8.6e5 < 1e-4
+    assert np.abs(M64 - M).max() / 8.6e5 > 1e-3
+
+
+def test_stepper_field_function_equals_alternating_parameters():
+    # the alternating field as a host function gives the device-evaluated
+    # alternating field's trajectory (the stage times and sines may differ in
+    # the last bit between host and device)
+    import math
+    from micromagnum_amd.solver import Stepper
+    offs, amp, w, phi = (1e4, 0.0, -5e3), (2e4, 3e4, 5e4), (2e10, 3e10, 5e10), (0.1, -0.7, 1.2)
+    fn = lambda t: tuple(offs[c] + amp[c] * math.sin(t * w[c] + phi[c]) for c in range(3))
+    shape, delta = (20, 16, 2), (3e-9, 3e-9, 2e-9)
+    kw = dict(Ms=8.6e5, alpha=0.1, A=1.3e-11, precision="fp64", method="dp54", eps_abs=1e-5, eps_rel=1e-5,
+              h0=1e-13)
+    M = vortex_field(shape, delta, (30e-9, 24e-9), Ms=8.6e5)
+    a = Stepper(shape, delta, h_ext=offs, h_ext_amp=amp, h_ext_freq=w, h_ext_phase=phi, **kw)
+    b = Stepper(shape, delta, h_ext_fn=fn, **kw)
+    for st in (a, b):
+        st.set_m(M)
+        st.run(12)
+    sa, sb = a.stats(), b.stats()
+    assert sa["attempts"] == sb["attempts"] and sa["t"] == pytest.approx(sb["t"], rel=1e-13)
+    assert np.abs(a.get_m() - b.get_m()).max() / 8.6e5 < 1e-12
+    a.close()
+    b.close()
+
+
+def test_stepper_field_function_errors_and_removal():
+    from micromagnum_amd.solver import Stepper
+    shape, delta = (8, 8, 1), (3e-9, 3e-9, 2e-9)
+    kw = dict(Ms=8.6e5, alpha=0.1, A=1.3e-11, precision="fp64", method="rkf45", demag=False)
+    M = vortex_field(shape, delta, (12e-9, 12e-9), Ms=8.6e5)
+
+    def bad(t):
+        raise ZeroDivisionError("from the field function")
+
+    st = Stepper(shape, delta, h_ext_fn=bad, **kw)
+    st.set_m(M)
+    with pytest.raises(ZeroDivisionError):
+        st.run(1)
+    with pytest.raises(ValueError):
+        Stepper(shape, delta, h_ext_fn=_pulse_field, h_ext=(1.0, 0.0, 0.0), **kw)
+    st.close()
+    # removing the function leaves a system without Zeeman field
+    st = Stepper(shape, delta, h_ext_fn=_pulse_field, **kw)
+    st.set_field_fn(None)
+    st.set_m(M)
+    plain = Stepper(shape, delta, **kw)
+    plain.set_m(M)
+    st.run(5)
+    plain.run(5)
+    assert np.array_equal(st.get_m(), plain.get_m())
+    assert st.energy()["E_ext"] == 0.0
+    st.close()
+    plain.close()
+
+
 @pytest.mark.parametrize("method", ["rkf45", "dp54", "rk4"])
 def test_stepper_run_until_vs_oracle(method):
     # stepping to a fixed end time: the last accepted step is redone with

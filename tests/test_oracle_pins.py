@@ -305,6 +305,62 @@ def test_alternating_field_precession_closed_form(method):
     assert np.abs(M.reshape(3) - want).max() / 8e5 < 2e-6
 
 
+@pytest.mark.parametrize("method", ["rkf45", "cc45", "rk4"])
+def test_field_function_ramp_closed_form(method):
+    # AlternatingField with a user function (alternating_field.py:60-75):
+    # H = (0, 0, a + b t) on one undamped cell precesses M in the x-y plane
+    # by theta(t) = gamma (a t + b t^2 / 2).  Evaluating the function at the
+    # attempt's start time instead of t0 + a_s h leaves an O(h) error.
+    gamma = 2.211e5
+    a, b = 4.0e4, 3.0e14
+    s = O.System((1, 1, 1), (2e-9,) * 3, Ms=8e5, alpha=0.0, h_ext_fn=lambda t: (0.0, 0.0, a + b * t))
+    M0 = np.zeros((3, 1, 1, 1))
+    M0[0] = 8e5
+    t_end = 1.9e-10
+    M, st = s.run_until(M0, t_end, method=method, eps_abs=1e-9, eps_rel=1e-9, h0=1e-14)
+    assert st["t"] == pytest.approx(t_end, rel=1e-15)
+    theta = gamma * (a * t_end + 0.5 * b * t_end ** 2)
+    assert theta > 2.5
+    want = 8e5 * np.array([np.cos(theta), np.sin(theta), 0.0])
+    assert np.abs(M.reshape(3) - want).max() / 8e5 < 2e-6
+
+
+@pytest.mark.parametrize("method", ["rkf45", "dp54"])
+def test_field_function_equals_alternating_parameters(method):
+    # the same field given as offs + amp sin(t freq + phase) parameters and
+    # as a function of t: the same trajectory, bit for bit
+    import math
+    offs, amp, w, phi = (1.0e4, 0.0, -5.0e3), (2.0e4, 3.0e4, 5.0e4), (2e10, 3e10, 5e10), (0.1, -0.7, 1.2)
+    fn = lambda t: tuple(offs[c] + amp[c] * math.sin(t * w[c] + phi[c]) for c in range(3))
+    shape, delta = (6, 5, 2), (3e-9, 3e-9, 2e-9)
+    kw = dict(Ms=8.6e5, alpha=0.1, A=1.3e-11)
+    M0 = np.random.default_rng(5).standard_normal((3, 2, 5, 6))
+    M0 *= 8.6e5 / np.sqrt((M0 ** 2).sum(axis=0))
+    s1 = O.System(shape, delta, h_ext=offs, h_ext_amp=amp, h_ext_freq=w, h_ext_phase=phi, **kw)
+    s2 = O.System(shape, delta, h_ext_fn=fn, **kw)
+    M1, st1 = s1.run(M0, method=method, steps=12, eps_abs=1e-5, eps_rel=1e-5)
+    M2, st2 = s2.run(M0, method=method, steps=12, eps_abs=1e-5, eps_rel=1e-5)
+    assert np.array_equal(M1, M2) and st1 == st2
+    t = st1["t"]
+    assert np.array_equal(s1.h_tot(M1, t), s2.h_tot(M1, t))
+    assert s1.energy(M1, t) == s2.energy(M1, t)
+
+
+def test_field_function_excludes_parameters():
+    # alternating_field.py:67-69: with a function, offs/amp/freq/phase must
+    # be zero vectors (zero vectors are accepted and dropped)
+    from micromagnum_amd.solver import _zeeman_fn_args
+    f = lambda t: (0.0, 0.0, 1.0)
+    with pytest.raises(ValueError):
+        _zeeman_fn_args(dict(h_ext_fn=f, h_ext=(1.0, 0.0, 0.0)))
+    with pytest.raises(ValueError):
+        _zeeman_fn_args(dict(h_ext_fn=f, h_ext_amp=(0.0, 2.0, 0.0)))
+    kw = dict(h_ext_fn=f, h_ext=(0.0, 0.0, 0.0), h_ext_phase=(0.0, 0.0, 0.0), A=1e-11)
+    assert _zeeman_fn_args(kw) is f and kw == dict(A=1e-11)
+    kw = dict(h_ext=(1.0, 0.0, 0.0))
+    assert _zeeman_fn_args(kw) is None and kw == dict(h_ext=(1.0, 0.0, 0.0))
+
+
 def test_alternating_field_value_and_energy_at_time():
     offs, amp, w, phi = (1.0e3, -2.0e3, 5.0e4), (3.0e4, 0.0, 1.0e4), (1e10, 2e10, 3e9), (0.1, 0.2, -0.4)
     s = O.System((2, 1, 1), (2e-9, 3e-9, 4e-9), Ms=8e5, alpha=0.1, h_ext=offs, h_ext_amp=amp, h_ext_freq=w,
