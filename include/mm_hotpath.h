@@ -266,6 +266,36 @@ typedef void (*mm_field_fn_t)(double t, double *h3, void *user);
 int mm_stepper_set_field_fn(mm_stepper_t *st, mm_field_fn_t fn, void *user);
 int mm_stepper_destroy(mm_stepper_t *st);
 
+/* ---------------------------------------------- launch shapes ("wisdom") */
+/* The reference keeps FFTW's tuned plans ("wisdom") in a file that
+ * magneto.initialize/deinitialize import and export
+ * (src/magneto/Magneto.cpp:151-190).  Here the tuned choices are launch
+ * shapes: x transform lines per workgroup, kx columns per workgroup of the y
+ * and z passes, rows per stage workgroup and threads of the stage launch
+ * (0 = the built-in choice).  A process-wide table keyed by mesh size,
+ * precision and device holds them; convolutions and steppers created later
+ * for that key use the entry (the MM_LPW_X, MM_TK_Y, MM_TK_Z, MM_RPW and
+ * MM_STAGE_THREADS environment variables take precedence).  Launch shapes do
+ * not change results: every shape computes the same values. */
+typedef struct {
+    int lpw_x, tk_y, tk_z, rpw, stage_threads;
+} mm_launch_shape_t;
+/* the shape a stepper runs with */
+int mm_stepper_launch_shape(mm_stepper_t *st, mm_launch_shape_t *out);
+/* Measure `steps` accepted steps of steppers for desc with the built-in shape
+ * and with variations of one knob at a time (coordinate descent), record the
+ * fastest in the table and return it with its time per step [ms].  Takes
+ * seconds: one stepper set-up per candidate. */
+int mm_autotune(const mm_system_desc_t *desc, int steps, mm_launch_shape_t *best, double *ms_per_step);
+/* Text file: '#' comments, then one entry per line,
+ * "<nx>x<ny>x<nz> <fp32|fp64> <arch>/<CUs> lpw_x tk_y tk_z rpw stage_threads".
+ * Import adds (replacing equal keys) all entries of a valid file and nothing
+ * from an invalid one (the reference logs an invalid FFTW wisdom file and
+ * goes on without it). */
+int mm_wisdom_import(const char *path, int *n_entries);
+int mm_wisdom_export(const char *path);
+int mm_wisdom_forget(void);
+
 /* ------------------------------------------------------ ensembles */
 /* Many independent systems (each a full mm_system_desc_t: own mesh, material,
  * method, precision) stepped together: the batched variant of the fused

@@ -51,6 +51,10 @@ class SystemDesc(ctypes.Structure):
                 ("use_ac", _i), ("h_amp", _d * 3), ("h_freq", _d * 3), ("h_phase", _d * 3)]
 
 
+class LaunchShape(ctypes.Structure):
+    _fields_ = [("lpw_x", _i), ("tk_y", _i), ("tk_z", _i), ("rpw", _i), ("stage_threads", _i)]
+
+
 # mm_field_fn_t: void (*)(double t, double *h3, void *user)
 FIELD_FN = ctypes.CFUNCTYPE(None, _d, _dp, _vp)
 
@@ -100,6 +104,11 @@ SIGNATURES = {
     "mm_stepper_relax": (_i, [_vp, _i, _d, _i, ctypes.c_longlong, _dp]),
     "mm_stepper_set_field_fn": (_i, [_vp, FIELD_FN, _vp]),
     "mm_stepper_destroy": (_i, [_vp]),
+    "mm_stepper_launch_shape": (_i, [_vp, ctypes.POINTER(LaunchShape)]),
+    "mm_autotune": (_i, [ctypes.POINTER(SystemDesc), _i, ctypes.POINTER(LaunchShape), _dp]),
+    "mm_wisdom_import": (_i, [ctypes.c_char_p, _ip]),
+    "mm_wisdom_export": (_i, [ctypes.c_char_p]),
+    "mm_wisdom_forget": (_i, []),
     "mm_batch_create": (_i, [ctypes.POINTER(SystemDesc), _i, _i, ctypes.POINTER(_vp)]),
     "mm_batch_set_m": (_i, [_vp, _i, _dp]),
     "mm_batch_get_m": (_i, [_vp, _i, _dp]),

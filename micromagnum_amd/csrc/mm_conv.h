@@ -165,4 +165,19 @@ struct mm_conv {
 void mm_lds_attr_once();
 int mm_max_lds_bytes();
 
+// Launch shapes of a mesh (0: not set, the built-in choice applies): x lines
+// per workgroup, kx columns per workgroup of the y and z passes, rows per
+// stage workgroup and stage threads.  Tuned shapes live in a process-wide
+// table keyed by mesh size, precision and device (the counterpart of the
+// reference's FFTW wisdom, Magneto.cpp:151-190).  mm_shape_for gives the
+// shape in force for a new convolution or stepper: the tuner's candidate
+// while it measures one (this thread), else the table's entry.  The MM_LPW_X,
+// MM_TK_Y, MM_TK_Z, MM_RPW and MM_STAGE_THREADS variables override both.
+struct MmShape {
+    int lpw_x, tk_y, tk_z, rpw, stage_threads;
+};
+MmShape mm_shape_for(int nx, int ny, int nz, size_t real_bytes);
+void mm_shape_set_candidate(const MmShape *s);
+void mm_wisdom_put(int nx, int ny, int nz, size_t real_bytes, const MmShape &s);
+
 #endif

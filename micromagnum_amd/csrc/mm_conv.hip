@@ -29,7 +29,11 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <map>
 #include <mutex>
+#include <sstream>
+#include <string>
 
 // ---------------------------------------------------------------------------
 // FFT plan
@@ -605,20 +609,145 @@ template <int SK> static size_t lines_lds(int nlines, int L, size_t csize)
     return ((size_t)nlines * lds_len<SK>(L) + (size_t)L) * csize;
 }
 
+// ---------------------------------------------------------------------------
+// launch-shape table ("wisdom")
+// ---------------------------------------------------------------------------
+static std::mutex g_wis_mu;
+static std::map<std::string, MmShape> g_wisdom;
+static thread_local MmShape g_candidate;
+static thread_local bool g_have_candidate = false;
+
+// "<arch>/<CUs>" of the current device, e.g. gfx950/256
+static std::string device_tag()
+{
+    static std::mutex mu;
+    static std::map<int, std::string> tags;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = tags.find(dev);
+    if (it != tags.end()) return it->second;
+    hipDeviceProp_t p;
+    std::string tag = "unknown";
+    if (hipGetDeviceProperties(&p, dev) == hipSuccess) {
+        std::string arch = p.gcnArchName;
+        tag = arch.substr(0, arch.find(':')) + "/" + std::to_string(p.multiProcessorCount);
+    }
+    tags[dev] = tag;
+    return tag;
+}
+
+static std::string shape_key(int nx, int ny, int nz, size_t real_bytes)
+{
+    return std::to_string(nx) + "x" + std::to_string(ny) + "x" + std::to_string(nz) +
+           (real_bytes == 4 ? " fp32 " : " fp64 ") + device_tag();
+}
+
+MmShape mm_shape_for(int nx, int ny, int nz, size_t real_bytes)
+{
+    if (g_have_candidate) return g_candidate;
+    const std::string key = shape_key(nx, ny, nz, real_bytes);
+    std::lock_guard<std::mutex> lk(g_wis_mu);
+    auto it = g_wisdom.find(key);
+    return it != g_wisdom.end() ? it->second : MmShape{0, 0, 0, 0, 0};
+}
+
+void mm_shape_set_candidate(const MmShape *s)
+{
+    g_have_candidate = s != nullptr;
+    if (s) g_candidate = *s;
+}
+
+void mm_wisdom_put(int nx, int ny, int nz, size_t real_bytes, const MmShape &s)
+{
+    const std::string key = shape_key(nx, ny, nz, real_bytes);
+    std::lock_guard<std::mutex> lk(g_wis_mu);
+    g_wisdom[key] = s;
+}
+
+// File format: '#' comment lines, then one entry per line,
+// "<nx>x<ny>x<nz> <fp32|fp64> <arch>/<CUs> lpw_x tk_y tk_z rpw stage_threads".
+// An import reads the whole file first and takes nothing from an invalid one.
+extern "C" int mm_wisdom_import(const char *path, int *n_entries)
+{
+    MM_CAPI_BEGIN
+    MM_REQUIRE(path, "null argument");
+    std::ifstream f(path);
+    MM_REQUIRE((bool)f, std::string("cannot open wisdom file ") + path);
+    std::map<std::string, MmShape> got;
+    std::string line;
+    int lineno = 0;
+    while (std::getline(f, line)) {
+        ++lineno;
+        const size_t b = line.find_first_not_of(" \t\r");
+        if (b == std::string::npos || line[b] == '#') continue;
+        std::istringstream ls(line);
+        std::string mesh, prec, dev, extra;
+        MmShape v;
+        const bool ok = (bool)(ls >> mesh >> prec >> dev >> v.lpw_x >> v.tk_y >> v.tk_z >> v.rpw >> v.stage_threads) &&
+                        !(ls >> extra);
+        int nx = 0, ny = 0, nz = 0;
+        char x1 = 0, x2 = 0;
+        std::istringstream ms(mesh);
+        const bool mesh_ok = (bool)(ms >> nx >> x1 >> ny >> x2 >> nz) && x1 == 'x' && x2 == 'x' && nx > 0 && ny > 0 &&
+                             nz > 0 && !(ms >> extra);
+        const bool vals_ok = v.lpw_x >= 0 && v.tk_y >= 0 && v.tk_z >= 0 && v.rpw >= 0 && v.stage_threads >= 0 &&
+                             v.lpw_x <= 1024 && v.tk_y <= 1024 && v.tk_z <= 1024 && v.rpw <= 1024 &&
+                             v.stage_threads <= 1024;
+        MM_REQUIRE(ok && mesh_ok && vals_ok && (prec == "fp32" || prec == "fp64"),
+                   std::string("invalid wisdom file ") + path + " at line " + std::to_string(lineno));
+        got[mesh + " " + prec + " " + dev] = v;
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_wis_mu);
+        for (const auto &kv : got) g_wisdom[kv.first] = kv.second;
+    }
+    if (n_entries) *n_entries = (int)got.size();
+    MM_CAPI_END
+}
+
+extern "C" int mm_wisdom_export(const char *path)
+{
+    MM_CAPI_BEGIN
+    MM_REQUIRE(path, "null argument");
+    std::ostringstream os;
+    os << "# libmmhip launch-shape wisdom: mesh precision device lpw_x tk_y tk_z rpw stage_threads\n";
+    {
+        std::lock_guard<std::mutex> lk(g_wis_mu);
+        for (const auto &kv : g_wisdom)
+            os << kv.first << " " << kv.second.lpw_x << " " << kv.second.tk_y << " " << kv.second.tk_z << " "
+               << kv.second.rpw << " " << kv.second.stage_threads << "\n";
+    }
+    std::ofstream f(path);
+    MM_REQUIRE((bool)f, std::string("cannot write wisdom file ") + path);
+    f << os.str();
+    MM_REQUIRE((bool)f, std::string("cannot write wisdom file ") + path);
+    MM_CAPI_END
+}
+
+extern "C" int mm_wisdom_forget(void)
+{
+    MM_CAPI_BEGIN
+    std::lock_guard<std::mutex> lk(g_wis_mu);
+    g_wisdom.clear();
+    MM_CAPI_END
+}
+
 static ConvLaunch choose_launch(const ConvGeom &g, const FftPlanHost &px, const FftPlanHost &py,
                                 const FftPlanHost &pz, size_t csize)
 {
     const int lds = mm_max_lds_bytes();
+    const MmShape w = mm_shape_for(g.nx, g.ny, g.nz, csize / 2);
     ConvLaunch ln;
-    ln.lpw_x = env_int("MM_LPW_X", std::max(1, 256 / px.tline));
+    ln.lpw_x = env_int("MM_LPW_X", w.lpw_x > 0 ? w.lpw_x : std::max(1, 256 / px.tline));
     while (ln.lpw_x > 1 && (lines_lds<MM_SKEW_X>(ln.lpw_x, px.L, csize) > (size_t)lds || ln.lpw_x * px.tline > 1024)) --ln.lpw_x;
     const int nl_y = g.is2d ? 3 : 1;
-    int tk = env_int("MM_TK_Y", 8);
+    int tk = env_int("MM_TK_Y", w.tk_y > 0 ? w.tk_y : 8);
     while (tk > 1 && ((size_t)nl_y * tk * py.tline > 1024 || lines_lds<MM_SKEW_Y>(nl_y * tk, py.L, csize) > (size_t)lds)) tk /= 2;
     ln.tk_y = tk;
     MM_REQUIRE((size_t)nl_y * tk * py.tline <= 1024 && lines_lds<MM_SKEW_Y>(nl_y * tk, py.L, csize) <= (size_t)lds,
                "y transform length too large for one workgroup");
-    int tz = env_int("MM_TK_Z", 16);
+    int tz = env_int("MM_TK_Z", w.tk_z > 0 ? w.tk_z : 16);
     while (tz > 1 && ((size_t)3 * tz * pz.tline > 1024 || lines_lds<MM_SKEW_Z>(3 * tz, pz.L, csize) > (size_t)lds)) tz /= 2;
     ln.tk_z = tz;
     return ln;

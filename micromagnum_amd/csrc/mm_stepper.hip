@@ -24,8 +24,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
+#include <memory>
 #include <vector>
 
 #include "mm_cell.h"
@@ -870,6 +873,7 @@ struct StepperBase {
     virtual void relax(int criterion, double thr, int every, long long max_steps, double *info3) = 0;
     virtual void run_until(double t_end, long long max_steps) = 0;
     virtual void set_field_fn(mm_field_fn_t fn, void *user) = 0;
+    virtual MmShape launch_shape() const = 0;
     // ensemble driving: arm(steps) sets the target, then rounds of
     // enqueue_some() on the stepper's stream and refresh() after the stream
     // is synchronised, while pending()
@@ -1093,6 +1097,8 @@ template <typename T> struct Stepper : public StepperBase {
             }
         }
 
+        // launch shape in force for this mesh (tuner candidate or table entry)
+        const MmShape wshape = mm_shape_for(d.nx, d.ny, d.nz, sizeof(T));
         if (d.N_host) {
             conv = new ConvImpl<T>();
             conv->init(d.N_host, d.nx, d.ny, d.nz, d.ex, d.ey, d.ez);
@@ -1110,6 +1116,7 @@ template <typename T> struct Stepper : public StepperBase {
                 // rows go through the x transforms in pairs (two real rows per
                 // complex line): np pairs, 3 np lines, rpw = 2 np rows per block
                 int np = std::max(1, 192 / (3 * tl));
+                if (wshape.rpw > 0) np = std::max(1, wshape.rpw / 2);
                 if (const char *e = std::getenv("MM_RPW")) np = std::max(1, std::atoi(e) / 2);
                 while (np > 1 && ((size_t)3 * np * lds_len<MM_SKEW_X>((int)L) + L) * cs > (size_t)mm_max_lds_bytes()) --np;
                 rpw = 2 * np;
@@ -1130,6 +1137,8 @@ template <typename T> struct Stepper : public StepperBase {
             // FP32 and 74.9 vs 83.8 us FP64.  MM_STAGE_THREADS overrides.
             const int fft_threads = threads;
             threads = std::max(fft_threads, std::min(1024, (rpw * d.nx / 2 + 63) / 64 * 64));
+            if (wshape.stage_threads > 0)
+                threads = std::min(1024, std::max(fft_threads, (wshape.stage_threads + 63) / 64 * 64));
             if (const char *e = std::getenv("MM_STAGE_THREADS"))
                 threads = std::min(1024, std::max(fft_threads, (std::atoi(e) + 63) / 64 * 64));
         }
@@ -1613,6 +1622,21 @@ template <typename T> struct Stepper : public StepperBase {
     double *fn_tab = nullptr;
     double *fn_host = nullptr;  // pinned staging rows, rewritten only with the stream idle
 
+    MmShape launch_shape() const override
+    {
+        MmShape r = {0, 0, 0, 0, 0};
+        if (conv) {
+            r.lpw_x = conv->ln.lpw_x;
+            r.tk_y = conv->ln.tk_y;
+            r.tk_z = conv->ln.tk_z;
+        }
+        if (conv && !split_x) {
+            r.rpw = rpw;
+            r.stage_threads = threads;
+        }
+        return r;
+    }
+
     void set_field_fn(mm_field_fn_t fn, void *user) override
     {
         // alternating_field.py:68-69: offs, amp, freq, phase must be zero
@@ -1745,29 +1769,165 @@ struct mm_stepper {
     StepperBase *impl;
 };
 
+static StepperBase *make_stepper(const mm_system_desc_t &d)
+{
+    MM_REQUIRE(d.precision == MM_FP64 || d.precision == MM_FP32, "bad precision");
+    if (d.precision == MM_FP64) {
+        auto *p = new Stepper<double>();
+        try {
+            p->init(d);
+        } catch (...) {
+            delete p;
+            throw;
+        }
+        return p;
+    }
+    auto *p = new Stepper<float>();
+    try {
+        p->init(d);
+    } catch (...) {
+        delete p;
+        throw;
+    }
+    return p;
+}
+
 extern "C" int mm_stepper_create(const mm_system_desc_t *desc, mm_stepper_t **out)
 {
     MM_CAPI_BEGIN
     MM_REQUIRE(desc && out, "null argument");
-    MM_REQUIRE(desc->precision == MM_FP64 || desc->precision == MM_FP32, "bad precision");
     mm_stepper *st = new mm_stepper();
-    st->impl = nullptr;
     try {
-        if (desc->precision == MM_FP64) {
-            auto *p = new Stepper<double>();
-            st->impl = p;
-            p->init(*desc);
-        } else {
-            auto *p = new Stepper<float>();
-            st->impl = p;
-            p->init(*desc);
-        }
+        st->impl = make_stepper(*desc);
     } catch (...) {
-        delete st->impl;
         delete st;
         throw;
     }
     *out = st;
+    MM_CAPI_END
+}
+
+extern "C" int mm_stepper_launch_shape(mm_stepper_t *st, mm_launch_shape_t *out)
+{
+    MM_CAPI_BEGIN
+    MM_REQUIRE(st && out, "null argument");
+    const MmShape r = st->impl->launch_shape();
+    out->lpw_x = r.lpw_x;
+    out->tk_y = r.tk_y;
+    out->tk_z = r.tk_z;
+    out->rpw = r.rpw;
+    out->stage_threads = r.stage_threads;
+    MM_CAPI_END
+}
+
+// Milliseconds per accepted step of a stepper for d built with launch shape
+// cand (its resolved shape into *got), from a fixed pseudo-random state.
+// Launch shapes change which thread computes what, not the arithmetic, so
+// every candidate takes the same steps.
+static double time_shape(const mm_system_desc_t &d, const MmShape &cand, int steps, MmShape *got)
+{
+    mm_shape_set_candidate(&cand);
+    std::unique_ptr<StepperBase> st;
+    try {
+        st.reset(make_stepper(d));
+    } catch (...) {
+        mm_shape_set_candidate(nullptr);
+        throw;
+    }
+    mm_shape_set_candidate(nullptr);
+    *got = st->launch_shape();
+    const size_t n = (size_t)d.nx * d.ny * d.nz;
+    std::vector<double> M(3 * n);
+    uint64_t x = 0x9E3779B97F4A7C15ull;
+    auto uni = [&x]() {
+        x = x * 6364136223846793005ull + 1442695040888963407ull;
+        return (double)(x >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
+    };
+    for (size_t i = 0; i < n; ++i) {
+        double v[3], q = 0.0;
+        do {
+            for (int c = 0; c < 3; ++c) v[c] = uni();
+            q = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+        } while (q < 1e-4 || q > 1.0);
+        const double ms = d.Ms_host ? d.Ms_host[i] : 0.0;
+        for (int c = 0; c < 3; ++c) M[(size_t)c * n + i] = ms * v[c] / std::sqrt(q);
+    }
+    st->set_m(M.data());
+    st->run(2);
+    st->sync();
+    const auto t0 = std::chrono::steady_clock::now();
+    st->run(steps);
+    st->sync();
+    const auto t1 = std::chrono::steady_clock::now();
+    return std::chrono::duration<double, std::milli>(t1 - t0).count() / steps;
+}
+
+extern "C" int mm_autotune(const mm_system_desc_t *desc, int steps, mm_launch_shape_t *best, double *ms_per_step)
+{
+    MM_CAPI_BEGIN
+    MM_REQUIRE(desc && best, "null argument");
+    MM_REQUIRE(steps >= 1, "steps must be positive");
+    MM_REQUIRE(desc->Ms_host && desc->alpha_host, "Ms and alpha are required");
+    // the built-in choice first, then one knob at a time around the best so far
+    MmShape cur;
+    double cur_ms = time_shape(*desc, MmShape{0, 0, 0, 0, 0}, steps, &cur);
+    std::vector<MmShape> seen = {cur};
+    auto same = [](const MmShape &a, const MmShape &b) {
+        return a.lpw_x == b.lpw_x && a.tk_y == b.tk_y && a.tk_z == b.tk_z && a.rpw == b.rpw &&
+               a.stage_threads == b.stage_threads;
+    };
+    auto consider = [&](MmShape cand) {
+        for (const MmShape &q : seen)
+            if (same(q, cand)) return;
+        MmShape got;
+        const double ms = time_shape(*desc, cand, steps, &got);
+        for (const MmShape &q : seen)
+            if (same(q, got)) return;  // clamped to a shape already measured
+        seen.push_back(got);
+        if (ms < 0.99 * cur_ms) {
+            cur = got;
+            cur_ms = ms;
+        }
+    };
+    if (cur.lpw_x > 0) {
+        const int base = cur.lpw_x;
+        for (int v : {std::max(1, base / 2), base * 2}) {
+            MmShape c = cur;
+            c.lpw_x = v;
+            consider(c);
+        }
+        for (int v : {4, 8, 16}) {
+            MmShape c = cur;
+            c.tk_y = v;
+            consider(c);
+        }
+        for (int v : {8, 16, 32}) {
+            MmShape c = cur;
+            c.tk_z = v;
+            consider(c);
+        }
+    }
+    if (cur.rpw > 0) {
+        for (int v : {2, 4}) {
+            MmShape c = cur;
+            c.rpw = v;
+            c.stage_threads = 0;
+            consider(c);
+        }
+        const int base = cur.stage_threads;
+        for (int v : {base / 2, base * 2}) {
+            MmShape c = cur;
+            c.stage_threads = std::min(1024, std::max(64, v));
+            consider(c);
+        }
+    }
+    mm_wisdom_put(desc->nx, desc->ny, desc->nz, desc->precision == MM_FP32 ? 4 : 8, cur);
+    best->lpw_x = cur.lpw_x;
+    best->tk_y = cur.tk_y;
+    best->tk_z = cur.tk_z;
+    best->rpw = cur.rpw;
+    best->stage_threads = cur.stage_threads;
+    if (ms_per_step) *ms_per_step = cur_ms;
     MM_CAPI_END
 }
 

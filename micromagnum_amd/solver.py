@@ -8,6 +8,7 @@ or classical rk4 with a fixed step.  The whole attempt loop runs on the device;
 see include/mm_hotpath.h (mm_stepper_*).
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -97,6 +98,41 @@ class SystemSpec:
         self.method = method
 
 
+SHAPE_FIELDS = ("lpw_x", "tk_y", "tk_z", "rpw", "stage_threads")
+
+
+def autotune(shape, delta, Ms, alpha, steps=20, **kw):
+    """Measure launch shapes for this system (mm_autotune) and record the
+    fastest in the process's launch-shape table, which steppers created
+    afterwards for the same mesh, precision and device use.  Returns
+    (shape dict, ms per step)."""
+    kw = dict(kw)
+    _zeeman_fn_args(kw)
+    spec = SystemSpec(shape, delta, Ms, alpha, **kw)
+    best = _lib.LaunchShape()
+    ms = ctypes.c_double()
+    _lib.check(_lib.device_lib().mm_autotune(ctypes.byref(spec.desc), int(steps), ctypes.byref(best),
+                                             ctypes.byref(ms)))
+    return {f: getattr(best, f) for f in SHAPE_FIELDS}, ms.value
+
+
+def wisdom_import(path):
+    """Add the entries of a launch-shape file (the reference's FFTW wisdom
+    import, Magneto.cpp:151-174); returns their number."""
+    n = ctypes.c_int()
+    _lib.check(_lib.load().mm_wisdom_import(os.fsencode(path), ctypes.byref(n)))
+    return n.value
+
+
+def wisdom_export(path):
+    """Write the launch-shape table (Magneto.cpp:176-190)."""
+    _lib.check(_lib.load().mm_wisdom_export(os.fsencode(path)))
+
+
+def wisdom_forget():
+    _lib.check(_lib.load().mm_wisdom_forget())
+
+
 def _zeeman_fn_args(kw):
     """Take h_ext_fn out of the keyword arguments.  As the reference's
     AlternatingField (alternating_field.py:67-69), a function excludes the
@@ -133,6 +169,12 @@ class Stepper:
         self._fn_error = None
         if fn is not None:
             self.set_field_fn(fn)
+
+    def launch_shape(self):
+        """The launch shape this stepper runs with (see autotune)."""
+        r = _lib.LaunchShape()
+        _lib.check(_lib.load().mm_stepper_launch_shape(self._h, ctypes.byref(r)))
+        return {f: getattr(r, f) for f in SHAPE_FIELDS}
 
     def set_field_fn(self, fn):
         """Zeeman field from fn(t) -> 3-vector (None removes it)."""
