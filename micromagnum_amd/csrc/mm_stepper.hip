@@ -690,38 +690,19 @@ __device__ __forceinline__ void accept_step(StepCtl *ctl, double h)
 // (micro_magnetics_stepsize_controller.py:35-73) + RungeKutta.evolve retry
 // and overshoot handling (runge_kutta.py:56-86).  Non-adaptive (rk4): every
 // attempt is accepted with the fixed h.
-__global__ void __launch_bounds__(1024) k_controller(StepCtl *ctl, const double *__restrict__ mpart, int nb,
-                                                    int order, double eps_abs, double eps_rel, int Ms_uniform,
-                                                    double Ms_u, int adaptive)
+__device__ __forceinline__ void controller_update(StepCtl &c, const double m[4], int order, double eps_abs,
+                                                  double eps_rel, int Ms_uniform, double Ms_u, int adaptive)
 {
-    if (ctl->stop) return;
-    // maxima of the per-block partials of this attempt's stage launches
-    double m[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 4
-    for (int b = threadIdx.x; b < nb; b += blockDim.x)
-        for (int q = 0; q < 4; ++q) m[q] = fmax(m[q], mpart[(size_t)q * nb + b]);
-    __shared__ double red[4][16];
-    for (int q = 0; q < 4; ++q) {
-        const double w = wave_max(m[q]);
-        if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = w;
-    }
-    __syncthreads();
-    if (threadIdx.x != 0) return;
-    for (int q = 0; q < 4; ++q) {
-        double v = red[q][0];
-        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) v = fmax(v, red[q][w]);
-        m[q] = v;
-    }
-    const double h = ctl->h;
+    const double h = c.h;
     double dmax = sqrt(m[1]);
     if (Ms_uniform) dmax /= Ms_u;
-    const bool relaxed = relax_check(ctl, dmax, m[2], m[3]);
+    const bool relaxed = relax_check(&c, dmax, m[2], m[3]);
     if (relaxed) return;
-    ctl->attempts += 1;
-    if (ctl->forced) {
-        ctl->forced = 0;
-        accept_step(ctl, h);
-        ctl->h = ctl->h_saved;
+    c.attempts += 1;
+    if (c.forced) {
+        c.forced = 0;
+        accept_step(&c, h);
+        c.h = c.h_saved;
         return;
     }
     int accept = 1;
@@ -748,17 +729,52 @@ __global__ void __launch_bounds__(1024) k_controller(StepCtl *ctl, const double 
         }
     }
     if (accept) {
-        if (ctl->tmax_on && ctl->t + h > ctl->t_max) {
-            ctl->forced = 1;
-            ctl->h_saved = hn;
-            ctl->h = ctl->t_max - ctl->t;
+        if (c.tmax_on && c.t + h > c.t_max) {
+            c.forced = 1;
+            c.h_saved = hn;
+            c.h = c.t_max - c.t;
             return;
         }
-        accept_step(ctl, h);
+        accept_step(&c, h);
     } else {
-        ctl->rejections += 1;
+        c.rejections += 1;
     }
-    ctl->h = hn;
+    c.h = hn;
+}
+
+// Step-size control after an attempt (one workgroup): the maxima of the
+// per-block partials of the attempt's stage launches, then controller_update
+// on the control block.
+__global__ void __launch_bounds__(1024) k_controller(StepCtl *ctl, const double *__restrict__ mpart, int nb,
+                                                    int order, double eps_abs, double eps_rel, int Ms_uniform,
+                                                    double Ms_u, int adaptive)
+{
+    // The control block (thread 0) and the partials load together; the stop
+    // flag is looked at only when the maxima are there (after a stopped run
+    // the reduction's result is simply not used), so the kernel waits for
+    // one memory round trip instead of two.
+    StepCtl c;
+    if (threadIdx.x == 0) c = *ctl;
+    // maxima of the per-block partials of this attempt's stage launches
+    double m[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+        for (int q = 0; q < 4; ++q) m[q] = fmax(m[q], mpart[(size_t)q * nb + b]);
+    __shared__ double red[4][16];
+    for (int q = 0; q < 4; ++q) {
+        const double w = wave_max(m[q]);
+        if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = w;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0 || c.stop) return;
+    for (int q = 0; q < 4; ++q) {
+        double v = red[q][0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) v = fmax(v, red[q][w]);
+        m[q] = v;
+    }
+    // the control block is updated in registers and written back once
+    controller_update(c, m, order, eps_abs, eps_rel, Ms_uniform, Ms_u, adaptive);
+    *ctl = c;
 }
 
 // ytmp_1 = y; ytmp_1 += k_0 * (h b10) for attempts that start from a FSAL k_0
