@@ -1482,6 +1482,7 @@ template <typename T> struct Stepper : public StepperBase {
     void set_m(const double *M_host) override
     {
         StepCtl c;
+        MM_HIP(hipStreamSynchronize(s));
         MM_HIP(hipMemcpy(&c, ctl, sizeof(c), hipMemcpyDeviceToHost));
         upload(yb[c.cur], M_host, 3 * n);
         have_fsal = false;
@@ -1896,7 +1897,12 @@ extern "C" int mm_autotune(const mm_system_desc_t *desc, int steps, mm_launch_sh
         for (const MmShape &q : seen)
             if (same(q, cand)) return;
         MmShape got;
-        const double ms = time_shape(*desc, cand, steps, &got);
+        double ms;
+        try {
+            ms = time_shape(*desc, cand, steps, &got);
+        } catch (const mm_error &) {
+            return;  // a shape this system cannot launch (the built-in one could)
+        }
         for (const MmShape &q : seen)
             if (same(q, got)) return;  // clamped to a shape already measured
         seen.push_back(got);
