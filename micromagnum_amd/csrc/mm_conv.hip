@@ -208,27 +208,38 @@ __global__ void k_xfwd(const TIN *__restrict__ in, size_t in_cstride, C2<T> *__r
     }
 }
 
+// n / d for the tile indices of a launch without an integer division (the
+// GPU has none; the compiler emits some twenty instructions for one):
+// m = floor(2^32 / d) + 1 gives floor(n / d) = umulhi(n, m) exactly when
+// n * d < 2^32.  mdiv returns 0 where that does not hold, and the division
+// is then done the long way.
+__device__ __forceinline__ unsigned udiv(unsigned n, unsigned d, unsigned m)
+{
+    return d == 1 ? n : (m ? __umulhi(n, m) : n / d);
+}
+
 // y forward: columns kx of planes (c, z); rows y < ny_in read, Py written
 template <typename T, int LC = 0>
 __global__ void k_yfwd(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, int Hx, int Hp, int ny_in,
-                       int nzl, FftPlanDev p, int tk, int remap)
+                       int nzl, FftPlanDev p, int tk, int remap, unsigned mtx, unsigned mzl)
 {
     constexpr int SK = MM_SKEW_Y;
     extern __shared__ char smem_raw[];
     C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
     const int L = PlanC<LC>::L(p), LP = lds_len<SK>(L);
-    // tiles (kx tile, z, c), kx tile fastest
-    const int tile = block_tile(remap), ntx = (int)gridDim.x;
-    const int kx0 = (tile % ntx) * tk;
-    const int z = (tile / ntx) % nzl, c = tile / (ntx * nzl);
+    // tiles (kx tile, z, c), kx tile fastest; tk is a power of two
+    const int tile = block_tile(remap), ntx = (int)gridDim.x, ltk = __ffs(tk) - 1;
+    const int q1 = (int)udiv((unsigned)tile, (unsigned)ntx, mtx), c = (int)udiv((unsigned)q1, (unsigned)nzl, mzl);
+    const int kx0 = (tile - q1 * ntx) * tk;
+    const int z = q1 - c * nzl;
     const C2<T> *src = in + ((size_t)c * nzl + z) * (size_t)ny_in * Hp;
     C2<T> *dst = out + ((size_t)c * nzl + z) * (size_t)L * Hp;
     C2<T> *twl = sm + (size_t)tk * LP;
     const TwFetch<T> twf = TwFetch<T>::fetch(p);
     // global passes: columns fastest across the threads (row segments of tk
     // values per load); LDS passes: one line per group of tline threads
-    const int lg = threadIdx.x % tk, ls = threadIdx.x / PlanC<LC>::tline(p);
-    const LineMap mg = {(int)(threadIdx.x / tk), true}, ms = {(int)(threadIdx.x % PlanC<LC>::tline(p)), ls < tk};
+    const int lg = threadIdx.x & (tk - 1), ls = threadIdx.x / PlanC<LC>::tline(p);
+    const LineMap mg = {(int)(threadIdx.x >> ltk), true}, ms = {(int)(threadIdx.x % PlanC<LC>::tline(p)), ls < tk};
     const int kx = kx0 + lg;
     fft_line<T, false, true, true, LC, SK>(
         p, twl, sm + (size_t)lg * LP, mg, sm + (size_t)ls * LP, ms,
@@ -241,21 +252,22 @@ __global__ void k_yfwd(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, in
 // y inverse: columns of S2 planes (c, z), all Py rows read, y < ny_out written
 template <typename T, int LC = 0>
 __global__ void k_yinv(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, int Hx, int Hp, int ny_out,
-                       int nzl, FftPlanDev p, int tk, int remap)
+                       int nzl, FftPlanDev p, int tk, int remap, unsigned mtx, unsigned mzl)
 {
     constexpr int SK = MM_SKEW_Y;
     extern __shared__ char smem_raw[];
     C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
     const int L = PlanC<LC>::L(p), LP = lds_len<SK>(L);
-    const int tile = block_tile(remap), ntx = (int)gridDim.x;
-    const int kx0 = (tile % ntx) * tk;
-    const int z = (tile / ntx) % nzl, c = tile / (ntx * nzl);
+    const int tile = block_tile(remap), ntx = (int)gridDim.x, ltk = __ffs(tk) - 1;
+    const int q1 = (int)udiv((unsigned)tile, (unsigned)ntx, mtx), c = (int)udiv((unsigned)q1, (unsigned)nzl, mzl);
+    const int kx0 = (tile - q1 * ntx) * tk;
+    const int z = q1 - c * nzl;
     const C2<T> *src = in + ((size_t)c * nzl + z) * (size_t)L * Hp;
     C2<T> *dst = out + ((size_t)c * nzl + z) * (size_t)ny_out * Hp;
     C2<T> *twl = sm + (size_t)tk * LP;
     const TwFetch<T> twf = TwFetch<T>::fetch(p);
-    const int lg = threadIdx.x % tk, ls = threadIdx.x / PlanC<LC>::tline(p);
-    const LineMap mg = {(int)(threadIdx.x / tk), true}, ms = {(int)(threadIdx.x % PlanC<LC>::tline(p)), ls < tk};
+    const int lg = threadIdx.x & (tk - 1), ls = threadIdx.x / PlanC<LC>::tline(p);
+    const LineMap mg = {(int)(threadIdx.x >> ltk), true}, ms = {(int)(threadIdx.x % PlanC<LC>::tline(p)), ls < tk};
     const int kx = kx0 + lg;
     fft_line<T, true, true, true, LC, SK>(
         p, twl, sm + (size_t)lg * LP, mg, sm + (size_t)ls * LP, ms,
@@ -400,7 +412,7 @@ __device__ __forceinline__ void mult3(C2<T> &mx, C2<T> &my, C2<T> &mz, const T k
 // keep z < nz.  Lines: 3 components x tk columns for one ky.
 template <typename T, int LC = 0>
 __global__ void k_zmi(C2<T> *__restrict__ S2, const T *__restrict__ K, size_t plane, int Hx, int Hp, int Hy2,
-                      int Py, int nz, FftPlanDev p, int tk, int remap)
+                      int Py, int nz, FftPlanDev p, int tk, int remap, unsigned mtx)
 {
     constexpr int SK = MM_SKEW_Z;
     extern __shared__ char smem_raw[];
@@ -408,9 +420,10 @@ __global__ void k_zmi(C2<T> *__restrict__ S2, const T *__restrict__ K, size_t pl
     const int L = PlanC<LC>::L(p), LP = lds_len<SK>(L);
     // tiles (kx tile, m), kx tile fastest.  With remap, rows come in the order
     // 0, 1, Py-1, 2, Py-2, ...: ky and Py-ky read the same kernel row
-    const int tile = block_tile(remap), ntx = (int)gridDim.x;
-    const int kx0 = (tile % ntx) * tk;
-    const int m = tile / ntx, hm = (m + 1) >> 1;
+    // tk is a power of two
+    const int tile = block_tile(remap), ntx = (int)gridDim.x, ltk = __ffs(tk) - 1;
+    const int m = (int)udiv((unsigned)tile, (unsigned)ntx, mtx), hm = (m + 1) >> 1;
+    const int kx0 = (tile - m * ntx) * tk;
     const int ky = !remap || m == 0 ? m : ((m & 1) ? hm : Py - hm);
     const size_t cstride = (size_t)nz * Py * Hp;
     const size_t zstride = (size_t)Py * Hp;
@@ -419,8 +432,8 @@ __global__ void k_zmi(C2<T> *__restrict__ S2, const T *__restrict__ K, size_t pl
     const TwFetch<T> twf = TwFetch<T>::fetch(p);
     // global passes: columns fastest, then components; LDS passes: one line
     // (component c, column) per group of tline threads
-    const int colg = threadIdx.x % tk, cg = (threadIdx.x / tk) % 3;
-    const LineMap mg = {(int)(threadIdx.x / (3 * tk)), true};
+    const int colg = threadIdx.x & (tk - 1), cg = (int)(threadIdx.x >> ltk) % 3;
+    const LineMap mg = {(int)(threadIdx.x >> ltk) / 3, true};
     C2<T> *lineG = sm + (size_t)(cg * tk + colg) * LP;
     const int l = threadIdx.x / PlanC<LC>::tline(p), t = threadIdx.x % PlanC<LC>::tline(p);
     const LineMap ms = {t, l < 3 * tk};
@@ -440,7 +453,7 @@ __global__ void k_zmi(C2<T> *__restrict__ S2, const T *__restrict__ K, size_t pl
 #pragma unroll
     for (int j = 0; j < ZM_ITEMS; ++j) {
         const int e = threadIdx.x + j * blockDim.x;
-        const int col = e % tk, kzf = e / tk, kx = kx0 + col;
+        const int col = e & (tk - 1), kzf = e >> ltk, kx = kx0 + col;
         if (e < nitems && kx < Hx) {
             const size_t o = ((size_t)kzf * Hy2 + kyf) * Hp + kx;
 #pragma unroll
@@ -454,7 +467,7 @@ __global__ void k_zmi(C2<T> *__restrict__ S2, const T *__restrict__ K, size_t pl
 #pragma unroll
     for (int j = 0; j < ZM_ITEMS; ++j) {
         const int e = threadIdx.x + j * blockDim.x;
-        const int col = e % tk, kzf = e / tk, kx = kx0 + col;
+        const int col = e & (tk - 1), kzf = e >> ltk, kx = kx0 + col;
         if (e < nitems && kx < Hx) {
             T k[6] = {kk[j][0], sy * kk[j][1], kk[j][2], kk[j][3], sy * kk[j][4], kk[j][5]};
             C2<T> *l0 = sm + (size_t)(0 * tk + col) * LP, *l1 = sm + (size_t)(1 * tk + col) * LP,
@@ -591,6 +604,16 @@ static int grid_1d(size_t n, int bs)
     size_t b = (n + bs - 1) / bs;
     return (int)std::min<size_t>(b, 65535 * 64);
 }
+
+// multiplier for udiv: n / d for every n < nmax (0: not exact, divide)
+static unsigned mdiv(unsigned d, size_t nmax)
+{
+    MM_REQUIRE(d >= 1, "zero divisor");
+    if (d == 1 || (unsigned long long)nmax * d >= (1ull << 32)) return 0u;
+    return (unsigned)((1ull << 32) / d + 1);
+}
+
+static int floor_pow2(int v) { return v < 1 ? 1 : 1 << (31 - __builtin_clz((unsigned)v)); }
 
 static int env_int(const char *name, int dflt)
 {
@@ -742,12 +765,12 @@ static ConvLaunch choose_launch(const ConvGeom &g, const FftPlanHost &px, const 
     ln.lpw_x = env_int("MM_LPW_X", w.lpw_x > 0 ? w.lpw_x : std::max(1, 256 / px.tline));
     while (ln.lpw_x > 1 && (lines_lds<MM_SKEW_X>(ln.lpw_x, px.L, csize) > (size_t)lds || ln.lpw_x * px.tline > 1024)) --ln.lpw_x;
     const int nl_y = g.is2d ? 3 : 1;
-    int tk = env_int("MM_TK_Y", w.tk_y > 0 ? w.tk_y : 8);
+    int tk = floor_pow2(env_int("MM_TK_Y", w.tk_y > 0 ? w.tk_y : 8));
     while (tk > 1 && ((size_t)nl_y * tk * py.tline > 1024 || lines_lds<MM_SKEW_Y>(nl_y * tk, py.L, csize) > (size_t)lds)) tk /= 2;
     ln.tk_y = tk;
     MM_REQUIRE((size_t)nl_y * tk * py.tline <= 1024 && lines_lds<MM_SKEW_Y>(nl_y * tk, py.L, csize) <= (size_t)lds,
                "y transform length too large for one workgroup");
-    int tz = env_int("MM_TK_Z", w.tk_z > 0 ? w.tk_z : 16);
+    int tz = floor_pow2(env_int("MM_TK_Z", w.tk_z > 0 ? w.tk_z : 16));
     while (tz > 1 && ((size_t)3 * tz * pz.tline > 1024 || lines_lds<MM_SKEW_Z>(3 * tz, pz.L, csize) > (size_t)lds)) tz /= 2;
     ln.tk_z = tz;
     return ln;
@@ -855,8 +878,9 @@ void ConvImpl<T>::init(const double *N_host, int nx, int ny, int nz, int ex, int
                            dN, 0, T1, 0, g.Hx, g.Hp, Py, Pz, nrows, Px, dx, lpw, (const double *)nullptr,
                            (const int *)nullptr);
         MM_HIP(hipGetLastError());
-        hipLaunchKernelGGL((k_yfwd<double>), dim3((g.Hx + tky - 1) / tky, Pz, 1), dim3(tky * py.tline), shy, 0,
-                           T1, T2, g.Hx, g.Hp, Py, Pz, dy, tky, 0);
+        const unsigned ntyk = (unsigned)((g.Hx + tky - 1) / tky), nbk = ntyk * Pz;
+        hipLaunchKernelGGL((k_yfwd<double>), dim3(ntyk, Pz, 1), dim3(tky * py.tline), shy, 0, T1, T2, g.Hx, g.Hp, Py,
+                           Pz, dy, tky, 0, mdiv(ntyk, nbk), mdiv((unsigned)Pz, nbk));
         MM_HIP(hipGetLastError());
         if (Pz > 1) {
             hipLaunchKernelGGL(k_zfwd_setup, dim3((g.Hx + tkz - 1) / tkz, Py, 1), dim3(tkz * pz.tline), shz, 0,
@@ -956,9 +980,10 @@ template <typename T> void ConvImpl<T>::spectral(hipStream_t s)
                            g.Hx, g.Hp, g.ny, g.nz, pyh.dev<T>(), (const C2<T> *)py.plain<T>(), tkp);
     } else {
         mm_with_len(py.L, fft_generic, [&](auto lc) {
-            hipLaunchKernelGGL((k_yfwd<T, decltype(lc)::value>), dim3((g.Hx + tk - 1) / tk, g.nz, 3),
-                               dim3(tk * py.tline), shy, s, S1, S2, g.Hx, g.Hp, g.ny, g.nz, py.dev<T>(), tk,
-                               xcd_remap);
+            const unsigned ntx = (unsigned)((g.Hx + tk - 1) / tk), nb = ntx * g.nz * 3;
+            hipLaunchKernelGGL((k_yfwd<T, decltype(lc)::value>), dim3(ntx, g.nz, 3), dim3(tk * py.tline), shy, s, S1,
+                               S2, g.Hx, g.Hp, g.ny, g.nz, py.dev<T>(), tk, xcd_remap, mdiv(ntx, nb),
+                               mdiv((unsigned)g.nz, nb));
         });
     }
     MM_HIP(hipGetLastError());
@@ -966,9 +991,11 @@ template <typename T> void ConvImpl<T>::spectral(hipStream_t s)
     const int tz = ln.tk_z;
     const size_t shz = lines_lds<MM_SKEW_Z>(3 * tz, pz.L, cs);
     MM_REQUIRE((size_t)tz * (pz.L / 2 + 1) <= (size_t)ZM_ITEMS * 3 * tz * pz.tline, "z pass items per thread");
+    const unsigned ntz = (unsigned)((g.Hx + tz - 1) / tz);
+    const unsigned mtz = mdiv(ntz, (size_t)ntz * g.Py);
     mm_with_len(pz.L, fft_generic, [&](auto lc) {
-        hipLaunchKernelGGL((k_zmi<T, decltype(lc)::value>), dim3((g.Hx + tz - 1) / tz, g.Py), dim3(3 * tz * pz.tline),
-                           shz, s, S2, kern, plane, g.Hx, g.Hp, g.Hy2, g.Py, g.nz, pz.dev<T>(), tz, xcd_remap);
+        hipLaunchKernelGGL((k_zmi<T, decltype(lc)::value>), dim3(ntz, g.Py), dim3(3 * tz * pz.tline), shz, s, S2, kern,
+                           plane, g.Hx, g.Hp, g.Hy2, g.Py, g.nz, pz.dev<T>(), tz, xcd_remap, mtz);
     });
     MM_HIP(hipGetLastError());
     if (klog) klog->mark(s, KC_ZMI, conv_bytes<T>(g, KC_ZMI));
@@ -977,9 +1004,10 @@ template <typename T> void ConvImpl<T>::spectral(hipStream_t s)
                            g.Hx, g.Hp, g.ny, g.nz, pyh.dev<T>(), (const C2<T> *)py.plain<T>(), tkp);
     } else {
         mm_with_len(py.L, fft_generic, [&](auto lc) {
-            hipLaunchKernelGGL((k_yinv<T, decltype(lc)::value>), dim3((g.Hx + tk - 1) / tk, g.nz, 3),
-                               dim3(tk * py.tline), shy, s, S2, S1, g.Hx, g.Hp, g.ny, g.nz, py.dev<T>(), tk,
-                               xcd_remap);
+            const unsigned ntx = (unsigned)((g.Hx + tk - 1) / tk), nb = ntx * g.nz * 3;
+            hipLaunchKernelGGL((k_yinv<T, decltype(lc)::value>), dim3(ntx, g.nz, 3), dim3(tk * py.tline), shy, s, S2,
+                               S1, g.Hx, g.Hp, g.ny, g.nz, py.dev<T>(), tk, xcd_remap, mdiv(ntx, nb),
+                               mdiv((unsigned)g.nz, nb));
         });
     }
     MM_HIP(hipGetLastError());
