@@ -410,9 +410,13 @@ __device__ __forceinline__ void mult3(C2<T> &mx, C2<T> &my, C2<T> &mz, const T k
 
 // z pass (3-D): forward over z (nz rows, zero-extended), multiply, inverse,
 // keep z < nz.  Lines: 3 components x tk columns for one ky.
-template <typename T, int LC = 0>
+// O32: S2 and K are under 4 GiB and a z step of S2 under 16 MiB (zsb bytes),
+// so every access is a 32-bit byte offset from the kernel-argument base
+// (global_load with an SGPR base: no 64-bit address arithmetic per element;
+// the offsets' z steps are 24-bit multiplies); pb: bytes of one K plane.
+template <typename T, int LC = 0, bool O32 = false>
 __global__ void k_zmi(C2<T> *__restrict__ S2, const T *__restrict__ K, size_t plane, int Hx, int Hp, int Hy2,
-                      int Py, int nz, FftPlanDev p, int tk, int remap, unsigned mtx)
+                      int Py, int nz, FftPlanDev p, int tk, int remap, unsigned mtx, unsigned zsb, unsigned pb)
 {
     constexpr int SK = MM_SKEW_Z;
     extern __shared__ char smem_raw[];
@@ -440,6 +444,11 @@ __global__ void k_zmi(C2<T> *__restrict__ S2, const T *__restrict__ K, size_t pl
     C2<T> *line = sm + (size_t)l * LP;
     const int kxg = kx0 + colg;
     C2<T> *colp = base + cg * cstride + kxg;
+    const unsigned cb = O32 ? (unsigned)(((size_t)cg * cstride + (size_t)ky * Hp + kxg) * sizeof(C2<T>)) : 0u;
+    auto col_at = [&](int z) -> C2<T> & {
+        if constexpr (O32) return *(C2<T> *)((char *)S2 + (cb + __umul24((unsigned)z, zsb)));
+        else return colp[z * zstride];
+    };
     // The kernel entries this thread multiplies load before the forward
     // transform, so their latency hides behind it.  Items (col, kzf) with
     // kzf <= L/2 each serve kz = kzf and the mirror kz = L - kzf, whose xz
@@ -456,13 +465,19 @@ __global__ void k_zmi(C2<T> *__restrict__ S2, const T *__restrict__ K, size_t pl
         const int col = e & (tk - 1), kzf = e >> ltk, kx = kx0 + col;
         if (e < nitems && kx < Hx) {
             const size_t o = ((size_t)kzf * Hy2 + kyf) * Hp + kx;
+            if constexpr (O32) {
+                const unsigned ob = (unsigned)o * (unsigned)sizeof(T);
 #pragma unroll
-            for (int q = 0; q < 6; ++q) kk[j][q] = K[q * plane + o];
+                for (int q = 0; q < 6; ++q) kk[j][q] = *(const T *)((const char *)K + (ob + (unsigned)q * pb));
+            } else {
+#pragma unroll
+                for (int q = 0; q < 6; ++q) kk[j][q] = K[q * plane + o];
+            }
         }
     }
     fft_line<T, false, true, false, LC, SK>(
         p, twl, lineG, mg, line, ms,
-        [&](int z) { return (z < nz && kxg < Hx) ? colp[z * zstride] : cmk<T>((T)0, (T)0); },
+        [&](int z) { return (z < nz && kxg < Hx) ? col_at(z) : cmk<T>((T)0, (T)0); },
         [](int, C2<T>) {}, [&] { twf.store(twl); });
 #pragma unroll
     for (int j = 0; j < ZM_ITEMS; ++j) {
@@ -485,7 +500,7 @@ __global__ void k_zmi(C2<T> *__restrict__ S2, const T *__restrict__ K, size_t pl
     fft_line<T, true, false, true, LC, SK>(
         p, twl, lineG, mg, line, ms, [](int) { return cmk<T>((T)0, (T)0); },
         [&](int z, C2<T> v) {
-            if (z < nz && kxg < Hx) colp[z * zstride] = v;
+            if (z < nz && kxg < Hx) col_at(z) = v;
         });
 }
 
@@ -993,9 +1008,17 @@ template <typename T> void ConvImpl<T>::spectral(hipStream_t s)
     MM_REQUIRE((size_t)tz * (pz.L / 2 + 1) <= (size_t)ZM_ITEMS * 3 * tz * pz.tline, "z pass items per thread");
     const unsigned ntz = (unsigned)((g.Hx + tz - 1) / tz);
     const unsigned mtz = mdiv(ntz, (size_t)ntz * g.Py);
+    const size_t zsb = (size_t)g.Py * g.Hp * cs;
+    const bool o32 = zsb < (1u << 24) && 3 * (size_t)g.nz * zsb < (1ull << 32) && kern_elems * sizeof(T) < (1ull << 32);
     mm_with_len(pz.L, fft_generic, [&](auto lc) {
-        hipLaunchKernelGGL((k_zmi<T, decltype(lc)::value>), dim3(ntz, g.Py), dim3(3 * tz * pz.tline), shz, s, S2, kern,
-                           plane, g.Hx, g.Hp, g.Hy2, g.Py, g.nz, pz.dev<T>(), tz, xcd_remap, mtz);
+        if (o32)
+            hipLaunchKernelGGL((k_zmi<T, decltype(lc)::value, true>), dim3(ntz, g.Py), dim3(3 * tz * pz.tline), shz, s,
+                               S2, kern, plane, g.Hx, g.Hp, g.Hy2, g.Py, g.nz, pz.dev<T>(), tz, xcd_remap, mtz,
+                               (unsigned)zsb, (unsigned)(plane * sizeof(T)));
+        else
+            hipLaunchKernelGGL((k_zmi<T, decltype(lc)::value, false>), dim3(ntz, g.Py), dim3(3 * tz * pz.tline), shz,
+                               s, S2, kern, plane, g.Hx, g.Hp, g.Hy2, g.Py, g.nz, pz.dev<T>(), tz, xcd_remap, mtz, 0u,
+                               0u);
     });
     MM_HIP(hipGetLastError());
     if (klog) klog->mark(s, KC_ZMI, conv_bytes<T>(g, KC_ZMI));
