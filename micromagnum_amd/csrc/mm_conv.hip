@@ -218,8 +218,10 @@ __device__ __forceinline__ unsigned udiv(unsigned n, unsigned d, unsigned m)
     return d == 1 ? n : (m ? __umulhi(n, m) : n / d);
 }
 
-// y forward: columns kx of planes (c, z); rows y < ny_in read, Py written
-template <typename T, int LC = 0>
+// y forward: columns kx of planes (c, z); rows y < ny_in read, Py written.
+// O32: both arrays under 4 GiB and a row under 16 MiB: 32-bit byte offsets
+// from the kernel-argument bases, rows stepped by 24-bit multiplies
+template <typename T, int LC = 0, bool O32 = false>
 __global__ void k_yfwd(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, int Hx, int Hp, int ny_in,
                        int nzl, FftPlanDev p, int tk, int remap, unsigned mtx, unsigned mzl)
 {
@@ -241,16 +243,27 @@ __global__ void k_yfwd(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, in
     const int lg = threadIdx.x & (tk - 1), ls = threadIdx.x / PlanC<LC>::tline(p);
     const LineMap mg = {(int)(threadIdx.x >> ltk), true}, ms = {(int)(threadIdx.x % PlanC<LC>::tline(p)), ls < tk};
     const int kx = kx0 + lg;
+    constexpr unsigned CS = sizeof(C2<T>);
+    const unsigned rs = (unsigned)Hp * CS, pl = ((unsigned)c * nzl + z);
+    const unsigned sb = O32 ? (pl * (unsigned)ny_in * Hp + kx) * CS : 0u, db = O32 ? (pl * (unsigned)L * Hp + kx) * CS : 0u;
     fft_line<T, false, true, true, LC, SK>(
         p, twl, sm + (size_t)lg * LP, mg, sm + (size_t)ls * LP, ms,
-        [&](int y) { return (y < ny_in && kx < Hx) ? src[(size_t)y * Hp + kx] : cmk<T>((T)0, (T)0); },
+        [&](int y) {
+            if (!(y < ny_in && kx < Hx)) return cmk<T>((T)0, (T)0);
+            if constexpr (O32) return *(const C2<T> *)((const char *)in + (sb + __umul24((unsigned)y, rs)));
+            else return src[(size_t)y * Hp + kx];
+        },
         [&](int y, C2<T> v) {
-            if (kx < Hx) dst[(size_t)y * Hp + kx] = v;
+            if (kx < Hx) {
+                if constexpr (O32) *(C2<T> *)((char *)out + (db + __umul24((unsigned)y, rs))) = v;
+                else dst[(size_t)y * Hp + kx] = v;
+            }
         }, [&] { twf.store(twl); });
 }
 
-// y inverse: columns of S2 planes (c, z), all Py rows read, y < ny_out written
-template <typename T, int LC = 0>
+// y inverse: columns of S2 planes (c, z), all Py rows read, y < ny_out
+// written; O32 as k_yfwd
+template <typename T, int LC = 0, bool O32 = false>
 __global__ void k_yinv(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, int Hx, int Hp, int ny_out,
                        int nzl, FftPlanDev p, int tk, int remap, unsigned mtx, unsigned mzl)
 {
@@ -269,11 +282,21 @@ __global__ void k_yinv(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, in
     const int lg = threadIdx.x & (tk - 1), ls = threadIdx.x / PlanC<LC>::tline(p);
     const LineMap mg = {(int)(threadIdx.x >> ltk), true}, ms = {(int)(threadIdx.x % PlanC<LC>::tline(p)), ls < tk};
     const int kx = kx0 + lg;
+    constexpr unsigned CS = sizeof(C2<T>);
+    const unsigned rs = (unsigned)Hp * CS, pl = ((unsigned)c * nzl + z);
+    const unsigned sb = O32 ? (pl * (unsigned)L * Hp + kx) * CS : 0u, db = O32 ? (pl * (unsigned)ny_out * Hp + kx) * CS : 0u;
     fft_line<T, true, true, true, LC, SK>(
         p, twl, sm + (size_t)lg * LP, mg, sm + (size_t)ls * LP, ms,
-        [&](int y) { return kx < Hx ? src[(size_t)y * Hp + kx] : cmk<T>((T)0, (T)0); },
+        [&](int y) {
+            if (!(kx < Hx)) return cmk<T>((T)0, (T)0);
+            if constexpr (O32) return *(const C2<T> *)((const char *)in + (sb + __umul24((unsigned)y, rs)));
+            else return src[(size_t)y * Hp + kx];
+        },
         [&](int y, C2<T> v) {
-            if (y < ny_out && kx < Hx) dst[(size_t)y * Hp + kx] = v;
+            if (y < ny_out && kx < Hx) {
+                if constexpr (O32) *(C2<T> *)((char *)out + (db + __umul24((unsigned)y, rs))) = v;
+                else dst[(size_t)y * Hp + kx] = v;
+            }
         }, [&] { twf.store(twl); });
 }
 
@@ -988,6 +1011,8 @@ template <typename T> void ConvImpl<T>::spectral(hipStream_t s)
     }
     const int tk = ln.tk_y;
     const size_t shy = lines_lds<MM_SKEW_Y>(tk, py.L, cs);
+    // S2 (3 nz Py rows) is the larger of the two arrays
+    const bool y_o32 = (size_t)g.Hp * cs < (1u << 24) && (size_t)3 * g.nz * g.Py * g.Hp * cs < (1ull << 32);
     const int tkp = tk_yp;
     const size_t shyp = lines_lds<MM_SKEW_Y>(tkp, pyh.L, cs);
     if (prune_y) {
@@ -996,9 +1021,14 @@ template <typename T> void ConvImpl<T>::spectral(hipStream_t s)
     } else {
         mm_with_len(py.L, fft_generic, [&](auto lc) {
             const unsigned ntx = (unsigned)((g.Hx + tk - 1) / tk), nb = ntx * g.nz * 3;
-            hipLaunchKernelGGL((k_yfwd<T, decltype(lc)::value>), dim3(ntx, g.nz, 3), dim3(tk * py.tline), shy, s, S1,
-                               S2, g.Hx, g.Hp, g.ny, g.nz, py.dev<T>(), tk, xcd_remap, mdiv(ntx, nb),
-                               mdiv((unsigned)g.nz, nb));
+            if (y_o32)
+                hipLaunchKernelGGL((k_yfwd<T, decltype(lc)::value, true>), dim3(ntx, g.nz, 3), dim3(tk * py.tline), shy,
+                                   s, S1, S2, g.Hx, g.Hp, g.ny, g.nz, py.dev<T>(), tk, xcd_remap, mdiv(ntx, nb),
+                                   mdiv((unsigned)g.nz, nb));
+            else
+                hipLaunchKernelGGL((k_yfwd<T, decltype(lc)::value, false>), dim3(ntx, g.nz, 3), dim3(tk * py.tline),
+                                   shy, s, S1, S2, g.Hx, g.Hp, g.ny, g.nz, py.dev<T>(), tk, xcd_remap, mdiv(ntx, nb),
+                                   mdiv((unsigned)g.nz, nb));
         });
     }
     MM_HIP(hipGetLastError());
@@ -1028,9 +1058,14 @@ template <typename T> void ConvImpl<T>::spectral(hipStream_t s)
     } else {
         mm_with_len(py.L, fft_generic, [&](auto lc) {
             const unsigned ntx = (unsigned)((g.Hx + tk - 1) / tk), nb = ntx * g.nz * 3;
-            hipLaunchKernelGGL((k_yinv<T, decltype(lc)::value>), dim3(ntx, g.nz, 3), dim3(tk * py.tline), shy, s, S2,
-                               S1, g.Hx, g.Hp, g.ny, g.nz, py.dev<T>(), tk, xcd_remap, mdiv(ntx, nb),
-                               mdiv((unsigned)g.nz, nb));
+            if (y_o32)
+                hipLaunchKernelGGL((k_yinv<T, decltype(lc)::value, true>), dim3(ntx, g.nz, 3), dim3(tk * py.tline), shy,
+                                   s, S2, S1, g.Hx, g.Hp, g.ny, g.nz, py.dev<T>(), tk, xcd_remap, mdiv(ntx, nb),
+                                   mdiv((unsigned)g.nz, nb));
+            else
+                hipLaunchKernelGGL((k_yinv<T, decltype(lc)::value, false>), dim3(ntx, g.nz, 3), dim3(tk * py.tline),
+                                   shy, s, S2, S1, g.Hx, g.Hp, g.ny, g.nz, py.dev<T>(), tk, xcd_remap, mdiv(ntx, nb),
+                                   mdiv((unsigned)g.nz, nb));
         });
     }
     MM_HIP(hipGetLastError());
