@@ -103,6 +103,19 @@ int mm_conv_execute_f32(mm_conv_t *conv, const float *M, float *H, void *stream)
 int mm_conv_fft_size(const mm_conv_t *conv, int *fft_xyz);
 int mm_conv_destroy(mm_conv_t *conv);
 
+/* SymmetricMatrixVectorConvolution_Simple
+ * (src/magneto/math/conv/SymmetricMatrixVectorConvolution_Simple.cpp:23-80,
+ * bindings/math.i:35-41): the direct sum H(r) = sum_r' N((r - r') mod E) M(r')
+ * over the mesh, FP64, O(n^2) -- what the reference's StrayFieldCalculator
+ * uses below 32 cells on the CPU (stray_field_calculator.py:109-121).  N_host:
+ * the tensor as for mm_conv_create (6 entries per point of the (ex, ey, ez)
+ * grid). */
+typedef struct mm_conv_simple mm_conv_simple_t;
+int mm_conv_simple_create(const double *N_host, int nx, int ny, int nz, int ex, int ey, int ez,
+                          mm_conv_simple_t **out);
+int mm_conv_simple_execute(mm_conv_simple_t *conv, const double *M, double *H, void *stream);
+int mm_conv_simple_destroy(mm_conv_simple_t *conv);
+
 /* ------------------------------------------------------ local field terms */
 /* exchange (src/magneto/mmm/exchange/exchange.cpp:31-77; bindings/mmm/exchange.i:24-39).
  * energy (may be NULL) receives -mu0/2 * dV * sum_i m_i.H_i. */

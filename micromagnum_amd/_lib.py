@@ -75,6 +75,9 @@ SIGNATURES = {
     "mm_conv_execute_f32": (_i, [_vp, _vp, _vp, _vp]),
     "mm_conv_fft_size": (_i, [_vp, _ip]),
     "mm_conv_destroy": (_i, [_vp]),
+    "mm_conv_simple_create": (_i, [_dp, _i, _i, _i, _i, _i, _i, ctypes.POINTER(_vp)]),
+    "mm_conv_simple_execute": (_i, [_vp, _vp, _vp, _vp]),
+    "mm_conv_simple_destroy": (_i, [_vp]),
     "mm_exchange": (_i, [_i, _i, _i, _d, _d, _d, _i, _i, _i, _vp, _vp, _vp, _vp, _dp, _vp]),
     "mm_exchange_f32": (_i, [_i, _i, _i, _d, _d, _d, _i, _i, _i, _vp, _vp, _vp, _vp, _vp]),
     "mm_uniaxial_anisotropy": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _dp, _vp]),

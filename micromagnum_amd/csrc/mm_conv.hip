@@ -1155,3 +1155,107 @@ extern "C" int mm_conv_destroy(mm_conv_t *conv)
     delete conv;
     MM_CAPI_END
 }
+
+// ---------------------------------------------------------------------------
+// direct-sum convolution
+// ---------------------------------------------------------------------------
+// H(r) = sum_r' N((r - r') mod E) M(r') over the whole mesh, the tensor in the
+// layout of GenerateDemagTensor (6 entries per point, x fastest).  One thread
+// per target cell; the workgroup walks the source cells in tiles staged in
+// LDS (M of 256 source cells, three components), the source order of every
+// thread z, y, x as the reference's loop.
+#define MM_SIMPLE_TILE 256
+__global__ void __launch_bounds__(MM_SIMPLE_TILE) k_conv_simple(const double *__restrict__ N, int nx, int ny, int nz,
+                                                               int ex, int ey, int ez, const double *__restrict__ M,
+                                                               double *__restrict__ H)
+{
+    __shared__ double ms[3][MM_SIMPLE_TILE];
+    const int n = nx * ny * nz;
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const bool act = i < n;
+    const int x = act ? i % nx : 0, y = act ? (i / nx) % ny : 0, z = act ? i / (nx * ny) : 0;
+    double hx = 0.0, hy = 0.0, hz = 0.0;
+    for (int j0 = 0; j0 < n; j0 += MM_SIMPLE_TILE) {
+        const int jl = j0 + (int)threadIdx.x;
+        __syncthreads();
+        for (int c = 0; c < 3; ++c) ms[c][threadIdx.x] = jl < n ? M[(size_t)c * n + jl] : 0.0;
+        __syncthreads();
+        if (!act) continue;
+        const int cnt = min(MM_SIMPLE_TILE, n - j0);
+        int m = j0 % nx, o = j0 / (nx * ny), nn = (j0 / nx) % ny;
+        for (int t = 0; t < cnt; ++t) {
+            const int X = (x - m + ex) % ex, Y = (y - nn + ey) % ey, Z = (z - o + ez) % ez;
+            const double *Np = N + 6 * ((size_t)X + (size_t)ex * ((size_t)Y + (size_t)ey * Z));
+            const double Mx = ms[0][t], My = ms[1][t], Mz = ms[2][t];
+            hx += Np[0] * Mx + Np[1] * My + Np[2] * Mz;
+            hy += Np[1] * Mx + Np[3] * My + Np[4] * Mz;
+            hz += Np[2] * Mx + Np[4] * My + Np[5] * Mz;
+            if (++m == nx) {
+                m = 0;
+                if (++nn == ny) {
+                    nn = 0;
+                    ++o;
+                }
+            }
+        }
+    }
+    if (act) {
+        H[i] = hx;
+        H[(size_t)n + i] = hy;
+        H[2 * (size_t)n + i] = hz;
+    }
+}
+
+struct mm_conv_simple {
+    double *N = nullptr;
+    int nx = 0, ny = 0, nz = 0, ex = 0, ey = 0, ez = 0;
+};
+
+extern "C" int mm_conv_simple_create(const double *N_host, int nx, int ny, int nz, int ex, int ey, int ez,
+                                     mm_conv_simple_t **out)
+{
+    MM_CAPI_BEGIN
+    MM_REQUIRE(N_host && out, "null argument");
+    MM_REQUIRE(nx >= 1 && ny >= 1 && nz >= 1 && ex >= nx && ey >= ny && ez >= nz, "bad sizes");
+    MM_REQUIRE((size_t)nx * ny * nz < (1u << 30), "mesh too large for the direct sum");
+    mm_conv_simple *c = new mm_conv_simple();
+    c->nx = nx;
+    c->ny = ny;
+    c->nz = nz;
+    c->ex = ex;
+    c->ey = ey;
+    c->ez = ez;
+    const size_t bytes = sizeof(double) * 6 * (size_t)ex * ey * ez;
+    if (hipMalloc(&c->N, bytes) != hipSuccess) {
+        delete c;
+        MM_REQUIRE(false, "device allocation failed");
+    }
+    const hipError_t e = hipMemcpy(c->N, N_host, bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(c->N);
+        delete c;
+        MM_HIP(e);
+    }
+    *out = c;
+    MM_CAPI_END
+}
+
+extern "C" int mm_conv_simple_execute(mm_conv_simple_t *c, const double *M, double *H, void *stream)
+{
+    MM_CAPI_BEGIN
+    MM_REQUIRE(c && M && H, "null argument");
+    const int n = c->nx * c->ny * c->nz;
+    hipLaunchKernelGGL(k_conv_simple, dim3((n + MM_SIMPLE_TILE - 1) / MM_SIMPLE_TILE), dim3(MM_SIMPLE_TILE), 0,
+                       (hipStream_t)stream, c->N, c->nx, c->ny, c->nz, c->ex, c->ey, c->ez, M, H);
+    MM_HIP(hipGetLastError());
+    MM_CAPI_END
+}
+
+extern "C" int mm_conv_simple_destroy(mm_conv_simple_t *c)
+{
+    MM_CAPI_BEGIN
+    if (!c) return MM_OK;
+    if (c->N) (void)hipFree(c->N);
+    delete c;
+    MM_CAPI_END
+}

@@ -247,6 +247,33 @@ class SymmetricMatrixVectorConvolution_FFT:
             pass
 
 
+class SymmetricMatrixVectorConvolution_Simple:
+    """SymmetricMatrixVectorConvolution_Simple(N, nx, ny, nz) (math.i:35-41):
+    the direct-sum convolution, FP64, O(n^2)."""
+
+    def __init__(self, N, nx, ny, nz):
+        L = _lib.device_lib()
+        N = np.ascontiguousarray(N, dtype=np.float64)
+        ez, ey, ex, six = N.shape
+        assert six == 6
+        self.shape = (nx, ny, nz)
+        h = ctypes.c_void_p()
+        _lib.check(L.mm_conv_simple_create(N.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), nx, ny, nz, ex, ey,
+                                           ez, ctypes.byref(h)))
+        self._h = h
+
+    def execute(self, M, H):
+        _lib.check(_lib.device_lib().mm_conv_simple_execute(self._h, _ptr(M, _F64), _wptr(H, _F64), _stream()))
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None) and _lib._LIB is not None:
+                _lib._LIB.mm_conv_simple_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
 class ButcherTableau:
     """Butcher tableau of a named method (runge_kutta.h:31-72, tableaus.py:54-139)."""
 

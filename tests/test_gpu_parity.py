@@ -302,6 +302,30 @@ def test_conv_compiled_plans_vs_oracle_and_runtime_plans(mg, shape, monkeypatch)
     assert rel_err(H32c, Ho) < 1e-5 and rel_err(H32, H32c) < 1e-6
 
 
+@pytest.mark.parametrize("shape,pbc", [((1, 1, 1), (0, 0, 0)), ((3, 2, 1), (0, 0, 0)), ((4, 4, 1), (0, 0, 0)),
+                                       ((2, 3, 5), (0, 0, 0)), ((5, 4, 3), (0, 0, 0)), ((17, 9, 2), (0, 0, 0)),
+                                       ((6, 5, 2), (1, 0, 0))])
+def test_conv_direct_sum_vs_oracle_and_fft(mg, shape, pbc):
+    # SymmetricMatrixVectorConvolution_Simple: the direct sum over the mesh
+    # (the reference's choice below 32 cells), against the oracle's direct sum
+    # and the FFT convolution of the same tensor
+    nx, ny, nz = shape
+    delta = (2e-9, 3e-9, 2.5e-9)
+    N = mg.GenerateDemagTensor(nx, ny, nz, *delta, *[bool(b) for b in pbc], repeat=2) if any(pbc) \
+        else mg.GenerateDemagTensor(nx, ny, nz, *delta)
+    M = random_unit_field(shape, 11) * 8e5
+    conv = mg.SymmetricMatrixVectorConvolution_Simple(N, nx, ny, nz)
+    H = like(dev(M))
+    conv.execute(dev(M), H)
+    Hs = host(H)
+    Ho = O.conv_simple(N, M)
+    assert rel_err(Hs, Ho) < 1e-13, rel_err(Hs, Ho)
+    fft = mg.SymmetricMatrixVectorConvolution_FFT(N, nx, ny, nz)
+    Hf = like(dev(M))
+    fft.execute(dev(M), Hf)
+    assert rel_err(Hs, host(Hf)) < 1e-12
+
+
 def test_conv_non_smooth_padding_reembedded(mg):
     # round4(2*13-1) = 28 = 4*7 is not radix-friendly: the library re-embeds the
     # tensor into 30; the linear convolution is unchanged.
