@@ -50,7 +50,10 @@ enum { MM_FP64 = 0, MM_FP32 = 1 };
 
 /* Runge-Kutta methods: src/magnum/evolver/tableaus.py:54-139 (+ classical RK4,
  * src/magnum/evolver/runge_kutta_4.py:28-59) */
-enum { MM_RKF45 = 0, MM_CC45 = 1, MM_DP54 = 2, MM_RK23 = 3, MM_RK4 = 4 };
+/* MM_EULER: the reference's Euler evolver (src/magnum/evolver/euler.py:27-36),
+ * y += h dy/dt with the fixed step h0; as there, its last step is not clipped
+ * to the end time of mm_stepper_run_until. */
+enum { MM_RKF45 = 0, MM_CC45 = 1, MM_DP54 = 2, MM_RK23 = 3, MM_RK4 = 4, MM_EULER = 5 };
 
 /* ---------------------------------------------------------------- library */
 /* Last error message of this thread (never NULL). */

@@ -13,8 +13,9 @@ LIB_PATH = os.path.join(_HERE, "libmmhip.so")
 
 MM_OK = 0
 MM_FP64, MM_FP32 = 0, 1
-MM_RKF45, MM_CC45, MM_DP54, MM_RK23, MM_RK4 = 0, 1, 2, 3, 4
-METHODS = {"rkf45": MM_RKF45, "cc45": MM_CC45, "dp54": MM_DP54, "rk23": MM_RK23, "rk4": MM_RK4}
+MM_RKF45, MM_CC45, MM_DP54, MM_RK23, MM_RK4, MM_EULER = 0, 1, 2, 3, 4, 5
+METHODS = {"rkf45": MM_RKF45, "cc45": MM_CC45, "dp54": MM_DP54, "rk23": MM_RK23, "rk4": MM_RK4,
+           "euler": MM_EULER}
 MM_RELAX_DMDT, MM_RELAX_DEG_PER_NS = 1, 2
 
 PADDING_DISABLE = 0

@@ -163,7 +163,7 @@ extern "C" int mm_generate_demag_tensor(int nx, int ny, int nz, double dx, doubl
     MM_CAPI_END
 }
 
-// tableaus.py:54-139; classical RK4 for MM_RK4
+// tableaus.py:54-139; classical RK4 for MM_RK4; euler.py:27-36 for MM_EULER
 extern "C" int mm_tableau(int method, mm_tableau_t *t)
 {
     MM_CAPI_BEGIN
@@ -225,6 +225,11 @@ extern "C" int mm_tableau(int method, mm_tableau_t *t)
             t->a[i] = a[i]; t->c[i] = c3[i]; t->ec[i] = c2[i] - c3[i];
             for (int j = 0; j < 3; ++j) t->b[i][j] = b[i][j];
         }
+        break;
+    }
+    case MM_EULER: {  // euler.py:27-36
+        t->num_steps = 1; t->order = 1; t->fsal = 0;
+        t->c[0] = 1.0;
         break;
     }
     case MM_RK4: {

@@ -690,8 +690,10 @@ __device__ __forceinline__ void accept_step(StepCtl *ctl, double h)
 // (micro_magnetics_stepsize_controller.py:35-73) + RungeKutta.evolve retry
 // and overshoot handling (runge_kutta.py:56-86).  Non-adaptive (rk4): every
 // attempt is accepted with the fixed h.
+// mode: 1 adaptive; 0 fixed step (rk4), last step clipped to t_max; 2 fixed
+// step without clipping (euler.py:27-36 ignores t_max)
 __device__ __forceinline__ void controller_update(StepCtl &c, const double m[4], int order, double eps_abs,
-                                                  double eps_rel, int Ms_uniform, double Ms_u, int adaptive)
+                                                  double eps_rel, int Ms_uniform, double Ms_u, int mode)
 {
     const double h = c.h;
     double dmax = sqrt(m[1]);
@@ -707,7 +709,7 @@ __device__ __forceinline__ void controller_update(StepCtl &c, const double m[4],
     }
     int accept = 1;
     double hn = h;
-    if (adaptive) {
+    if (mode == 1) {
         double e_abs = sqrt(m[0]);
         if (Ms_uniform) e_abs /= Ms_u;
         const double HEADROOM = 0.85, MINS = 0.2, MAXS = 5.0, MINH = 1e-15, MAXH = 1e-10;
@@ -729,7 +731,7 @@ __device__ __forceinline__ void controller_update(StepCtl &c, const double m[4],
         }
     }
     if (accept) {
-        if (c.tmax_on && c.t + h > c.t_max) {
+        if (mode != 2 && c.tmax_on && c.t + h > c.t_max) {
             c.forced = 1;
             c.h_saved = hn;
             c.h = c.t_max - c.t;
@@ -747,7 +749,7 @@ __device__ __forceinline__ void controller_update(StepCtl &c, const double m[4],
 // on the control block.
 __global__ void __launch_bounds__(1024) k_controller(StepCtl *ctl, const double *__restrict__ mpart, int nb,
                                                     int order, double eps_abs, double eps_rel, int Ms_uniform,
-                                                    double Ms_u, int adaptive)
+                                                    double Ms_u, int mode)
 {
     // The control block (thread 0) and the partials load together; the stop
     // flag is looked at only when the maxima are there (after a stopped run
@@ -773,7 +775,7 @@ __global__ void __launch_bounds__(1024) k_controller(StepCtl *ctl, const double 
         m[q] = v;
     }
     // the control block is updated in registers and written back once
-    controller_update(c, m, order, eps_abs, eps_rel, Ms_uniform, Ms_u, adaptive);
+    controller_update(c, m, order, eps_abs, eps_rel, Ms_uniform, Ms_u, mode);
     *ctl = c;
 }
 
@@ -1427,9 +1429,9 @@ template <typename T> struct Stepper : public StepperBase {
             s1_ready = G.fwd != 0;
         }
         if (fsal) have_fsal = true;
-        const bool adaptive = d.method != MM_RK4;
+        const int mode = d.method == MM_RK4 ? 0 : (d.method == MM_EULER ? 2 : 1);
         hipLaunchKernelGGL(k_controller, dim3(1), dim3(1024), 0, s, ctl, mpart, grid_rows, tab.order, d.eps_abs,
-                           d.eps_rel, sys.Ms_uniform, sys.Ms_u, adaptive ? 1 : 0);
+                           d.eps_rel, sys.Ms_uniform, sys.Ms_u, mode);
         MM_HIP(hipGetLastError());
         if (klog) klog->mark(s, KC_CTRL, 0.0);
     }

@@ -4,7 +4,7 @@ Counterpart of the reference's create_solver/Solver.solve loop
 (src/magnum/micromagnetics/create_solver.py:27-79, src/magnum/solver/solver.py:76-142)
 for the modules StrayField, ExchangeField, AnisotropyField, ExternalField and
 SpinTorque, in that order, driven by rkf45/cc45 (MicroMagneticsStepSizeController)
-or classical rk4 with a fixed step.  The whole attempt loop runs on the device;
+or classical rk4 / euler with a fixed step.  The whole attempt loop runs on the device;
 see include/mm_hotpath.h (mm_stepper_*).
 """
 import ctypes

@@ -695,6 +695,9 @@ void orc_tableau_get(int method, orc_tableau *t)
             t->a[i] = a[i]; t->c[i] = c3[i]; t->ec[i] = c2[i] - c3[i];
             for (int j = 0; j < 3; ++j) t->b[i][j] = b[i][j];
         }
+    } else if (method == ORC_EULER) { /* euler.py:27-36: y += h dy/dt */
+        t->num_steps = 1; t->order = 1; t->fsal = 0;
+        t->c[0] = 1.0;
     } else { /* classical RK4, fixed step (runge_kutta_4.py:28-59, stage 4 at h) */
         t->num_steps = 4; t->order = 4; t->fsal = 0;
         t->a[1] = 0.5; t->a[2] = 0.5; t->a[3] = 1.0;
@@ -983,13 +986,14 @@ static long run_impl(const orc_system *s, int method, double eps_abs, double eps
         }
         memcpy(y0, M, sizeof(double) * m);
         for (;;) {
-            double h = method == ORC_RK4 ? h0 : h_try;
+            const int fixed = method == ORC_RK4 || method == ORC_EULER;
+            double h = fixed ? h0 : h_try;
             attempts++;
             apply_attempt(s, method, &t, f1, f2, k, M, ytmp, yerr, fsal, &have_fsal, tnow, h, k0_ready, n);
             k0_ready = 0;
             double hn = h;
             int acc = 1;
-            if (method != ORC_RK4) {
+            if (!fixed) {
                 const double max_err = orc_scaled_abs_max(n, yerr, s->Ms, s->Ms_uniform);
                 const double max_dydt = orc_scaled_abs_max(n, k[0], s->Ms, s->Ms_uniform);
                 acc = controller(eps_abs, eps_rel, h, t.order, max_err, max_dydt, &hn);
@@ -1000,7 +1004,7 @@ static long run_impl(const orc_system *s, int method, double eps_abs, double eps
                 h_try = hn;
                 continue;
             }
-            if (tmax_on && tnow + h > t_max) {
+            if (tmax_on && method != ORC_EULER && tnow + h > t_max) {
                 h = t_max - tnow;
                 memcpy(M, y0, sizeof(double) * m);
                 attempts++;

@@ -64,7 +64,10 @@ void orc_average(int n, const double *M, double out[3]);
 void orc_normalize(int n, double *M, const double *len, int mode);
 
 /* ---- Runge-Kutta tableaus and stage algebra ---- */
-enum { ORC_RKF45 = 0, ORC_CC45 = 1, ORC_DP54 = 2, ORC_RK23 = 3, ORC_RK4 = 4 };
+/* ORC_EULER: the reference's Euler evolver (euler.py:27-36): y += h dy/dt with
+ * the fixed step h0, then finish_step; it does not clip its last step to
+ * t_max, so a run to t_end ends at the first multiple of h0 at or past it */
+enum { ORC_RKF45 = 0, ORC_CC45 = 1, ORC_DP54 = 2, ORC_RK23 = 3, ORC_RK4 = 4, ORC_EULER = 5 };
 typedef struct {
     int num_steps, order, fsal;
     double a[7], b[7][7], c[7], ec[7];

@@ -14,8 +14,8 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
-RKF45, CC45, DP54, RK23, RK4 = 0, 1, 2, 3, 4
-METHODS = {"rkf45": RKF45, "cc45": CC45, "dp54": DP54, "rk23": RK23, "rk4": RK4}
+RKF45, CC45, DP54, RK23, RK4, EULER = 0, 1, 2, 3, 4, 5
+METHODS = {"rkf45": RKF45, "cc45": CC45, "dp54": DP54, "rk23": RK23, "rk4": RK4, "euler": EULER}
 
 _dp = ctypes.POINTER(ctypes.c_double)
 

@@ -600,6 +600,35 @@ def test_stepper_field_function_errors_and_removal():
     plain.close()
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_stepper_euler_vs_oracle(prec):
+    # euler.py:27-36: fixed step, y += h dM/dt, normalise; run_until ends at
+    # the first multiple of h at or past t_end (no clipping)
+    from micromagnum_amd.solver import Stepper
+    shape, delta = (20, 16, 2), (3e-9, 3e-9, 2e-9)
+    kw = dict(Ms=8.6e5, alpha=0.1, A=1.3e-11, k_uni=1e5, axis1=(0.0, 0.0, 1.0), h_ext=(2e4, 0.0, 0.0))
+    h = 5e-15
+    st = Stepper(shape, delta, precision=prec, method="euler", h0=h, **kw)
+    M = vortex_field(shape, delta, (30e-9, 24e-9), Ms=8.6e5)
+    st.set_m(M)
+    st.run(40)
+    s40 = st.stats()
+    sysO = O.System(shape, delta, N=st.N, **kw)
+    Mo, so = sysO.run(M, method="euler", steps=40, h0=h)
+    assert s40["attempts"] == so["attempts"] == 40 and s40["rejections"] == 0
+    assert s40["t"] == pytest.approx(so["t"], rel=1e-14)
+    tol = 1e-12 if prec == "fp64" else 1e-5
+    assert np.abs(st.get_m() - Mo).max() / 8.6e5 < tol
+    t_end = s40["t"] + 2.5 * h
+    st.run_until(t_end)
+    su = st.stats()
+    assert su["attempts"] == 43 and su["t"] == pytest.approx(43 * h, rel=1e-12)
+    Mu, sou = sysO.run_until(Mo, t_end - so["t"], method="euler", h0=h)
+    assert sou["steps"] == 3
+    assert np.abs(st.get_m() - Mu).max() / 8.6e5 < tol
+    st.close()
+
+
 @pytest.mark.parametrize("method", ["rkf45", "dp54", "rk4"])
 def test_stepper_run_until_vs_oracle(method):
     # stepping to a fixed end time: the last accepted step is redone with

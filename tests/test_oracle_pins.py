@@ -237,6 +237,29 @@ def test_rk4_converges_with_order_4():
     assert 3.5 < order < 4.6, errs
 
 
+def test_euler_converges_with_order_1_and_does_not_clip():
+    # euler.py:27-36: y += h dM/dt with the fixed step, then normalise; the
+    # damped single spin (alpha 0.2) converges to the analytic solution with
+    # order 1 (a stage-time or coefficient slip breaks the order)
+    errs = []
+    for h in (4e-14, 2e-14):
+        steps = int(round(1e-11 / h))
+        M, Mref, st = _single_spin(1.0, 0.0, 0.2, 1e5, "euler", steps, h)
+        assert st["attempts"] == steps and st["rejections"] == 0 and st["t"] == pytest.approx(steps * h, rel=1e-12)
+        errs.append(np.abs(M - Mref).max())
+    order = np.log2(errs[0] / errs[1])
+    assert 0.85 < order < 1.15, errs
+    # run_until: the reference's Euler ignores t_max, so the run ends at the
+    # first multiple of h at or past t_end
+    s = O.System((1, 1, 1), (2e-9,) * 3, Ms=8e5, alpha=0.2, h_ext=(0.0, 0.0, 1e5))
+    M0 = np.zeros((3, 1, 1, 1))
+    M0[0] = 8e5
+    h, t_end = 3e-14, 1.0e-12
+    _, su = s.run_until(M0, t_end, method="euler", h0=h)
+    n = int(np.ceil(t_end / h))
+    assert su["steps"] == n and su["t"] == pytest.approx(n * h, rel=1e-12) and su["t"] > t_end
+
+
 @pytest.mark.parametrize("method", ["rkf45", "dp54", "rk4"])
 def test_run_until_lands_on_t_end_and_matches_analytic_llg(method):
     # the last step is clipped to t_end (runge_kutta.py:74-78); the state at
