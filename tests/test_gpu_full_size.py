@@ -142,6 +142,36 @@ def test_film_1024x1024x2_spin_torque_full_size():
     assert np.abs(M64 - M0).max() / Ms > 1e-3
 
 
+def test_thin_film_rk4_1000_steps_full_length():
+    # the thin-film workload at its full length: 128x32x1 cells of 4x4x3 nm,
+    # demag + exchange, alpha 0.5, RK4 with h = 1e-13 s for 1000 steps, <M>
+    # every 10 steps; FP64 against the oracle, the mixed path against FP64
+    from micromagnum_amd import magneto as mg
+    from micromagnum_amd.solver import Stepper
+    import oracle as O
+    from conftest import random_unit_field
+    shape, delta = (128, 32, 1), (4e-9, 4e-9, 3e-9)
+    kw = dict(Ms=8e5, alpha=0.5, A=1.3e-11)
+    N = mg.GenerateDemagTensor(*shape, *delta)
+    M0 = random_unit_field(shape, 0x5EED) * 8e5
+    avg, fin = {}, {}
+    for prec in ("fp64", "fp32"):
+        st = Stepper(shape, delta, N=N, precision=prec, method="rk4", h0=1e-13, **kw)
+        st.set_m(M0)
+        a = []
+        for _ in range(100):
+            st.run(10)
+            a.append(st.average_m())
+        avg[prec], fin[prec] = np.array(a), st.get_m()
+        assert st.stats()["t"] == pytest.approx(1e-10, rel=1e-12)
+        st.close()
+    Mo, so = O.System(shape, delta, N=N, **kw).run(M0, method="rk4", steps=1000, h0=1e-13, record_avg=True)
+    assert np.abs(fin["fp64"] - Mo).max() / 8e5 < 1e-10
+    assert np.abs(avg["fp64"] - so["avg"][9::10]).max() / 8e5 < 1e-10
+    assert np.abs(avg["fp32"] - avg["fp64"]).max() / 8e5 < 1e-5
+    assert np.abs(avg["fp64"][-1] - avg["fp64"][0]).max() / 8e5 > 1e-3  # the state moved
+
+
 def _dipole_tensor(shape, delta):
     """Point-dipole cell-cell tensor in the padded layout and sign convention
     of GenerateDemagTensor (H = N M; round4(2n-1) per open axis, offsets
