@@ -132,6 +132,7 @@ struct ConvImpl {
     bool fft_generic = false;  // runtime FFT plans only (no compile-time lengths)
     int xcd_remap = 1;         // XCD-contiguous tile order (xcd_tile)
     bool prune_y = false;
+    bool wide = false;         // general index arithmetic only (MM_WIDE_INDEX)
     int tk_yp = 1;          // kx columns per WG of the pruned y transforms
     T *kern = nullptr;      // 6 real planes [e][kz<Hz2][ky<Hy2][kx<Hx], pitch Hp, scaled 1/(PxPyPz)
     C2<T> *S1 = nullptr;    // [c][z<nz][y<ny][kx<Hx], pitch Hp

@@ -643,11 +643,12 @@ static int grid_1d(size_t n, int bs)
     return (int)std::min<size_t>(b, 65535 * 64);
 }
 
-// multiplier for udiv: n / d for every n < nmax (0: not exact, divide)
-static unsigned mdiv(unsigned d, size_t nmax)
+// multiplier for udiv: n / d for every n < nmax (0: not exact, divide;
+// wide: always divide)
+static unsigned mdiv(unsigned d, size_t nmax, bool wide = false)
 {
     MM_REQUIRE(d >= 1, "zero divisor");
-    if (d == 1 || (unsigned long long)nmax * d >= (1ull << 32)) return 0u;
+    if (d == 1 || (unsigned long long)nmax * d >= (1ull << 32) || wide) return 0u;
     return (unsigned)((1ull << 32) / d + 1);
 }
 
@@ -839,6 +840,10 @@ void ConvImpl<T>::init(const double *N_host, int nx, int ny, int nz, int ex, int
         const char *e = std::getenv("MM_FFT_GENERIC");
         fft_generic = e && std::atoi(e) != 0;
     }
+    // MM_WIDE_INDEX=1: the general forms of the index arithmetic (integer
+    // division, 64-bit element addressing) that grids beyond the 32-bit
+    // ranges take, for any grid (tests compare both; the results are equal)
+    wide = env_int("MM_WIDE_INDEX", 0) != 0;
     // MM_XCD_REMAP=0: hardware workgroup order for the tiles (measurements)
     xcd_remap = env_int("MM_XCD_REMAP", 1) != 0 ? 1 : 0;
     // pruned y transforms when the padding is at least half of the line
@@ -1012,7 +1017,8 @@ template <typename T> void ConvImpl<T>::spectral(hipStream_t s)
     const int tk = ln.tk_y;
     const size_t shy = lines_lds<MM_SKEW_Y>(tk, py.L, cs);
     // S2 (3 nz Py rows) is the larger of the two arrays
-    const bool y_o32 = (size_t)g.Hp * cs < (1u << 24) && (size_t)3 * g.nz * g.Py * g.Hp * cs < (1ull << 32);
+    const bool y_o32 =
+        (size_t)g.Hp * cs < (1u << 24) && (size_t)3 * g.nz * g.Py * g.Hp * cs < (1ull << 32) && !wide;
     const int tkp = tk_yp;
     const size_t shyp = lines_lds<MM_SKEW_Y>(tkp, pyh.L, cs);
     if (prune_y) {
@@ -1023,12 +1029,12 @@ template <typename T> void ConvImpl<T>::spectral(hipStream_t s)
             const unsigned ntx = (unsigned)((g.Hx + tk - 1) / tk), nb = ntx * g.nz * 3;
             if (y_o32)
                 hipLaunchKernelGGL((k_yfwd<T, decltype(lc)::value, true>), dim3(ntx, g.nz, 3), dim3(tk * py.tline), shy,
-                                   s, S1, S2, g.Hx, g.Hp, g.ny, g.nz, py.dev<T>(), tk, xcd_remap, mdiv(ntx, nb),
-                                   mdiv((unsigned)g.nz, nb));
+                                   s, S1, S2, g.Hx, g.Hp, g.ny, g.nz, py.dev<T>(), tk, xcd_remap, mdiv(ntx, nb, wide),
+                                   mdiv((unsigned)g.nz, nb, wide));
             else
                 hipLaunchKernelGGL((k_yfwd<T, decltype(lc)::value, false>), dim3(ntx, g.nz, 3), dim3(tk * py.tline),
-                                   shy, s, S1, S2, g.Hx, g.Hp, g.ny, g.nz, py.dev<T>(), tk, xcd_remap, mdiv(ntx, nb),
-                                   mdiv((unsigned)g.nz, nb));
+                                   shy, s, S1, S2, g.Hx, g.Hp, g.ny, g.nz, py.dev<T>(), tk, xcd_remap, mdiv(ntx, nb, wide),
+                                   mdiv((unsigned)g.nz, nb, wide));
         });
     }
     MM_HIP(hipGetLastError());
@@ -1037,9 +1043,10 @@ template <typename T> void ConvImpl<T>::spectral(hipStream_t s)
     const size_t shz = lines_lds<MM_SKEW_Z>(3 * tz, pz.L, cs);
     MM_REQUIRE((size_t)tz * (pz.L / 2 + 1) <= (size_t)ZM_ITEMS * 3 * tz * pz.tline, "z pass items per thread");
     const unsigned ntz = (unsigned)((g.Hx + tz - 1) / tz);
-    const unsigned mtz = mdiv(ntz, (size_t)ntz * g.Py);
+    const unsigned mtz = mdiv(ntz, (size_t)ntz * g.Py, wide);
     const size_t zsb = (size_t)g.Py * g.Hp * cs;
-    const bool o32 = zsb < (1u << 24) && 3 * (size_t)g.nz * zsb < (1ull << 32) && kern_elems * sizeof(T) < (1ull << 32);
+    const bool o32 = zsb < (1u << 24) && 3 * (size_t)g.nz * zsb < (1ull << 32) &&
+                     kern_elems * sizeof(T) < (1ull << 32) && !wide;
     mm_with_len(pz.L, fft_generic, [&](auto lc) {
         if (o32)
             hipLaunchKernelGGL((k_zmi<T, decltype(lc)::value, true>), dim3(ntz, g.Py), dim3(3 * tz * pz.tline), shz, s,
@@ -1060,12 +1067,12 @@ template <typename T> void ConvImpl<T>::spectral(hipStream_t s)
             const unsigned ntx = (unsigned)((g.Hx + tk - 1) / tk), nb = ntx * g.nz * 3;
             if (y_o32)
                 hipLaunchKernelGGL((k_yinv<T, decltype(lc)::value, true>), dim3(ntx, g.nz, 3), dim3(tk * py.tline), shy,
-                                   s, S2, S1, g.Hx, g.Hp, g.ny, g.nz, py.dev<T>(), tk, xcd_remap, mdiv(ntx, nb),
-                                   mdiv((unsigned)g.nz, nb));
+                                   s, S2, S1, g.Hx, g.Hp, g.ny, g.nz, py.dev<T>(), tk, xcd_remap, mdiv(ntx, nb, wide),
+                                   mdiv((unsigned)g.nz, nb, wide));
             else
                 hipLaunchKernelGGL((k_yinv<T, decltype(lc)::value, false>), dim3(ntx, g.nz, 3), dim3(tk * py.tline),
-                                   shy, s, S2, S1, g.Hx, g.Hp, g.ny, g.nz, py.dev<T>(), tk, xcd_remap, mdiv(ntx, nb),
-                                   mdiv((unsigned)g.nz, nb));
+                                   shy, s, S2, S1, g.Hx, g.Hp, g.ny, g.nz, py.dev<T>(), tk, xcd_remap, mdiv(ntx, nb, wide),
+                                   mdiv((unsigned)g.nz, nb, wide));
         });
     }
     MM_HIP(hipGetLastError());

@@ -326,6 +326,19 @@ def test_conv_direct_sum_vs_oracle_and_fft(mg, shape, pbc):
     assert rel_err(Hs, host(Hf)) < 1e-12
 
 
+@pytest.mark.parametrize("shape", [(24, 20, 6), (64, 16, 1), (13, 11, 3), (40, 36, 16)])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_conv_wide_index_forms_give_identical_results(mg, shape, precision, monkeypatch):
+    # the index arithmetic large grids take (integer division, 64-bit element
+    # addressing; MM_WIDE_INDEX=1 forces it) against the 32-bit forms: the
+    # same arithmetic on the same elements, so bit-identical fields
+    H, Ho, _ = _conv_case(mg, shape, (2e-9, 2.5e-9, 3e-9), precision, seed=9)
+    monkeypatch.setenv("MM_WIDE_INDEX", "1")
+    Hw, _, _ = _conv_case(mg, shape, (2e-9, 2.5e-9, 3e-9), precision, seed=9)
+    assert np.array_equal(H, Hw)
+    assert rel_err(H, Ho) < (TOL64 if precision == "fp64" else 1e-5)
+
+
 def test_conv_non_smooth_padding_reembedded(mg):
     # round4(2*13-1) = 28 = 4*7 is not radix-friendly: the library re-embeds the
     # tensor into 30; the linear convolution is unchanged.
