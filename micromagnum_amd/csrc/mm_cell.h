@@ -55,15 +55,20 @@ template <typename T> __device__ __forceinline__ void store3(T *__restrict__ a, 
     a[2 * n + i] = v.z;
 }
 
-// |v| from the float-cast components with every product and sum rounded to
-// float and no contraction: the norm CPUDevice::normalize3 computes
-// (src/magneto/matrix/device/cpu/CPUDevice.cpp:255-259).
+// |v| as CPUDevice::normalize3 computes it
+// (src/magneto/matrix/device/cpu/CPUDevice.cpp:255-259): the components cast
+// to float, every product and sum rounded to float with no contraction, and
+// std::sqrt of that float expression, i.e. the float overload, so the norm
+// itself is a float.  The square root is taken in double and rounded to
+// float: double rounding is harmless for sqrt when 53 >= 2*24 + 2 bits, so
+// this equals the correctly rounded float square root whatever the
+// compiler's single-precision sqrt options are.
 __device__ __forceinline__ double float_cast_norm(double a, double b, double c)
 {
 #pragma clang fp contract(off)
     const float e0 = (float)a, e1 = (float)b, e2 = (float)c;
     const float s = e0 * e0 + e1 * e1 + e2 * e2;
-    return sqrt((double)s);
+    return (double)(float)sqrt((double)s);
 }
 
 struct MeshDev {

@@ -16,8 +16,14 @@
  *   llge       : exact known-answer vector of llge_test.py.
  *   zhangli    : <dM> regression value of spintorque_test.py, zero gradient.
  *   RK         : tableau order conditions, convergence order on scalar ODEs.
- *   controller : restated from source text; parity unpinned beyond the
- *                scaled_abs_max known answers.
+ *   controller : fixed-point error ratios 0.85^(p+1) / 0.85^p under power-law
+ *                errors, scale clamps, step bounds, forced accept, zero
+ *                denominator; closed-loop step size against the acceptance
+ *                boundary on uniform precession (every method).
+ *   FSAL reuse : the whole retry sequence of dp54 on one spin, emulated
+ *                with the published coefficients and the stale k_0.
+ *   normalize  : mode 0 against numpy float32 arithmetic, bit for bit.
+ *   relax      : stop step against the closed-form damped single spin.
  */
 #include "mm_oracle.h"
 
@@ -616,7 +622,10 @@ void orc_average(int n, const double *M, double out[3])
     }
 }
 
-/* CPUDevice.cpp:244-271 (mode 0) */
+/* CPUDevice.cpp:244-271 (mode 0).  The reference's `std::sqrt(e0*e0+e1*e1+
+ * e2*e2)` has a float argument (e0..e2 are float, CPUDevice.cpp:255-258), so
+ * it is the float overload: the norm is the correctly rounded float square
+ * root of the float sum, then widened to double for `len / norm`. */
 void orc_normalize(int n, double *M, const double *len, int mode)
 {
 #pragma omp parallel for schedule(static)
@@ -625,7 +634,7 @@ void orc_normalize(int n, double *M, const double *len, int mode)
         if (mode == 0) {
             const float e0 = (float)M[i], e1 = (float)M[n + i], e2 = (float)M[2 * n + i];
             const float s = e0 * e0 + e1 * e1 + e2 * e2;
-            norm = sqrt((double)s);
+            norm = (double)sqrtf(s);
         } else {
             const double a = M[i], b = M[n + i], c = M[2 * n + i];
             norm = sqrt(a * a + b * b + c * c);
@@ -877,6 +886,12 @@ static int controller(double eps_abs, double eps_rel, double h, int order,
     if (hn == MINH) accept = 1;
     *h_next = hn;
     return accept;
+}
+
+int orc_controller(double eps_abs, double eps_rel, double h, int order, double max_y_err,
+                   double max_dydt, double *h_next)
+{
+    return controller(eps_abs, eps_rel, h, order, max_y_err, max_dydt, h_next);
 }
 
 /* Relaxation rule (mm_stepper_relax semantics): checked before a step whose

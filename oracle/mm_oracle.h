@@ -135,6 +135,12 @@ long orc_relax(const orc_system *s, int method, double eps_abs, double eps_rel, 
 long orc_run_until(const orc_system *s, int method, double eps_abs, double eps_rel, double h0,
                    double t_end, long max_steps, double *M, double *stats);
 
+/* MicroMagneticsStepSizeController.adjust_stepsize
+ * (micro_magnetics_stepsize_controller.py:35-73) on the two maxima it takes
+ * from scaled_abs_max: returns accept, writes the next step size. */
+int orc_controller(double eps_abs, double eps_rel, double h, int order, double max_y_err,
+                   double max_dydt, double *h_next);
+
 /* E_stray, E_exch, E_aniso, E_ext, E_tot at M [J]. */
 void orc_energy(const orc_system *s, const double *M, double *E5);
 void orc_energy_at(const orc_system *s, const double *M, double t, double *E5);

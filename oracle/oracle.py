@@ -169,6 +169,17 @@ def scaled_abs_max(M, s, uniform=None):
     return lib().orc_scaled_abs_max(s.size, _p(M), _p(s), int(uniform))
 
 
+def controller(eps_abs, eps_rel, h, order, max_y_err, max_dydt):
+    """MicroMagneticsStepSizeController.adjust_stepsize on the two maxima:
+    returns (accept, h_next)."""
+    L = lib()
+    d = ctypes.c_double
+    hn = d(0.0)
+    acc = L.orc_controller(d(eps_abs), d(eps_rel), d(h), int(order), d(max_y_err), d(max_dydt),
+                           ctypes.byref(hn))
+    return bool(acc), hn.value
+
+
 def normalize(M, length, mode=0):
     M = _c(M).copy()
     length = _c(np.broadcast_to(length, M.shape[1:]))
