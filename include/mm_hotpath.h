@@ -227,6 +227,11 @@ int mm_stepper_run_until(mm_stepper_t *st, double t_end, long long max_steps);
 int mm_stepper_sync(mm_stepper_t *st);
 /* stats: [0]=t [1]=h of last accepted step [2]=next h [3]=attempts [4]=rejections */
 int mm_stepper_stats(mm_stepper_t *st, double *stats5);
+/* Set the simulation time t and, if h_next > 0, the step size of the next
+ * attempt: the state.t and the next trial step that RungeKutta.evolve keeps
+ * on the state (src/magnum/evolver/runge_kutta.py:55-58,81-82), for a caller
+ * whose state was changed outside the stepper. */
+int mm_stepper_set_time(mm_stepper_t *st, double t, double h_next);
 /* <M> per component (FP64 sums). */
 int mm_stepper_average_m(mm_stepper_t *st, double *avg3);
 void *mm_stepper_stream(mm_stepper_t *st);

@@ -884,6 +884,7 @@ struct StepperBase {
     virtual void run(long long steps) = 0;
     virtual void sync() = 0;
     virtual void stats(double *s5) = 0;
+    virtual void set_time(double t, double h_next) = 0;
     virtual void average(double *a3) = 0;
     virtual hipStream_t stream() = 0;
     virtual void profile(int attempts, double *out) = 0;
@@ -1768,6 +1769,17 @@ template <typename T> struct Stepper : public StepperBase {
         s5[4] = (double)c.rejections;
     }
 
+    // state.t and the evolver's next trial step (runge_kutta.py:55-58,81-82)
+    void set_time(double t, double h_next) override
+    {
+        StepCtl c;
+        MM_HIP(hipStreamSynchronize(s));
+        MM_HIP(hipMemcpy(&c, ctl, sizeof(c), hipMemcpyDeviceToHost));
+        c.t = t;
+        if (h_next > 0.0) c.h = h_next;
+        MM_HIP(hipMemcpy(ctl, &c, sizeof(c), hipMemcpyHostToDevice));
+    }
+
     void average(double *a3) override
     {
         const int grid = (int)std::max<size_t>(1, std::min<size_t>(MM_RED_BLOCKS, (n + 255) / 256));
@@ -2008,6 +2020,14 @@ extern "C" int mm_stepper_stats(mm_stepper_t *st, double *stats5)
     MM_CAPI_BEGIN
     MM_REQUIRE(st && stats5, "null argument");
     st->impl->stats(stats5);
+    MM_CAPI_END
+}
+
+extern "C" int mm_stepper_set_time(mm_stepper_t *st, double t, double h_next)
+{
+    MM_CAPI_BEGIN
+    MM_REQUIRE(st && std::isfinite(t) && std::isfinite(h_next) && h_next >= 0.0, "bad argument");
+    st->impl->set_time(t, h_next);
     MM_CAPI_END
 }
 
