@@ -89,6 +89,26 @@ int mm_generate_demag_tensor(int nx, int ny, int nz, double dx, double dy, doubl
                              int px, int py, int pz, int repeat, int padding,
                              const char *cache_dir, int nthreads, double *N_host, int *exp_out);
 
+/* GenerateDemagTensor for open boundaries with the GPU doing the work
+ * (demag_tensor.cpp:116-186; tensor.cpp:35-191).  One GPU thread per entry of
+ * the positive octant evaluates the six components in FP64 and writes their
+ * mirror images; entries whose branch is a finite difference (the near field
+ * and the partly expanded shells, where FP64 loses ~6 log10(r) digits to
+ * cancellation at distance r cells) or whose branch choice is within 1e-12 of
+ * a threshold are evaluated on the host in long double, bit-identical to
+ * mm_generate_demag_tensor -- unless near_on_gpu, which keeps them in FP64 on
+ * the GPU.  Output as mm_generate_demag_tensor (padding as there, periodic
+ * axes not taken).  info4 (may be NULL): [0] entries done on the GPU, [1] on
+ * the host, [2] GPU seconds (incl. the copy), [3] host seconds.  Uses the
+ * device selected by mm_init. */
+int mm_generate_demag_tensor_gpu(int nx, int ny, int nz, double dx, double dy, double dz, int padding,
+                                 int near_on_gpu, int nthreads, double *N_host, int *exp_out,
+                                 double *info4);
+/* The open-boundary tensor's value at (i, j, k), 0 <= i < nx etc., of the
+ * padded field mm_generate_demag_tensor returns (six components, long
+ * double arithmetic as there), for checking sampled entries of large meshes. */
+int mm_demag_tensor_entry(int i, int j, int k, double dx, double dy, double dz, double *v6);
+
 /* ------------------------------------------------------ FFT convolution */
 /* SymmetricMatrixVectorConvolution_FFT(N, nx, ny, nz)
  * (src/magneto/math/conv/SymmetricMatrixVectorConvolution_FFT.cpp:31-65).

@@ -71,6 +71,8 @@ SIGNATURES = {
     "mm_device_copy": (_i, [_vp, _vp, ctypes.c_size_t, _vp]),
     "mm_round_tensor_dimension": (_i, [_i, _i, _i, _ip]),
     "mm_generate_demag_tensor": (_i, [_i, _i, _i, _d, _d, _d, _i, _i, _i, _i, _i, ctypes.c_char_p, _i, _dp, _ip]),
+    "mm_generate_demag_tensor_gpu": (_i, [_i, _i, _i, _d, _d, _d, _i, _i, _i, _dp, _ip, _dp]),
+    "mm_demag_tensor_entry": (_i, [_i, _i, _i, _d, _d, _d, _dp]),
     "mm_conv_create": (_i, [_dp, _i, _i, _i, _i, _i, _i, _i, ctypes.POINTER(_vp)]),
     "mm_conv_execute": (_i, [_vp, _vp, _vp, _vp]),
     "mm_conv_execute_f32": (_i, [_vp, _vp, _vp, _vp]),

@@ -464,3 +464,61 @@ void mm_demag_tensor_impl(int nx, int ny, int nz, double dx, double dy, double d
     }
     for (size_t q = 0; q < tot; ++q) N[q] = -N[q];
 }
+
+// One open-boundary entry before mirroring, as `entry` above computes it
+// without periodic images: acc[e] = -I_e(i,j,k) / mcs rounded to double.
+void mm_demag_entry_open_ld(int i, int j, int k, double dx, double dy, double dz, double acc[6])
+{
+    static const int CA[6] = {0, 0, 0, 1, 1, 2}, CB[6] = {0, 1, 2, 1, 2, 2};
+    int mcs = 1;
+    if (i == 0) mcs *= 2;
+    if (j == 0) mcs *= 2;
+    if (k == 0) mcs *= 2;
+    for (int e = 0; e < 6; ++e) {
+        int o[3] = {0, 0, 0};
+        ++o[CA[e]];
+        ++o[CB[e]];
+        acc[e] = (double)(0.0L - cell_integral(o[0], o[1], o[2], (ld)dx, (ld)dy, (ld)dz, i, j, k) / (ld)mcs);
+    }
+}
+
+// Mirror one entry into the 8 octants of the padded grid, accumulating in
+// the loop order of `fold` above (tensor.cpp:167-180), open boundaries.
+void mm_demag_fold_open(int i, int j, int k, int ex, int ey, int ez, const double v[6], double *N)
+{
+    for (int o = -1; o <= 1; o += 2)
+        for (int p = -1; p <= 1; p += 2)
+            for (int q = -1; q <= 1; q += 2) {
+                const int I = (o * i + ex) % ex, J = (p * j + ey) % ey, K = (q * k + ez) % ez;
+                double *e = N + 6 * ((size_t)I + (size_t)ex * ((size_t)J + (size_t)ey * K));
+                e[0] += v[0];
+                e[1] += o * p * v[1];
+                e[2] += o * q * v[2];
+                e[3] += v[3];
+                e[4] += p * q * v[4];
+                e[5] += v[5];
+            }
+}
+
+// The padded open-boundary tensor's value at (i, j, k) (0 <= i < nx, ...):
+// the mirror images of entry (i, j, k) that land there, accumulated in the
+// loop order of `fold` and sign-flipped, i.e. what mm_demag_tensor_impl
+// stores at that location (for any padding >= 2n - 1).
+void mm_demag_tensor_entry_impl(int i, int j, int k, double dx, double dy, double dz, double v6[6])
+{
+    double acc[6];
+    mm_demag_entry_open_ld(i, j, k, dx, dy, dz, acc);
+    double e[6] = {0, 0, 0, 0, 0, 0};
+    for (int o = -1; o <= 1; o += 2)
+        for (int p = -1; p <= 1; p += 2)
+            for (int q = -1; q <= 1; q += 2) {
+                if ((o < 0 && i != 0) || (p < 0 && j != 0) || (q < 0 && k != 0)) continue;
+                e[0] += acc[0];
+                e[1] += o * p * acc[1];
+                e[2] += o * q * acc[2];
+                e[3] += acc[3];
+                e[4] += p * q * acc[4];
+                e[5] += acc[5];
+            }
+    for (int c = 0; c < 6; ++c) v6[c] = -e[c];
+}

@@ -45,6 +45,9 @@ int mm_round_tensor_dimension_impl(int dim, bool periodic, int padding);
 void mm_demag_tensor_impl(int nx, int ny, int nz, double dx, double dy, double dz,
                           int ex, int ey, int ez, double *N, int nthreads,
                           int px, int py, int pz, int repeat);
+void mm_demag_entry_open_ld(int i, int j, int k, double dx, double dy, double dz, double acc[6]);
+void mm_demag_fold_open(int i, int j, int k, int ex, int ey, int ez, const double v[6], double *N);
+void mm_demag_tensor_entry_impl(int i, int j, int k, double dx, double dy, double dz, double v6[6]);
 
 // constants (src/magneto/mmm/constants.h:26-40)
 static const long double MM_PI_L = 3.141592653589793238462643383279502884197169399L;

@@ -163,6 +163,15 @@ extern "C" int mm_generate_demag_tensor(int nx, int ny, int nz, double dx, doubl
     MM_CAPI_END
 }
 
+extern "C" int mm_demag_tensor_entry(int i, int j, int k, double dx, double dy, double dz, double *v6)
+{
+    MM_CAPI_BEGIN
+    MM_REQUIRE(i >= 0 && j >= 0 && k >= 0 && v6, "bad argument");
+    MM_REQUIRE(dx > 0 && dy > 0 && dz > 0, "cell sizes must be positive");
+    mm_demag_tensor_entry_impl(i, j, k, dx, dy, dz, v6);
+    MM_CAPI_END
+}
+
 // tableaus.py:54-139; classical RK4 for MM_RK4; euler.py:27-36 for MM_EULER
 extern "C" int mm_tableau(int method, mm_tableau_t *t)
 {

@@ -17,7 +17,7 @@ from ._lib import (PADDING_DISABLE, PADDING_ROUND_2, PADDING_ROUND_4, PADDING_RO
 
 __all__ = ["exchange", "uniaxial_anisotropy", "cubic_anisotropy", "llge", "fdm_zhangli",
            "scaled_abs_max", "abs_max", "dot_sum", "average", "normalize",
-           "GenerateDemagTensor", "round_tensor_dimension", "SymmetricMatrixVectorConvolution_FFT",
+           "GenerateDemagTensor", "demag_tensor_entry", "round_tensor_dimension", "SymmetricMatrixVectorConvolution_FFT",
            "ButcherTableau", "rk_prepare_step", "rk_combine_result",
            "PADDING_DISABLE", "PADDING_ROUND_2", "PADDING_ROUND_4", "PADDING_ROUND_8",
            "PADDING_ROUND_POT", "PADDING_SMALL_PRIME_FACTORS"]
@@ -192,11 +192,32 @@ def round_tensor_dimension(dim, periodic, padding):
 
 
 def GenerateDemagTensor(nx, ny, nz, dx, dy, dz, px=False, py=False, pz=False, repeat=1,
-                        padding=PADDING_ROUND_4, cache_dir=None, nthreads=0):
+                        padding=PADDING_ROUND_4, cache_dir=None, nthreads=0, device=None, near_on_gpu=False,
+                        info=None):
     """Demag tensor field (demag.i:36-42; demag_tensor.cpp:116-186).  Periodic
     axes keep the mesh size and sum `repeat` images plus the far-image
     correction.  Returns a float64 numpy array of shape (ez, ey, ex, 6) whose
-    memory order is the reference's Shape(6, ex, ey, ez)."""
+    memory order is the reference's Shape(6, ex, ey, ez).
+
+    device="gpu" (open boundaries only): the GPU evaluates the series and
+    asymptotic entries in FP64 and the host the finite-difference entries in
+    long double (near_on_gpu: those in FP64 on the GPU too); `info`, a dict,
+    receives the split and the times (mm_generate_demag_tensor_gpu)."""
+    if device == "gpu":
+        if px or py or pz:
+            raise ValueError("GenerateDemagTensor(device='gpu') takes open boundaries only")
+        L = _lib.device_lib()
+        ex = (ctypes.c_int * 3)()
+        _lib.check(L.mm_generate_demag_tensor_gpu(nx, ny, nz, dx, dy, dz, int(padding), int(bool(near_on_gpu)),
+                                                  int(nthreads), None, ex, None))
+        N = np.empty((ex[2], ex[1], ex[0], 6), dtype=np.float64)
+        inf = (ctypes.c_double * 4)()
+        _lib.check(L.mm_generate_demag_tensor_gpu(nx, ny, nz, dx, dy, dz, int(padding), int(bool(near_on_gpu)),
+                                                  int(nthreads), N.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                  ex, inf))
+        if info is not None:
+            info.update(gpu_entries=int(inf[0]), host_entries=int(inf[1]), gpu_s=inf[2], host_s=inf[3])
+        return N
     L = _lib.load()
     ex = (ctypes.c_int * 3)()
     cd = cache_dir.encode() if cache_dir else None
@@ -207,6 +228,15 @@ def GenerateDemagTensor(nx, ny, nz, dx, dy, dz, px=False, py=False, pz=False, re
                                           int(padding), cd, int(nthreads),
                                           N.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ex))
     return N
+
+
+def demag_tensor_entry(i, j, k, dx, dy, dz):
+    """The six components (xx, xy, xz, yy, yz, zz) of the open-boundary
+    tensor at (i, j, k) of the padded field GenerateDemagTensor returns,
+    computed on the host in long double (mm_demag_tensor_entry)."""
+    v = (ctypes.c_double * 6)()
+    _lib.check(_lib.load().mm_demag_tensor_entry(int(i), int(j), int(k), dx, dy, dz, v))
+    return np.array(v[:])
 
 
 class SymmetricMatrixVectorConvolution_FFT:

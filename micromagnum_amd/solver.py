@@ -34,10 +34,13 @@ class SystemSpec:
                  axis1=None, axis2=None, h_ext=None, P=None, xi=None, j=None, do_precess=True,
                  method="rkf45", eps_abs=1e-4, eps_rel=1e-4, h0=1e-13, precision="fp64",
                  normalize_mode=0, padding=PADDING_ROUND_4, pbc=(False, False, False), periodic_repeat=None,
-                 cache_dir=None, h_ext_amp=None, h_ext_freq=None, h_ext_phase=None):
+                 cache_dir=None, h_ext_amp=None, h_ext_freq=None, h_ext_phase=None, tensor="auto"):
         """h_ext: the Zeeman field (the offset of the reference's
         ExternalField); h_ext_amp, h_ext_freq, h_ext_phase (3-tuples) add
-        amp sin(t freq + phase) per component (alternating_field.py:57-84)."""
+        amp sin(t freq + phase) per component (alternating_field.py:57-84).
+        tensor: where GenerateDemagTensor runs when N is not given -- "host",
+        "gpu" (open boundaries), or "auto": the GPU for open meshes of at
+        least 2^18 cells, the host otherwise."""
         nx, ny, nz = shape
         self.shape = tuple(shape)
         self.delta = tuple(float(v) for v in delta)
@@ -86,8 +89,13 @@ class SystemSpec:
         self.pbc = tuple(bool(b) for b in pbc)
         if demag:
             if N is None:
-                N = GenerateDemagTensor(nx, ny, nz, *self.delta, *self.pbc, repeat=periodic_repeat,
-                                        padding=padding, cache_dir=cache_dir)
+                on_gpu = tensor == "gpu" or (tensor == "auto" and not any(self.pbc) and self.n >= 1 << 18
+                                             and cache_dir is None)
+                if on_gpu:
+                    N = GenerateDemagTensor(nx, ny, nz, *self.delta, padding=padding, device="gpu")
+                else:
+                    N = GenerateDemagTensor(nx, ny, nz, *self.delta, *self.pbc, repeat=periodic_repeat,
+                                            padding=padding, cache_dir=cache_dir)
             N = np.ascontiguousarray(N, dtype=np.float64)
             keep.append(N)
             d.N_host = N.ctypes.data_as(ctypes.POINTER(ctypes.c_double))

@@ -202,3 +202,17 @@ def test_tensor_cache_round_trip(tmp_path, monkeypatch):
     # open boundaries: no periodic suffix; fast computations are not saved
     mg.GenerateDemagTensor(3, 3, 1, 2e-9, 2e-9, 2e-9, cache_dir=str(tmp_path))
     assert not (tmp_path / "Demag--3-3-1--8-8-1--2000-2000-2000.dat").exists()
+
+
+@pytest.mark.parametrize("shape,delta", [((7, 5, 3), (1e-9, 1e-9, 1e-9)), ((9, 4, 1), (4e-9, 4e-9, 3e-9)),
+                                         ((5, 6, 4), (1e-9, 2e-9, 5e-9))])
+def test_single_entry_equals_generated_field(shape, delta):
+    # mm_demag_tensor_entry gives, bit for bit, the padded field's value at
+    # each octant location (the checker of sampled entries on large grids)
+    from micromagnum_amd import magneto as mg
+    N = mg.GenerateDemagTensor(*shape, *delta)
+    nx, ny, nz = shape
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                assert np.array_equal(mg.demag_tensor_entry(i, j, k, *delta), N[k, j, i]), (i, j, k)
