@@ -247,6 +247,9 @@ int mm_stepper_run_until(mm_stepper_t *st, double t_end, long long max_steps);
 int mm_stepper_sync(mm_stepper_t *st);
 /* stats: [0]=t [1]=h of last accepted step [2]=next h [3]=attempts [4]=rejections */
 int mm_stepper_stats(mm_stepper_t *st, double *stats5);
+/* Accepted steps since creation (the reference's state.step when the
+ * stepper started from step 0). */
+int mm_stepper_step_count(mm_stepper_t *st, long long *steps);
 /* Set the simulation time t and, if h_next > 0, the step size of the next
  * attempt: the state.t and the next trial step that RungeKutta.evolve keeps
  * on the state (src/magnum/evolver/runge_kutta.py:55-58,81-82), for a caller

@@ -104,6 +104,7 @@ SIGNATURES = {
     "mm_stepper_sync": (_i, [_vp]),
     "mm_stepper_stats": (_i, [_vp, _dp]),
     "mm_stepper_set_time": (_i, [_vp, _d, _d]),
+    "mm_stepper_step_count": (_i, [_vp, ctypes.POINTER(ctypes.c_longlong)]),
     "mm_stepper_average_m": (_i, [_vp, _dp]),
     "mm_stepper_stream": (_vp, [_vp]),
     "mm_stepper_kernel_profile": (_i, [_vp, _i, _dp]),

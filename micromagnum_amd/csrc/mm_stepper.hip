@@ -885,6 +885,7 @@ struct StepperBase {
     virtual void sync() = 0;
     virtual void stats(double *s5) = 0;
     virtual void set_time(double t, double h_next) = 0;
+    virtual long long step_count() = 0;
     virtual void average(double *a3) = 0;
     virtual hipStream_t stream() = 0;
     virtual void profile(int attempts, double *out) = 0;
@@ -1769,6 +1770,14 @@ template <typename T> struct Stepper : public StepperBase {
         s5[4] = (double)c.rejections;
     }
 
+    long long step_count() override
+    {
+        StepCtl c;
+        MM_HIP(hipStreamSynchronize(s));
+        MM_HIP(hipMemcpy(&c, ctl, sizeof(c), hipMemcpyDeviceToHost));
+        return c.accepted;
+    }
+
     // state.t and the evolver's next trial step (runge_kutta.py:55-58,81-82)
     void set_time(double t, double h_next) override
     {
@@ -2020,6 +2029,14 @@ extern "C" int mm_stepper_stats(mm_stepper_t *st, double *stats5)
     MM_CAPI_BEGIN
     MM_REQUIRE(st && stats5, "null argument");
     st->impl->stats(stats5);
+    MM_CAPI_END
+}
+
+extern "C" int mm_stepper_step_count(mm_stepper_t *st, long long *steps)
+{
+    MM_CAPI_BEGIN
+    MM_REQUIRE(st && steps, "null argument");
+    *steps = st->impl->step_count();
     MM_CAPI_END
 }
 

@@ -341,6 +341,19 @@ class RectangularMesh:
         dx, dy, dz = self.delta
         return dx * dy * dz
 
+    @property
+    def size(self):
+        """(nx dx, ny dy, nz dz), RectangularMesh.getSize (RectangularMesh.h:42)"""
+        return tuple(n * d for n, d in zip(self.num_nodes, self.delta))
+
+    def getPosition(self, linidx):
+        """centre of the cell with linear index linidx (RectangularMesh.h:45-58)"""
+        nx, ny, _ = self.num_nodes
+        dx, dy, dz = self.delta
+        z, r = divmod(int(linidx), nx * ny)
+        y, x = divmod(r, nx)
+        return ((x + 0.5) * dx, (y + 0.5) * dy, (z + 0.5) * dz)
+
     def __repr__(self):
         return "RectangularMesh(%r, %r, periodic_bc=%r, periodic_repeat=%r)" % (
             self.num_nodes, self.delta, self.periodic_bc, self.periodic_repeat)

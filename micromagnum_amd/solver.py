@@ -256,6 +256,16 @@ class Stepper:
         _lib.check(_lib.load().mm_stepper_average_m(self._h, self._dp(a)))
         return a
 
+    def step_count(self):
+        """Accepted steps since the stepper was created."""
+        n = ctypes.c_longlong()
+        _lib.check(_lib.load().mm_stepper_step_count(self._h, ctypes.byref(n)))
+        return n.value
+
+    def set_time(self, t, h_next=0.0):
+        """Set t and (h_next > 0) the step size of the next attempt."""
+        _lib.check(_lib.load().mm_stepper_set_time(self._h, float(t), float(h_next)))
+
     ENERGY_TERMS = ("E_stray", "E_exch", "E_aniso", "E_ext", "E_tot")
 
     def energy(self):
