@@ -31,7 +31,6 @@ Extensions: create_solver(..., precision="fp32") runs the mixed-precision
 bulk path; normalize_mode=1 the FP64 renormalisation.
 """
 import math
-import sys
 
 import numpy as np
 
@@ -401,25 +400,8 @@ class Relaxed(Condition):
 
 
 # ------------------------------------------------------------ step handlers
-class StepHandler(object):
-    """step_handler.py:19-36"""
-
-    def handle(self, state):
-        raise NotImplementedError("StepHandler.handle(): Implement me!")
-
-    def done(self):
-        pass
-
-
-class ScreenLog(StepHandler):
-    """One line per handled state: step, t, h, <M> (stephandler/screen_log.py)."""
-
-    def __init__(self, out=None):
-        self.out = out or sys.stdout
-
-    def handle(self, state):
-        a = state.M.average()
-        self.out.write("step=%d t=%.6e h=%.6e <M>=(%.6e, %.6e, %.6e)\n" % (state.step, state.t, state.h, *a))
+from .stephandler import StepHandler, ScreenLog  # noqa: E402,F401
+from .datatable import DataTableLog  # noqa: E402,F401
 
 
 # ------------------------------------------------------------ state
