@@ -244,6 +244,15 @@ int mm_stepper_run(mm_stepper_t *st, int steps);
  * taken without error check; the controller's step size resumes after it.
  * This is what solve(Time(t_end)) does step by step. */
 int mm_stepper_run_until(mm_stepper_t *st, double t_end, long long max_steps);
+/* As mm_stepper_run_until, and also stop after the first accepted step that
+ * ends at t >= t_stop, without shortening that step (t_stop = +inf: no such
+ * stop).  This is the solver loop of src/magnum/solver/solver.py:110-112 for a
+ * stop condition on t that names no time of interest, such as
+ * TimeGreaterEq (src/magnum/solver/condition.py:145-157): solve(Time(t))
+ * steps with step_with_t_max(1e100) until state.t >= t, so its last step
+ * overshoots t.  t_end remains the time of interest that steps are clipped
+ * to (step handlers' get_time_of_interest, solver.py:76-82). */
+int mm_stepper_run_until_stop(mm_stepper_t *st, double t_end, double t_stop, long long max_steps);
 int mm_stepper_sync(mm_stepper_t *st);
 /* stats: [0]=t [1]=h of last accepted step [2]=next h [3]=attempts [4]=rejections */
 int mm_stepper_stats(mm_stepper_t *st, double *stats5);

@@ -101,6 +101,7 @@ SIGNATURES = {
     "mm_stepper_dmdt": (_i, [_vp, _dp, _dp]),
     "mm_stepper_run": (_i, [_vp, _i]),
     "mm_stepper_run_until": (_i, [_vp, _d, ctypes.c_longlong]),
+    "mm_stepper_run_until_stop": (_i, [_vp, _d, _d, ctypes.c_longlong]),
     "mm_stepper_sync": (_i, [_vp]),
     "mm_stepper_stats": (_i, [_vp, _dp]),
     "mm_stepper_set_time": (_i, [_vp, _d, _d]),

@@ -54,6 +54,12 @@ struct StepCtl {
     // taken without error check (forced); the controller's h resumes after it
     int tmax_on, forced;
     double t_max, h_saved;
+    // stop time without clipping (the solver loop's stop conditions on t,
+    // solver.py:110-112 with condition.py:145-157, which name no time of
+    // interest): the run stops after the first accepted step that ends at
+    // t >= t_stop, and that step keeps its full length
+    int tstop_on;
+    double t_stop;
 };
 
 // a per-cell parameter that is either uniform (p == null) or an array
@@ -683,7 +689,9 @@ __device__ __forceinline__ void accept_step(StepCtl *ctl, double h)
     ctl->h_last = h;
     ctl->cur = 1 - ctl->cur;
     ctl->accepted += 1;
-    if (ctl->accepted >= ctl->target || (ctl->tmax_on && ctl->t >= ctl->t_max)) ctl->stop = 1;
+    if (ctl->accepted >= ctl->target || (ctl->tmax_on && ctl->t >= ctl->t_max) ||
+        (ctl->tstop_on && ctl->t >= ctl->t_stop))
+        ctl->stop = 1;
 }
 
 // MicroMagneticsStepSizeController.adjust_stepsize
@@ -891,7 +899,7 @@ struct StepperBase {
     virtual void profile(int attempts, double *out) = 0;
     virtual void energy(double *E5) = 0;
     virtual void relax(int criterion, double thr, int every, long long max_steps, double *info3) = 0;
-    virtual void run_until(double t_end, long long max_steps) = 0;
+    virtual void run_until(double t_end, long long max_steps, double t_stop) = 0;
     virtual void set_field_fn(mm_field_fn_t fn, void *user) = 0;
     virtual MmShape launch_shape() const = 0;
     // ensemble driving: arm(steps) sets the target, then rounds of
@@ -1555,6 +1563,7 @@ template <typename T> struct Stepper : public StepperBase {
         c.target = c.accepted + max_steps;
         c.stop = max_steps == 0 ? 1 : 0;
         c.tmax_on = 0;
+        c.tstop_on = 0;
         c.relax_mode = criterion;
         c.relax_every = every;
         c.relax_thr = thr;
@@ -1710,6 +1719,7 @@ template <typename T> struct Stepper : public StepperBase {
         cached.target = cached.accepted + std::max(0ll, steps);
         cached.stop = steps > 0 ? 0 : 1;
         cached.tmax_on = 0;
+        cached.tstop_on = 0;
         MM_HIP(hipMemcpy(ctl, &cached, sizeof(cached), hipMemcpyHostToDevice));
     }
 
@@ -1732,24 +1742,30 @@ template <typename T> struct Stepper : public StepperBase {
         c.target = c.accepted + steps;
         c.stop = 0;
         c.tmax_on = 0;
+        c.tstop_on = 0;
         MM_HIP(hipMemcpy(ctl, &c, sizeof(c), hipMemcpyHostToDevice));
         drive(c, 1ll << 40);
     }
 
-    // accepted steps until t reaches t_end (last step clipped) or max_steps
-    void run_until(double t_end, long long max_steps) override
+    // accepted steps until t reaches t_end (last step clipped), a step ends
+    // at t >= t_stop (not clipped; t_stop = inf: no such stop) or max_steps
+    void run_until(double t_end, long long max_steps, double t_stop) override
     {
         StepCtl c;
         MM_HIP(hipStreamSynchronize(s));
         MM_HIP(hipMemcpy(&c, ctl, sizeof(c), hipMemcpyDeviceToHost));
+        const bool ts = std::isfinite(t_stop);
         c.target = c.accepted + max_steps;
-        c.stop = (max_steps <= 0 || c.t >= t_end) ? 1 : 0;
+        c.stop = (max_steps <= 0 || c.t >= t_end || (ts && c.t >= t_stop)) ? 1 : 0;
         c.tmax_on = 1;
         c.t_max = t_end;
         c.forced = 0;
+        c.tstop_on = ts ? 1 : 0;
+        c.t_stop = ts ? t_stop : 0.0;
         MM_HIP(hipMemcpy(ctl, &c, sizeof(c), hipMemcpyHostToDevice));
         drive(c, 64);
         c.tmax_on = 0;
+        c.tstop_on = 0;
         c.forced = 0;
         c.stop = 0;
         c.target = c.accepted;
@@ -2012,7 +2028,16 @@ extern "C" int mm_stepper_run_until(mm_stepper_t *st, double t_end, long long ma
 {
     MM_CAPI_BEGIN
     MM_REQUIRE(st, "null argument");
-    st->impl->run_until(t_end, max_steps);
+    st->impl->run_until(t_end, max_steps, INFINITY);
+    MM_CAPI_END
+}
+
+extern "C" int mm_stepper_run_until_stop(mm_stepper_t *st, double t_end, double t_stop, long long max_steps)
+{
+    MM_CAPI_BEGIN
+    MM_REQUIRE(st, "null argument");
+    MM_REQUIRE(!std::isnan(t_stop) && !std::isnan(t_end), "NaN time");
+    st->impl->run_until(t_end, max_steps, t_stop);
     MM_CAPI_END
 }
 

@@ -248,6 +248,13 @@ class Condition(object):
     def steps_left(self, state):
         return 1
 
+    def stop_time(self, state):
+        """None, or a time from which on the value may change after a step
+        that is not clipped to it: the device returns after the first step
+        that ends at or past it (TimeGreaterEq names no time of interest, so
+        the reference's solve(Time(t)) overshoots t)."""
+        return None
+
     def __and__(self, other):
         return AndCondition(self, other)
 
@@ -263,6 +270,11 @@ def _toi(*conds, state):
     return None if t == 1e100 else t
 
 
+def _stop(*conds, state):
+    ts = [t for t in (c.stop_time(state) for c in conds) if t is not None]
+    return min(ts) if ts else None
+
+
 class AndCondition(Condition):
     def __init__(self, a, b):
         super(AndCondition, self).__init__(lambda s: a.check(s) and b.check(s))
@@ -273,6 +285,9 @@ class AndCondition(Condition):
 
     def steps_left(self, state):
         return min(c.steps_left(state) for c in self._ops)
+
+    def stop_time(self, state):
+        return _stop(*self._ops, state=state)
 
 
 class OrCondition(AndCondition):
@@ -291,6 +306,9 @@ class InvertCondition(Condition):
 
     def steps_left(self, state):
         return self._a.steps_left(state)
+
+    def stop_time(self, state):
+        return self._a.stop_time(state)
 
 
 class Always(Condition):
@@ -349,29 +367,38 @@ class AfterNthStep(Condition):
 
 
 class TimeGreaterEq(Condition):
-    """True once t >= t_max (condition.py:145-157)."""
+    """True once t >= t_max (condition.py:145-157).  It names no time of
+    interest: steps are not clipped to t_max, the first step that ends at or
+    past it makes the condition true."""
 
     def __init__(self, t_max):
         super(TimeGreaterEq, self).__init__(lambda s: s.t >= t_max)
         self.t_max = t_max
 
     def steps_left(self, state):
-        return 1 << 40  # the solver clips the steps to times of interest
+        return 1 if state.t >= self.t_max else 1 << 40
+
+    def stop_time(self, state):
+        return self.t_max if state.t < self.t_max else None
 
 
 Time = TimeGreaterEq
 
 
 class TimeBetween(Condition):
+    """True for t0 <= t < t1 (condition.py:159-167); like TimeGreaterEq it
+    names no time of interest, so no step is clipped to t0."""
+
     def __init__(self, t0, t1):
         super(TimeBetween, self).__init__(lambda s: t0 <= s.t < t1)
         self.t0, self.t1 = t0, t1
 
-    def get_time_of_interest(self, state):
-        return self.t0 if state.t < self.t0 else None
-
     def steps_left(self, state):
-        return 1 << 40 if state.t < self.t0 else 1
+        # true on every step in [t0, t1), false for good after t1
+        return 1 if self.t0 <= state.t < self.t1 else 1 << 40
+
+    def stop_time(self, state):
+        return self.t0 if state.t < self.t0 else None
 
 
 class Relaxed(Condition):
@@ -760,12 +787,13 @@ class Solver(object):
                 h.handle(self.state)
 
     # -- the solver loop
-    def _advance(self, t_max, steps):
+    def _advance(self, t_max, steps, t_stop=None):
         """Up to `steps` accepted steps on the device, the last one clipped
-        to t_max (runge_kutta.py:74-78); the state follows."""
+        to t_max (runge_kutta.py:74-78), ending early after a step that ends
+        at t >= t_stop (not clipped); the state follows."""
         st = self._stepper_now()
         before = st.step_count()
-        st.run_until(t_max, max(1, int(steps)))
+        st.run_until(t_max, max(1, int(steps)), t_stop=t_stop)
         made = st.step_count() - before
         s = st.stats()
         self.state.__dict__.update(t=s["t"], h=s["h"], step=self.state.step + made)
@@ -797,7 +825,8 @@ class Solver(object):
                      + [c.get_time_of_interest(self.state) for _, c in self._step_handlers]
                      if t is not None and t > t0)
             n = min([stop_condition.steps_left(self.state)] + [c.steps_left(self.state) for _, c in self._step_handlers])
-            self._advance(t1, n)
+            t_stop = _stop(stop_condition, *[c for _, c in self._step_handlers], state=self.state)
+            self._advance(t1, n, t_stop)
             self._call_step_handlers()
         self.state.flush_cache()
 
@@ -813,7 +842,8 @@ class Solver(object):
             return
         before = st.step_count()
         # as solve's loop test, the rule is checked at the current state first
-        self.last_relax = st.relax("deg_per_ns", cond.max_degree_per_ns, check_every=cond.every)
+        self.last_relax = st.relax(threshold=cond.max_degree_per_ns, criterion="deg_per_ns",
+                                   check_every=cond.every)
         made = st.step_count() - before
         s = st.stats()
         self.state.__dict__.update(t=s["t"], h=s["h"], step=self.state.step + made)

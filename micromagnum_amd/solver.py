@@ -237,10 +237,16 @@ class Stepper:
         _lib.check(_lib.load().mm_stepper_run(self._h, int(steps)))
         self._fn_check()
 
-    def run_until(self, t_end, max_steps=10000000):
+    def run_until(self, t_end, max_steps=10000000, t_stop=None):
         """Step until t == t_end (the last step clipped as in the reference's
-        RungeKutta.evolve) or max_steps accepted steps."""
-        _lib.check(_lib.load().mm_stepper_run_until(self._h, float(t_end), int(max_steps)))
+        RungeKutta.evolve) or max_steps accepted steps; with t_stop, also
+        stop after the first step that ends at t >= t_stop (that step is not
+        clipped: the solver loop's stop condition Time(t_stop))."""
+        L = _lib.load()
+        if t_stop is None:
+            _lib.check(L.mm_stepper_run_until(self._h, float(t_end), int(max_steps)))
+        else:
+            _lib.check(L.mm_stepper_run_until_stop(self._h, float(t_end), float(t_stop), int(max_steps)))
         self._fn_check()
 
     def sync(self):

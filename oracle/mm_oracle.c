@@ -967,7 +967,10 @@ static void apply_attempt(const orc_system *s, int method, const orc_tableau *t,
  * finish_step = normalize(Ms) (micro_magnetics.py:49-50); classical RK4 with
  * the fixed h0 for ORC_RK4.  t_max (tmax_on): an accepted step past it is
  * redone from the same state with h = t_max - t (runge_kutta.py:74-78) and
- * stepping ends once t >= t_max. */
+ * stepping ends once t >= t_max.  tmax_on == 2: no clipping, stepping ends
+ * after the first step that ends at t >= t_max (the solver loop of
+ * solver.py:110-112 with TimeGreaterEq, condition.py:145-157, which names no
+ * time of interest, so evolve runs with t_max = 1e100). */
 static long run_impl(const orc_system *s, int method, double eps_abs, double eps_rel,
                      double h0, long steps, double *M, double *stats, double *avg_out,
                      relax_state *rx, int tmax_on, double t_max)
@@ -1019,7 +1022,7 @@ static long run_impl(const orc_system *s, int method, double eps_abs, double eps
                 h_try = hn;
                 continue;
             }
-            if (tmax_on && method != ORC_EULER && tnow + h > t_max) {
+            if (tmax_on == 1 && method != ORC_EULER && tnow + h > t_max) {
                 h = t_max - tnow;
                 memcpy(M, y0, sizeof(double) * m);
                 attempts++;
@@ -1051,6 +1054,12 @@ long orc_run_until(const orc_system *s, int method, double eps_abs, double eps_r
                    double t_end, long max_steps, double *M, double *stats)
 {
     return run_impl(s, method, eps_abs, eps_rel, h0, max_steps, M, stats, NULL, NULL, 1, t_end);
+}
+
+long orc_run_to_time(const orc_system *s, int method, double eps_abs, double eps_rel, double h0,
+                     double t_stop, long max_steps, double *M, double *stats)
+{
+    return run_impl(s, method, eps_abs, eps_rel, h0, max_steps, M, stats, NULL, NULL, 2, t_stop);
 }
 
 long orc_relax(const orc_system *s, int method, double eps_abs, double eps_rel, double h0,

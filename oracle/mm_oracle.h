@@ -134,6 +134,10 @@ long orc_relax(const orc_system *s, int method, double eps_abs, double eps_rel, 
  * runge_kutta.py:74-78) or max_steps accepted steps.  Returns the steps made. */
 long orc_run_until(const orc_system *s, int method, double eps_abs, double eps_rel, double h0,
                    double t_end, long max_steps, double *M, double *stats);
+/* steps until the first accepted step that ends at t >= t_stop, none clipped
+ * (solve(Time(t_stop)), solver.py:110-112); returns the accepted steps */
+long orc_run_to_time(const orc_system *s, int method, double eps_abs, double eps_rel, double h0,
+                     double t_stop, long max_steps, double *M, double *stats);
 
 /* MicroMagneticsStepSizeController.adjust_stepsize
  * (micro_magnetics_stepsize_controller.py:35-73) on the two maxima it takes

@@ -350,6 +350,19 @@ class System:
                    value=float(info[2]))
         return M, out
 
+    def run_to_time(self, M, t_stop, method="rkf45", max_steps=1000000, eps_abs=1e-4, eps_rel=1e-4, h0=1e-13):
+        """Accepted steps until the first one that ends at t >= t_stop, none
+        clipped (the reference's solve(Time(t_stop)))."""
+        M = _c(M).copy()
+        stats = np.zeros(5)
+        L = lib()
+        L.orc_run_to_time.restype = ctypes.c_long
+        done = L.orc_run_to_time(ctypes.byref(self.s), METHODS[method], ctypes.c_double(eps_abs),
+                                 ctypes.c_double(eps_rel), ctypes.c_double(h0), ctypes.c_double(t_stop),
+                                 ctypes.c_long(int(max_steps)), _p(M), _p(stats))
+        return M, dict(t=stats[0], h=stats[1], h_next=stats[2], attempts=int(stats[3]),
+                       rejections=int(stats[4]), steps=int(done))
+
     def run_until(self, M, t_end, method="rkf45", max_steps=1000000, eps_abs=1e-4, eps_rel=1e-4, h0=1e-13):
         """Accepted steps until t_end, the last one clipped (runge_kutta.py:74-78)."""
         M = _c(M).copy()

@@ -80,6 +80,24 @@ def test_conditions():
     assert e.get_time_of_interest(_S(0, 2.5e-11)) == pytest.approx(3e-11)
 
 
+def test_time_conditions_name_no_time_of_interest():
+    # condition.py:145-167: TimeGreaterEq and TimeBetween inherit
+    # get_time_of_interest() -> None, so solve(Time(t)) steps with
+    # step_with_t_max(1e100) and its last step overshoots t; the front end
+    # instead hands the device a stop time that does not clip
+    t = mm.Time(1e-9)
+    assert t.get_time_of_interest(_S(0, 5e-10)) is None
+    assert t.stop_time(_S(0, 5e-10)) == 1e-9 and t.stop_time(_S(0, 1e-9)) is None
+    assert t.steps_left(_S(0, 5e-10)) > 1000 and t.steps_left(_S(0, 2e-9)) == 1
+    b = mm.TimeBetween(1e-9, 2e-9)
+    assert b.get_time_of_interest(_S(0, 0.0)) is None and b.stop_time(_S(0, 0.0)) == 1e-9
+    assert b.steps_left(_S(0, 1.5e-9)) == 1 and b.steps_left(_S(0, 3e-9)) > 1000
+    assert (mm.Time(3e-9) | mm.Time(2e-9)).stop_time(_S(0, 0.0)) == 2e-9
+    assert (mm.EveryNthStep(3) & mm.Time(2e-9)).stop_time(_S(0, 0.0)) == 2e-9
+    assert (~mm.Time(2e-9)).stop_time(_S(0, 0.0)) == 2e-9
+    assert mm.EveryNthStep(3).stop_time(_S(0, 0.0)) is None
+
+
 def test_relaxed_condition_needs_a_previous_check_and_a_decrease():
     # condition.py:177-198
     r = mm.Relaxed(5.0, 3)

@@ -45,7 +45,8 @@ def _octant(N, shape):
     ((72, 40, 4), (1e-9, 1e-9, 1e-9)),        # cubic cells
     ((128, 32, 1), (4e-9, 4e-9, 3e-9)),       # the thin-film workload's cells
     ((96, 64, 2), (3e-9, 3e-9, 2e-9)),        # the spin-torque film's cells
-    ((60, 30, 3), (1e-9, 2e-9, 5e-9)),        # elongated: partly expanded shells
+    ((160, 40, 3), (1e-9, 1.5e-9, 2e-9)),     # elongated cells: all three axes expanded only
+                                              # beyond 60 cells (radius 30 x the longest edge)
 ])
 def test_gpu_tensor_matches_host_generator(mg, shape, delta):
     Ng, Nh, info = _split(mg, shape, delta)

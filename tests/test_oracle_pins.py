@@ -297,6 +297,25 @@ def test_run_until_takes_the_unclipped_steps_of_run():
     assert sb["steps"] == 5 and np.array_equal(Mb, M5)
 
 
+@pytest.mark.parametrize("method", ["rkf45", "dp54"])
+def test_run_to_time_is_the_first_unclipped_step_past_t(method):
+    # solve(Time(t)) (solver.py:110-112): plain steps while state.t < t, so
+    # the run is run() up to the first step count whose end time reaches t
+    shape = (6, 5, 2)
+    s = O.System(shape, (2e-9,) * 3, Ms=8e5, alpha=0.2, A=1.3e-11, k_uni=1e5, axis1=(0.0, 0.0, 1.0))
+    M0 = np.ascontiguousarray(np.broadcast_to(np.array([0.6, 0.0, 0.8]).reshape(3, 1, 1, 1) * 8e5, (3, 2, 5, 6)))
+    ts = [s.run(M0, method=method, steps=k)[1]["t"] for k in range(1, 9)]
+    t_stop = 0.5 * (ts[4] + ts[5])
+    k = next(q + 1 for q, t in enumerate(ts) if t >= t_stop)
+    Mr, sr = s.run_to_time(M0, t_stop, method=method)
+    Mk, sk = s.run(M0, method=method, steps=k)
+    assert k == 6 and sr["steps"] == k and sr["t"] == sk["t"] > t_stop
+    assert np.array_equal(Mr, Mk) and sr["attempts"] == sk["attempts"]
+    # a stop time on a step boundary stops there
+    Mb, sb = s.run_to_time(M0, ts[4], method=method)
+    assert sb["steps"] == 5 and sb["t"] == ts[4]
+
+
 # ---------------------------------------------------------------- alternating field
 @pytest.mark.parametrize("method", ["rkf45", "dp54", "rk23", "cc45"])
 def test_alternating_field_precession_closed_form(method):
