@@ -71,6 +71,14 @@ __device__ __forceinline__ double float_cast_norm(double a, double b, double c)
     return (double)(float)sqrt((double)s);
 }
 
+// |v| in FP64, products and sums rounded separately (no contraction), as
+// the CPU path's sqrt(a*a + b*b + c*c)
+__device__ __forceinline__ double fp64_norm(double a, double b, double c)
+{
+#pragma clang fp contract(off)
+    return sqrt(a * a + b * b + c * c);
+}
+
 struct MeshDev {
     int nx, ny, nz;
     int px, py, pz;

@@ -20,13 +20,13 @@
 // reduction scratch
 // ---------------------------------------------------------------------------
 static std::mutex g_red_mtx;
-static double *g_red_partial = nullptr; // MM_RED_BLOCKS * 3
+static double *g_red_partial = nullptr; // MM_RED_BLOCKS * 6: up to three compensated pairs
 static double *g_red_out = nullptr;     // 4
 
 static void red_scratch()
 {
     if (!g_red_partial) {
-        MM_HIP(hipMalloc(&g_red_partial, sizeof(double) * MM_RED_BLOCKS * 4));
+        MM_HIP(hipMalloc(&g_red_partial, sizeof(double) * MM_RED_BLOCKS * 6));
         MM_HIP(hipMalloc(&g_red_out, sizeof(double) * 4));
     }
 }
@@ -41,7 +41,7 @@ __global__ void k_exchange(MeshDev m, T wx, T wy, T wz, T coef, const T *__restr
                            const T *__restrict__ M, T *__restrict__ H, double *__restrict__ epart)
 {
     const size_t n = (size_t)m.nx * m.ny * m.nz;
-    double e = 0.0;
+    Dd e = {0.0, 0.0};
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         const int x = (int)(i % m.nx);
         const int y = (int)((i / m.nx) % m.ny);
@@ -53,23 +53,24 @@ __global__ void k_exchange(MeshDev m, T wx, T wy, T wz, T coef, const T *__restr
         }
         const V3<T> h = exchange_cell<T>(m, x, y, z, i, n, wx, wy, wz, coef, Ms, A, M, ms);
         store3<T>(H, n, i, h);
-        if (epart) e += (double)(M[i] / ms) * h.x + (double)(M[n + i] / ms) * h.y + (double)(M[2 * n + i] / ms) * h.z;
+        if (epart)
+            dd_add(e, (double)(M[i] / ms) * h.x + (double)(M[n + i] / ms) * h.y + (double)(M[2 * n + i] / ms) * h.z);
     }
-    if (epart) block_sum_store(e, epart);
+    if (epart) block_sum_dd_store(e, epart, MM_RED_BLOCKS);
 }
 
 template <typename T>
 __global__ void k_uniaxial(size_t n, const T *__restrict__ axis, const T *__restrict__ k, const T *__restrict__ Ms,
                            const T *__restrict__ M, T *__restrict__ H, T mu0, double *__restrict__ epart)
 {
-    double e = 0.0;
+    Dd e = {0.0, 0.0};
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         T ei;
         const V3<T> h = uniaxial_cell<T>(load3(M, n, i), load3(axis, n, i), k[i], Ms[i], mu0, &ei);
         store3<T>(H, n, i, h);
-        e += (double)ei;
+        dd_add(e, (double)ei);
     }
-    if (epart) block_sum_store(e, epart);
+    if (epart) block_sum_dd_store(e, epart, MM_RED_BLOCKS);
 }
 
 template <typename T>
@@ -77,14 +78,14 @@ __global__ void k_cubic(size_t n, const T *__restrict__ a1, const T *__restrict_
                         const T *__restrict__ Ms, const T *__restrict__ M, T *__restrict__ H, T cm2mu0,
                         double *__restrict__ epart)
 {
-    double e = 0.0;
+    Dd e = {0.0, 0.0};
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         T ei;
         const V3<T> h = cubic_cell<T>(load3(M, n, i), load3(a1, n, i), load3(a2, n, i), k[i], Ms[i], cm2mu0, &ei);
         store3<T>(H, n, i, h);
-        e += (double)ei;
+        dd_add(e, (double)ei);
     }
-    if (epart) block_sum_store(e, epart);
+    if (epart) block_sum_dd_store(e, epart, MM_RED_BLOCKS);
 }
 
 template <typename T>
@@ -132,23 +133,27 @@ __global__ void k_maxsq(size_t n, const T *__restrict__ M, const T *__restrict__
 template <typename T>
 __global__ void k_dot(size_t n, const T *__restrict__ A, const T *__restrict__ B, double *__restrict__ part)
 {
-    double acc = 0.0;
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        acc += (double)A[i] * B[i] + (double)A[n + i] * B[n + i] + (double)A[2 * n + i] * B[2 * n + i];
-    block_sum_store(acc, part);
+    Dd acc = {0.0, 0.0};
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        dd_add(acc, (double)A[i] * B[i]);
+        dd_add(acc, (double)A[n + i] * B[n + i]);
+        dd_add(acc, (double)A[2 * n + i] * B[2 * n + i]);
+    }
+    block_sum_dd_store(acc, part, MM_RED_BLOCKS);
 }
 
 template <typename T> __global__ void k_sum3(size_t n, const T *__restrict__ A, double *__restrict__ part)
 {
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    Dd s0 = {0.0, 0.0}, s1 = {0.0, 0.0}, s2 = {0.0, 0.0};
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        s0 += (double)A[i];
-        s1 += (double)A[n + i];
-        s2 += (double)A[2 * n + i];
+        dd_add(s0, (double)A[i]);
+        dd_add(s1, (double)A[n + i]);
+        dd_add(s2, (double)A[2 * n + i]);
     }
-    block_sum_store(s0, part);
-    block_sum_store(s1, part + MM_RED_BLOCKS);
-    block_sum_store(s2, part + 2 * MM_RED_BLOCKS);
+    // component c's pair at part[2c MM_RED_BLOCKS + b], part[(2c+1) MM_RED_BLOCKS + b]
+    block_sum_dd_store(s0, part, MM_RED_BLOCKS);
+    block_sum_dd_store(s1, part + 2 * MM_RED_BLOCKS, MM_RED_BLOCKS);
+    block_sum_dd_store(s2, part + 4 * MM_RED_BLOCKS, MM_RED_BLOCKS);
 }
 
 // CPUDevice.cpp:244-271 (mode 0) or FP64 norm (mode 1)
@@ -160,8 +165,7 @@ __global__ void k_normalize(size_t n, T *__restrict__ M, const T *__restrict__ l
         if (mode == 0) {
             norm = float_cast_norm((double)M[i], (double)M[n + i], (double)M[2 * n + i]);
         } else {
-            const double a = M[i], b = M[n + i], c = M[2 * n + i];
-            norm = sqrt(a * a + b * b + c * c);
+            norm = fp64_norm((double)M[i], (double)M[n + i], (double)M[2 * n + i]);
         }
         if (norm == 0.0) continue;
         const double sc = (double)len[i] / norm;
@@ -236,9 +240,11 @@ __global__ void k_rk_combine_seq(size_t m, T *__restrict__ y, T *__restrict__ er
 // ---------------------------------------------------------------------------
 static int grid_for(size_t n) { return (int)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, 16384)); }
 
+// part: compensated pairs (part[b], part[MM_RED_BLOCKS + b]) of nblocks blocks
 static double finish_sum(hipStream_t s, double *part, int nblocks)
 {
-    hipLaunchKernelGGL(k_reduce_final_sum, dim3(1), dim3(MM_RED_THREADS), 0, s, part, nblocks, g_red_out);
+    hipLaunchKernelGGL(k_reduce_final_sum_dd, dim3(1), dim3(MM_RED_THREADS), 0, s, part, (size_t)MM_RED_BLOCKS,
+                       nblocks, g_red_out);
     MM_HIP(hipGetLastError());
     double v = 0.0;
     MM_HIP(hipMemcpyAsync(&v, g_red_out, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -436,7 +442,7 @@ extern "C" int mm_average(int n, const double *M, double *out3, void *stream)
     const int grid = red_grid(n);
     hipLaunchKernelGGL(k_sum3<double>, dim3(grid), dim3(MM_RED_THREADS), 0, s, (size_t)n, M, g_red_partial);
     MM_HIP(hipGetLastError());
-    for (int c = 0; c < 3; ++c) out3[c] = finish_sum(s, g_red_partial + c * MM_RED_BLOCKS, grid) / n;
+    for (int c = 0; c < 3; ++c) out3[c] = finish_sum(s, g_red_partial + 2 * c * MM_RED_BLOCKS, grid) / n;
     MM_CAPI_END
 }
 

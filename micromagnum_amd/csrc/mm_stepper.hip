@@ -195,6 +195,84 @@ __device__ __forceinline__ V3<T> exchange_cell_uniform(const MeshDev &m, int x, 
     return v3<T>(f * s.x, f * s.y, f * s.z);
 }
 
+// 32-bit byte-offset access for arrays under 4 GiB: base is a pointer the
+// whole workgroup shares (held in SGPRs), off a per-lane byte offset, so the
+// access is one global load/store with an SGPR base and a 32-bit VGPR offset
+// (no 64-bit address arithmetic per element).
+template <typename T> __device__ __forceinline__ T ld32(const T *__restrict__ base, unsigned off)
+{
+    return *(const T *)((const char *)base + off);
+}
+template <typename T> __device__ __forceinline__ void st32(T *__restrict__ base, unsigned off, T v)
+{
+    *(T *)((char *)base + off) = v;
+}
+
+// exchange_cell_uniform with byte offsets: ib = i sizeof(T), nb = n sizeof(T)
+// (3 nb < 2^32); same neighbours, loads and summation order
+template <typename T>
+__device__ __forceinline__ V3<T> exchange_cell_uniform32(const MeshDev &m, int x, int y, int z, unsigned ib,
+                                                         unsigned nb, T wx, T wy, T wz, T f, const V3<T> &mi,
+                                                         const T *__restrict__ M)
+{
+    constexpr unsigned sx = sizeof(T);
+    const unsigned sy = (unsigned)m.nx * sx, sz = (unsigned)m.nx * (unsigned)m.ny * sx;
+    unsigned j[6];
+    j[0] = x > 0 ? ib - sx : (m.px ? ib + sx * (unsigned)(m.nx - 1) : ib);
+    j[1] = x < m.nx - 1 ? ib + sx : (m.px ? ib - sx * (unsigned)(m.nx - 1) : ib);
+    j[2] = y > 0 ? ib - sy : (m.py ? ib + sy * (unsigned)(m.ny - 1) : ib);
+    j[3] = y < m.ny - 1 ? ib + sy : (m.py ? ib - sy * (unsigned)(m.ny - 1) : ib);
+    j[4] = z > 0 ? ib - sz : (m.pz ? ib + sz * (unsigned)(m.nz - 1) : ib);
+    j[5] = z < m.nz - 1 ? ib + sz : (m.pz ? ib - sz * (unsigned)(m.nz - 1) : ib);
+    T a[6][3];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        a[q][0] = ld32(M, j[q]);
+        a[q][1] = ld32(M, j[q] + nb);
+        a[q][2] = ld32(M, j[q] + 2 * nb);
+    }
+    const T w[6] = {wx, wx, wy, wy, wz, wz};
+    V3<T> s = v3<T>(0, 0, 0);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        s.x += (a[q][0] - mi.x) * w[q];
+        s.y += (a[q][1] - mi.y) * w[q];
+        s.z += (a[q][2] - mi.z) * w[q];
+    }
+    return v3<T>(f * s.x, f * s.y, f * s.z);
+}
+
+// zhangli_cell_uniform with byte offsets (as exchange_cell_uniform32)
+template <typename T>
+__device__ __forceinline__ V3<T> zhangli_cell_uniform32(const MeshDev &m, int x, int y, int z, unsigned ib,
+                                                        unsigned nb, const T *inv_d, T f1, T f2, const V3<T> &j,
+                                                        const V3<T> &Mv, const T *__restrict__ M)
+{
+    auto grad = [&](int pos, int dim, unsigned st, T id) -> V3<T> {
+        if (dim <= 1) return v3<T>(0, 0, 0);
+        unsigned a = ib - st, b = ib + st;
+        T f = id * (T)0.5;
+        if (pos == 0) {
+            a = ib;
+            f = id;
+        } else if (pos == dim - 1) {
+            b = ib;
+            f = id;
+        }
+        return v3<T>((ld32(M, b) - ld32(M, a)) * f, (ld32(M, b + nb) - ld32(M, a + nb)) * f,
+                     (ld32(M, b + 2 * nb) - ld32(M, a + 2 * nb)) * f);
+    };
+    constexpr unsigned sx = sizeof(T);
+    const V3<T> gx = grad(x, m.nx, sx, inv_d[0]);
+    const V3<T> gy = grad(y, m.ny, (unsigned)m.nx * sx, inv_d[1]);
+    const V3<T> gz = grad(z, m.nz, (unsigned)m.nx * (unsigned)m.ny * sx, inv_d[2]);
+    const V3<T> jg = v3<T>(j.x * gx.x + j.y * gy.x + j.z * gz.x, j.x * gx.y + j.y * gy.y + j.z * gz.y,
+                           j.x * gx.z + j.y * gy.z + j.z * gz.z);
+    const V3<T> B = cross3(Mv, jg);
+    const V3<T> Av = cross3(Mv, B);
+    return v3<T>(f1 * Av.x + f2 * B.x, f1 * Av.y + f2 * B.y, f1 * Av.z + f2 * B.z);
+}
+
 // stored stage vectors one stage combination reads at most (DP54's last
 // stage: k_0 .. k_5)
 #define MM_STAGE_MAXK 6
@@ -252,36 +330,61 @@ struct StageRed {
     double mraw;  // |y|^2 of the attempt's start state
 };
 
-// per-thread sums of M.H per term (energy launches only)
+// per-thread sums of M.H per term (energy launches only), compensated
 struct EnergyAcc {
-    double demag, exch, aniso, zee;
+    Dd demag, exch, aniso, zee;
 };
 
 // Per-cell epilogue: everything after the demag field of one cell.  STT:
 // the Zhang-Li term is compiled in; GEN: cubic anisotropy and the exchange
 // with per-cell Ms/A are compiled in (without them the epilogue is shorter
-// and needs fewer registers).
+// and needs fewer registers).  Without GEN the state arrays are addressed by
+// 32-bit byte offsets from their (workgroup-uniform) bases, which the host
+// guarantees by taking GEN for state arrays of 4 GiB or more.
 template <typename T, bool EN, bool STT, bool GEN>
 __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T> &G, const StepCtl *ctl,
                                            const T *__restrict__ Y, int x, int y, int z, const V3<T> &Hd,
                                            const V3<T> &hz, StageRed &red, EnergyAcc &ea, T *nxt)
 {
+    constexpr bool O32 = !GEN;
     const size_t n = S.n;
     const size_t i = (size_t)x + (size_t)S.m.nx * ((size_t)y + (size_t)S.m.ny * z);
-    const T ms = S.Ms.at(i);
-    const V3<T> Mv = v3<T>(Y[i], Y[n + i], Y[2 * n + i]);
+    // byte offsets of cell i and of one component plane (O32 only)
+    const unsigned ib = O32 ? ((unsigned)x + (unsigned)S.m.nx * ((unsigned)y + (unsigned)S.m.ny * (unsigned)z)) *
+                                  (unsigned)sizeof(T)
+                            : 0u;
+    const unsigned nb = O32 ? (unsigned)n * (unsigned)sizeof(T) : 0u;
+    auto ldv = [&](const T *__restrict__ a) -> V3<T> {
+        if constexpr (O32) return v3<T>(ld32(a, ib), ld32(a, ib + nb), ld32(a, ib + 2 * nb));
+        else return v3<T>(a[i], a[n + i], a[2 * n + i]);
+    };
+    auto stv = [&](T *__restrict__ a, const V3<T> &v) {
+        if constexpr (O32) {
+            st32(a, ib, v.x);
+            st32(a, ib + nb, v.y);
+            st32(a, ib + 2 * nb, v.z);
+        } else {
+            store3<T>(a, n, i, v);
+        }
+    };
+    const T ms = S.Ms_uniform ? S.Ms.u : (O32 ? ld32(gfield(S.Ms.p), ib) : S.Ms.at(i));
+    const V3<T> Mv = ldv(Y);
     V3<T> H = Hd;
-    if (EN) ea.demag += dotd<T>(Mv, Hd);
+    if (EN) dd_add(ea.demag, dotd<T>(Mv, Hd));
     if (S.has_exch && ms != (T)0) {
-        const V3<T> he = (!GEN || S.exch_uniform)
-                             ? exchange_cell_uniform<T>(S.m, x, y, z, i, n, S.wx, S.wy, S.wz, S.exch_f, Y)
-                             : exchange_cell<T>(S.m, x, y, z, i, n, S.wx, S.wy, S.wz, S.coef_ex,
-                                                S.Ms.p ? gfield(S.Ms.p) : nullptr, S.A.p ? gfield(S.A.p) : nullptr,
-                                                Y, ms);
+        V3<T> he;
+        if constexpr (O32) {
+            he = exchange_cell_uniform32<T>(S.m, x, y, z, ib, nb, S.wx, S.wy, S.wz, S.exch_f, Mv, Y);
+        } else {
+            he = S.exch_uniform
+                     ? exchange_cell_uniform<T>(S.m, x, y, z, i, n, S.wx, S.wy, S.wz, S.exch_f, Y)
+                     : exchange_cell<T>(S.m, x, y, z, i, n, S.wx, S.wy, S.wz, S.coef_ex,
+                                        S.Ms.p ? gfield(S.Ms.p) : nullptr, S.A.p ? gfield(S.A.p) : nullptr, Y, ms);
+        }
         H.x += he.x;
         H.y += he.y;
         H.z += he.z;
-        if (EN) ea.exch += dotd<T>(Mv, he);
+        if (EN) dd_add(ea.exch, dotd<T>(Mv, he));
     }
     if (S.has_uni || (GEN && S.has_cub)) {
         T e0 = (T)0, e1 = (T)0;
@@ -300,7 +403,7 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
             } else {
                 hu = uniaxial_cell<T>(Mv, S.axis1.at(n, i), S.kuni.at(i), ms, S.mu0, &e0);
             }
-            if (EN) ea.aniso += (double)e0;
+            if (EN) dd_add(ea.aniso, (double)e0);
             if (GEN && S.has_cub) {
                 ha.x += hu.x;
                 ha.y += hu.y;
@@ -309,7 +412,7 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
                 ha = hu;
             }
         }
-        if (EN) ea.aniso += (double)e1;
+        if (EN && GEN && S.has_cub) dd_add(ea.aniso, (double)e1);
         H.x += ha.x;
         H.y += ha.y;
         H.z += ha.z;
@@ -318,22 +421,27 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
         H.x += hz.x;
         H.y += hz.y;
         H.z += hz.z;
-        if (EN) ea.zee += (double)Mv.x * hz.x + (double)Mv.y * hz.y + (double)Mv.z * hz.z;
+        if (EN) dd_add(ea.zee, (double)Mv.x * hz.x + (double)Mv.y * hz.y + (double)Mv.z * hz.z);
     }
     if (EN) return;
     V3<T> k = llge_cell<T>(Mv, H, S.f1.at(i), S.f2.at(i));
     if (STT && S.has_stt) {
-        const V3<T> st =
-            S.stt_fast ? zhangli_cell_uniform<T>(S.m, x, y, z, i, n, S.inv_d, S.stt_f1, S.stt_f2,
-                                                 v3<T>(S.j.u[0], S.j.u[1], S.j.u[2]), Y)
-                       : zhangli_cell<T>(S.m, x, y, z, i, n, S.dx, S.dy, S.dz, S.do_precess != 0, S.P.at(i),
-                                         S.xi.at(i), ms, S.alpha.at(i), S.j.at(n, i), Y, S.mu_b, S.e_charge);
+        V3<T> st;
+        if (O32 && S.stt_fast)
+            st = zhangli_cell_uniform32<T>(S.m, x, y, z, ib, nb, S.inv_d, S.stt_f1, S.stt_f2,
+                                           v3<T>(S.j.u[0], S.j.u[1], S.j.u[2]), Mv, Y);
+        else if (S.stt_fast)
+            st = zhangli_cell_uniform<T>(S.m, x, y, z, i, n, S.inv_d, S.stt_f1, S.stt_f2,
+                                         v3<T>(S.j.u[0], S.j.u[1], S.j.u[2]), Y);
+        else
+            st = zhangli_cell<T>(S.m, x, y, z, i, n, S.dx, S.dy, S.dz, S.do_precess != 0, S.P.at(i), S.xi.at(i), ms,
+                                 S.alpha.at(i), S.j.at(n, i), Y, S.mu_b, S.e_charge);
         k.x += st.x;
         k.y += st.y;
         k.z += st.z;
     }
-    if (G.kout) store3<T>(gfield(G.kout), n, i, k);
-    if (G.Hout) store3<T>(gfield(G.Hout), n, i, H);
+    if (G.kout) stv(gfield(G.kout), k);
+    if (G.Hout) stv(gfield(G.Hout), H);
     if (G.want_dydt) {
         const double a = k.x, b = k.y, c = k.z;
         double q = a * a + b * b + c * c;
@@ -352,14 +460,19 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
     // computation's live registers)
     const T *yc = ctl->cur ? gfield(G.yb1) : gfield(G.yb0);
     T kv[MM_STAGE_MAXK][3], ycv[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) ycv[c] = yc[(size_t)c * n + i];
+    {
+        const V3<T> v = ldv(yc);
+        ycv[0] = v.x;
+        ycv[1] = v.y;
+        ycv[2] = v.z;
+    }
 #pragma unroll
     for (int q = 0; q < MM_STAGE_MAXK; ++q)
         if (q < G.nk) {
-            const T *kq = gfield(G.kin[q]);
-#pragma unroll
-            for (int c = 0; c < 3; ++c) kv[q][c] = kq[(size_t)c * n + i];
+            const V3<T> v = ldv(gfield(G.kin[q]));
+            kv[q][0] = v.x;
+            kv[q][1] = v.y;
+            kv[q][2] = v.z;
         }
     const double h = ctl->h;
     T kc[3] = {k.x, k.y, k.z};
@@ -414,10 +527,7 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
         }
     }
     if (G.prep == 1) {
-        T *po = gfield(G.prep_out);
-        po[i] = out[0];
-        po[n + i] = out[1];
-        po[2 * n + i] = out[2];
+        stv(gfield(G.prep_out), v3<T>(out[0], out[1], out[2]));
         nxt[0] = out[0];
         nxt[1] = out[1];
         nxt[2] = out[2];
@@ -425,8 +535,8 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
     }
     // final stage: error norm and normalised new state into the other buffer
     if (G.dydt_from_k0) {
-        const T *k0 = gfield(G.kin[0]);
-        const double a = k0[i], b = k0[n + i], c = k0[2 * n + i];
+        const V3<T> k0 = ldv(gfield(G.kin[0]));
+        const double a = k0.x, b = k0.y, c = k0.z;
         double q = a * a + b * b + c * c;
         red.draw = fmax(red.draw, q);
         const double y0 = ycv[0], y1 = ycv[1], y2 = ycv[2];
@@ -437,7 +547,7 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
         }
         red.dydt = fmax(red.dydt, q);
     }
-    if (G.fsal_out) store3<T>(gfield(G.fsal_out), n, i, k);
+    if (G.fsal_out) stv(gfield(G.fsal_out), k);
     {
         double q = er[0] * er[0] + er[1] * er[1] + er[2] * er[2];
         if (!S.Ms_uniform) {
@@ -451,7 +561,7 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
     if (G.norm_mode == 0) {
         norm = float_cast_norm((double)out[0], (double)out[1], (double)out[2]);
     } else {
-        norm = sqrt((double)out[0] * out[0] + (double)out[1] * out[1] + (double)out[2] * out[2]);
+        norm = fp64_norm((double)out[0], (double)out[1], (double)out[2]);
     }
     if (norm != 0.0) {
         const double sc = (double)ms / norm;
@@ -459,16 +569,15 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
         out[1] = (T)(out[1] * sc);
         out[2] = (T)(out[2] * sc);
     }
-    yn[i] = out[0];
-    yn[n + i] = out[1];
-    yn[2 * n + i] = out[2];
+    stv(yn, v3<T>(out[0], out[1], out[2]));
 }
 
 // x inverse of the demag spectrum rows + epilogue.  rpw rows per block, all
 // three components of a row in the same block.  has_demag: 0 no stray field,
 // 1 inverse x transform of S1 rows in LDS, 2 H_demag read from Hdem (lines
 // too long for three in one workgroup's LDS).  EN: instead of the stage
-// epilogue, per-block sums of M.H per term into epart[q * gridDim.x + block].
+// epilogue, per-block compensated sums of M.H per term: the pair of term q
+// at epart[2q gridDim.x + block] (s) and epart[(2q+1) gridDim.x + block] (e).
 template <typename T, bool EN, int LC, bool STT, bool GEN>
 __global__ void k_stage(const SysDev<T> *__restrict__ Sp, const StageDev<T> *__restrict__ Gp, StepCtl *ctl,
                         const C2<T> *__restrict__ S1, FftPlanDev p, int Hx, int Hp, int rpw, int has_demag,
@@ -527,7 +636,7 @@ __global__ void k_stage(const SysDev<T> *__restrict__ Sp, const StageDev<T> *__r
             [](int, C2<T>) {}, [&] { twf.store(twl); });
     }
     StageRed red = {0.0, 0.0, 0.0, 0.0};
-    EnergyAcc ea = {0.0, 0.0, 0.0, 0.0};
+    EnergyAcc ea = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
     const bool fused = !EN && has_demag == 1 && G.fwd;
     // Zeeman field of this evaluation (alternating_field.py:57-84), at the
     // stage time t + a_s h
@@ -606,18 +715,29 @@ __global__ void k_stage(const SysDev<T> *__restrict__ Sp, const StageDev<T> *__r
     __shared__ double wred[4][16];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (int)((blockDim.x + 63) >> 6);
     if (EN) {
-        const double a = wave_sum(ea.demag), b = wave_sum(ea.exch), c = wave_sum(ea.aniso), d = wave_sum(ea.zee);
+        __shared__ double wsum[8][16];
+        const Dd a = wave_sum_dd(ea.demag), b = wave_sum_dd(ea.exch), c = wave_sum_dd(ea.aniso),
+                 d = wave_sum_dd(ea.zee);
         if (lane == 0) {
-            wred[0][w] = a;
-            wred[1][w] = b;
-            wred[2][w] = c;
-            wred[3][w] = d;
+            wsum[0][w] = a.s;
+            wsum[1][w] = a.e;
+            wsum[2][w] = b.s;
+            wsum[3][w] = b.e;
+            wsum[4][w] = c.s;
+            wsum[5][w] = c.e;
+            wsum[6][w] = d.s;
+            wsum[7][w] = d.e;
         }
         __syncthreads();
         if (threadIdx.x < 4) {
-            double acc = 0.0;
-            for (int q = 0; q < nw; ++q) acc += wred[threadIdx.x][q];
-            epart[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = acc;
+            const int q = threadIdx.x;
+            Dd acc = {wsum[2 * q][0], wsum[2 * q + 1][0]};
+            for (int v = 1; v < nw; ++v) {
+                const Dd o = {wsum[2 * q][v], wsum[2 * q + 1][v]};
+                acc = dd_merge(acc, o);
+            }
+            epart[(size_t)(2 * q) * gridDim.x + blockIdx.x] = acc.s;
+            epart[(size_t)(2 * q + 1) * gridDim.x + blockIdx.x] = acc.e;
         }
         return;
     }
@@ -867,18 +987,20 @@ template <typename T> __global__ void k_cast_out(size_t n, const T *__restrict__
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         b[i] = (double)a[i];
 }
+// per-block compensated sums of the three components of y[cur]: component
+// c's pair at part[2c MM_RED_BLOCKS + b] and part[(2c+1) MM_RED_BLOCKS + b]
 template <typename T> __global__ void k_sum3_sel(size_t n, const T *y0, const T *y1, const StepCtl *ctl, double *part)
 {
     const T *A = ctl->cur ? y1 : y0;
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    Dd s0 = {0.0, 0.0}, s1 = {0.0, 0.0}, s2 = {0.0, 0.0};
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        s0 += (double)A[i];
-        s1 += (double)A[n + i];
-        s2 += (double)A[2 * n + i];
+        dd_add(s0, (double)A[i]);
+        dd_add(s1, (double)A[n + i]);
+        dd_add(s2, (double)A[2 * n + i]);
     }
-    block_sum_store(s0, part);
-    block_sum_store(s1, part + MM_RED_BLOCKS);
-    block_sum_store(s2, part + 2 * MM_RED_BLOCKS);
+    block_sum_dd_store(s0, part, MM_RED_BLOCKS);
+    block_sum_dd_store(s1, part + 2 * MM_RED_BLOCKS, MM_RED_BLOCKS);
+    block_sum_dd_store(s2, part + 4 * MM_RED_BLOCKS, MM_RED_BLOCKS);
 }
 
 // ---------------------------------------------------------------------------
@@ -1014,7 +1136,7 @@ template <typename T> struct Stepper : public StepperBase {
             MM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
         }
         MM_HIP(hipMalloc(&dbuf, sizeof(double) * 3 * n));
-        MM_HIP(hipMalloc(&part, sizeof(double) * 3 * MM_RED_BLOCKS));
+        MM_HIP(hipMalloc(&part, sizeof(double) * 6 * MM_RED_BLOCKS));
         MM_HIP(hipMalloc(&ctl, sizeof(StepCtl)));
         int rc = mm_tableau(d.method, &tab);
         if (rc != MM_OK) throw mm_error(rc, mm_last_error());
@@ -1176,7 +1298,7 @@ template <typename T> struct Stepper : public StepperBase {
             const bool zcol = e ? std::atoi(e) != 0 : true;
             sys.tile_zcol = (zcol && d.nz > 1 && d.ny % rpw == 0) ? d.ny / rpw : 0;
         }
-        MM_HIP(hipMalloc(&epart, sizeof(double) * 4 * grid_rows));
+        MM_HIP(hipMalloc(&epart, sizeof(double) * 8 * grid_rows));
         MM_HIP(hipMalloc(&mpart, sizeof(double) * 4 * grid_rows));
         MM_HIP(hipMemset(mpart, 0, sizeof(double) * 4 * grid_rows));
         {
@@ -1273,7 +1395,10 @@ template <typename T> struct Stepper : public StepperBase {
     // Gd: the stage record in device memory (gtab)
     template <bool EN> void stage_kernel(const StageDev<T> *Gd, double *ep)
     {
-        const bool stt = !EN && sys.has_stt, gen = sys.has_cub || (sys.has_exch && !sys.exch_uniform);
+        // GEN also for state arrays of 4 GiB or more: the other instantiation
+        // addresses them by 32-bit byte offsets
+        const bool big = 3.0 * (double)n * sizeof(T) >= 4294967296.0;
+        const bool stt = !EN && sys.has_stt, gen = sys.has_cub || (sys.has_exch && !sys.exch_uniform) || big;
         mm_with_len(stage_len(), conv && conv->fft_generic, [&](auto lc) {
             constexpr int LC = decltype(lc)::value;
             if (stt) {
@@ -1529,14 +1654,17 @@ template <typename T> struct Stepper : public StepperBase {
         }
         stage_kernel<true>(gtab + tab.num_steps + 1, epart);
         MM_HIP(hipGetLastError());
-        std::vector<double> h((size_t)4 * grid_rows);
+        std::vector<double> h((size_t)8 * grid_rows);
         MM_HIP(hipMemcpyAsync(h.data(), epart, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
         MM_HIP(hipStreamSynchronize(s));
         double sum[4];
         for (int q = 0; q < 4; ++q) {
-            double acc = 0.0;
-            for (int b = 0; b < grid_rows; ++b) acc += h[(size_t)q * grid_rows + b];
-            sum[q] = acc;
+            Dd acc = {0.0, 0.0};
+            for (int b = 0; b < grid_rows; ++b) {
+                const Dd o = {h[(size_t)(2 * q) * grid_rows + b], h[(size_t)(2 * q + 1) * grid_rows + b]};
+                acc = dd_merge(acc, o);
+            }
+            sum[q] = dd_value(acc);
         }
         // the reference's front-end evaluates these with MU0 = 4 pi 1e-7 in
         // Python doubles (constants.py:21)
@@ -1810,13 +1938,16 @@ template <typename T> struct Stepper : public StepperBase {
         const int grid = (int)std::max<size_t>(1, std::min<size_t>(MM_RED_BLOCKS, (n + 255) / 256));
         hipLaunchKernelGGL(k_sum3_sel<T>, dim3(grid), dim3(MM_RED_THREADS), 0, s, n, yb[0], yb[1], ctl, part);
         MM_HIP(hipGetLastError());
-        std::vector<double> h((size_t)3 * MM_RED_BLOCKS);
-        MM_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * 3 * MM_RED_BLOCKS, hipMemcpyDeviceToHost, s));
+        std::vector<double> h((size_t)6 * MM_RED_BLOCKS);
+        MM_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * 6 * MM_RED_BLOCKS, hipMemcpyDeviceToHost, s));
         MM_HIP(hipStreamSynchronize(s));
         for (int c = 0; c < 3; ++c) {
-            double acc = 0.0;
-            for (int b = 0; b < grid; ++b) acc += h[(size_t)c * MM_RED_BLOCKS + b];
-            a3[c] = acc / (double)n;
+            Dd acc = {0.0, 0.0};
+            for (int b = 0; b < grid; ++b) {
+                const Dd o = {h[(size_t)(2 * c) * MM_RED_BLOCKS + b], h[(size_t)(2 * c + 1) * MM_RED_BLOCKS + b]};
+                acc = dd_merge(acc, o);
+            }
+            a3[c] = dd_value(acc) / (double)n;
         }
     }
 };
