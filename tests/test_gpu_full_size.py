@@ -1,6 +1,9 @@
-"""GPU runs at the full sizes of the benchmark workloads, checked through
-size-independent properties (the oracle does not run at these sizes):
+"""GPU runs at the full sizes of the benchmark workloads:
 
+  * the block (256x256x16) and the spin-torque film (1024x1024x2): H_tot
+    of the FP64 path against the FP64 oracle on the same state (<= 1e-12)
+    and of the mixed FP32 path (<= 1e-5); term energies of the FP64 path
+    against the oracle;
   * the demag convolution is linear, and reciprocal: sum M1.H(M2) equals
     sum M2.H(M1) because the cell-cell tensor is symmetric and even;
   * trace rule: with M uniform along each axis in turn, the volume means of
@@ -10,11 +13,12 @@ size-independent properties (the oracle does not run at these sizes):
   * RK steps keep |M| = Ms in every cell (normalisation after each step);
   * the mixed FP32 path tracks the FP64 path (same attempts and a nearby
     state, or, where FP32 rounding changes the step sequence, a nearby state
-    at a common end time).
-
-Workloads: block 256x256x16 cells (RKF45), film 1024x1024x2 with spin torque
-(DP54), grid 512x512x64 (its Newell tensor takes about a minute and a half on
-the host, so that grid runs a point-dipole tensor of the same symmetries).
+    at a common end time);
+  * the 512x512x64 grid runs its Newell tensor built on the GPU (the oracle
+    needs minutes to set up its FFT kernel at this size, so the FP32 path is
+    checked against the FP64 path there), including the relaxation stop;
+  * the 512-system ensemble draw: members equal to single steppers bit for
+    bit, an FP64 member against the oracle.
 """
 import numpy as np
 import pytest
@@ -105,6 +109,44 @@ def test_block_256x256x16_full_size():
                       eps_abs=bench.TOL, eps_rel=bench.TOL, **p)
 
 
+def _fields_vs_oracle(shape, delta, N, M0, tol32, **params):
+    """H_tot and energies of both paths at M0 against the FP64 oracle"""
+    from micromagnum_amd.solver import Stepper
+    import oracle as O
+    from conftest import rel_err
+    so = O.System(shape, delta, N=N, **params)
+    Ho = so.h_tot(M0)
+    Eo = so.energy(M0)
+    for prec, tol in (("fp64", 1e-12), ("fp32", tol32)):
+        st = Stepper(shape, delta, N=N, precision=prec, **params)
+        st.set_m(M0)
+        _, H = st.dmdt()
+        e = rel_err(H, Ho)
+        print("%s %s: H_tot vs oracle %.2e" % (shape, prec, e))
+        assert e < tol
+        if prec == "fp64":
+            E = st.energy()
+            for k, v in Eo.items():
+                if v != 0.0:
+                    assert E[k] == pytest.approx(v, rel=1e-12), k
+        st.close()
+
+
+def test_block_fields_vs_oracle_full_size():
+    from micromagnum_amd import magneto as mg
+    import bench
+    N = mg.GenerateDemagTensor(*bench.SHAPE, *bench.DELTA)
+    _fields_vs_oracle(bench.SHAPE, bench.DELTA, N, bench.initial_m(bench.SHAPE, bench.PARAMS["Ms"]), 1e-5,
+                      **bench.PARAMS)
+
+
+def test_stt_film_fields_vs_oracle_full_size():
+    from micromagnum_amd import magneto as mg
+    import bench
+    N = mg.GenerateDemagTensor(*bench.STT_SHAPE, *bench.STT_DELTA, device="gpu")
+    _fields_vs_oracle(bench.STT_SHAPE, bench.STT_DELTA, N, bench.stt_initial_m(), 1e-5, **bench.STT_PARAMS)
+
+
 def test_film_1024x1024x2_spin_torque_full_size():
     from micromagnum_amd import magneto as mg
     shape, delta = (1024, 1024, 2), (3e-9, 3e-9, 2e-9)
@@ -172,74 +214,76 @@ def test_thin_film_rk4_1000_steps_full_length():
     assert np.abs(avg["fp64"][-1] - avg["fp64"][0]).max() / 8e5 > 1e-3  # the state moved
 
 
-def _dipole_tensor(shape, delta):
-    """Point-dipole cell-cell tensor in the padded layout and sign convention
-    of GenerateDemagTensor (H = N M; round4(2n-1) per open axis, offsets
-    0..n-1 first, -(n-1)..-1 last, zero between; self term -1/3): symmetric
-    and even like the Newell tensor.  Evaluated on the octant of non-negative
-    offsets and mirrored (xy, xz, yz odd in the axes they name)."""
-    from micromagnum_amd import magneto as mg
-    nx, ny, nz = shape
-    ex, ey, ez = (mg.round_tensor_dimension(n, False, mg.PADDING_ROUND_4) for n in shape)
-    x = np.arange(nx) * delta[0]
-    y = np.arange(ny) * delta[1]
-    f0 = delta[0] * delta[1] * delta[2] / (4 * np.pi)
-    O = np.empty((nz, ny, nx, 6))
-    X, Y = x[None, :], y[:, None]
-    for k in range(nz):
-        z = k * delta[2]
-        r2 = X * X + Y * Y + z * z
-        r2[0, 0] = np.inf if k == 0 else r2[0, 0]
-        i3 = f0 / (r2 * np.sqrt(r2))
-        i5 = 3.0 * i3 / r2
-        O[k, :, :, 0] = i5 * X * X - i3
-        O[k, :, :, 1] = i5 * X * Y
-        O[k, :, :, 2] = i5 * X * z
-        O[k, :, :, 3] = i5 * Y * Y - i3
-        O[k, :, :, 4] = i5 * Y * z
-        O[k, :, :, 5] = i5 * z * z - i3
-    O[0, 0, 0] = (-1.0 / 3.0, 0.0, 0.0, -1.0 / 3.0, 0.0, -1.0 / 3.0)
-    N = np.zeros((ez, ey, ex, 6))
-    odd = np.array([[0, 1, 1, 0, 0, 0], [0, 1, 0, 0, 1, 0], [0, 0, 1, 0, 1, 0]])  # odd in x, y, z
-    for sz in (1, -1):
-        for sy in (1, -1):
-            for sx in (1, -1):
-                sign = np.ones(6)
-                src, dst = [], []
-                for a, (s_, n, e) in enumerate(((sz, nz, ez), (sy, ny, ey), (sx, nx, ex))):
-                    if s_ > 0:
-                        src.append(slice(0, n))
-                        dst.append(slice(0, n))
-                    else:
-                        src.append(slice(1, n))
-                        dst.append(slice(e - 1, e - n, -1))
-                        sign = sign * np.where(odd[2 - a] == 1, -1.0, 1.0)
-                if any(d.start == d.stop for d in dst):
-                    continue
-                N[tuple(dst)] = O[tuple(src)] * sign
-    return N
-
-
 def test_grid_512x512x64_full_size():
     from micromagnum_amd import magneto as mg
     from micromagnum_amd.solver import Stepper
-    shape, delta = (512, 512, 64), (2e-9, 2e-9, 2e-9)
-    N = _dipole_tensor(shape, delta)
+    from conftest import rel_err
+    import bench
+    shape, delta = bench.GRID_SHAPE, bench.GRID_DELTA
+    N = mg.GenerateDemagTensor(*shape, *delta, device="gpu")
     conv = mg.SymmetricMatrixVectorConvolution_FFT(N, *shape)
     assert conv.fft_size == (1024, 1024, 128)
-    _check_conv_properties(conv, shape)
+    _check_conv_properties(conv, shape, trace_tol=2e-5)
     del conv
-    # the relaxation workload's physics (cubic anisotropy, alpha 1), a few
-    # steps in FP32: the norm stays, the state moves, nothing overflows
-    Ms = 8e5
-    M0 = _random_m(shape, 3, Ms)
-    st = Stepper(shape, delta, N=N, precision="fp32", method="rkf45", Ms=Ms, alpha=1.0, A=1.3e-11,
-                 k_cub=-5e3, axis1=(1.0, 0.0, 0.0), axis2=(0.0, 1.0, 0.0), eps_abs=1e-4, eps_rel=1e-4)
-    st.set_m(M0)
-    st.run(5)
-    M = st.get_m()
-    s = st.stats()
-    st.close()
-    assert s["attempts"] - s["rejections"] == 5 and np.isfinite(M).all()
-    assert _norm_err(M, Ms) < 1e-6
-    assert np.abs(M - M0).max() > 1e-3 * Ms
+    # the relaxation workload (cubic anisotropy, alpha 1) in both precisions:
+    # H_tot of the initial state, 20 steps with the same attempts, norms kept,
+    # energies of the two paths after them; the relaxation rule evaluated on
+    # the device equals max |dM/dt| / Ms of the state
+    p = dict(bench.GRID_PARAMS)
+    Ms = p.pop("Ms")
+    M0 = bench.initial_m(shape, Ms)
+    out = {}
+    for prec in ("fp64", "fp32"):
+        st = Stepper(shape, delta, N=N, precision=prec, method="rkf45", Ms=Ms, eps_abs=1e-4, eps_rel=1e-4, **p)
+        st.set_m(M0)
+        H0 = st.dmdt()[1]
+        st.run(20)
+        M, s = st.get_m(), st.stats()
+        dM = st.dmdt()[0]
+        dmax = float(np.sqrt((dM ** 2).sum(axis=0)).max() / Ms)
+        r = st.relax(threshold=1e30, criterion="dmdt", check_every=1, max_steps=10)
+        assert r["relaxed"] and r["steps"] == 0
+        assert r["value"] == pytest.approx(dmax, rel=1e-12 if prec == "fp64" else 1e-5)
+        r = st.relax(threshold=1e7, criterion="dmdt", check_every=1, max_steps=3)
+        assert not r["relaxed"] and r["steps"] == 3
+        out[prec] = (H0, M, s, st.energy()["E_tot"])
+        st.close()
+    (H64, M64, s64, E64), (H32, M32, s32, E32) = out["fp64"], out["fp32"]
+    assert rel_err(H32, H64) < 1e-5
+    assert s32["attempts"] == s64["attempts"] and s32["rejections"] == s64["rejections"]
+    assert _norm_err(M64, Ms) < 1e-6 and _norm_err(M32, Ms) < 1e-6
+    assert np.abs(M32 - M64).max() / Ms < 1e-3
+    assert abs(E32 - E64) < 1e-4 * abs(E64)
+    assert np.abs(M64 - M0).max() > 1e-3 * Ms
+
+
+def test_ensemble_512_systems():
+    # the ensemble workload's full draw (FP32), a few steps; sampled members
+    # equal independent steppers bit for bit; one member switched to FP64
+    # against the oracle
+    from micromagnum_amd.solver import Ensemble, Stepper
+    import oracle as O
+    import bench
+    specs, M0 = bench.ensemble_specs()
+    small = int(np.argmin([np.prod(s["shape"]) for s in specs]))
+    specs[small] = dict(specs[small], precision="fp64")
+    ens = Ensemble(specs)
+    for i, M in enumerate(M0):
+        ens.set_m(i, M)
+    ens.run(3)
+    for i in sorted({0, 137, 301, 511, small}):
+        sp = specs[i]
+        kw = {k: v for k, v in sp.items() if k not in ("shape", "delta", "Ms", "alpha", "tensor")}
+        st = Stepper(sp["shape"], sp["delta"], sp["Ms"], sp["alpha"], N=ens.specs[i].N, **kw)
+        st.set_m(M0[i])
+        st.run(3)
+        assert np.array_equal(ens.get_m(i), st.get_m()), i
+        assert ens.stats(i) == st.stats(), i
+        st.close()
+    sp = specs[small]
+    Mo, so = O.System(sp["shape"], sp["delta"], Ms=sp["Ms"], alpha=sp["alpha"], N=ens.specs[small].N, A=sp["A"],
+                      k_uni=sp["k_uni"], axis1=sp["axis1"]).run(M0[small], method="rkf45", steps=3)
+    s = ens.stats(small)
+    assert s["attempts"] == so["attempts"] and s["rejections"] == so["rejections"]
+    assert np.abs(ens.get_m(small) - Mo).max() / sp["Ms"] < 1e-10
+    ens.close()
