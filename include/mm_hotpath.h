@@ -169,6 +169,17 @@ int mm_fdm_zhangli(int nx, int ny, int nz, double dx, double dy, double dz, int 
  * uniform scale -> absMax(M)/scale[0]; else max over scale!=0 of |M|/scale. */
 int mm_scaled_abs_max(int n, const double *M, const double *scale, int scale_uniform,
                       double *out, void *stream);
+/* No counterpart in the reference.  mm_scaled_abs_max without allocating:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * result goes to the device pointer out_dev (one double) on `stream`; the
+ * call neither allocates nor synchronises.  The per-block partial maxima live
+ * in a file-scope device array of 1024 doubles (8 KiB) that each GPU holds
+ * until the process ends; any n up to INT_MAX fits it (at most 1024 blocks
+ * stride over the cells).  Calls on one GPU share that array: calls that
+ * overlap on one GPU race. */
+int mm_scaled_abs_max_noalloc(int n, const double *M, const double *scale, int scale_uniform,
+                              double *out_dev, void *stream);
 /* VectorMatrix::absMax / dotSum / average (VectorMatrix.cpp:187-274) */
 int mm_abs_max(int n, const double *M, double *out, void *stream);
 int mm_dot_sum(int n, const double *A, const double *B, double *out, void *stream);

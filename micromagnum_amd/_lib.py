@@ -88,6 +88,7 @@ SIGNATURES = {
     "mm_llge": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _vp]),
     "mm_fdm_zhangli": (_i, [_i, _i, _i, _d, _d, _d, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "mm_scaled_abs_max": (_i, [_i, _vp, _vp, _i, _dp, _vp]),
+    "mm_scaled_abs_max_noalloc": (_i, [_i, _vp, _vp, _i, _vp, _vp]),
     "mm_abs_max": (_i, [_i, _vp, _dp, _vp]),
     "mm_dot_sum": (_i, [_i, _vp, _vp, _dp, _vp]),
     "mm_average": (_i, [_i, _vp, _dp, _vp]),

@@ -18,6 +18,10 @@ struct mm_error : public std::runtime_error {
 
 void mm_set_error(const std::string &msg);
 
+// looks up, on the current device, the scratch array of the
+// allocation-free entry point (mm_local.hip); called by mm_init
+void mm_noalloc_setup();
+
 #define MM_REQUIRE(cond, msg)                                                     \
     do {                                                                          \
         if (!(cond)) throw mm_error(MM_ERR_INVALID, std::string(__func__) + ": " + (msg)); \

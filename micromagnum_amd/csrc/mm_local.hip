@@ -34,6 +34,42 @@ static void red_scratch()
 static int red_grid(size_t n) { return (int)std::max<size_t>(1, std::min<size_t>(MM_RED_BLOCKS, (n + 255) / 256)); }
 
 // ---------------------------------------------------------------------------
+// scratch of the allocation-free entry point (mm_scaled_abs_max_noalloc):
+// the per-block partial maxima, one per block of at most MM_RED_BLOCKS
+// blocks, in a file-scope device array that every GPU the library runs on
+// holds until the process ends (8 KiB).  Its name is a host-side stand-in in
+// host code: kernels get the address hipGetSymbolAddress returns on the
+// device, looked up once per device (noalloc_scratch) with that device
+// current.  Calls on one GPU share the array: calls that overlap on one GPU
+// race.
+// ---------------------------------------------------------------------------
+#define MM_NOALLOC_PARTIALS MM_RED_BLOCKS
+#define MM_NOALLOC_MAX_DEVICES 64
+static __device__ __align__(256) double scratch[MM_NOALLOC_PARTIALS];
+static std::mutex g_scratch_mtx;
+static double *g_scratch_ptr[MM_NOALLOC_MAX_DEVICES];
+
+// the current device's address of `scratch` (looked up on the first call
+// per device)
+static double *noalloc_scratch()
+{
+    int dev = -1;
+    MM_HIP(hipGetDevice(&dev));
+    MM_REQUIRE(dev >= 0 && dev < MM_NOALLOC_MAX_DEVICES, "device index beyond the scratch address table");
+    std::lock_guard<std::mutex> lk(g_scratch_mtx);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        MM_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        MM_REQUIRE(ptr != nullptr, "no device address for the scratch array");
+        g_scratch_ptr[dev] = (double *)ptr;
+    }
+    return g_scratch_ptr[dev];
+}
+
+// set-up from mm_init with the device current
+void mm_noalloc_setup() { (void)noalloc_scratch(); }
+
+// ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
 template <typename T>
@@ -400,6 +436,44 @@ extern "C" int mm_scaled_abs_max(int n, const double *M, const double *scale, in
         MM_HIP(hipGetLastError());
         *out = std::sqrt(finish_max(s, g_red_partial, grid));
     }
+    MM_CAPI_END
+}
+
+// out[0] = sqrt(max of the partials) (/ scale[0] for a uniform scale): the
+// final step of mm_scaled_abs_max on the device
+static __global__ void k_scaled_abs_max_final(const double *__restrict__ part, int nb,
+                                              const double *__restrict__ scale, int uniform,
+                                              double *__restrict__ out)
+{
+    double s = -1.0e308;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) s = fmax(s, part[i]);
+    __shared__ double sred[MM_RED_THREADS / 64];
+    s = wave_max(s);
+    if ((threadIdx.x & 63) == 0) sred[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = sred[0];
+        for (int q = 1; q < (int)(blockDim.x >> 6); ++q) t = fmax(t, sred[q]);
+        t = sqrt(t);
+        out[0] = uniform ? t / scale[0] : t;
+    }
+}
+
+extern "C" int mm_scaled_abs_max_noalloc(int n, const double *M, const double *scale, int scale_uniform,
+                                         double *out_dev, void *stream)
+{
+    MM_CAPI_BEGIN
+    MM_REQUIRE(n > 0 && M && scale && out_dev, "bad argument");
+    const int grid = red_grid(n);
+    MM_REQUIRE(grid <= MM_NOALLOC_PARTIALS, "more blocks than the scratch array holds partial maxima");
+    hipStream_t s = (hipStream_t)stream;
+    double *part = noalloc_scratch();
+    hipLaunchKernelGGL(k_maxsq<double>, dim3(grid), dim3(MM_RED_THREADS), 0, s, (size_t)n, M,
+                       scale_uniform ? (const double *)nullptr : scale, part);
+    MM_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_scaled_abs_max_final, dim3(1), dim3(MM_RED_THREADS), 0, s, (const double *)part, grid,
+                       scale, scale_uniform, out_dev);
+    MM_HIP(hipGetLastError());
     MM_CAPI_END
 }
 

@@ -30,6 +30,7 @@ extern "C" int mm_init(int device)
     MM_HIP(hipSetDevice(device));
     MM_HIP(hipFree(nullptr));
     (void)mm_max_lds_bytes();
+    mm_noalloc_setup();
     MM_CAPI_END
 }
 
