@@ -122,6 +122,56 @@ struct KernelLog {
     }
 };
 
+// ---- batched passes: one launch runs one pass for many systems ----------
+// Each system of a batch has a descriptor per pass; the launch's global
+// workgroup g (in XCD order, xcd_tile) belongs to system tsys[g] and is that
+// system's workgroup g - tbase.  A system whose stop flag is set skips the
+// pass.  Lengths differ between systems, so the transforms take the runtime
+// FFT plans.
+template <typename T> struct BxDesc {  // x forward of the current state y[*sel]
+    const T *y0, *y1;
+    const int *sel, *stop;
+    C2<T> *S1;
+    size_t in_cstride, out_cstride;
+    int Hx, Hp, ny, nz, nrows, nx, lpw, tbase;
+    FftPlanDev p;
+};
+template <typename T> struct ByDesc {  // y forward S1 -> S2 and inverse S2 -> S1
+    C2<T> *S1, *S2;
+    const int *stop;
+    int Hx, Hp, ny, nz, tk, ntx, tbase;
+    unsigned mtx, mzl;
+    FftPlanDev p;
+};
+template <typename T> struct BzDesc {  // z forward, multiply, z inverse on S2
+    C2<T> *S2;
+    const T *K;
+    size_t plane;
+    const int *stop;
+    int Hx, Hp, Hy2, Py, nz, tk, ntx, tbase;
+    unsigned mtx, zsb, pb;
+    FftPlanDev p;
+};
+template <typename T> struct B2Desc {  // 2-D: y forward, multiply, y inverse on S1
+    C2<T> *S1;
+    const T *K;
+    size_t plane;
+    const int *stop;
+    int Hx, Hp, Hy2, ny, tk, tbase;
+    FftPlanDev p;
+};
+
+// One pass of a batch: descriptors and tile table in device memory, the
+// launch's workgroups, threads per workgroup and dynamic LDS.  o32: every
+// system's spectra fit the 32-bit offset forms.
+struct BatchPass {
+    void *desc = nullptr;
+    int *tsys = nullptr;
+    int ntiles = 0, threads = 0;
+    size_t lds = 0;
+    bool o32 = true;
+};
+
 template <typename T>
 struct ConvImpl {
     KernelLog *klog = nullptr;
@@ -152,7 +202,27 @@ struct ConvImpl {
         spectral(s);
         inverse_x(H, s);
     }
+    // this system's part of a batched pass launched with `threads` threads
+    // per workgroup: its descriptor (tbase left to the caller), workgroup
+    // count, the dynamic LDS it needs and whether it fits the 32-bit offset
+    // forms.  Tile widths follow the batch's thread count.
+    int batch_x(BxDesc<T> &d, int threads, size_t &lds, const T *y0, const T *y1, const int *sel,
+                const int *stop) const;
+    int batch_y(ByDesc<T> &d, int threads, size_t &lds, bool &o32, const int *stop) const;
+    int batch_z(BzDesc<T> &d, int threads, size_t &lds, bool &o32, const int *stop) const;
+    int batch_2d(B2Desc<T> &d, int threads, size_t &lds, const int *stop) const;
+    // threads per workgroup this system's passes need at least
+    int batch_threads_x() const { return px.tline; }
+    int batch_threads_y() const { return (g.is2d ? 3 : 1) * py.tline; }
+    int batch_threads_z() const { return 3 * pz.tline; }
 };
+
+// batched launches (all systems of bp in one launch on stream s)
+template <typename T> void mm_batch_xfwd(const BatchPass &bp, hipStream_t s);
+template <typename T> void mm_batch_yfwd(const BatchPass &bp, hipStream_t s);
+template <typename T> void mm_batch_yinv(const BatchPass &bp, hipStream_t s);
+template <typename T> void mm_batch_zmi(const BatchPass &bp, hipStream_t s);
+template <typename T> void mm_batch_ymi2d(const BatchPass &bp, hipStream_t s);
 
 // device-side FFT kernels reused by the fused stepper
 template <typename T> size_t conv_lds_xinv(const ConvImpl<T> &c, int lines) { return (size_t)lines * (c.px.L + 1) * sizeof(C2<T>); }

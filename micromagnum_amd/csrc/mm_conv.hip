@@ -24,6 +24,7 @@
 // working precision.  The reference keeps the full complex kernel; the
 // dropped imaginary parts are rounding noise (~1e-17 relative).
 #include "mm_conv.h"
+#include "mm_cell.h"
 
 #include <algorithm>
 #include <cmath>
@@ -159,23 +160,22 @@ template <typename K> static void set_lds(K, size_t bytes)
 // kernels
 // ---------------------------------------------------------------------------
 
-// x forward: rows (c, z, y); input rows of nx_in reals with row stride nx_in
-// sel (may be null): device index choosing in (0) or in_alt (1)
+// x forward: rows (c, z, y); input rows of nx_in reals with row stride nx_in;
+// workgroup blk takes lines blk lpw .. (blk + 1) lpw - 1 (threads beyond the
+// lpw lines stay idle)
 template <typename T, typename TIN, int LC = 0>
-__global__ void k_xfwd(const TIN *__restrict__ in, size_t in_cstride, C2<T> *__restrict__ out,
-                       size_t out_cstride, int Hx, int Hp, int ny, int nz, int nrows, int nx_in,
-                       FftPlanDev p, int lpw, const TIN *__restrict__ in_alt = nullptr,
-                       const int *__restrict__ sel = nullptr)
+__device__ __forceinline__ void xfwd_body(const TIN *__restrict__ in, size_t in_cstride, C2<T> *__restrict__ out,
+                                          size_t out_cstride, int Hx, int Hp, int ny, int nz, int nrows, int nx_in,
+                                          const FftPlanDev &p, int lpw, int blk)
 {
     constexpr int SK = MM_SKEW_X;
-    if (sel && *sel) in = in_alt;
     extern __shared__ char smem_raw[];
     C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
     const int L = PlanC<LC>::L(p), LP = lds_len<SK>(L), TL = PlanC<LC>::tline(p);
     // two real rows per complex line: line l of the block holds rows
     // 2 g and 2 g + 1 (g = blockIdx.x lpw + l) as real and imaginary part
     const int l = threadIdx.x / TL, t = threadIdx.x % TL;
-    const int g0 = 2 * (blockIdx.x * lpw + l);
+    const int g0 = 2 * (blk * lpw + l);
     const bool active = l < lpw && g0 < nrows;
     const bool has1 = g0 + 1 < nrows;
     C2<T> *line = sm + (size_t)l * LP;
@@ -195,7 +195,7 @@ __global__ void k_xfwd(const TIN *__restrict__ in, size_t in_cstride, C2<T> *__r
         [](int, C2<T>) {}, [&] { twf.store(twl); });
     // X0[k] = (Z[k] + conj Z[L-k]) / 2, X1[k] = (Z[k] - conj Z[L-k]) / 2i
     for (int l2 = 0; l2 < lpw; ++l2) {
-        const int h0 = 2 * (blockIdx.x * lpw + l2);
+        const int h0 = 2 * (blk * lpw + l2);
         if (h0 >= nrows) break;
         const C2<T> *ln = sm + (size_t)l2 * LP;
         C2<T> *d0 = out + (h0 / rows_c) * out_cstride + (size_t)(h0 % rows_c) * Hp;
@@ -206,6 +206,17 @@ __global__ void k_xfwd(const TIN *__restrict__ in, size_t in_cstride, C2<T> *__r
             if (d1) d1[k] = cmk<T>((zk.y + zm.y) * (T)0.5, (zm.x - zk.x) * (T)0.5);
         }
     }
+}
+
+// sel (may be null): device index choosing in (0) or in_alt (1)
+template <typename T, typename TIN, int LC = 0>
+__global__ void k_xfwd(const TIN *__restrict__ in, size_t in_cstride, C2<T> *__restrict__ out,
+                       size_t out_cstride, int Hx, int Hp, int ny, int nz, int nrows, int nx_in,
+                       FftPlanDev p, int lpw, const TIN *__restrict__ in_alt = nullptr,
+                       const int *__restrict__ sel = nullptr)
+{
+    if (sel && *sel) in = in_alt;
+    xfwd_body<T, TIN, LC>(in, in_cstride, out, out_cstride, Hx, Hp, ny, nz, nrows, nx_in, p, lpw, (int)blockIdx.x);
 }
 
 // n / d for the tile indices of a launch without an integer division (the
@@ -221,16 +232,18 @@ __device__ __forceinline__ unsigned udiv(unsigned n, unsigned d, unsigned m)
 // y forward: columns kx of planes (c, z); rows y < ny_in read, Py written.
 // O32: both arrays under 4 GiB and a row under 16 MiB: 32-bit byte offsets
 // from the kernel-argument bases, rows stepped by 24-bit multiplies
+// tile: (kx tile, z, c) with the kx tile fastest, ntx kx tiles per plane
 template <typename T, int LC = 0, bool O32 = false>
-__global__ void k_yfwd(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, int Hx, int Hp, int ny_in,
-                       int nzl, FftPlanDev p, int tk, int remap, unsigned mtx, unsigned mzl)
+__device__ __forceinline__ void yfwd_body(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, int Hx, int Hp,
+                                          int ny_in, int nzl, const FftPlanDev &p, int tk, int tile, int ntx,
+                                          unsigned mtx, unsigned mzl)
 {
     constexpr int SK = MM_SKEW_Y;
     extern __shared__ char smem_raw[];
     C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
     const int L = PlanC<LC>::L(p), LP = lds_len<SK>(L);
-    // tiles (kx tile, z, c), kx tile fastest; tk is a power of two
-    const int tile = block_tile(remap), ntx = (int)gridDim.x, ltk = __ffs(tk) - 1;
+    // tk is a power of two
+    const int ltk = __ffs(tk) - 1;
     const int q1 = (int)udiv((unsigned)tile, (unsigned)ntx, mtx), c = (int)udiv((unsigned)q1, (unsigned)nzl, mzl);
     const int kx0 = (tile - q1 * ntx) * tk;
     const int z = q1 - c * nzl;
@@ -241,7 +254,8 @@ __global__ void k_yfwd(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, in
     // global passes: columns fastest across the threads (row segments of tk
     // values per load); LDS passes: one line per group of tline threads
     const int lg = threadIdx.x & (tk - 1), ls = threadIdx.x / PlanC<LC>::tline(p);
-    const LineMap mg = {(int)(threadIdx.x >> ltk), true}, ms = {(int)(threadIdx.x % PlanC<LC>::tline(p)), ls < tk};
+    const int tg = (int)(threadIdx.x >> ltk);
+    const LineMap mg = {tg, tg < PlanC<LC>::tline(p)}, ms = {(int)(threadIdx.x % PlanC<LC>::tline(p)), ls < tk};
     const int kx = kx0 + lg;
     constexpr unsigned CS = sizeof(C2<T>);
     const unsigned rs = (unsigned)Hp * CS, pl = ((unsigned)c * nzl + z);
@@ -261,17 +275,25 @@ __global__ void k_yfwd(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, in
         }, [&] { twf.store(twl); });
 }
 
+template <typename T, int LC = 0, bool O32 = false>
+__global__ void k_yfwd(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, int Hx, int Hp, int ny_in,
+                       int nzl, FftPlanDev p, int tk, int remap, unsigned mtx, unsigned mzl)
+{
+    yfwd_body<T, LC, O32>(in, out, Hx, Hp, ny_in, nzl, p, tk, block_tile(remap), (int)gridDim.x, mtx, mzl);
+}
+
 // y inverse: columns of S2 planes (c, z), all Py rows read, y < ny_out
 // written; O32 as k_yfwd
 template <typename T, int LC = 0, bool O32 = false>
-__global__ void k_yinv(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, int Hx, int Hp, int ny_out,
-                       int nzl, FftPlanDev p, int tk, int remap, unsigned mtx, unsigned mzl)
+__device__ __forceinline__ void yinv_body(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, int Hx, int Hp,
+                                          int ny_out, int nzl, const FftPlanDev &p, int tk, int tile, int ntx,
+                                          unsigned mtx, unsigned mzl)
 {
     constexpr int SK = MM_SKEW_Y;
     extern __shared__ char smem_raw[];
     C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
     const int L = PlanC<LC>::L(p), LP = lds_len<SK>(L);
-    const int tile = block_tile(remap), ntx = (int)gridDim.x, ltk = __ffs(tk) - 1;
+    const int ltk = __ffs(tk) - 1;
     const int q1 = (int)udiv((unsigned)tile, (unsigned)ntx, mtx), c = (int)udiv((unsigned)q1, (unsigned)nzl, mzl);
     const int kx0 = (tile - q1 * ntx) * tk;
     const int z = q1 - c * nzl;
@@ -280,7 +302,8 @@ __global__ void k_yinv(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, in
     C2<T> *twl = sm + (size_t)tk * LP;
     const TwFetch<T> twf = TwFetch<T>::fetch(p);
     const int lg = threadIdx.x & (tk - 1), ls = threadIdx.x / PlanC<LC>::tline(p);
-    const LineMap mg = {(int)(threadIdx.x >> ltk), true}, ms = {(int)(threadIdx.x % PlanC<LC>::tline(p)), ls < tk};
+    const int tg = (int)(threadIdx.x >> ltk);
+    const LineMap mg = {tg, tg < PlanC<LC>::tline(p)}, ms = {(int)(threadIdx.x % PlanC<LC>::tline(p)), ls < tk};
     const int kx = kx0 + lg;
     constexpr unsigned CS = sizeof(C2<T>);
     const unsigned rs = (unsigned)Hp * CS, pl = ((unsigned)c * nzl + z);
@@ -298,6 +321,13 @@ __global__ void k_yinv(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, in
                 else dst[(size_t)y * Hp + kx] = v;
             }
         }, [&] { twf.store(twl); });
+}
+
+template <typename T, int LC = 0, bool O32 = false>
+__global__ void k_yinv(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, int Hx, int Hp, int ny_out,
+                       int nzl, FftPlanDev p, int tk, int remap, unsigned mtx, unsigned mzl)
+{
+    yinv_body<T, LC, O32>(in, out, Hx, Hp, ny_out, nzl, p, tk, block_tile(remap), (int)gridDim.x, mtx, mzl);
 }
 
 // Pruned y transforms for zero-padded lines (P = 2h even, data only in
@@ -438,8 +468,9 @@ __device__ __forceinline__ void mult3(C2<T> &mx, C2<T> &my, C2<T> &mz, const T k
 // (global_load with an SGPR base: no 64-bit address arithmetic per element;
 // the offsets' z steps are 24-bit multiplies); pb: bytes of one K plane.
 template <typename T, int LC = 0, bool O32 = false>
-__global__ void k_zmi(C2<T> *__restrict__ S2, const T *__restrict__ K, size_t plane, int Hx, int Hp, int Hy2,
-                      int Py, int nz, FftPlanDev p, int tk, int remap, unsigned mtx, unsigned zsb, unsigned pb)
+__device__ __forceinline__ void zmi_body(C2<T> *__restrict__ S2, const T *__restrict__ K, size_t plane, int Hx,
+                                         int Hp, int Hy2, int Py, int nz, const FftPlanDev &p, int tk, int remap,
+                                         int tile, int ntx, unsigned mtx, unsigned zsb, unsigned pb)
 {
     constexpr int SK = MM_SKEW_Z;
     extern __shared__ char smem_raw[];
@@ -448,7 +479,7 @@ __global__ void k_zmi(C2<T> *__restrict__ S2, const T *__restrict__ K, size_t pl
     // tiles (kx tile, m), kx tile fastest.  With remap, rows come in the order
     // 0, 1, Py-1, 2, Py-2, ...: ky and Py-ky read the same kernel row
     // tk is a power of two
-    const int tile = block_tile(remap), ntx = (int)gridDim.x, ltk = __ffs(tk) - 1;
+    const int ltk = __ffs(tk) - 1;
     const int m = (int)udiv((unsigned)tile, (unsigned)ntx, mtx), hm = (m + 1) >> 1;
     const int kx0 = (tile - m * ntx) * tk;
     const int ky = !remap || m == 0 ? m : ((m & 1) ? hm : Py - hm);
@@ -460,7 +491,8 @@ __global__ void k_zmi(C2<T> *__restrict__ S2, const T *__restrict__ K, size_t pl
     // global passes: columns fastest, then components; LDS passes: one line
     // (component c, column) per group of tline threads
     const int colg = threadIdx.x & (tk - 1), cg = (int)(threadIdx.x >> ltk) % 3;
-    const LineMap mg = {(int)(threadIdx.x >> ltk) / 3, true};
+    const int tg = (int)(threadIdx.x >> ltk) / 3;
+    const LineMap mg = {tg, tg < PlanC<LC>::tline(p)};
     C2<T> *lineG = sm + (size_t)(cg * tk + colg) * LP;
     const int l = threadIdx.x / PlanC<LC>::tline(p), t = threadIdx.x % PlanC<LC>::tline(p);
     const LineMap ms = {t, l < 3 * tk};
@@ -527,22 +559,31 @@ __global__ void k_zmi(C2<T> *__restrict__ S2, const T *__restrict__ K, size_t pl
         });
 }
 
+template <typename T, int LC = 0, bool O32 = false>
+__global__ void k_zmi(C2<T> *__restrict__ S2, const T *__restrict__ K, size_t plane, int Hx, int Hp, int Hy2,
+                      int Py, int nz, FftPlanDev p, int tk, int remap, unsigned mtx, unsigned zsb, unsigned pb)
+{
+    zmi_body<T, LC, O32>(S2, K, plane, Hx, Hp, Hy2, Py, nz, p, tk, remap, block_tile(remap), (int)gridDim.x, mtx,
+                         zsb, pb);
+}
+
 // 2-D: y forward (ny rows, zero-extended) + multiply (kz = 0) + y inverse,
 // keep y < ny, in place on S1 planes [c][y][kx].
 template <typename T, int LC = 0>
-__global__ void k_ymi2d(C2<T> *__restrict__ S1, const T *__restrict__ K, size_t plane, int Hx, int Hp, int Hy2,
-                        int ny, FftPlanDev p, int tk)
+__device__ __forceinline__ void ymi2d_body(C2<T> *__restrict__ S1, const T *__restrict__ K, size_t plane, int Hx,
+                                           int Hp, int Hy2, int ny, const FftPlanDev &p, int tk, int blk)
 {
     constexpr int SK = MM_SKEW_Y;
     extern __shared__ char smem_raw[];
     C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
     const int L = PlanC<LC>::L(p), LP = lds_len<SK>(L);
-    const int kx0 = blockIdx.x * tk;
+    const int kx0 = blk * tk;
     const size_t cstride = (size_t)ny * Hp;
     C2<T> *twl = sm + (size_t)3 * tk * LP;
     const TwFetch<T> twf = TwFetch<T>::fetch(p);
     const int colg = threadIdx.x % tk, cg = (threadIdx.x / tk) % 3;
-    const LineMap mg = {(int)(threadIdx.x / (3 * tk)), true};
+    const int tg = (int)(threadIdx.x / (3 * tk));
+    const LineMap mg = {tg, tg < PlanC<LC>::tline(p)};
     C2<T> *lineG = sm + (size_t)(cg * tk + colg) * LP;
     const int l = threadIdx.x / PlanC<LC>::tline(p), t = threadIdx.x % PlanC<LC>::tline(p);
     const LineMap ms = {t, l < 3 * tk};
@@ -567,6 +608,13 @@ __global__ void k_ymi2d(C2<T> *__restrict__ S1, const T *__restrict__ K, size_t 
         [&](int y, C2<T> v) {
             if (y < ny && kxg < Hx) colp[(size_t)y * Hp] = v;
         });
+}
+
+template <typename T, int LC = 0>
+__global__ void k_ymi2d(C2<T> *__restrict__ S1, const T *__restrict__ K, size_t plane, int Hx, int Hp, int Hy2,
+                        int ny, FftPlanDev p, int tk)
+{
+    ymi2d_body<T, LC>(S1, K, plane, Hx, Hp, Hy2, ny, p, tk, (int)blockIdx.x);
 }
 
 // x inverse (C2R): rows (c, z, y) of S1, Hermitian extension, keep x < nx
@@ -1092,8 +1140,217 @@ template <typename T> void ConvImpl<T>::inverse_x(T *H, hipStream_t s)
     MM_HIP(hipGetLastError());
 }
 
+// ---------------------------------------------------------------------------
+// batched passes
+// ---------------------------------------------------------------------------
+template <typename T>
+int ConvImpl<T>::batch_x(BxDesc<T> &d, int threads, size_t &lds, const T *y0, const T *y1, const int *sel,
+                         const int *stop) const
+{
+    const size_t cs = sizeof(C2<T>);
+    MM_REQUIRE(threads >= px.tline, "batch launch has fewer threads than one x line needs");
+    int lpw = std::max(1, threads / px.tline);
+    while (lpw > 1 && lines_lds<MM_SKEW_X>(lpw, px.L, cs) > (size_t)mm_max_lds_bytes()) --lpw;
+    const int nrows = 3 * g.ny * g.nz;
+    d.y0 = y0;
+    d.y1 = y1;
+    d.sel = sel;
+    d.stop = stop;
+    d.S1 = S1;
+    d.in_cstride = (size_t)g.nx * g.ny * g.nz;
+    d.out_cstride = (size_t)g.ny * g.nz * g.Hp;
+    d.Hx = g.Hx;
+    d.Hp = g.Hp;
+    d.ny = g.ny;
+    d.nz = g.nz;
+    d.nrows = nrows;
+    d.nx = g.nx;
+    d.lpw = lpw;
+    d.tbase = 0;
+    d.p = px.template dev<T>();
+    lds = lines_lds<MM_SKEW_X>(lpw, px.L, cs);
+    return ((nrows + 1) / 2 + lpw - 1) / lpw;
+}
+
+template <typename T>
+int ConvImpl<T>::batch_y(ByDesc<T> &d, int threads, size_t &lds, bool &o32, const int *stop) const
+{
+    const size_t cs = sizeof(C2<T>);
+    MM_REQUIRE(!g.is2d && threads >= py.tline, "batch y pass: 2-D system or too few threads");
+    int tk = std::min(16, floor_pow2(std::max(1, threads / py.tline)));
+    while (tk > 1 && lines_lds<MM_SKEW_Y>(tk, py.L, cs) > (size_t)mm_max_lds_bytes()) tk /= 2;
+    const unsigned ntx = (unsigned)((g.Hx + tk - 1) / tk), nb = ntx * g.nz * 3;
+    d.S1 = S1;
+    d.S2 = S2;
+    d.stop = stop;
+    d.Hx = g.Hx;
+    d.Hp = g.Hp;
+    d.ny = g.ny;
+    d.nz = g.nz;
+    d.tk = tk;
+    d.ntx = (int)ntx;
+    d.tbase = 0;
+    d.mtx = mdiv(ntx, nb);
+    d.mzl = mdiv((unsigned)g.nz, nb);
+    d.p = py.template dev<T>();
+    o32 = (size_t)g.Hp * cs < (1u << 24) && (size_t)3 * g.nz * g.Py * g.Hp * cs < (1ull << 32);
+    lds = lines_lds<MM_SKEW_Y>(tk, py.L, cs);
+    return (int)nb;
+}
+
+template <typename T>
+int ConvImpl<T>::batch_z(BzDesc<T> &d, int threads, size_t &lds, bool &o32, const int *stop) const
+{
+    const size_t cs = sizeof(C2<T>);
+    MM_REQUIRE(!g.is2d && threads >= 3 * pz.tline, "batch z pass: 2-D system or too few threads");
+    int tz = std::min(16, floor_pow2(std::max(1, threads / (3 * pz.tline))));
+    while (tz > 1 && lines_lds<MM_SKEW_Z>(3 * tz, pz.L, cs) > (size_t)mm_max_lds_bytes()) tz /= 2;
+    MM_REQUIRE((size_t)tz * (pz.L / 2 + 1) <= (size_t)ZM_ITEMS * threads, "z pass items per thread");
+    const unsigned ntz = (unsigned)((g.Hx + tz - 1) / tz), nb = ntz * (unsigned)g.Py;
+    const size_t plane = (size_t)g.Hp * g.Hy2 * g.Hz2;
+    const size_t zsb = (size_t)g.Py * g.Hp * cs;
+    d.S2 = S2;
+    d.K = kern;
+    d.plane = plane;
+    d.stop = stop;
+    d.Hx = g.Hx;
+    d.Hp = g.Hp;
+    d.Hy2 = g.Hy2;
+    d.Py = g.Py;
+    d.nz = g.nz;
+    d.tk = tz;
+    d.ntx = (int)ntz;
+    d.tbase = 0;
+    d.mtx = mdiv(ntz, nb);
+    d.zsb = (unsigned)zsb;
+    d.pb = (unsigned)(plane * sizeof(T));
+    d.p = pz.template dev<T>();
+    o32 = zsb < (1u << 24) && 3 * (size_t)g.nz * zsb < (1ull << 32) && kern_elems * sizeof(T) < (1ull << 32);
+    lds = lines_lds<MM_SKEW_Z>(3 * tz, pz.L, cs);
+    return (int)nb;
+}
+
+template <typename T> int ConvImpl<T>::batch_2d(B2Desc<T> &d, int threads, size_t &lds, const int *stop) const
+{
+    const size_t cs = sizeof(C2<T>);
+    MM_REQUIRE(g.is2d && threads >= 3 * py.tline, "batch 2-D pass: 3-D system or too few threads");
+    int tk = std::min(16, floor_pow2(std::max(1, threads / (3 * py.tline))));
+    while (tk > 1 && lines_lds<MM_SKEW_Y>(3 * tk, py.L, cs) > (size_t)mm_max_lds_bytes()) tk /= 2;
+    d.S1 = S1;
+    d.K = kern;
+    d.plane = (size_t)g.Hp * g.Hy2 * g.Hz2;
+    d.stop = stop;
+    d.Hx = g.Hx;
+    d.Hp = g.Hp;
+    d.Hy2 = g.Hy2;
+    d.ny = g.ny;
+    d.tk = tk;
+    d.tbase = 0;
+    d.p = py.template dev<T>();
+    lds = lines_lds<MM_SKEW_Y>(3 * tk, py.L, cs);
+    return (g.Hx + tk - 1) / tk;
+}
+
 template struct ConvImpl<float>;
 template struct ConvImpl<double>;
+
+// The batched kernels: the global workgroup index in XCD order picks the
+// system (tsys) and that system's workgroup; descriptor fields are read
+// through global-address-space pointer types (gfield) so that the accesses
+// through them are global, not flat, operations.
+template <typename T> __global__ void kb_xfwd(const BxDesc<T> *__restrict__ D, const int *__restrict__ tsys)
+{
+    const int gi = block_tile(1);
+    const BxDesc<T> &d = D[tsys[gi]];
+    if (gfield(d.stop)[0]) return;
+    const T *in = gfield(d.sel)[0] ? gfield(d.y1) : gfield(d.y0);
+    xfwd_body<T, T, 0>(in, d.in_cstride, gfield(d.S1), d.out_cstride, d.Hx, d.Hp, d.ny, d.nz, d.nrows, d.nx, d.p,
+                       d.lpw, gi - d.tbase);
+}
+
+template <typename T, bool INV, bool O32>
+__global__ void kb_y(const ByDesc<T> *__restrict__ D, const int *__restrict__ tsys)
+{
+    const int gi = block_tile(1);
+    const ByDesc<T> &d = D[tsys[gi]];
+    if (gfield(d.stop)[0]) return;
+    if (INV)
+        yinv_body<T, 0, O32>(gfield(d.S2), gfield(d.S1), d.Hx, d.Hp, d.ny, d.nz, d.p, d.tk, gi - d.tbase, d.ntx,
+                             d.mtx, d.mzl);
+    else
+        yfwd_body<T, 0, O32>(gfield(d.S1), gfield(d.S2), d.Hx, d.Hp, d.ny, d.nz, d.p, d.tk, gi - d.tbase, d.ntx,
+                             d.mtx, d.mzl);
+}
+
+template <typename T, bool O32> __global__ void kb_zmi(const BzDesc<T> *__restrict__ D, const int *__restrict__ tsys)
+{
+    const int gi = block_tile(1);
+    const BzDesc<T> &d = D[tsys[gi]];
+    if (gfield(d.stop)[0]) return;
+    zmi_body<T, 0, O32>(gfield(d.S2), gfield(d.K), d.plane, d.Hx, d.Hp, d.Hy2, d.Py, d.nz, d.p, d.tk, 1, gi - d.tbase,
+                        d.ntx, d.mtx, d.zsb, d.pb);
+}
+
+template <typename T> __global__ void kb_ymi2d(const B2Desc<T> *__restrict__ D, const int *__restrict__ tsys)
+{
+    const int gi = block_tile(1);
+    const B2Desc<T> &d = D[tsys[gi]];
+    if (gfield(d.stop)[0]) return;
+    ymi2d_body<T, 0>(gfield(d.S1), gfield(d.K), d.plane, d.Hx, d.Hp, d.Hy2, d.ny, d.p, d.tk, gi - d.tbase);
+}
+
+template <typename T> void mm_batch_xfwd(const BatchPass &bp, hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_xfwd<T>, dim3(bp.ntiles), dim3(bp.threads), bp.lds, s, (const BxDesc<T> *)bp.desc,
+                       (const int *)bp.tsys);
+    MM_HIP(hipGetLastError());
+}
+template <typename T> void mm_batch_yfwd(const BatchPass &bp, hipStream_t s)
+{
+    if (bp.o32)
+        hipLaunchKernelGGL((kb_y<T, false, true>), dim3(bp.ntiles), dim3(bp.threads), bp.lds, s,
+                           (const ByDesc<T> *)bp.desc, (const int *)bp.tsys);
+    else
+        hipLaunchKernelGGL((kb_y<T, false, false>), dim3(bp.ntiles), dim3(bp.threads), bp.lds, s,
+                           (const ByDesc<T> *)bp.desc, (const int *)bp.tsys);
+    MM_HIP(hipGetLastError());
+}
+template <typename T> void mm_batch_yinv(const BatchPass &bp, hipStream_t s)
+{
+    if (bp.o32)
+        hipLaunchKernelGGL((kb_y<T, true, true>), dim3(bp.ntiles), dim3(bp.threads), bp.lds, s,
+                           (const ByDesc<T> *)bp.desc, (const int *)bp.tsys);
+    else
+        hipLaunchKernelGGL((kb_y<T, true, false>), dim3(bp.ntiles), dim3(bp.threads), bp.lds, s,
+                           (const ByDesc<T> *)bp.desc, (const int *)bp.tsys);
+    MM_HIP(hipGetLastError());
+}
+template <typename T> void mm_batch_zmi(const BatchPass &bp, hipStream_t s)
+{
+    if (bp.o32)
+        hipLaunchKernelGGL((kb_zmi<T, true>), dim3(bp.ntiles), dim3(bp.threads), bp.lds, s,
+                           (const BzDesc<T> *)bp.desc, (const int *)bp.tsys);
+    else
+        hipLaunchKernelGGL((kb_zmi<T, false>), dim3(bp.ntiles), dim3(bp.threads), bp.lds, s,
+                           (const BzDesc<T> *)bp.desc, (const int *)bp.tsys);
+    MM_HIP(hipGetLastError());
+}
+template <typename T> void mm_batch_ymi2d(const BatchPass &bp, hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_ymi2d<T>, dim3(bp.ntiles), dim3(bp.threads), bp.lds, s, (const B2Desc<T> *)bp.desc,
+                       (const int *)bp.tsys);
+    MM_HIP(hipGetLastError());
+}
+template void mm_batch_xfwd<float>(const BatchPass &, hipStream_t);
+template void mm_batch_xfwd<double>(const BatchPass &, hipStream_t);
+template void mm_batch_yfwd<float>(const BatchPass &, hipStream_t);
+template void mm_batch_yfwd<double>(const BatchPass &, hipStream_t);
+template void mm_batch_yinv<float>(const BatchPass &, hipStream_t);
+template void mm_batch_yinv<double>(const BatchPass &, hipStream_t);
+template void mm_batch_zmi<float>(const BatchPass &, hipStream_t);
+template void mm_batch_zmi<double>(const BatchPass &, hipStream_t);
+template void mm_batch_ymi2d<float>(const BatchPass &, hipStream_t);
+template void mm_batch_ymi2d<double>(const BatchPass &, hipStream_t);
 
 // ---------------------------------------------------------------------------
 // C ABI

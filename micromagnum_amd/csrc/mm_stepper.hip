@@ -28,7 +28,9 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <cstddef>
 #include <vector>
 
 #include "mm_cell.h"
@@ -577,11 +579,15 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
 // 1 inverse x transform of S1 rows in LDS, 2 H_demag read from Hdem (lines
 // too long for three in one workgroup's LDS).  EN: instead of the stage
 // epilogue, per-block compensated sums of M.H per term: the pair of term q
-// at epart[2q gridDim.x + block] (s) and epart[(2q+1) gridDim.x + block] (e).
+// at epart[2q nblk + blk] (s) and epart[(2q+1) nblk + blk] (e).  blk: this
+// workgroup's index among the nblk of the system's launch; remap: order the
+// tiles by xcd_tile(blk, nblk) (the single-system launch; a batched launch
+// orders its global workgroups before it picks the system).
 template <typename T, bool EN, int LC, bool STT, bool GEN>
-__global__ void k_stage(const SysDev<T> *__restrict__ Sp, const StageDev<T> *__restrict__ Gp, StepCtl *ctl,
-                        const C2<T> *__restrict__ S1, FftPlanDev p, int Hx, int Hp, int rpw, int has_demag,
-                        double *__restrict__ epart, const T *__restrict__ Hdem)
+__device__ __forceinline__ void stage_body(const SysDev<T> *__restrict__ Sp, const StageDev<T> *__restrict__ Gp,
+                                           StepCtl *ctl, const C2<T> *__restrict__ S1, const FftPlanDev &p, int Hx,
+                                           int Hp, int rpw, int has_demag, double *__restrict__ epart,
+                                           const T *__restrict__ Hdem, int blk, int nblk, bool remap)
 {
     // the system and stage records live in device memory (read where used)
     // rather than in the kernel arguments: by value they need ~160 SGPRs,
@@ -596,7 +602,7 @@ __global__ void k_stage(const SysDev<T> *__restrict__ Sp, const StageDev<T> *__r
     C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
     const int nx = S.m.nx, ny = S.m.ny, nz = S.m.nz;
     const int nrows = ny * nz;
-    int tile = S.xcd_remap ? xcd_tile((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
+    int tile = (remap && S.xcd_remap) ? xcd_tile(blk, nblk) : blk;
     if (S.tile_zcol > 0) {
         const int ty = tile / nz;
         tile = (tile - ty * nz) * S.tile_zcol + ty;
@@ -736,8 +742,8 @@ __global__ void k_stage(const SysDev<T> *__restrict__ Sp, const StageDev<T> *__r
                 const Dd o = {wsum[2 * q][v], wsum[2 * q + 1][v]};
                 acc = dd_merge(acc, o);
             }
-            epart[(size_t)(2 * q) * gridDim.x + blockIdx.x] = acc.s;
-            epart[(size_t)(2 * q + 1) * gridDim.x + blockIdx.x] = acc.e;
+            epart[(size_t)(2 * q) * nblk + blk] = acc.s;
+            epart[(size_t)(2 * q + 1) * nblk + blk] = acc.e;
         }
         return;
     }
@@ -758,16 +764,24 @@ __global__ void k_stage(const SysDev<T> *__restrict__ Sp, const StageDev<T> *__r
                 ec = fmax(ec, wred[2][q]);
                 ed = fmax(ed, wred[3][q]);
             }
-            const size_t nb = gridDim.x, b = blockIdx.x;
             double *mp = gfield(G.mpart);
-            if (G.prep == 2) mp[b] = ea0;
+            if (G.prep == 2) mp[blk] = ea0;
             if (G.want_dydt || G.dydt_from_k0) {
-                mp[nb + b] = eb;
-                mp[2 * nb + b] = ec;
-                mp[3 * nb + b] = ed;
+                mp[(size_t)nblk + blk] = eb;
+                mp[2 * (size_t)nblk + blk] = ec;
+                mp[3 * (size_t)nblk + blk] = ed;
             }
         }
     }
+}
+
+template <typename T, bool EN, int LC, bool STT, bool GEN>
+__global__ void k_stage(const SysDev<T> *__restrict__ Sp, const StageDev<T> *__restrict__ Gp, StepCtl *ctl,
+                        const C2<T> *__restrict__ S1, FftPlanDev p, int Hx, int Hp, int rpw, int has_demag,
+                        double *__restrict__ epart, const T *__restrict__ Hdem)
+{
+    stage_body<T, EN, LC, STT, GEN>(Sp, Gp, ctl, S1, p, Hx, Hp, rpw, has_demag, epart, Hdem, (int)blockIdx.x,
+                                    (int)gridDim.x, true);
 }
 
 // Relaxation rule on the state the attempt started from; true: stop there
@@ -875,9 +889,8 @@ __device__ __forceinline__ void controller_update(StepCtl &c, const double m[4],
 // Step-size control after an attempt (one workgroup): the maxima of the
 // per-block partials of the attempt's stage launches, then controller_update
 // on the control block.
-__global__ void __launch_bounds__(1024) k_controller(StepCtl *ctl, const double *__restrict__ mpart, int nb,
-                                                    int order, double eps_abs, double eps_rel, int Ms_uniform,
-                                                    double Ms_u, int mode)
+__device__ __forceinline__ void controller_body(StepCtl *ctl, const double *__restrict__ mpart, int nb, int order,
+                                                double eps_abs, double eps_rel, int Ms_uniform, double Ms_u, int mode)
 {
     // The control block (thread 0) and the partials load together; the stop
     // flag is looked at only when the maxima are there (after a stopped run
@@ -905,6 +918,13 @@ __global__ void __launch_bounds__(1024) k_controller(StepCtl *ctl, const double 
     // the control block is updated in registers and written back once
     controller_update(c, m, order, eps_abs, eps_rel, Ms_uniform, Ms_u, mode);
     *ctl = c;
+}
+
+__global__ void __launch_bounds__(1024) k_controller(StepCtl *ctl, const double *__restrict__ mpart, int nb,
+                                                    int order, double eps_abs, double eps_rel, int Ms_uniform,
+                                                    double Ms_u, int mode)
+{
+    controller_body(ctl, mpart, nb, order, eps_abs, eps_rel, Ms_uniform, Ms_u, mode);
 }
 
 // ytmp_1 = y; ytmp_1 += k_0 * (h b10) for attempts that start from a FSAL k_0
@@ -1393,12 +1413,21 @@ template <typename T> struct Stepper : public StepperBase {
                            conv ? conv->g.Hx : 0, conv ? conv->g.Hp : 0, rpw, demag_mode(), ep, Hdem);
     }
     // Gd: the stage record in device memory (gtab)
+    // the stage kernel's physics class: STT (Zhang-Li) and GEN (cubic
+    // anisotropy, exchange with per-cell Ms/A, or state arrays of 4 GiB or
+    // more, which the other instantiation addresses by 32-bit byte offsets)
+    void physics_class(bool &stt, bool &gen) const
+    {
+        const bool big = 3.0 * (double)n * sizeof(T) >= 4294967296.0;
+        stt = sys.has_stt != 0;
+        gen = sys.has_cub || (sys.has_exch && !sys.exch_uniform) || big;
+    }
+
     template <bool EN> void stage_kernel(const StageDev<T> *Gd, double *ep)
     {
-        // GEN also for state arrays of 4 GiB or more: the other instantiation
-        // addresses them by 32-bit byte offsets
-        const bool big = 3.0 * (double)n * sizeof(T) >= 4294967296.0;
-        const bool stt = !EN && sys.has_stt, gen = sys.has_cub || (sys.has_exch && !sys.exch_uniform) || big;
+        bool stt, gen;
+        physics_class(stt, gen);
+        stt = stt && !EN;
         mm_with_len(stage_len(), conv && conv->fft_generic, [&](auto lc) {
             constexpr int LC = decltype(lc)::value;
             if (stt) {
@@ -2252,16 +2281,365 @@ extern "C" int mm_stepper_relax(mm_stepper_t *st, int criterion, double threshol
 // streams so that the small launches of different systems overlap; every
 // system keeps its own control block (own h, accept/reject, t)
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// batched ensembles: every pass of an attempt is one launch for all systems
+// ---------------------------------------------------------------------------
+// Per-system records of the batched stage and controller launches (the conv
+// passes' records are in mm_conv.h).  The global workgroup index in XCD order
+// picks the system through the pass's tile table, as in the conv passes.
+template <typename T> struct BStageDesc {
+    const SysDev<T> *S;
+    const StageDev<T> *G0;  // the system's stage records; the launch picks stage st
+    StepCtl *ctl;
+    const C2<T> *S1;
+    int Hx, Hp, rpw, nb, tbase;
+    FftPlanDev p;
+};
+
+struct BCtlDesc {
+    StepCtl *ctl;
+    const double *mpart;
+    int nb, order, Ms_uniform, mode;
+    double eps_abs, eps_rel, Ms_u;
+};
+
+template <typename T, bool STT, bool GEN>
+__global__ void kb_stage(const BStageDesc<T> *__restrict__ D, const int *__restrict__ tsys, int st)
+{
+    const int gi = block_tile(1);
+    const BStageDesc<T> &d = D[tsys[gi]];
+    stage_body<T, false, 0, STT, GEN>(gfield(d.S), gfield(d.G0) + st, gfield(d.ctl), gfield(d.S1), d.p, d.Hx, d.Hp,
+                                      d.rpw, 1, nullptr, nullptr, gi - d.tbase, d.nb, false);
+}
+
+// one workgroup per system
+__global__ void __launch_bounds__(256) kb_controller(const BCtlDesc *__restrict__ D)
+{
+    const BCtlDesc &d = D[blockIdx.x];
+    controller_body(gfield(d.ctl), gfield(d.mpart), d.nb, d.order, d.eps_abs, d.eps_rel, d.Ms_uniform, d.Ms_u,
+                    d.mode);
+}
+
+// out[0] = number of systems whose run has not stopped
+__global__ void kb_running(StepCtl *const *__restrict__ ctls, int n, int *__restrict__ out)
+{
+    int c = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) c += gfield(ctls[i])->stop ? 0 : 1;
+    __shared__ int sred[16];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, 64);
+    if ((threadIdx.x & 63) == 0) sred[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += sred[w];
+        out[0] = t;
+    }
+}
+
+struct BatchRunnerBase {
+    virtual ~BatchRunnerBase() {}
+    virtual void run(long long steps) = 0;
+};
+
+// Systems of one precision and one method without FSAL, all with the demag
+// field in the fused stage (no split x pass).  An attempt of every system is
+// one x forward launch, per stage the y, z (or 2-D) and stage launches (one
+// per physics class present), and one controller launch: 2 + 4 s launches
+// for the whole ensemble (s stages), recorded once as a HIP graph.  Systems
+// that reached their step target skip the passes.
+template <typename T> struct BatchRunner : BatchRunnerBase {
+    std::vector<Stepper<T> *> sys;  // not owned
+    hipStream_t s = nullptr;
+    int ns = 0;
+    bool any3d = false, any2d = false;
+    BatchPass bx, by, bz, b2;
+    struct StageGroup {
+        BatchPass bp;
+        bool stt, gen;
+    };
+    std::vector<StageGroup> stg;
+    BCtlDesc *cdesc = nullptr;
+    StepCtl **ctls = nullptr;
+    int *running = nullptr;
+    int *running_host = nullptr;
+    std::vector<void *> allocs;
+    hipGraphExec_t gexec = nullptr;
+
+    static int round64(int v) { return std::max(64, (v + 63) / 64 * 64); }
+
+    template <typename D> void upload_pass(BatchPass &bp, const std::vector<D> &desc, const std::vector<int> &tsys)
+    {
+        void *dd = nullptr, *dt = nullptr;
+        MM_HIP(hipMalloc(&dd, sizeof(D) * std::max<size_t>(1, desc.size())));
+        allocs.push_back(dd);
+        MM_HIP(hipMalloc(&dt, sizeof(int) * std::max<size_t>(1, tsys.size())));
+        allocs.push_back(dt);
+        MM_HIP(hipMemcpy(dd, desc.data(), sizeof(D) * desc.size(), hipMemcpyHostToDevice));
+        MM_HIP(hipMemcpy(dt, tsys.data(), sizeof(int) * tsys.size(), hipMemcpyHostToDevice));
+        bp.desc = dd;
+        bp.tsys = (int *)dt;
+        bp.ntiles = (int)tsys.size();
+        MM_REQUIRE(bp.lds <= (size_t)mm_max_lds_bytes(), "batched pass needs more LDS than the device has");
+        MM_REQUIRE(bp.threads <= 1024, "batched pass needs more than 1024 threads per workgroup");
+    }
+
+    BatchRunner(const std::vector<Stepper<T> *> &systems)
+    {
+        sys = systems;
+        MM_REQUIRE(!sys.empty(), "empty batch");
+        MM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        ns = sys[0]->tab.num_steps;
+        const int n = (int)sys.size();
+        // threads per workgroup of each pass: the largest a system's own
+        // launch takes
+        int tx = 64, ty = 64, tz = 64, t2 = 64, tsg = 64;
+        for (Stepper<T> *q : sys) {
+            const ConvImpl<T> &c = *q->conv;
+            MM_REQUIRE(q->tab.num_steps == ns && !q->tab.fsal && q->d.method == sys[0]->d.method,
+                       "batched systems share a method without FSAL");
+            MM_REQUIRE(!q->split_x, "batched systems run the fused x transform");
+            tx = std::max(tx, c.ln.lpw_x * c.px.tline);
+            if (c.g.is2d) {
+                any2d = true;
+                t2 = std::max(t2, 3 * c.ln.tk_y * c.py.tline);
+            } else {
+                any3d = true;
+                ty = std::max(ty, c.ln.tk_y * c.py.tline);
+                tz = std::max(tz, 3 * c.ln.tk_z * c.pz.tline);
+            }
+            tsg = std::max(tsg, q->threads);
+        }
+        bx.threads = round64(tx);
+        by.threads = round64(ty);
+        bz.threads = round64(tz);
+        b2.threads = round64(t2);
+        // per-system records and tile tables
+        std::vector<BxDesc<T>> dx;
+        std::vector<ByDesc<T>> dy;
+        std::vector<BzDesc<T>> dz;
+        std::vector<B2Desc<T>> d2;
+        std::vector<int> tx_s, ty_s, tz_s, t2_s;
+        auto add_tiles = [](std::vector<int> &t, int count, int sysi) {
+            const int base = (int)t.size();
+            t.insert(t.end(), (size_t)count, sysi);
+            return base;
+        };
+        for (int i = 0; i < n; ++i) {
+            Stepper<T> *q = sys[i];
+            const ConvImpl<T> &c = *q->conv;
+            const int *sel = (const int *)((char *)q->ctl + offsetof(StepCtl, cur));
+            const int *stop = (const int *)((char *)q->ctl + offsetof(StepCtl, stop));
+            size_t lds = 0;
+            bool o32 = true;
+            BxDesc<T> a;
+            int t = c.batch_x(a, bx.threads, lds, q->yb[0], q->yb[1], sel, stop);
+            a.tbase = add_tiles(tx_s, t, i);
+            bx.lds = std::max(bx.lds, lds);
+            dx.push_back(a);
+            if (c.g.is2d) {
+                B2Desc<T> e;
+                t = c.batch_2d(e, b2.threads, lds, stop);
+                e.tbase = add_tiles(t2_s, t, (int)d2.size());
+                b2.lds = std::max(b2.lds, lds);
+                d2.push_back(e);
+            } else {
+                ByDesc<T> e;
+                t = c.batch_y(e, by.threads, lds, o32, stop);
+                e.tbase = add_tiles(ty_s, t, (int)dy.size());
+                by.lds = std::max(by.lds, lds);
+                by.o32 = by.o32 && o32;
+                dy.push_back(e);
+                BzDesc<T> f;
+                t = c.batch_z(f, bz.threads, lds, o32, stop);
+                f.tbase = add_tiles(tz_s, t, (int)dz.size());
+                bz.lds = std::max(bz.lds, lds);
+                bz.o32 = bz.o32 && o32;
+                dz.push_back(f);
+            }
+        }
+        upload_pass(bx, dx, tx_s);
+        if (any3d) {
+            upload_pass(by, dy, ty_s);
+            upload_pass(bz, dz, tz_s);
+        }
+        if (any2d) upload_pass(b2, d2, t2_s);
+        // stage launches, one per physics class present
+        for (int cls = 0; cls < 4; ++cls) {
+            const bool stt = cls & 1, gen = (cls >> 1) & 1;
+            std::vector<BStageDesc<T>> ds;
+            std::vector<int> ts;
+            StageGroup gr;
+            gr.stt = stt;
+            gr.gen = gen;
+            gr.bp.threads = round64(tsg);
+            for (int i = 0; i < n; ++i) {
+                Stepper<T> *q = sys[i];
+                bool a, b;
+                q->physics_class(a, b);
+                if (a != stt || b != gen) continue;
+                BStageDesc<T> e;
+                e.S = q->sysd;
+                e.G0 = q->gtab;
+                e.ctl = q->ctl;
+                e.S1 = q->conv->S1;
+                e.Hx = q->conv->g.Hx;
+                e.Hp = q->conv->g.Hp;
+                e.rpw = q->rpw;
+                e.nb = q->grid_rows;
+                e.p = q->conv->px.template dev<T>();
+                e.tbase = add_tiles(ts, q->grid_rows, (int)ds.size());
+                gr.bp.lds = std::max(gr.bp.lds, q->stage_lds);
+                ds.push_back(e);
+            }
+            if (ds.empty()) continue;
+            upload_pass(gr.bp, ds, ts);
+            stg.push_back(gr);
+        }
+        // controllers and the running count
+        std::vector<BCtlDesc> dc;
+        std::vector<StepCtl *> cp;
+        for (Stepper<T> *q : sys) {
+            BCtlDesc e;
+            e.ctl = q->ctl;
+            e.mpart = q->mpart;
+            e.nb = q->grid_rows;
+            e.order = q->tab.order;
+            e.Ms_uniform = q->sys.Ms_uniform;
+            e.mode = q->d.method == MM_RK4 ? 0 : (q->d.method == MM_EULER ? 2 : 1);
+            e.eps_abs = q->d.eps_abs;
+            e.eps_rel = q->d.eps_rel;
+            e.Ms_u = q->sys.Ms_u;
+            dc.push_back(e);
+            cp.push_back(q->ctl);
+        }
+        MM_HIP(hipMalloc(&cdesc, sizeof(BCtlDesc) * dc.size()));
+        allocs.push_back(cdesc);
+        MM_HIP(hipMemcpy(cdesc, dc.data(), sizeof(BCtlDesc) * dc.size(), hipMemcpyHostToDevice));
+        MM_HIP(hipMalloc(&ctls, sizeof(StepCtl *) * cp.size()));
+        allocs.push_back(ctls);
+        MM_HIP(hipMemcpy(ctls, cp.data(), sizeof(StepCtl *) * cp.size(), hipMemcpyHostToDevice));
+        MM_HIP(hipMalloc(&running, sizeof(int)));
+        allocs.push_back(running);
+        MM_HIP(hipHostMalloc((void **)&running_host, sizeof(int), hipHostMallocDefault));
+    }
+
+    ~BatchRunner() override
+    {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        for (void *p : allocs) (void)hipFree(p);
+        if (running_host) (void)hipHostFree(running_host);
+        if (s) (void)hipStreamDestroy(s);
+    }
+
+    void stage_launch(const StageGroup &gr, int st)
+    {
+        const BStageDesc<T> *D = (const BStageDesc<T> *)gr.bp.desc;
+        const dim3 grid(gr.bp.ntiles), block(gr.bp.threads);
+        if (gr.stt) {
+            if (gr.gen) hipLaunchKernelGGL((kb_stage<T, true, true>), grid, block, gr.bp.lds, s, D, gr.bp.tsys, st);
+            else hipLaunchKernelGGL((kb_stage<T, true, false>), grid, block, gr.bp.lds, s, D, gr.bp.tsys, st);
+        } else {
+            if (gr.gen) hipLaunchKernelGGL((kb_stage<T, false, true>), grid, block, gr.bp.lds, s, D, gr.bp.tsys, st);
+            else hipLaunchKernelGGL((kb_stage<T, false, false>), grid, block, gr.bp.lds, s, D, gr.bp.tsys, st);
+        }
+        MM_HIP(hipGetLastError());
+    }
+
+    // one attempt of every system (the records of stage st < ns - 1 also
+    // transform the next stage input along x, so only stage 0 has an x pass)
+    void enqueue_attempt()
+    {
+        mm_batch_xfwd<T>(bx, s);
+        for (int st = 0; st < ns; ++st) {
+            if (any3d) {
+                mm_batch_yfwd<T>(by, s);
+                mm_batch_zmi<T>(bz, s);
+                mm_batch_yinv<T>(by, s);
+            }
+            if (any2d) mm_batch_ymi2d<T>(b2, s);
+            for (const StageGroup &gr : stg) stage_launch(gr, st);
+        }
+        hipLaunchKernelGGL(kb_controller, dim3((unsigned)sys.size()), dim3(256), 0, s, (const BCtlDesc *)cdesc);
+        MM_HIP(hipGetLastError());
+    }
+
+    void launch_attempt()
+    {
+        if (!gexec) {
+            hipGraph_t graph = nullptr;
+            MM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            try {
+                enqueue_attempt();
+            } catch (...) {
+                (void)hipStreamEndCapture(s, &graph);
+                if (graph) (void)hipGraphDestroy(graph);
+                throw;
+            }
+            MM_HIP(hipStreamEndCapture(s, &graph));
+            const hipError_t e = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            MM_HIP(e);
+        }
+        MM_HIP(hipGraphLaunch(gexec, s));
+    }
+
+    int count_running()
+    {
+        hipLaunchKernelGGL(kb_running, dim3(1), dim3(256), 0, s, (StepCtl *const *)ctls, (int)sys.size(), running);
+        MM_HIP(hipGetLastError());
+        MM_HIP(hipMemcpyAsync(running_host, running, sizeof(int), hipMemcpyDeviceToHost, s));
+        MM_HIP(hipStreamSynchronize(s));
+        return *running_host;
+    }
+
+    // `steps` accepted steps of every system (the systems were armed)
+    void run(long long steps) override
+    {
+        if (steps <= 0) return;
+        for (Stepper<T> *q : sys) MM_HIP(hipStreamSynchronize(q->s));
+        // attempts between two looks at the running count: every system needs
+        // at least `steps`, then the rejected ones
+        long long batch = steps;
+        while (count_running() > 0) {
+            for (long long r = 0; r < batch; ++r) launch_attempt();
+            batch = std::max(1ll, std::min(8ll, steps / 8));
+        }
+    }
+};
+
 struct mm_batch {
     std::vector<StepperBase *> sims;
     std::vector<hipStream_t> streams;
+    std::vector<BatchRunnerBase *> runners;
+    std::vector<char> in_runner;  // system i is stepped by a runner
 };
 
 static void batch_free(mm_batch *b)
 {
+    for (BatchRunnerBase *r : b->runners) delete r;
     for (StepperBase *p : b->sims) delete p;
     for (hipStream_t st : b->streams) (void)hipStreamDestroy(st);
     delete b;
+}
+
+// batched runners for the systems that can take them: per precision and
+// method, systems with the demag field in the fused stage and no FSAL
+template <typename T> static void batch_group(mm_batch *b, int precision)
+{
+    std::map<int, std::vector<int>> by_method;
+    for (int i = 0; i < (int)b->sims.size(); ++i) {
+        auto *q = dynamic_cast<Stepper<T> *>(b->sims[i]);
+        if (!q || q->d.precision != precision) continue;
+        if (!q->conv || q->split_x || q->tab.fsal) continue;
+        by_method[q->d.method].push_back(i);
+    }
+    for (auto &kv : by_method) {
+        std::vector<Stepper<T> *> v;
+        for (int i : kv.second) v.push_back(dynamic_cast<Stepper<T> *>(b->sims[i]));
+        b->runners.push_back(new BatchRunner<T>(v));
+        for (int i : kv.second) b->in_runner[i] = 1;
+    }
 }
 
 extern "C" int mm_batch_create(const mm_system_desc_t *descs, int count, int nstreams, mm_batch_t **out)
@@ -2289,6 +2667,13 @@ extern "C" int mm_batch_create(const mm_system_desc_t *descs, int count, int nst
                 b->sims.push_back(p);
                 p->init(d, st);
             }
+        }
+        b->in_runner.assign(b->sims.size(), 0);
+        // MM_BATCH_LAUNCH=0: every system on its own graphs over the streams
+        const char *e = std::getenv("MM_BATCH_LAUNCH");
+        if (!(e && std::atoi(e) == 0)) {
+            batch_group<double>(b, MM_FP64);
+            batch_group<float>(b, MM_FP32);
         }
     } catch (...) {
         batch_free(b);
@@ -2341,17 +2726,20 @@ extern "C" int mm_batch_run(mm_batch_t *b, int steps)
     MM_CAPI_BEGIN
     MM_REQUIRE(b && steps >= 0, "bad argument");
     for (StepperBase *p : b->sims) p->arm(steps);
+    for (BatchRunnerBase *r : b->runners) r->run(steps);
     for (;;) {
         bool any = false;
-        for (StepperBase *p : b->sims)
-            if (p->pending()) {
-                p->enqueue_some(1ll << 40);
+        for (size_t i = 0; i < b->sims.size(); ++i)
+            if (!b->in_runner[i] && b->sims[i]->pending()) {
+                b->sims[i]->enqueue_some(1ll << 40);
                 any = true;
             }
         if (!any) break;
         for (hipStream_t st : b->streams) MM_HIP(hipStreamSynchronize(st));
-        for (StepperBase *p : b->sims) p->refresh();
+        for (size_t i = 0; i < b->sims.size(); ++i)
+            if (!b->in_runner[i]) b->sims[i]->refresh();
     }
+    for (StepperBase *p : b->sims) p->refresh();
     MM_CAPI_END
 }
 

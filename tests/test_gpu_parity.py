@@ -853,10 +853,16 @@ def _ensemble_specs(count, seed, precision="fp32", max_xy=96):
     return specs
 
 
-def test_ensemble_equals_single_steppers_and_oracle():
+@pytest.mark.parametrize("batched", ["1", "0"])
+def test_ensemble_equals_single_steppers_and_oracle(batched, monkeypatch):
+    # batched: every pass of an attempt is one launch for all systems of a
+    # precision (runtime FFT plans, so the single steppers compared with
+    # take the runtime plans too); "0": each system on its own graphs
     from micromagnum_amd.solver import Ensemble, Stepper
+    monkeypatch.setenv("MM_BATCH_LAUNCH", batched)
     specs = _ensemble_specs(6, 61)
     specs[0]["precision"] = "fp64"
+    specs[3]["shape"] = (40, 36, 1)  # a 2-D member
     ens = Ensemble(specs, nstreams=3)
     Ms = []
     for i, sp in enumerate(specs):
@@ -864,6 +870,7 @@ def test_ensemble_equals_single_steppers_and_oracle():
         Ms.append(M)
         ens.set_m(i, M)
     ens.run(12)
+    monkeypatch.setenv("MM_FFT_GENERIC", batched)
     for i, sp in enumerate(specs):
         kw = {k: v for k, v in sp.items() if k not in ("shape", "delta", "Ms", "alpha")}
         st = Stepper(sp["shape"], sp["delta"], sp["Ms"], sp["alpha"], N=ens.specs[i].N, **kw)
@@ -872,6 +879,7 @@ def test_ensemble_equals_single_steppers_and_oracle():
         assert np.array_equal(ens.get_m(i), st.get_m()), i
         assert ens.stats(i) == st.stats(), i
         st.close()
+    monkeypatch.delenv("MM_FFT_GENERIC")
     # the FP64 member against the oracle
     sp = specs[0]
     kw = {k: v for k, v in sp.items() if k in ("A", "k_uni", "axis1")}
@@ -880,6 +888,9 @@ def test_ensemble_equals_single_steppers_and_oracle():
     s = ens.stats(0)
     assert s["attempts"] == so["attempts"] and s["rejections"] == so["rejections"]
     assert np.abs(ens.get_m(0) - Mo).max() / sp["Ms"] < 1e-10
+    # a second run continues from where the first stopped
+    ens.run(3)
+    assert all(ens.stats(i)["attempts"] - ens.stats(i)["rejections"] == 15 for i in range(len(specs)))
     ens.close()
 
 
