@@ -634,10 +634,16 @@ __device__ __forceinline__ void stage_body(const SysDev<T> *__restrict__ Sp, con
             [&](int k) {
                 const bool lo = k < Hx;
                 const int kk = lo ? k : L - k;
+                // both loads unconditional (src1 is src0 for an absent second
+                // row) and the cases folded into a factor f (0 at k = 0 and
+                // L/2, +1 below Hx, -1 above; f*v is exact): with branches
+                // around the loads, the compiler waited for every element's
+                // loads before issuing the next element's
                 const C2<T> a = src0[kk];
-                const C2<T> b = has1 ? src1[kk] : cmk<T>((T)0, (T)0);
-                if (k == 0 || 2 * k == L) return cmk<T>(a.x, b.x);
-                return lo ? cmk<T>(a.x - b.y, a.y + b.x) : cmk<T>(a.x + b.y, b.x - a.y);
+                const C2<T> b = src1[kk];
+                const T bx = has1 ? b.x : (T)0, by = has1 ? b.y : (T)0;
+                const T f = ((k == 0) | (2 * k == L)) ? (T)0 : (lo ? (T)1 : (T)-1);
+                return cmk<T>(a.x - f * by, bx + f * a.y);
             },
             [](int, C2<T>) {}, [&] { twf.store(twl); });
     }
@@ -1347,10 +1353,11 @@ template <typename T> struct Stepper : public StepperBase {
     void release()
     {
         for (int g = 0; g < 2; ++g)
-            if (gexec[g]) {
-                (void)hipGraphExecDestroy(gexec[g]);
-                gexec[g] = nullptr;
-            }
+            for (int w = 0; w < 2; ++w)
+                if (gexec[g][w]) {
+                    (void)hipGraphExecDestroy(gexec[g][w]);
+                    gexec[g][w] = nullptr;
+                }
         if (conv) {
             conv->release();
             delete conv;
@@ -1593,12 +1600,15 @@ template <typename T> struct Stepper : public StepperBase {
             s1_ready = G.fwd != 0;
         }
         if (fsal) have_fsal = true;
-        const int mode = d.method == MM_RK4 ? 0 : (d.method == MM_EULER ? 2 : 1);
         hipLaunchKernelGGL(k_controller, dim3(1), dim3(1024), 0, s, ctl, mpart, grid_rows, tab.order, d.eps_abs,
-                           d.eps_rel, sys.Ms_uniform, sys.Ms_u, mode);
+                           d.eps_rel, sys.Ms_uniform, sys.Ms_u, ctl_mode());
         MM_HIP(hipGetLastError());
         if (klog) klog->mark(s, KC_CTRL, 0.0);
     }
+
+    // controller mode: 1 adaptive, 0 fixed step clipped to t_max (rk4), 2
+    // fixed step without clipping (euler)
+    int ctl_mode() const { return d.method == MM_RK4 ? 0 : (d.method == MM_EULER ? 2 : 1); }
 
     // Enqueue `attempts` attempts with an event after every kernel; per
     // kernel class: [avg ms per launch, launches, avg algorithmic bytes].
@@ -1756,49 +1766,63 @@ template <typename T> struct Stepper : public StepperBase {
         }
         while (!c.stop) {
             const long long q = std::max(1ll, std::min(batch, c.target - c.accepted));
-            for (long long r = 0; r < q; ++r) launch_attempt();
+            launch_attempts(q);
             MM_HIP(hipStreamSynchronize(s));
             MM_HIP(hipMemcpy(&c, ctl, sizeof(c), hipMemcpyDeviceToHost));
         }
     }
 
-    // One attempt as a replay of a captured HIP graph (one graph for attempts
-    // that compute k_0, one for FSAL attempts that reuse it): every kernel
+    // Attempts as replays of captured HIP graphs (graphs for attempts that
+    // compute k_0, graphs for FSAL attempts that reuse it): every kernel
     // argument is fixed at set-up and all step state lives in device memory,
-    // so the recorded launches are valid for every attempt.  MM_STEPPER_GRAPHS=0
-    // launches the kernels one by one instead.
-    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    // so the recorded launches are valid for every attempt.  A graph holds
+    // one attempt or graph_attempts of them in a row (fewer graph launches,
+    // so fewer gaps between them; attempts after the run has stopped do
+    // nothing).  MM_STEPPER_GRAPHS=0 launches the kernels one by one instead;
+    // MM_GRAPH_ATTEMPTS sets the attempts per long graph.
+    hipGraphExec_t gexec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     int use_graphs = -1;
+    int graph_attempts = 4;
 
-    void launch_attempt()
+    void launch_attempt() { launch_attempts(1); }
+
+    void launch_attempts(long long q)
     {
         if (use_graphs < 0) {
             const char *e = std::getenv("MM_STEPPER_GRAPHS");
             use_graphs = (e && std::atoi(e) == 0) ? 0 : 1;
+            const char *ga = std::getenv("MM_GRAPH_ATTEMPTS");
+            if (ga) graph_attempts = std::max(1, std::min(64, std::atoi(ga)));
         }
         if (!use_graphs || klog) {
-            enqueue_attempt();
+            for (long long r = 0; r < q; ++r) enqueue_attempt();
             return;
         }
-        const int g = (tab.fsal && have_fsal) ? 1 : 0;
-        if (!gexec[g]) {
-            hipGraph_t graph = nullptr;
-            MM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            try {
-                enqueue_attempt();
-            } catch (...) {
-                (void)hipStreamEndCapture(s, &graph);
-                if (graph) (void)hipGraphDestroy(graph);
-                throw;
+        while (q > 0) {
+            // whole long graphs, then single attempts for the rest
+            const int w = (graph_attempts > 1 && q >= graph_attempts) ? 1 : 0;
+            const int na = w ? graph_attempts : 1;
+            const int g = (tab.fsal && have_fsal) ? 1 : 0;
+            if (!gexec[g][w]) {
+                hipGraph_t graph = nullptr;
+                MM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                try {
+                    for (int r = 0; r < na; ++r) enqueue_attempt();
+                } catch (...) {
+                    (void)hipStreamEndCapture(s, &graph);
+                    if (graph) (void)hipGraphDestroy(graph);
+                    throw;
+                }
+                MM_HIP(hipStreamEndCapture(s, &graph));
+                const hipError_t e = hipGraphInstantiate(&gexec[g][w], graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                MM_HIP(e);
+            } else if (tab.fsal) {
+                have_fsal = true;  // what enqueue_attempt records for the next attempt
             }
-            MM_HIP(hipStreamEndCapture(s, &graph));
-            const hipError_t e = hipGraphInstantiate(&gexec[g], graph, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(graph);
-            MM_HIP(e);
-        } else if (tab.fsal) {
-            have_fsal = true;  // what enqueue_attempt records for the next attempt
+            MM_HIP(hipGraphLaunch(gexec[g][w], s));
+            q -= na;
         }
-        MM_HIP(hipGraphLaunch(gexec[g], s));
     }
 
     // Zeeman field from a host function (AlternatingField with H_ext_fn,
@@ -1885,7 +1909,7 @@ template <typename T> struct Stepper : public StepperBase {
     void enqueue_some(long long batch) override
     {
         const long long q = std::max(1ll, std::min(batch, cached.target - cached.accepted));
-        for (long long r = 0; r < q; ++r) launch_attempt();
+        launch_attempts(q);
     }
 
     void refresh() override { MM_HIP(hipMemcpy(&cached, ctl, sizeof(cached), hipMemcpyDeviceToHost)); }
