@@ -637,7 +637,15 @@ __global__ void k_xinv(const C2<T> *__restrict__ in, TOUT *__restrict__ out, siz
     TOUT *dst = out + c * out_cstride + (size_t)r * nx;
     const LineMap m = {t, active};
     fft_line<T, true, true, true, LC, SK>(
-        p, twl, line, m, line, m, [&](int k) { return k < Hx ? src[k] : cconj<T>(src[L - k]); },
+        p, twl, line, m, line, m,
+        [&](int k) {
+            // one unconditional load per element (the conjugate by an exact
+            // sign): a load in each arm of a branch made the compiler wait
+            // for every element's load before issuing the next
+            const bool lo = k < Hx;
+            const C2<T> v = src[lo ? k : L - k];
+            return cmk<T>(v.x, (lo ? (T)1 : (T)-1) * v.y);
+        },
         [&](int x, C2<T> v) {
             if (x < nx) dst[x] = (TOUT)v.x;
         }, [&] { twf.store(twl); });
@@ -852,12 +860,19 @@ static ConvLaunch choose_launch(const ConvGeom &g, const FftPlanHost &px, const 
     ln.lpw_x = env_int("MM_LPW_X", w.lpw_x > 0 ? w.lpw_x : std::max(1, 256 / px.tline));
     while (ln.lpw_x > 1 && (lines_lds<MM_SKEW_X>(ln.lpw_x, px.L, csize) > (size_t)lds || ln.lpw_x * px.tline > 1024)) --ln.lpw_x;
     const int nl_y = g.is2d ? 3 : 1;
-    int tk = floor_pow2(env_int("MM_TK_Y", w.tk_y > 0 ? w.tk_y : 8));
+    // default tile widths by line length, measured on MI355X (FP32 steps/s,
+    // same-run A/B): y 8 columns, 2 for lines of 2048 or more (1024x1024x2
+    // cells: +3 %; 256x256x16: 8 best; 512x512x64: 8 better than 4 by 4 %);
+    // z 512 / Pz columns within [16, 64] (256x256x16: 16 best of 8..64;
+    // 1024x1024x2, Pz 4: 64 +4 % against 16)
+    const int tk_y0 = g.Py >= 2048 ? 2 : 8;
+    const int tk_z0 = std::max(16, std::min(64, 512 / std::max(1, g.Pz)));
+    int tk = floor_pow2(env_int("MM_TK_Y", w.tk_y > 0 ? w.tk_y : tk_y0));
     while (tk > 1 && ((size_t)nl_y * tk * py.tline > 1024 || lines_lds<MM_SKEW_Y>(nl_y * tk, py.L, csize) > (size_t)lds)) tk /= 2;
     ln.tk_y = tk;
     MM_REQUIRE((size_t)nl_y * tk * py.tline <= 1024 && lines_lds<MM_SKEW_Y>(nl_y * tk, py.L, csize) <= (size_t)lds,
                "y transform length too large for one workgroup");
-    int tz = floor_pow2(env_int("MM_TK_Z", w.tk_z > 0 ? w.tk_z : 16));
+    int tz = floor_pow2(env_int("MM_TK_Z", w.tk_z > 0 ? w.tk_z : tk_z0));
     while (tz > 1 && ((size_t)3 * tz * pz.tline > 1024 || lines_lds<MM_SKEW_Z>(3 * tz, pz.L, csize) > (size_t)lds)) tz /= 2;
     ln.tk_z = tz;
     return ln;

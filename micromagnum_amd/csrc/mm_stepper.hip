@@ -250,24 +250,30 @@ __device__ __forceinline__ V3<T> zhangli_cell_uniform32(const MeshDev &m, int x,
                                                         unsigned nb, const T *inv_d, T f1, T f2, const V3<T> &j,
                                                         const V3<T> &Mv, const T *__restrict__ M)
 {
-    auto grad = [&](int pos, int dim, unsigned st, T id) -> V3<T> {
-        if (dim <= 1) return v3<T>(0, 0, 0);
-        unsigned a = ib - st, b = ib + st;
-        T f = id * (T)0.5;
-        if (pos == 0) {
-            a = ib;
-            f = id;
-        } else if (pos == dim - 1) {
-            b = ib;
-            f = id;
+    // one-sided differences at the ends, central inside; a dimension of one
+    // cell has a = b (difference 0).  All 18 loads are unconditional and go
+    // out together (with a branch per axis the compiler waited for each
+    // axis's loads before issuing the next axis's).
+    T gv[3][2][3], f[3];
+    auto fetch = [&](int d, int pos, int dim, unsigned st, T id) {
+        const bool lo = pos == 0, hi = pos == dim - 1;
+        const unsigned a = lo ? ib : ib - st, b = hi ? ib : ib + st;
+        f[d] = (lo || hi) ? id : id * (T)0.5;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            gv[d][0][c] = ld32(M, a + (unsigned)c * nb);
+            gv[d][1][c] = ld32(M, b + (unsigned)c * nb);
         }
-        return v3<T>((ld32(M, b) - ld32(M, a)) * f, (ld32(M, b + nb) - ld32(M, a + nb)) * f,
-                     (ld32(M, b + 2 * nb) - ld32(M, a + 2 * nb)) * f);
     };
     constexpr unsigned sx = sizeof(T);
-    const V3<T> gx = grad(x, m.nx, sx, inv_d[0]);
-    const V3<T> gy = grad(y, m.ny, (unsigned)m.nx * sx, inv_d[1]);
-    const V3<T> gz = grad(z, m.nz, (unsigned)m.nx * (unsigned)m.ny * sx, inv_d[2]);
+    fetch(0, x, m.nx, sx, inv_d[0]);
+    fetch(1, y, m.ny, (unsigned)m.nx * sx, inv_d[1]);
+    fetch(2, z, m.nz, (unsigned)m.nx * (unsigned)m.ny * sx, inv_d[2]);
+    auto grad = [&](int d) -> V3<T> {
+        return v3<T>((gv[d][1][0] - gv[d][0][0]) * f[d], (gv[d][1][1] - gv[d][0][1]) * f[d],
+                     (gv[d][1][2] - gv[d][0][2]) * f[d]);
+    };
+    const V3<T> gx = grad(0), gy = grad(1), gz = grad(2);
     const V3<T> jg = v3<T>(j.x * gx.x + j.y * gy.x + j.z * gz.x, j.x * gx.y + j.y * gy.y + j.z * gz.y,
                            j.x * gx.z + j.y * gy.z + j.z * gz.z);
     const V3<T> B = cross3(Mv, jg);
