@@ -864,9 +864,13 @@ static ConvLaunch choose_launch(const ConvGeom &g, const FftPlanHost &px, const 
     // same-run A/B): y 8 columns, 2 for lines of 2048 or more (1024x1024x2
     // cells: +3 %; 256x256x16: 8 best; 512x512x64: 8 better than 4 by 4 %);
     // z 512 / Pz columns within [16, 64] (256x256x16: 16 best of 8..64;
-    // 1024x1024x2, Pz 4: 64 +4 % against 16)
+    // 1024x1024x2, Pz 4: 64 +4 % against 16), halved while a workgroup
+    // needs more than 64 KiB of LDS (512x512x64 FP64, Pz 128: 8 columns,
+    // 56 KiB, +11 % against 16 at 111 KiB, one workgroup per CU; FP32 keeps
+    // 16 at 55 KiB, 8 and 4 slower)
     const int tk_y0 = g.Py >= 2048 ? 2 : 8;
-    const int tk_z0 = std::max(16, std::min(64, 512 / std::max(1, g.Pz)));
+    int tk_z0 = std::max(16, std::min(64, 512 / std::max(1, g.Pz)));
+    while (tk_z0 > 1 && lines_lds<MM_SKEW_Z>(3 * tk_z0, pz.L, csize) > 65536) tk_z0 /= 2;
     int tk = floor_pow2(env_int("MM_TK_Y", w.tk_y > 0 ? w.tk_y : tk_y0));
     while (tk > 1 && ((size_t)nl_y * tk * py.tline > 1024 || lines_lds<MM_SKEW_Y>(nl_y * tk, py.L, csize) > (size_t)lds)) tk /= 2;
     ln.tk_y = tk;
