@@ -306,12 +306,27 @@ __device__ __forceinline__ void fft_pass_io(C2<T> *lineG, LineMap g, C2<T> *line
     constexpr bool useG = LDIN || STOUT;
     const LineMap in = useG ? g : sm_;
     C2<T> *lin = useG ? lineG : lineS;
+    if constexpr (LDIN) {
+        // all items' loads in one block: an item past nb loads element r nb
+        // (in range, not used) instead of branching, since a branch per item
+        // made the compiler wait for each item's loads before the next
+        if (in.active) {
 #pragma unroll
-    for (int q = 0; q < MB; ++q) {
-        const int b = in.t + q * tline;
-        if (in.active && b < nb) {
+            for (int q = 0; q < MB; ++q) {
+                const int b = in.t + q * tline;
+                const int bb = b < nb ? b : 0;
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[q][r] = LDIN ? ld(b + r * nb) : lin[sk<SK>(b + r * nb)];
+                for (int r = 0; r < R; ++r) v[q][r] = ld(bb + r * nb);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < MB; ++q) {
+            const int b = in.t + q * tline;
+            if (in.active && b < nb) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[q][r] = lin[sk<SK>(b + r * nb)];
+            }
         }
     }
     // first pass: the caller's hook runs while this pass's reads are in
