@@ -1300,7 +1300,11 @@ template <typename T> struct Stepper : public StepperBase {
                 int np = std::max(1, 192 / (3 * tl));
                 if (wshape.rpw > 0) np = std::max(1, wshape.rpw / 2);
                 if (const char *e = std::getenv("MM_RPW")) np = std::max(1, std::atoi(e) / 2);
-                while (np > 1 && ((size_t)3 * np * lds_len<MM_SKEW_X>((int)L) + L) * cs > (size_t)mm_max_lds_bytes()) --np;
+                // a tuned, imported or environment row count is cut to what one
+                // workgroup holds: LDS and 1024 threads (3 np lines of tl)
+                while (np > 1 && (((size_t)3 * np * lds_len<MM_SKEW_X>((int)L) + L) * cs > (size_t)mm_max_lds_bytes() ||
+                                  3 * np * tl > 1024))
+                    --np;
                 rpw = 2 * np;
                 threads = 3 * np * tl;
                 stage_lds = ((size_t)3 * np * lds_len<MM_SKEW_X>((int)L) + L) * cs;

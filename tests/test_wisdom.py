@@ -108,3 +108,32 @@ def test_autotune_records_a_shape_that_gives_the_same_results(tmp_path, prec):
     assert shape2 == other
     assert np.array_equal(M2, M0) and s2 == s0
     solver.wisdom_forget()
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not gpu_available(), reason="needs a HIP GPU")
+def test_wisdom_row_count_beyond_one_workgroup_is_cut(tmp_path):
+    # an entry whose stage rows would need more than 1024 threads (3 lines
+    # per row pair x 64 threads per 512-point line: 24 rows -> 2304 threads)
+    # is cut to what one workgroup holds, and the results do not change
+    from micromagnum_amd import magneto as mg
+    from micromagnum_amd import solver
+    shape, delta = (256, 16, 2), (3e-9, 3e-9, 3e-9)
+    N = mg.GenerateDemagTensor(*shape, *delta)
+    M = vortex_field(shape, delta, (384e-9, 24e-9), Ms=8.6e5)
+    solver.wisdom_forget()
+    M0, s0, shape0 = _run(shape, delta, N, M, "fp32")
+    path = tmp_path / "w"
+    # the key (mesh, precision, device) as this GPU writes it: from an export
+    solver.autotune(shape, delta, N=N, precision="fp32", method="rkf45", Ms=8.6e5, alpha=0.1, A=1.3e-11,
+                    k_uni=1e5, axis1=(0.0, 0.0, 1.0), h_ext=(1e4, 0.0, 0.0), eps_abs=1e-5, eps_rel=1e-5, steps=2)
+    solver.wisdom_export(str(path))
+    (line,) = [ln for ln in open(path) if not ln.startswith("#")]
+    key = " ".join(line.split()[:3])
+    path.write_text("%s 0 0 0 24 0\n" % key)
+    solver.wisdom_forget()
+    assert solver.wisdom_import(str(path)) == 1
+    M1, s1, shape1 = _run(shape, delta, N, M, "fp32")
+    solver.wisdom_forget()
+    assert 2 <= shape1["rpw"] <= 10 and shape1["stage_threads"] <= 1024
+    assert np.array_equal(M1, M0) and s1 == s0
