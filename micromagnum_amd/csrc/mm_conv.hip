@@ -1222,7 +1222,9 @@ int ConvImpl<T>::batch_z(BzDesc<T> &d, int threads, size_t &lds, bool &o32, cons
 {
     const size_t cs = sizeof(C2<T>);
     MM_REQUIRE(!g.is2d && threads >= 3 * pz.tline, "batch z pass: 2-D system or too few threads");
-    int tz = std::min(16, floor_pow2(std::max(1, threads / (3 * pz.tline))));
+    // the system's own z tile width (wide for short z lines), within the
+    // launch's threads
+    int tz = std::min(ln.tk_z, floor_pow2(std::max(1, threads / (3 * pz.tline))));
     while (tz > 1 && lines_lds<MM_SKEW_Z>(3 * tz, pz.L, cs) > (size_t)mm_max_lds_bytes()) tz /= 2;
     MM_REQUIRE((size_t)tz * (pz.L / 2 + 1) <= (size_t)ZM_ITEMS * threads, "z pass items per thread");
     const unsigned ntz = (unsigned)((g.Hx + tz - 1) / tz), nb = ntz * (unsigned)g.Py;
