@@ -483,55 +483,70 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
             kv[q][2] = v.z;
         }
     const double h = ctl->h;
-    T kc[3] = {k.x, k.y, k.z};
+    // stage vectors outermost, components inside: each tableau coefficient is
+    // read once per cell (with the components outermost, the guarded loops
+    // loaded every coefficient again per component); per component the sums
+    // keep the order q = 0, 1, ..., self
+    const int nk = G.nk, form = G.form, prep = G.prep;
+    const T kc[3] = {k.x, k.y, k.z};
     T out[3];
     double er[3] = {0.0, 0.0, 0.0};
+    if (form == FORM_SUM) {
+        T sum[3];
+        double e[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        if (G.form == FORM_SUM) {
-            T s = (T)0;
-            bool first = true;
+        for (int q = 0; q < MM_STAGE_MAXK; ++q) {
+            if (q < nk) {
+                const T bq = (T)G.b[q];
 #pragma unroll
-            for (int q = 0; q < MM_STAGE_MAXK; ++q) {
-                if (q < G.nk) {
-                    const T term = (T)G.b[q] * kv[q][c];
-                    s = first ? term : s + term;
-                    first = false;
+                for (int c = 0; c < 3; ++c) {
+                    const T term = bq * kv[q][c];
+                    sum[c] = q == 0 ? term : sum[c] + term;
                 }
-            }
-            const T ts = (T)G.bself * kc[c];
-            s = first ? ts : s + ts;
-            out[c] = ycv[c] + (T)h * s;
-            if (G.prep == 2) {
-                double e = 0.0;
-                bool f2 = true;
+                if (prep == 2) {
+                    const double ecq = G.ec[q];
 #pragma unroll
-                for (int q = 0; q < MM_STAGE_MAXK; ++q) {
-                    if (q < G.nk) {
-                        const double term = G.ec[q] * (double)kv[q][c];
-                        e = f2 ? term : e + term;
-                        f2 = false;
+                    for (int c = 0; c < 3; ++c) {
+                        const double term = ecq * (double)kv[q][c];
+                        e[c] = q == 0 ? term : e[c] + term;
                     }
                 }
-                const double tse = G.ecself * (double)kc[c];
-                e = f2 ? tse : e + tse;
-                er[c] = h * e;
             }
-        } else {
-            T v = ycv[c];
-            double e = 0.0;
+        }
+        const T bself = (T)G.bself;
+        const double ecself = G.ecself;
 #pragma unroll
-            for (int q = 0; q < MM_STAGE_MAXK; ++q) {
-                if (q < G.nk) {
+        for (int c = 0; c < 3; ++c) {
+            const T ts = bself * kc[c];
+            const T sc = nk == 0 ? ts : sum[c] + ts;
+            out[c] = ycv[c] + (T)h * sc;
+            if (prep == 2) {
+                const double tse = ecself * (double)kc[c];
+                er[c] = h * (nk == 0 ? tse : e[c] + tse);
+            }
+        }
+    } else {
+        T v[3] = {ycv[0], ycv[1], ycv[2]};
+#pragma unroll
+        for (int q = 0; q < MM_STAGE_MAXK; ++q) {
+            if (q < nk) {
+                const T f = (T)(h * G.b[q]);
+                const double fe = prep == 2 ? h * G.ec[q] : 0.0;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
                     const T kq = kv[q][c];
-                    v += kq * (T)(h * G.b[q]);
-                    if (G.prep == 2) e += (double)kq * (h * G.ec[q]);
+                    v[c] += kq * f;
+                    if (prep == 2) er[c] += (double)kq * fe;
                 }
             }
-            v += kc[c] * (T)(h * G.bself);
-            if (G.prep == 2) e += (double)kc[c] * (h * G.ecself);
-            out[c] = v;
-            er[c] = e;
+        }
+        const T fs = (T)(h * G.bself);
+        const double fes = h * G.ecself;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            v[c] += kc[c] * fs;
+            if (prep == 2) er[c] += (double)kc[c] * fes;
+            out[c] = v[c];
         }
     }
     if (G.prep == 1) {
