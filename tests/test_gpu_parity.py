@@ -804,23 +804,27 @@ def test_stepper_periodic_vs_oracle(pbc, method):
     st.close()
 
 
+@pytest.mark.parametrize("per_graph", ["4", "16"])
 @pytest.mark.parametrize("method", ["rkf45", "dp54", "rk4"])
-def test_stepper_graph_replays_equal_eager_launches(method, monkeypatch):
-    # attempts replayed from captured HIP graphs against the same attempts
-    # launched kernel by kernel: bitwise equal after every call, with eager
-    # work (dM/dt evaluation, <M>) and synchronisation between the replays
+def test_stepper_graph_replays_equal_eager_launches(method, per_graph, monkeypatch):
+    # attempts replayed from captured HIP graphs (per_graph attempts to a
+    # long graph, the rest one to a graph; attempts after the run has stopped
+    # do nothing) against the same attempts launched kernel by kernel:
+    # bitwise equal after every call, with eager work (dM/dt evaluation,
+    # <M>) and synchronisation between the replays
     from micromagnum_amd.solver import Stepper
     shape, delta = (20, 16, 3), (3e-9, 3e-9, 2e-9)
     kw = _system(shape, with_stt=True)
     M = random_unit_field(shape, 51) * 1.1e6
     out = {}
+    monkeypatch.setenv("MM_GRAPH_ATTEMPTS", per_graph)
     for graphs in ("1", "0"):
         monkeypatch.setenv("MM_STEPPER_GRAPHS", graphs)
         st = Stepper(shape, delta, precision="fp64", method=method, eps_abs=1e-4, eps_rel=1e-4, h0=5e-14, **kw)
         st.set_m(M)
         states = []
         for rep in range(4):
-            st.run(3 + rep)
+            st.run(3 + 7 * rep)
             st.sync()
             st.dmdt()
             states.append((st.get_m(), st.stats(), st.average_m()))
