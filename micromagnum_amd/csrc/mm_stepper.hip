@@ -375,14 +375,23 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
             store3<T>(a, n, i, v);
         }
     };
-    const T ms = S.Ms_uniform ? S.Ms.u : (O32 ? ld32(gfield(S.Ms.p), ib) : S.Ms.at(i));
+    // the record's flags and uniform parameters of this cell's path, read in
+    // one block (merged scalar loads, one wait) instead of inside each
+    // guarded section
+    const int Ms_uniform = S.Ms_uniform, has_exch = S.has_exch, has_uni = S.has_uni, uni_fast = S.uni_fast;
+    const int has_zee = S.has_zee;
+    const T Ms_u = S.Ms.u, wx = S.wx, wy = S.wy, wz = S.wz, exch_f = S.exch_f, uni_c = S.uni_c;
+    const T ax0 = S.axis1.u[0], ax1 = S.axis1.u[1], ax2 = S.axis1.u[2];
+    const T *const f1p = S.f1.p, *const f2p = S.f2.p;
+    const T f1u = S.f1.u, f2u = S.f2.u;
+    const T ms = Ms_uniform ? Ms_u : (O32 ? ld32(gfield(S.Ms.p), ib) : S.Ms.at(i));
     const V3<T> Mv = ldv(Y);
     V3<T> H = Hd;
     if (EN) dd_add(ea.demag, dotd<T>(Mv, Hd));
-    if (S.has_exch && ms != (T)0) {
+    if (has_exch && ms != (T)0) {
         V3<T> he;
         if constexpr (O32) {
-            he = exchange_cell_uniform32<T>(S.m, x, y, z, ib, nb, S.wx, S.wy, S.wz, S.exch_f, Mv, Y);
+            he = exchange_cell_uniform32<T>(S.m, x, y, z, ib, nb, wx, wy, wz, exch_f, Mv, Y);
         } else {
             he = S.exch_uniform
                      ? exchange_cell_uniform<T>(S.m, x, y, z, i, n, S.wx, S.wy, S.wz, S.exch_f, Y)
@@ -394,15 +403,15 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
         H.z += he.z;
         if (EN) dd_add(ea.exch, dotd<T>(Mv, he));
     }
-    if (S.has_uni || (GEN && S.has_cub)) {
+    if (has_uni || (GEN && S.has_cub)) {
         T e0 = (T)0, e1 = (T)0;
         V3<T> ha = v3<T>(0, 0, 0);
         if (GEN && S.has_cub) ha = cubic_cell<T>(Mv, S.axis1.at(n, i), S.axis2.at(n, i), S.kcub.at(i), ms, S.cm2mu0, &e1);
-        if (S.has_uni) {
+        if (has_uni) {
             V3<T> hu;
-            if (S.uni_fast) {
-                const V3<T> ax = v3<T>(S.axis1.u[0], S.axis1.u[1], S.axis1.u[2]);
-                const T f = S.uni_c * dot3(Mv, ax);
+            if (uni_fast) {
+                const V3<T> ax = v3<T>(ax0, ax1, ax2);
+                const T f = uni_c * dot3(Mv, ax);
                 hu = v3<T>(f * ax.x, f * ax.y, f * ax.z);
                 if (EN) {
                     const V3<T> c = cross3(ax, Mv);
@@ -425,14 +434,14 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
         H.y += ha.y;
         H.z += ha.z;
     }
-    if (S.has_zee) {
+    if (has_zee) {
         H.x += hz.x;
         H.y += hz.y;
         H.z += hz.z;
         if (EN) dd_add(ea.zee, (double)Mv.x * hz.x + (double)Mv.y * hz.y + (double)Mv.z * hz.z);
     }
     if (EN) return;
-    V3<T> k = llge_cell<T>(Mv, H, S.f1.at(i), S.f2.at(i));
+    V3<T> k = llge_cell<T>(Mv, H, f1p ? gfield(f1p)[i] : f1u, f2p ? gfield(f2p)[i] : f2u);
     if (STT && S.has_stt) {
         V3<T> st;
         if (O32 && S.stt_fast)
