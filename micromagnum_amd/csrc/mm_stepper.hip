@@ -349,7 +349,7 @@ struct EnergyAcc {
 // and needs fewer registers).  Without GEN the state arrays are addressed by
 // 32-bit byte offsets from their (workgroup-uniform) bases, which the host
 // guarantees by taking GEN for state arrays of 4 GiB or more.
-template <typename T, bool EN, bool STT, bool GEN>
+template <typename T, bool EN, bool STT, bool GEN, bool P1S = false>
 __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T> &G, const StepCtl *ctl,
                                            const T *__restrict__ Y, int x, int y, int z, const V3<T> &Hd,
                                            const V3<T> &hz, StageRed &red, EnergyAcc &ea, T *nxt)
@@ -470,7 +470,7 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
         }
         red.dydt = fmax(red.dydt, q);
     }
-    if (!G.prep) return;
+    if (!P1S && !G.prep) return;
     // the combination's inputs (stored stage vectors, the attempt's start
     // state) load in one batch of independent loads before any arithmetic
     // (loading them here rather than at the top keeps them out of the field
@@ -496,7 +496,7 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
     // read once per cell (with the components outermost, the guarded loops
     // loaded every coefficient again per component); per component the sums
     // keep the order q = 0, 1, ..., self
-    const int nk = G.nk, form = G.form, prep = G.prep;
+    const int nk = G.nk, form = P1S ? FORM_SUM : G.form, prep = P1S ? 1 : G.prep;
     const T kc[3] = {k.x, k.y, k.z};
     T out[3];
     double er[3] = {0.0, 0.0, 0.0};
@@ -558,7 +558,7 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
             out[c] = v[c];
         }
     }
-    if (G.prep == 1) {
+    if (prep == 1) {
         stv(gfield(G.prep_out), v3<T>(out[0], out[1], out[2]));
         nxt[0] = out[0];
         nxt[1] = out[1];
@@ -613,7 +613,7 @@ __device__ __forceinline__ void stage_cell(const SysDev<T> &S, const StageDev<T>
 // workgroup's index among the nblk of the system's launch; remap: order the
 // tiles by xcd_tile(blk, nblk) (the single-system launch; a batched launch
 // orders its global workgroups before it picks the system).
-template <typename T, bool EN, int LC, bool STT, bool GEN>
+template <typename T, bool EN, int LC, bool STT, bool GEN, bool P1S = false>
 __device__ __forceinline__ void stage_body(const SysDev<T> *__restrict__ Sp, const StageDev<T> *__restrict__ Gp,
                                            StepCtl *ctl, const C2<T> *__restrict__ S1, const FftPlanDev &p, int Hx,
                                            int Hp, int rpw, int has_demag, double *__restrict__ epart,
@@ -627,7 +627,7 @@ __device__ __forceinline__ void stage_body(const SysDev<T> *__restrict__ Sp, con
     constexpr int SK = MM_SKEW_X;
     // stages of an attempt (prep != 0) do nothing once the run has stopped;
     // evaluations outside attempts (dM/dt, energies) always run
-    if (!EN && G.prep && ctl->stop) return;
+    if (!EN && (P1S || G.prep) && ctl->stop) return;
     extern __shared__ char smem_raw[];
     C2<T> *sm = reinterpret_cast<C2<T> *>(smem_raw);
     const int nx = S.m.nx, ny = S.m.ny, nz = S.m.nz;
@@ -719,7 +719,7 @@ __device__ __forceinline__ void stage_body(const SysDev<T> *__restrict__ Sp, con
                 Hd = v3<T>(Hdem[i], Hdem[S.n + i], Hdem[2 * S.n + i]);
             }
             T nxt[3];
-            stage_cell<T, EN, STT, GEN>(S, G, ctl, Y, x, yr, zr, Hd, hz, red, ea, nxt);
+            stage_cell<T, EN, STT, GEN, P1S>(S, G, ctl, Y, x, yr, zr, Hd, hz, red, ea, nxt);
             // the half slot (row, x) of the three lines is this thread's
             // alone: the next stage input replaces H_demag in place
             if (fused)
@@ -783,7 +783,7 @@ __device__ __forceinline__ void stage_body(const SysDev<T> *__restrict__ Sp, con
         }
         return;
     }
-    if (G.prep == 2 || G.want_dydt || G.dydt_from_k0) {
+    if ((!P1S && (G.prep == 2 || G.dydt_from_k0)) || G.want_dydt) {
         const double a = wave_max(red.err), b = wave_max(red.dydt), c = wave_max(red.draw), d = wave_max(red.mraw);
         if (lane == 0) {
             wred[0][w] = a;
@@ -801,8 +801,8 @@ __device__ __forceinline__ void stage_body(const SysDev<T> *__restrict__ Sp, con
                 ed = fmax(ed, wred[3][q]);
             }
             double *mp = gfield(G.mpart);
-            if (G.prep == 2) mp[blk] = ea0;
-            if (G.want_dydt || G.dydt_from_k0) {
+            if (!P1S && G.prep == 2) mp[blk] = ea0;
+            if (G.want_dydt || (!P1S && G.dydt_from_k0)) {
                 mp[(size_t)nblk + blk] = eb;
                 mp[2 * (size_t)nblk + blk] = ec;
                 mp[3 * (size_t)nblk + blk] = ed;
@@ -811,12 +811,15 @@ __device__ __forceinline__ void stage_body(const SysDev<T> *__restrict__ Sp, con
     }
 }
 
-template <typename T, bool EN, int LC, bool STT, bool GEN>
+// P1S: the record is an inner stage with a summed combination (prep 1,
+// FORM_SUM, no FSAL reads), so the final-stage code and its record reads
+// are compiled out (RKF45 / CC45: five of the six stage launches)
+template <typename T, bool EN, int LC, bool STT, bool GEN, bool P1S = false>
 __global__ void k_stage(const SysDev<T> *__restrict__ Sp, const StageDev<T> *__restrict__ Gp, StepCtl *ctl,
                         const C2<T> *__restrict__ S1, FftPlanDev p, int Hx, int Hp, int rpw, int has_demag,
                         double *__restrict__ epart, const T *__restrict__ Hdem)
 {
-    stage_body<T, EN, LC, STT, GEN>(Sp, Gp, ctl, S1, p, Hx, Hp, rpw, has_demag, epart, Hdem, (int)blockIdx.x,
+    stage_body<T, EN, LC, STT, GEN, P1S>(Sp, Gp, ctl, S1, p, Hx, Hp, rpw, has_demag, epart, Hdem, (int)blockIdx.x,
                                     (int)gridDim.x, true);
 }
 
@@ -1460,9 +1463,10 @@ template <typename T> struct Stepper : public StepperBase {
     // the stage kernel instantiated for the x transform length and the
     // physics present: STT (Zhang-Li) and GEN (cubic anisotropy or exchange
     // with per-cell Ms/A); the energy launch (EN) has no spin torque term
-    template <bool EN, int LC, bool STT, bool GEN> void stage_launch(const StageDev<T> *Gd, double *ep)
+    template <bool EN, int LC, bool STT, bool GEN, bool P1S = false>
+    void stage_launch(const StageDev<T> *Gd, double *ep)
     {
-        hipLaunchKernelGGL((k_stage<T, EN, LC, STT, GEN>), dim3(grid_rows), dim3(threads), stage_lds, s, sysd, Gd,
+        hipLaunchKernelGGL((k_stage<T, EN, LC, STT, GEN, P1S>), dim3(grid_rows), dim3(threads), stage_lds, s, sysd, Gd,
                            ctl, conv ? conv->S1 : nullptr, conv ? conv->px.template dev<T>() : FftPlanDev(),
                            conv ? conv->g.Hx : 0, conv ? conv->g.Hp : 0, rpw, demag_mode(), ep, Hdem);
     }
@@ -1495,7 +1499,9 @@ template <typename T> struct Stepper : public StepperBase {
         return nb;
     }
 
-    template <bool EN> void stage_kernel(const StageDev<T> *Gd, double *ep)
+    // p1s: the record is an inner stage with a summed combination (the
+    // compact physics class has a kernel compiled for it)
+    template <bool EN> void stage_kernel(const StageDev<T> *Gd, double *ep, bool p1s = false)
     {
         bool stt, gen;
         physics_class(stt, gen);
@@ -1506,8 +1512,17 @@ template <typename T> struct Stepper : public StepperBase {
                 if (gen) stage_launch<EN, LC, !EN, true>(Gd, ep);
                 else stage_launch<EN, LC, !EN, false>(Gd, ep);
             } else {
-                if (gen) stage_launch<EN, LC, false, true>(Gd, ep);
-                else stage_launch<EN, LC, false, false>(Gd, ep);
+                if (gen) {
+                    stage_launch<EN, LC, false, true>(Gd, ep);
+                } else {
+                    if constexpr (!EN) {
+                        if (p1s) {
+                            stage_launch<EN, LC, false, false, true>(Gd, ep);
+                            return;
+                        }
+                    }
+                    stage_launch<EN, LC, false, false>(Gd, ep);
+                }
             }
         });
     }
@@ -1524,7 +1539,7 @@ template <typename T> struct Stepper : public StepperBase {
             conv->spectral(s);
             if (split_x) conv->inverse_x(Hdem, s);
         }
-        stage_kernel<false>(Gd, nullptr);
+        stage_kernel<false>(Gd, nullptr, G.prep == 1 && G.form == FORM_SUM && !G.dydt_from_k0 && !G.fsal_out);
         MM_HIP(hipGetLastError());
         if (klog) klog->mark(s, KC_STAGE, stage_bytes(G));
     }
