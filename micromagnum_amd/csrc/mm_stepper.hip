@@ -2407,12 +2407,12 @@ struct BCtlDesc {
     double eps_abs, eps_rel, Ms_u;
 };
 
-template <typename T, bool STT, bool GEN>
+template <typename T, bool STT, bool GEN, bool P1S = false>
 __global__ void kb_stage(const BStageDesc<T> *__restrict__ D, const int *__restrict__ tsys, int st)
 {
     const int gi = block_tile(1);
     const BStageDesc<T> &d = D[tsys[gi]];
-    stage_body<T, false, 0, STT, GEN>(gfield(d.S), gfield(d.G0) + st, gfield(d.ctl), gfield(d.S1), d.p, d.Hx, d.Hp,
+    stage_body<T, false, 0, STT, GEN, P1S>(gfield(d.S), gfield(d.G0) + st, gfield(d.ctl), gfield(d.S1), d.p, d.Hx, d.Hp,
                                       d.rpw, 1, nullptr, nullptr, gi - d.tbase, d.nb, false);
 }
 
@@ -2644,8 +2644,15 @@ template <typename T> struct BatchRunner : BatchRunnerBase {
             if (gr.gen) hipLaunchKernelGGL((kb_stage<T, true, true>), grid, block, gr.bp.lds, s, D, gr.bp.tsys, st);
             else hipLaunchKernelGGL((kb_stage<T, true, false>), grid, block, gr.bp.lds, s, D, gr.bp.tsys, st);
         } else {
-            if (gr.gen) hipLaunchKernelGGL((kb_stage<T, false, true>), grid, block, gr.bp.lds, s, D, gr.bp.tsys, st);
-            else hipLaunchKernelGGL((kb_stage<T, false, false>), grid, block, gr.bp.lds, s, D, gr.bp.tsys, st);
+            // inner stages of a six-stage tableau (summed combination, no
+            // FSAL in a batch): the kernel compiled for them
+            const bool p1s = ns == 6 && st < ns - 1;
+            if (gr.gen)
+                hipLaunchKernelGGL((kb_stage<T, false, true>), grid, block, gr.bp.lds, s, D, gr.bp.tsys, st);
+            else if (p1s)
+                hipLaunchKernelGGL((kb_stage<T, false, false, true>), grid, block, gr.bp.lds, s, D, gr.bp.tsys, st);
+            else
+                hipLaunchKernelGGL((kb_stage<T, false, false>), grid, block, gr.bp.lds, s, D, gr.bp.tsys, st);
         }
         MM_HIP(hipGetLastError());
     }
