@@ -813,9 +813,13 @@ __device__ __forceinline__ void stage_body(const SysDev<T> *__restrict__ Sp, con
 
 // P1S: the record is an inner stage with a summed combination (prep 1,
 // FORM_SUM, no FSAL reads), so the final-stage code and its record reads
-// are compiled out (RKF45 / CC45: five of the six stage launches)
+// are compiled out (RKF45 / CC45: five of the six stage launches).  In FP32
+// that kernel needs 55-59 VGPRs and is held to 7 waves per SIMD by its 106
+// SGPRs; asking for 8 (SGPRs 78, the rest spilled to VGPR lanes, no
+// scratch) lets a CU hold 8 of its 256-thread workgroups, one round for
+// 256x256x16 cells: stage 32.4 -> 30.7 us (MI355X, same-run A/B).
 template <typename T, bool EN, int LC, bool STT, bool GEN, bool P1S = false>
-__global__ void k_stage(const SysDev<T> *__restrict__ Sp, const StageDev<T> *__restrict__ Gp, StepCtl *ctl,
+__global__ void __attribute__((amdgpu_waves_per_eu((P1S && sizeof(T) == 4) ? 8 : 1))) k_stage(const SysDev<T> *__restrict__ Sp, const StageDev<T> *__restrict__ Gp, StepCtl *ctl,
                         const C2<T> *__restrict__ S1, FftPlanDev p, int Hx, int Hp, int rpw, int has_demag,
                         double *__restrict__ epart, const T *__restrict__ Hdem)
 {
