@@ -1504,7 +1504,7 @@ template <typename T> struct Stepper : public StepperBase {
     }
 
     // p1s: the record is an inner stage with a summed combination (the
-    // compact physics class has a kernel compiled for it)
+    // classes without spin torque have a kernel compiled for it)
     template <bool EN> void stage_kernel(const StageDev<T> *Gd, double *ep, bool p1s = false)
     {
         bool stt, gen;
@@ -1517,6 +1517,12 @@ template <typename T> struct Stepper : public StepperBase {
                 else stage_launch<EN, LC, !EN, false>(Gd, ep);
             } else {
                 if (gen) {
+                    if constexpr (!EN) {
+                        if (p1s) {
+                            stage_launch<EN, LC, false, true, true>(Gd, ep);
+                            return;
+                        }
+                    }
                     stage_launch<EN, LC, false, true>(Gd, ep);
                 } else {
                     if constexpr (!EN) {
